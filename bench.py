@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the MI355X matcher.
+
+Workload (one GPU): the plain FP32 matcher on a 384-channel, 512-template
+probe (Gaussian footprints sigma = 2 channels, rates log-uniform 0.5-20 Hz,
+amplitudes N(12, 3), banded-correlation whitening), streamed as
+3000-sample chunks framed by 61-sample margins (W = 3122 columns).  One
+step = one chunk through the whole path (int16 -> whitening -> PC conv ->
+projection -> peel loop -> events + PC features on the host).  The recording
+is 60 s of synthetic data held in HBM before the timed region.
+
+Prints one JSON line: realtime factor (seconds of data per wall second),
+p50/p99 chunk latency, the dominant kernel's roofline figure and the CPU
+baseline (the FP32 CPU restatement of the same algorithm on the host cores,
+over a bounded sample of the same chunks).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "realtime factor, p99 chunk latency (384ch); BF16/gated vs FP32 @99% spike match"
+FS = 30000.0
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_data(cache: Path, seconds: float):
+    from livespikesorter_amd import synth
+    cache.mkdir(parents=True, exist_ok=True)
+    d = cache / "cfg2_oss_input"
+    m = synth.make_model(**synth.CONFIGS["cfg2"])
+    if not (d / "ctc.npy").exists():
+        synth.write_oss_input(m, d)
+    rec_path = cache / f"cfg2_rec_{int(seconds)}s.npy"
+    if rec_path.exists():
+        x = np.load(rec_path)
+    else:
+        x, _ = synth.make_recording(m, int(seconds * FS))
+        np.save(rec_path, x)
+    return m, d, x
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--seconds", type=float, default=60.0, help="length of the synthetic recording")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-steps", type=int, default=100, help="chunks timed with host-resident input")
+    ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the matcher runs one independent instance per probe "
+              "stream per GPU and this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    from livespikesorter_amd import OnlineSpikesV2
+
+    N, M = 3000, 61
+    W = N + 2 * M
+    t0 = time.time()
+    m, d, x = make_data(Path(args.cache), args.seconds)
+    n_batches = (len(x) - W) // N + 1
+    log(f"data ready in {time.time() - t0:.1f}s: {len(x)} samples x {x.shape[1]} ch, {n_batches} chunks")
+
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    torch.cuda.synchronize()
+    base = dx.data_ptr()
+    C = x.shape[1]
+
+    sp = OnlineSpikesV2(d, chunk=N, profile=True)
+    info = sp.info
+
+    def step(j):
+        b = j % n_batches
+        return sp.process_batch_device(base + b * N * C * 2, W, C, first_sample=b * N)
+
+    for j in range(args.warmup):
+        step(j)
+    sp.stage_times(reset=True)
+    lat, iters, spikes = [], [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for j in range(args.warmup, args.warmup + args.steps):
+        t1 = time.perf_counter()
+        ev, _ = step(j)
+        lat.append((time.perf_counter() - t1) * 1e3)
+        q = sp.query()
+        iters.append(q.last_iters)
+        spikes.append(q.last_spikes)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    stage_ms, nb = sp.stage_times(reset=True)
+    rtf = args.steps * N / FS / elapsed
+    log(f"timed {args.steps} chunks in {elapsed:.3f}s: {rtf:.1f}x realtime, p50 {pct(lat, 50):.3f} ms, "
+        f"p99 {pct(lat, 99):.3f} ms; mean iterations {np.mean(iters):.1f}, spikes {np.mean(spikes):.1f}")
+    avg = {k: v / max(nb, 1) for k, v in stage_ms.items()}
+    log("device ms per chunk by stage: " + ", ".join(f"{k} {v:.4f}" for k, v in avg.items()))
+
+    # host-resident input: chunk latency from int16 on the host to events on the host
+    host_lat = []
+    for j in range(args.host_steps):
+        b = j % n_batches
+        t1 = time.perf_counter()
+        sp.process_batch(x[b * N:b * N + W], first_sample=b * N)
+        host_lat.append((time.perf_counter() - t1) * 1e3)
+    sp.stage_times(reset=True)
+
+    # roofline of the dominant device stage (algorithmic work per launch / measured time)
+    T, K = info.T, info.K
+    Wc = (W + 63) // 64 * 64
+    I = float(np.mean(iters))
+    S = float(np.mean(spikes))
+    work = {
+        "whiten": ("mfma", 2.0 * info.white_kblocks * 64 * 16 * Wc / 1e12, "TFLOP/s", FP32_MFMA_PEAK_TFLOPS),
+        "pcconv": ("hbm", (4.0 * C * W + 4.0 * K * C * W) / 1e9, "GB/s", HBM_PEAK_GBS),
+        "project": ("mfma", 2.0 * info.proj_kblocks * 64 * 16 * Wc / 1e12, "TFLOP/s", FP32_MFMA_PEAK_TFLOPS),
+        "peel": ("hbm", (I * 4.0 * T * W + I * 8.0 * W + S * T * (2 * M + 1) * 12.0
+                         + S * 10 * M * 4.0) / 1e9, "GB/s", HBM_PEAK_GBS),
+    }
+    dom = max(work, key=lambda k: avg[k])
+    bound, amount, unit, peak = work[dom]
+    achieved = amount / (avg[dom] / 1e3)
+    roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 5), "traffic": None, "kernel": dom,
+                "ms_per_launch": round(avg[dom], 5)}
+    stages = {k: {"ms": round(avg[k], 5), "bound": work[k][0] if k in work else "pcie",
+                  "achieved": round(work[k][1] / (avg[k] / 1e3), 3) if k in work and avg[k] > 0 else None,
+                  "unit": work[k][2] if k in work else None} for k in avg if k != "h2d_copy"}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+        from tests.parity import agreement
+        tab = oracle.load_tables(d)
+        threads = len(os.sched_getaffinity(0))
+        done, cpu_t, ident_n, within_n, tot_n, amp_err = 0, 0.0, 0, 0, 0, 0.0
+        j = 0
+        while cpu_t < args.cpu_seconds and j < n_batches:
+            batch = x[j * N:j * N + W]
+            t1 = time.perf_counter()
+            ref = oracle.run_model_batch(tab, batch, bits=32, threads=threads)
+            cpu_t += time.perf_counter() - t1
+            ev, _ = sp.process_batch(batch, first_sample=j * N)
+            rs = ref.w + j * N - M // 2 + 20
+            ident, within, pairs = agreement(rs, ref.t, ev["sample"], ev["template"])
+            n = max(len(rs), len(ev))
+            ident_n += ident * n
+            within_n += within * n
+            tot_n += n
+            if pairs:
+                ri = np.array([p[0] for p in pairs])
+                gi = np.array([p[1] for p in pairs])
+                amp_err = max(amp_err, float(np.max(np.abs(ev["amplitude"][gi] - ref.amp[ri]) / np.abs(ref.amp[ri]))))
+            done += 1
+            j += 1
+        cpu = {"value": round(done * N / FS / cpu_t, 4), "unit": "x realtime", "cores": threads,
+               "kind": "port",
+               "sample": f"first {done} chunks of the same recording (W={W}), FP32 C restatement "
+                         f"(Kilosort rules, 100 iterations), OpenMP {threads} threads, {cpu_t:.1f}s"}
+        parity = {"vs": "CPU FP32 restatement", "chunks": done, "events": tot_n,
+                  "identical": round(ident_n / max(tot_n, 1), 6),
+                  "within_1_sample": round(within_n / max(tot_n, 1), 6), "amp_rel_err_max": amp_err}
+        log(f"cpu baseline {cpu['value']}x realtime over {done} chunks; parity {parity}")
+    sp.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(rtf, 3),
+        "unit": "x realtime",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator: 384 ch, 512 Gaussian-footprint templates, banded whitening)",
+        "config": {"workload": "plain FP32 matcher, 384 channels, 512 templates, 3000-sample chunks "
+                               "+ 61-sample margins (W=3122), 60 s recording resident in HBM",
+                   "channels": C, "templates": T, "chunk": N, "batch_columns": W, "semantics": "kilosort",
+                   "max_iter": info.max_iter},
+        "p50_ms": round(pct(lat, 50), 4),
+        "p99_ms": round(pct(lat, 99), 4),
+        "p99_ms_host_input": round(pct(host_lat, 99), 4),
+        "p50_ms_host_input": round(pct(host_lat, 50), 4),
+        "mean_iterations": round(I, 2),
+        "mean_spikes_per_chunk": round(S, 2),
+        "stages": stages,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,148 @@
+/*
+ * lss.h -- C ABI of the MI355X online template matcher.
+ *
+ * The drop-in boundary for the matcher of the reference's OnlineSpikesV2
+ * (src/SpikeSorter/OnlineSpikesV2.h:29-56, src/SpikeSorter/OnlineSpikesV2.cu:248-1001).
+ * Plain C types only: no HIP or torch types cross this interface, no C++
+ * exception escapes it.  Every function returns LSS_OK (0) or a negative
+ * status; lss_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * One handle per probe stream and host thread; handles share nothing, so
+ * several handles may run on several threads without locks.  The caller owns
+ * every host buffer passed in; the library owns the device buffers.
+ */
+#ifndef LSS_H
+#define LSS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSS_OK 0
+#define LSS_ERR_ARG -1       /* bad argument or size beyond what the handle was created for */
+#define LSS_ERR_IO -2        /* oss_input file missing or malformed */
+#define LSS_ERR_HIP -3       /* HIP runtime failure */
+#define LSS_ERR_CAPACITY -4  /* more events than the caller's buffer holds (buffer filled, *n_ev = total) */
+#define LSS_ERR_STATE -5     /* call out of order (e.g. lss_flush before any chunk) */
+
+/* Peak-acceptance rules.  KILOSORT follows run_matching
+ * (KS/template_matching.py:159-198: Cmax > Th^2, exact peak, one template per
+ * sample, 100 iterations).  LSS follows the CUDA matcher
+ * (myGPUhelpers.cu:419-478, OnlineSpikesV2.cu:903-1001: Cmax >= Th^2, 1e-4
+ * tie tolerance per template, Th truncated to an integer, 50 iterations). */
+#define LSS_SEM_KILOSORT 0
+#define LSS_SEM_LSS 1
+
+#define LSS_PREC_FP32 0          /* FP32 projection (FP32 MFMA) */
+#define LSS_PREC_BF16_RESCORE 1  /* BF16 MFMA projection, FP32 accumulate, near-threshold FP32 rescore */
+
+typedef struct lss_config {
+    int32_t device;        /* HIP device ordinal (OnlineSpikesV2.cu:356-357) */
+    int32_t chunk;         /* samples per streaming chunk, for lss_process_chunk (default 3000) */
+    int32_t max_batch;     /* largest batch, margins included (default chunk + 2*M) */
+    int32_t max_iter;      /* peel iterations; 0 = semantics default (100 / 50) */
+    int32_t semantics;     /* LSS_SEM_* */
+    int32_t precision;     /* LSS_PREC_* */
+    int32_t dedup;         /* per-template duplicate suppression within dt samples
+                              (saveSpikes, OnlineSpikesV2.cu:1192-1194); -1 = on for LSS only */
+    int32_t max_events;    /* event capacity per batch on the device (default 65536) */
+    int32_t keep_stages;   /* nonzero: keep per-stage device copies for lss_debug_stage */
+    int32_t sparse;        /* nonzero (default): skip exact-zero template/channel blocks */
+    int32_t profile;       /* nonzero: time each device stage with HIP events (lss_stage_times) */
+    int32_t reserved[5];
+} lss_config;
+
+/* One spike: (sample, template, amplitude), as OnlineSpikesPayload carries them
+ * (src/SpikeSorter/onlineSpikesPayload.h:8-48).  `sample` follows Kilosort's
+ * convention st = column - nt//2 + nt0min + (absolute index of column 0)
+ * (KS/template_matching.py:68). */
+typedef struct lss_event {
+    int64_t sample;
+    int32_t template_id;
+    float amplitude;
+} lss_event;
+
+typedef struct lss_info {
+    int32_t C, T, M, K, nC;
+    int32_t nt0min, dt, semantics;
+    float Th;
+    int32_t max_batch, chunk, max_iter;
+    int64_t batches;        /* batches processed */
+    int64_t events;         /* events emitted */
+    int32_t last_iters;     /* peel iterations that found spikes in the last batch */
+    int32_t last_spikes;    /* spikes found in the last batch */
+    double last_ms;         /* wall time of the last batch, entry to events on host */
+    int64_t white_kblocks;  /* 64x16 whitening operand blocks computed per 64-sample column tile */
+    int64_t proj_kblocks;   /* 64x16 projection operand blocks computed per 64-sample column tile */
+    int64_t proj_kblocks_dense;  /* the same without exact-zero skipping */
+} lss_info;
+
+/* Fill *cfg with defaults. */
+int lss_config_init(lss_config* cfg);
+
+/* Load an oss_input directory and allocate the device state.
+ * Replaces OnlineSpikesV2::OnlineSpikesV2 / initializeSorter
+ * (OnlineSpikesV2.cu:248-281,322-382) and its loaders (:454-660). */
+int lss_create(const char* oss_input_dir, const lss_config* cfg, struct lss_handle** out);
+
+/* Match one batch whose columns are already framed with margins: W = n_samples
+ * columns of int16 samples, row stride `ld` (>= C) in the host buffer,
+ * channel c of the matcher read from column chan_map[c] (NULL = identity).
+ * Column 0 is absolute sample first_sample.  Spikes are only accepted in
+ * columns [M, W-M).  Events are sorted by sample.  feat (nullable) receives
+ * [ev_cap][nC][K] PC features.  Replaces kilosortMatchingPursuit
+ * (OnlineSpikesV2.h:56, OnlineSpikesV2.cu:903-1001) plus computeClosestClusters'
+ * feature part (OnlineSpikesV2.cu:853-875). */
+int lss_process_batch(struct lss_handle* h, const int16_t* samples, int64_t n_samples, int32_t ld,
+                      const int32_t* chan_map, int64_t first_sample,
+                      lss_event* ev, int64_t ev_cap, int64_t* n_ev, float* feat);
+
+/* As lss_process_batch, with `d_samples` a device pointer (HBM-resident input;
+ * chan_map is a host array). */
+int lss_process_batch_device(struct lss_handle* h, const int16_t* d_samples, int64_t n_samples,
+                             int32_t ld, const int32_t* chan_map, int64_t first_sample,
+                             lss_event* ev, int64_t ev_cap, int64_t* n_ev, float* feat);
+
+/* Streaming: hand over the next n_samples of the stream (first_sample = its
+ * absolute index; chunks must be contiguous).  The library frames the batch
+ * with the previous 2*M samples (edge-replicated before the first chunk, as
+ * Kilosort's first batch, KS/io.py:616-619) and emits the events of the
+ * samples whose right margin is complete.  Replaces the per-batch body of
+ * runSpikeSorting (OnlineSpikesV2.cu:700-845). */
+int lss_process_chunk(struct lss_handle* h, const int16_t* samples, int64_t n_samples, int32_t ld,
+                      const int32_t* chan_map, int64_t first_sample,
+                      lss_event* ev, int64_t ev_cap, int64_t* n_ev, float* feat);
+
+/* Emit the events held back by the right margin, padding the end of the
+ * stream by repeating its last sample (KS/io.py:620-623). */
+int lss_flush(struct lss_handle* h, lss_event* ev, int64_t ev_cap, int64_t* n_ev, float* feat);
+
+int lss_query(struct lss_handle* h, lss_info* info);
+
+/* Copy a device stage of the last batch to host memory (test support; needs
+ * keep_stages).  stage: 0 Xw [C][W] f32, 1 Bpc [K][C][W] f32, 2 B before the
+ * peel [T][W] f32, 3 Cfmax of the first iteration [W] f32, 4 imax [W] i32,
+ * 5 template norms [T] f32.  *bytes: in = capacity, out = size written. */
+int lss_debug_stage(struct lss_handle* h, int32_t stage, void* out, int64_t* bytes);
+
+/* Device time per stage accumulated over the batches run with cfg.profile set:
+ * ms[0] input copy, ms[1] whitening, ms[2] PC conv, ms[3] projection, ms[4] peel
+ * loop with events and features (LSS_NSTAGES entries); *batches = batches timed.
+ * reset != 0 clears the accumulators after reading.  No reference counterpart:
+ * the reference's per-stage Timer scopes print nothing (src/Helpers/Timer.h:12-16). */
+#define LSS_NSTAGES 5
+int lss_stage_times(struct lss_handle* h, double* ms, int64_t* batches, int32_t reset);
+
+int lss_destroy(struct lss_handle* h);
+
+/* Message of the last error on this thread ("" if none). */
+const char* lss_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSS_H */

@@ -1,0 +1,108 @@
+"""ctypes binding of ``liblss.so`` (the C ABI declared in include/lss.h).
+
+The library is the product: it holds every HIP kernel.  This module only
+loads it and converts numpy arrays to pointers.  If the library is missing
+or cannot be loaded, every entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "liblss.so"
+
+LSS_OK = 0
+LSS_ERR_ARG = -1
+LSS_ERR_IO = -2
+LSS_ERR_HIP = -3
+LSS_ERR_CAPACITY = -4
+LSS_ERR_STATE = -5
+
+SEM_KILOSORT = 0
+SEM_LSS = 1
+PREC_FP32 = 0
+PREC_BF16_RESCORE = 1
+
+ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_process_batch_device",
+                 "lss_process_chunk", "lss_flush", "lss_query", "lss_debug_stage", "lss_stage_times",
+                 "lss_destroy", "lss_last_error")
+NSTAGES = 5
+STAGE_NAMES = ("h2d_copy", "whiten", "pcconv", "project", "peel")
+
+
+class Config(ct.Structure):
+    _fields_ = [("device", ct.c_int32), ("chunk", ct.c_int32), ("max_batch", ct.c_int32),
+                ("max_iter", ct.c_int32), ("semantics", ct.c_int32), ("precision", ct.c_int32),
+                ("dedup", ct.c_int32), ("max_events", ct.c_int32), ("keep_stages", ct.c_int32),
+                ("sparse", ct.c_int32), ("profile", ct.c_int32), ("reserved", ct.c_int32 * 5)]
+
+
+class Event(ct.Structure):
+    _fields_ = [("sample", ct.c_int64), ("template_id", ct.c_int32), ("amplitude", ct.c_float)]
+
+
+EVENT_DTYPE = np.dtype([("sample", np.int64), ("template", np.int32), ("amplitude", np.float32)])
+assert EVENT_DTYPE.itemsize == ct.sizeof(Event) == 16
+
+
+class Info(ct.Structure):
+    _fields_ = [("C", ct.c_int32), ("T", ct.c_int32), ("M", ct.c_int32), ("K", ct.c_int32),
+                ("nC", ct.c_int32), ("nt0min", ct.c_int32), ("dt", ct.c_int32),
+                ("semantics", ct.c_int32), ("Th", ct.c_float), ("max_batch", ct.c_int32),
+                ("chunk", ct.c_int32), ("max_iter", ct.c_int32), ("batches", ct.c_int64),
+                ("events", ct.c_int64), ("last_iters", ct.c_int32), ("last_spikes", ct.c_int32),
+                ("last_ms", ct.c_double), ("white_kblocks", ct.c_int64), ("proj_kblocks", ct.c_int64),
+                ("proj_kblocks_dense", ct.c_int64)]
+
+
+class LssError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib() -> ct.CDLL:
+    """Load liblss.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(the HIP library is required; there is no CPU fallback)")
+        L = ct.CDLL(str(LIB_PATH))
+        L.lss_last_error.restype = ct.c_char_p
+        L.lss_config_init.argtypes = [ct.POINTER(Config)]
+        L.lss_create.argtypes = [ct.c_char_p, ct.POINTER(Config), ct.POINTER(ct.c_void_p)]
+        batch_args = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_int64,
+                      ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64), ct.c_void_p]
+        L.lss_process_batch.argtypes = batch_args
+        L.lss_process_batch_device.argtypes = batch_args
+        L.lss_process_chunk.argtypes = batch_args
+        L.lss_flush.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64), ct.c_void_p]
+        L.lss_query.argtypes = [ct.c_void_p, ct.POINTER(Info)]
+        L.lss_debug_stage.argtypes = [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.POINTER(ct.c_int64)]
+        L.lss_stage_times.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64), ct.c_int32]
+        L.lss_destroy.argtypes = [ct.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def check(code: int) -> None:
+    if code != LSS_OK:
+        raise LssError(code, lib().lss_last_error().decode(errors="replace"))
+
+
+def default_config(**kw) -> Config:
+    cfg = Config()
+    check(lib().lss_config_init(ct.byref(cfg)))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise TypeError(f"unknown config field {k}")
+        setattr(cfg, k, int(v))
+    return cfg

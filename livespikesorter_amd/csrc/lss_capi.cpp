@@ -1,0 +1,901 @@
+// C-ABI host layer of the MI355X matcher: oss_input loading, device state,
+// batch orchestration and streaming.  Mirrors the reference's OnlineSpikesV2
+// (src/SpikeSorter/OnlineSpikesV2.h:29-134, OnlineSpikesV2.cu:248-1001) without
+// its per-iteration host round trips: one batch is five kernel launches on
+// one stream and a single synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lss_kernels.h"
+#include "npy.hpp"
+
+using namespace lss;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct LssError : std::runtime_error {
+    int code;
+    LssError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw LssError(LSS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return LSS_OK;
+    } catch (const LssError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return LSS_ERR_IO;
+    }
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    HIPCHK(hipMemset(p, 0, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void upload(T* d, const std::vector<T>& h) {
+    if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+std::map<std::string, std::string> read_misc(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw LssError(LSS_ERR_IO, "cannot open " + path);
+    std::map<std::string, std::string> kv;
+    std::string line;
+    while (std::getline(in, line)) {
+        auto pos = line.find(':');
+        if (pos == std::string::npos) continue;
+        auto trim = [](std::string s) {
+            while (!s.empty() && (s.back() == '\r' || s.back() == ' ' || s.back() == '\n')) s.pop_back();
+            size_t i = 0;
+            while (i < s.size() && s[i] == ' ') ++i;
+            return s.substr(i);
+        };
+        kv[trim(line.substr(0, pos))] = trim(line.substr(pos + 1));
+    }
+    return kv;
+}
+
+bool file_exists(const std::string& p) {
+    std::ifstream f(p);
+    return f.good();
+}
+
+void expect_shape(const NpyArray& a, std::vector<int64_t> want, const std::string& name) {
+    if (a.shape != want) {
+        std::string got = "(", exp = "(";
+        for (auto s : a.shape) got += std::to_string(s) + ",";
+        for (auto s : want) exp += std::to_string(s) + ",";
+        throw LssError(LSS_ERR_IO, name + ": shape " + got + ") expected " + exp + ")");
+    }
+}
+
+}  // namespace
+
+struct lss_handle {
+    lss_config cfg{};
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int C = 0, T = 0, M = 0, K = 0, nC = 0, nt0min = 0, dt = 0;
+    int CP = 0, TP = 0, KCP = 0;
+    float Th = 0.f;
+    int sem = 0, max_iter = 0;
+    int Wmax = 0, XS = 0, BS = 0;
+    int peel_cap = 0;     // co-resident peel workgroups
+
+    // static device tables (internal template order)
+    float *d_Wr = nullptr, *d_U2 = nullptr, *d_U = nullptr, *d_wpca = nullptr;
+    float *d_nm = nullptr, *d_rnm = nullptr, *d_ctc = nullptr, *d_Gt = nullptr;
+    double* d_nm64 = nullptr;
+    int *d_orig = nullptr, *d_tlo = nullptr, *d_thi = nullptr, *d_iCC = nullptr, *d_iU = nullptr;
+    int *d_cmap = nullptr;
+    int *d_kbw_ptr = nullptr, *d_kbw_idx = nullptr, *d_kbp_ptr = nullptr, *d_kbp_idx = nullptr;
+    // per-batch device buffers
+    float *d_Xw = nullptr, *d_Bpc = nullptr, *d_B = nullptr;
+    float* d_Cfmax = nullptr;
+    int* d_imax = nullptr;
+    Spike *d_slots = nullptr, *d_list = nullptr;
+    int *d_slot_cnt = nullptr, *d_list_cnt = nullptr, *d_iter_cnt = nullptr, *d_err = nullptr;
+    unsigned* d_bar = nullptr;
+    int slot_cap = 0, cap_g = 0, max_groups = 0;
+    // debug copies
+    float *d_dbgB = nullptr, *d_dbgCf = nullptr;
+    int* d_dbgIm = nullptr;
+    // input staging
+    int16_t* d_x = nullptr;
+    size_t d_x_elems = 0;
+    // mapped host output
+    void* h_out = nullptr;
+    OutHeader* hdr = nullptr;
+    lss_event* ev = nullptr;
+    float* feat = nullptr;
+    OutHeader* d_hdr = nullptr;
+    lss_event* d_ev = nullptr;
+    float* d_feat = nullptr;
+    int ev_cap = 0;
+
+    std::vector<int> orig, cmap_default, cmap_cur;
+    int last_W = 0;
+
+    // streaming state
+    bool started = false, flushed = false;
+    int64_t next_sample = 0;
+    int32_t stream_ld = 0;
+    std::vector<int16_t> hist;   // last 2*M samples, [2M][C] gathered channel order
+    std::vector<int16_t> frame;  // host frame of the current batch, [W][C]
+    std::vector<int64_t> last_spike;
+
+    lss_info info{};
+
+    // stage timing (cfg.profile)
+    hipEvent_t evt[LSS_NSTAGES + 1] = {};
+    double stage_ms[LSS_NSTAGES] = {};
+    int64_t timed_batches = 0;
+    bool copy_timed = false;
+
+    ~lss_handle() { release(); }
+
+    void release() {
+        void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
+                        d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
+                        d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x};
+        (void)hipSetDevice(dev);
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        if (h_out) (void)hipHostFree(h_out);
+        for (auto& e : evt)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        h_out = nullptr;
+        stream = nullptr;
+    }
+};
+
+namespace {
+
+// K-block lists for block-sparse GEMMs: for each 64-row tile of A [rows][cols]
+// (row-major, cols a multiple of 16), the 16-column blocks holding any nonzero.
+void build_kblocks(const std::vector<float>& A, int rows, int cols, bool sparse, std::vector<int>& ptr,
+                   std::vector<int>& idx) {
+    const int mt = rows / GEMM_BM, kbs = cols / GEMM_BK;
+    ptr.assign(mt + 1, 0);
+    idx.clear();
+    for (int t = 0; t < mt; ++t) {
+        for (int kb = 0; kb < kbs; ++kb) {
+            bool nz = !sparse;
+            for (int r = 0; r < GEMM_BM && !nz; ++r) {
+                const float* a = &A[(size_t)(t * GEMM_BM + r) * cols + kb * GEMM_BK];
+                for (int c = 0; c < GEMM_BK; ++c)
+                    if (a[c] != 0.f) { nz = true; break; }
+            }
+            if (nz) idx.push_back(kb);
+        }
+        ptr[t + 1] = (int)idx.size();
+    }
+    if (idx.empty()) idx.push_back(0);
+}
+
+void load(lss_handle* h, const std::string& dir_in) {
+    std::string dir = dir_in;
+    if (!dir.empty() && dir.back() != '/') dir += '/';
+    // misc.txt (OnlineSpikesV2.cu:465-487); Th_learned parsed as float, truncated for LSS (:485)
+    auto kv = read_misc(dir + "misc.txt");
+    for (const char* key : {"nt0min", "numNearestChans", "Th_learned", "duplicate_spike_bins"})
+        if (!kv.count(key)) throw LssError(LSS_ERR_IO, std::string("misc.txt lacks ") + key);
+    h->nt0min = std::stoi(kv["nt0min"]);
+    h->nC = std::stoi(kv["numNearestChans"]);
+    h->Th = std::stof(kv["Th_learned"]);
+    h->dt = std::stoi(kv["duplicate_spike_bins"]);
+    if (h->sem == LSS_SEM_LSS) h->Th = (float)(int)h->Th;
+
+    // templates.npy gives M and C (loadTemplatesShape, OnlineSpikesV2.cu:454-462)
+    auto tpl = npy_load(dir + "templates.npy");
+    if (tpl.shape.size() != 3) throw LssError(LSS_ERR_IO, "templates.npy must be [T][M][C]");
+    h->M = (int)tpl.shape[1];
+    h->C = (int)tpl.shape[2];
+    // Wall3 gives T (unclu_T) and K (loadPreclusterShapes, :489-497)
+    auto wall3 = npy_load(dir + "Wall3.npy");
+    if (wall3.shape.size() != 3 || wall3.shape[2] != h->C)
+        throw LssError(LSS_ERR_IO, "Wall3.npy must be [T][K][C]");
+    h->T = (int)wall3.shape[0];
+    h->K = (int)wall3.shape[1];
+    const int C = h->C, T = h->T, M = h->M, K = h->K, nC = h->nC;
+    if (M % 2 == 0 || M > MAX_M) throw LssError(LSS_ERR_IO, "template length must be odd and <= 63");
+    if (K < 1 || K > MAX_K) throw LssError(LSS_ERR_IO, "PC count must be 1..8");
+    if (nC < 1 || nC * K > 64 || nC > C) throw LssError(LSS_ERR_IO, "numNearestChans out of range");
+    auto U = wall3.as<float>();
+    auto wr_a = npy_load(dir + "whiteningMat.npy");
+    expect_shape(wr_a, {C, C}, "whiteningMat.npy");
+    auto Wr = wr_a.as<double>();
+    auto wp_a = npy_load(dir + "wPCA.npy");
+    expect_shape(wp_a, {K, M}, "wPCA.npy");
+    auto wpca = wp_a.as<double>();
+
+    // drift matrix (a4'): the file holds get_drift_matrix(...).T (run_online_spikes.py:628);
+    // Kilosort and LSS both apply that transpose back, X <- D (Wr X) (OnlineKilosort.py:139-140).
+    // It is folded into the whitening matrix in FP64 when it is not the identity.
+    if (file_exists(dir + "drift_matrix.npy")) {
+        auto dm_a = npy_load(dir + "drift_matrix.npy");
+        expect_shape(dm_a, {C, C}, "drift_matrix.npy");
+        auto Dt = dm_a.as<double>();
+        bool ident = true;
+        for (int i = 0; i < C && ident; ++i)
+            for (int j = 0; j < C; ++j)
+                if (Dt[(size_t)i * C + j] != (i == j ? 1.0 : 0.0)) { ident = false; break; }
+        if (!ident) {
+            std::vector<double> R((size_t)C * C, 0.0);
+            for (int i = 0; i < C; ++i)
+                for (int k = 0; k < C; ++k) {
+                    const double d = Dt[(size_t)k * C + i];   // D[i][k] = file[k][i]
+                    if (d == 0.0) continue;
+                    for (int j = 0; j < C; ++j) R[(size_t)i * C + j] += d * Wr[(size_t)k * C + j];
+                }
+            Wr.swap(R);
+        }
+    }
+
+    // channel map (loadChannelMap, OnlineSpikesV2.cu:519-530)
+    h->cmap_default.resize(C);
+    std::iota(h->cmap_default.begin(), h->cmap_default.end(), 0);
+    if (file_exists(dir + "channelMap.npy")) {
+        auto cm = npy_load(dir + "channelMap.npy");
+        if (cm.numel() != C) throw LssError(LSS_ERR_IO, "channelMap.npy must have C entries");
+        h->cmap_default = cm.as<int>();
+    }
+
+    // iCC / iU (prepare_extract, KS/template_matching.py:20-26)
+    std::vector<int64_t> iCC, iU;
+    if (file_exists(dir + "iCC.npy") && file_exists(dir + "iU.npy")) {
+        auto a = npy_load(dir + "iCC.npy");
+        if (a.shape.size() != 2 || a.shape[1] != C || a.shape[0] < nC)
+            throw LssError(LSS_ERR_IO, "iCC.npy must be [>=nC][C]");
+        iCC = a.as<int64_t>();
+        auto b = npy_load(dir + "iU.npy");
+        expect_shape(b, {T}, "iU.npy");
+        iU = b.as<int64_t>();
+    } else {
+        auto xa = npy_load(dir + "xc.npy"), ya = npy_load(dir + "yc.npy");
+        auto xc = xa.as<double>(), yc = ya.as<double>();
+        if ((int)xc.size() != C || (int)yc.size() != C) throw LssError(LSS_ERR_IO, "xc/yc must have C entries");
+        iCC.assign((size_t)nC * C, 0);
+        std::vector<int> ord(C);
+        std::vector<double> ds(C);
+        for (int c = 0; c < C; ++c) {
+            for (int r = 0; r < C; ++r) ds[r] = (xc[c] - xc[r]) * (xc[c] - xc[r]) + (yc[c] - yc[r]) * (yc[c] - yc[r]);
+            std::iota(ord.begin(), ord.end(), 0);
+            std::stable_sort(ord.begin(), ord.end(), [&](int a2, int b2) { return ds[a2] < ds[b2]; });
+            for (int j = 0; j < nC; ++j) iCC[(size_t)j * C + c] = ord[j];
+        }
+        iU.assign(T, 0);
+        for (int t = 0; t < T; ++t) {
+            double best = -1;
+            for (int c = 0; c < C; ++c) {
+                double s = 0;
+                for (int k = 0; k < K; ++k) {
+                    const double u = U[((size_t)t * K + k) * C + c];
+                    s += u * u;
+                }
+                if (s > best) { best = s; iU[t] = c; }
+            }
+        }
+    }
+    for (auto v : iCC)
+        if (v < 0 || v >= C) throw LssError(LSS_ERR_IO, "iCC out of range");
+    for (auto v : iU)
+        if (v < 0 || v >= C) throw LssError(LSS_ERR_IO, "iU out of range");
+
+    // internal template order: by main channel (then file index) so that 64-template
+    // tiles of the projection touch few channel blocks; identity when sparse is off.
+    h->orig.resize(T);
+    std::iota(h->orig.begin(), h->orig.end(), 0);
+    if (h->cfg.sparse)
+        std::stable_sort(h->orig.begin(), h->orig.end(), [&](int a, int b) { return iU[a] < iU[b]; });
+
+    h->CP = (int)round_up(C, GEMM_BM);
+    h->TP = (int)round_up(T, GEMM_BM);
+    h->KCP = K * h->CP;
+    const int CP = h->CP, TP = h->TP, KCP = h->KCP;
+
+    // whitening matrix, padded [CP][CP]
+    std::vector<float> Wrp((size_t)CP * CP, 0.f);
+    for (int i = 0; i < C; ++i)
+        for (int j = 0; j < C; ++j) Wrp[(size_t)i * CP + j] = (float)Wr[(size_t)i * C + j];
+    // projection operand U2[t'][k*CP + c] and internal U[t'][k][c]
+    std::vector<float> U2((size_t)TP * KCP, 0.f), Ui((size_t)T * K * C);
+    for (int tp = 0; tp < T; ++tp) {
+        const int t = h->orig[tp];
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < C; ++c) {
+                const float u = U[((size_t)t * K + k) * C + c];
+                U2[(size_t)tp * KCP + (size_t)k * CP + c] = u;
+                Ui[((size_t)tp * K + k) * C + c] = u;
+            }
+    }
+    std::vector<int> kbw_ptr, kbw_idx, kbp_ptr, kbp_idx;
+    build_kblocks(Wrp, CP, CP, h->cfg.sparse != 0, kbw_ptr, kbw_idx);
+    build_kblocks(U2, TP, KCP, h->cfg.sparse != 0, kbp_ptr, kbp_idx);
+    h->info.white_kblocks = kbw_ptr.back();
+    h->info.proj_kblocks = kbp_ptr.back();
+    h->info.proj_kblocks_dense = (int64_t)(TP / GEMM_BM) * (KCP / GEMM_BK);
+
+    // ctc (KS/template_matching.py:110-121) re-laid as ctc_i[s'][t'][l] (LP-padded) in internal order
+    std::vector<float> ctc_i((size_t)T * T * LP, 0.f);
+    std::vector<int> tlo(T, 0), thi(T, 0);
+    {
+        const int L = 2 * M + 1;
+        std::vector<float> ctc;
+        if (file_exists(dir + "ctc.npy")) {
+            auto a = npy_load(dir + "ctc.npy");
+            expect_shape(a, {T, T, L}, "ctc.npy");
+            if (a.kind == 'f' && a.itemsize == 4) {
+                ctc.resize((size_t)a.numel());
+                std::memcpy(ctc.data(), a.bytes.data(), a.bytes.size());
+            } else {
+                ctc = a.as<float>();
+            }
+        } else {
+            // load-time precompute from Wall3 and wPCA (FP64)
+            std::vector<double> Wt((size_t)K * K * L, 0.0);
+            for (int k1 = 0; k1 < K; ++k1)
+                for (int k2 = 0; k2 < K; ++k2)
+                    for (int l = 0; l < L; ++l) {
+                        const int d = l - M;
+                        double s = 0;
+                        for (int m = 0; m < M; ++m) {
+                            const int mi = m - d;
+                            if (mi >= 0 && mi < M) s += wpca[(size_t)k1 * M + mi] * wpca[(size_t)k2 * M + m];
+                        }
+                        Wt[((size_t)k1 * K + k2) * L + l] = s;
+                    }
+            ctc.assign((size_t)T * T * L, 0.f);
+            std::vector<double> utu((size_t)K * K);
+            for (int i = 0; i < T; ++i)
+                for (int j = 0; j < T; ++j) {
+                    for (int k1 = 0; k1 < K; ++k1)
+                        for (int k2 = 0; k2 < K; ++k2) {
+                            double s = 0;
+                            const float* a = &U[((size_t)i * K + k1) * C];
+                            const float* b = &U[((size_t)j * K + k2) * C];
+                            for (int c = 0; c < C; ++c) s += (double)a[c] * b[c];
+                            utu[k1 * K + k2] = s;
+                        }
+                    for (int l = 0; l < L; ++l) {
+                        double s = 0;
+                        for (int q = 0; q < K * K; ++q) s += utu[q] * Wt[(size_t)q * L + l];
+                        ctc[((size_t)i * T + j) * L + l] = (float)s;
+                    }
+                }
+        }
+        for (int sp = 0; sp < T; ++sp) {
+            const int s = h->orig[sp];
+            int lo = T, hi = 0;
+            for (int tp = 0; tp < T; ++tp) {
+                const int t = h->orig[tp];
+                const float* src = &ctc[((size_t)t * T + s) * L];
+                float* dst = &ctc_i[((size_t)sp * T + tp) * LP];
+                bool nz = false;
+                for (int l = 0; l < L; ++l) {
+                    dst[l] = src[l];
+                    nz |= src[l] != 0.f;
+                }
+                if (nz) { lo = std::min(lo, tp); hi = tp + 1; }
+            }
+            if (!h->cfg.sparse) { lo = 0; hi = T; }
+            if (lo > hi) lo = hi = 0;
+            tlo[sp] = lo;
+            thi[sp] = hi;
+        }
+    }
+
+    // G[k'][k][d + M - 1] = sum_m W[k'][m + d] W[k][m]: PC projection of a shifted waveform,
+    // used to take the residual's subtracted spikes out of the features.
+    std::vector<float> Gt((size_t)K * K * (2 * M - 1), 0.f);
+    for (int k1 = 0; k1 < K; ++k1)
+        for (int k2 = 0; k2 < K; ++k2)
+            for (int d = -(M - 1); d <= M - 1; ++d) {
+                double s = 0;
+                for (int m = 0; m < M; ++m) {
+                    const int mi = m + d;
+                    if (mi >= 0 && mi < M) s += wpca[(size_t)k1 * M + mi] * wpca[(size_t)k2 * M + m];
+                }
+                Gt[((size_t)k1 * K + k2) * (2 * M - 1) + d + M - 1] = (float)s;
+            }
+
+    std::vector<int> iCCi((size_t)nC * C), iUi(T);
+    for (int j = 0; j < nC; ++j)
+        for (int c = 0; c < C; ++c) iCCi[(size_t)j * C + c] = (int)iCC[(size_t)j * C + c];
+    for (int tp = 0; tp < T; ++tp) iUi[tp] = (int)iU[h->orig[tp]];
+    std::vector<float> wpf((size_t)K * M);
+    for (size_t i = 0; i < wpf.size(); ++i) wpf[i] = (float)wpca[i];
+
+    // ---- device allocation ----
+    HIPCHK(hipSetDevice(h->dev));
+    const int Wmax = h->Wmax;
+    h->BS = (int)round_up(round_up(Wmax, GEMM_BN), 256) + 256;
+    h->XS = h->BS + 2 * XOFF + 64;
+    h->d_Wr = dalloc<float>(Wrp.size()); upload(h->d_Wr, Wrp);
+    h->d_U2 = dalloc<float>(U2.size()); upload(h->d_U2, U2);
+    h->d_U = dalloc<float>(Ui.size()); upload(h->d_U, Ui);
+    h->d_wpca = dalloc<float>(wpf.size()); upload(h->d_wpca, wpf);
+    h->d_ctc = dalloc<float>(ctc_i.size()); upload(h->d_ctc, ctc_i);
+    h->d_tlo = dalloc<int>(T); upload(h->d_tlo, tlo);
+    h->d_thi = dalloc<int>(T); upload(h->d_thi, thi);
+    h->d_Gt = dalloc<float>(Gt.size()); upload(h->d_Gt, Gt);
+    h->d_orig = dalloc<int>(T); upload(h->d_orig, h->orig);
+    h->d_iCC = dalloc<int>(iCCi.size()); upload(h->d_iCC, iCCi);
+    h->d_iU = dalloc<int>(T); upload(h->d_iU, iUi);
+    h->d_cmap = dalloc<int>(CP);
+    h->d_kbw_ptr = dalloc<int>(kbw_ptr.size()); upload(h->d_kbw_ptr, kbw_ptr);
+    h->d_kbw_idx = dalloc<int>(kbw_idx.size()); upload(h->d_kbw_idx, kbw_idx);
+    h->d_kbp_ptr = dalloc<int>(kbp_ptr.size()); upload(h->d_kbp_ptr, kbp_ptr);
+    h->d_kbp_idx = dalloc<int>(kbp_idx.size()); upload(h->d_kbp_idx, kbp_idx);
+    h->d_nm = dalloc<float>(T);
+    h->d_rnm = dalloc<float>(T);
+    h->d_nm64 = dalloc<double>(T);
+    h->d_Xw = dalloc<float>((size_t)CP * h->XS);
+    h->d_Bpc = dalloc<float>((size_t)KCP * h->BS);
+    h->d_B = dalloc<float>((size_t)TP * h->BS);
+    h->d_Cfmax = dalloc<float>(Wmax);
+    h->d_imax = dalloc<int>(Wmax);
+    h->peel_cap = peel_max_blocks(h->dev);
+    if (h->peel_cap <= 0) throw LssError(LSS_ERR_HIP, "cannot query peel kernel occupancy");
+    h->max_groups = (Wmax + 15) / 16;
+    h->slot_cap = 256;
+    h->cap_g = MAX_LIST;
+    h->d_slots = dalloc<Spike>((size_t)h->max_groups * h->slot_cap);
+    h->d_list = dalloc<Spike>((size_t)h->max_groups * h->cap_g);
+    h->d_slot_cnt = dalloc<int>(h->max_groups);
+    h->d_list_cnt = dalloc<int>(h->max_groups);
+    h->d_iter_cnt = dalloc<int>(std::max(1, h->max_iter));
+    h->d_err = dalloc<int>(1);
+    h->d_bar = dalloc<unsigned>(1);
+    if (h->cfg.keep_stages) {
+        h->d_dbgB = dalloc<float>((size_t)TP * h->BS);
+        h->d_dbgCf = dalloc<float>(Wmax);
+        h->d_dbgIm = dalloc<int>(Wmax);
+    }
+    // mapped, coherent host memory: the peel kernel writes events straight into it
+    h->ev_cap = h->cfg.max_events;
+    const size_t out_bytes = sizeof(OutHeader) + sizeof(lss_event) * (size_t)h->ev_cap +
+                             sizeof(float) * (size_t)h->ev_cap * nC * K;
+    HIPCHK(hipHostMalloc(&h->h_out, out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(h->h_out, 0, out_bytes);
+    void* dout = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&dout, h->h_out, 0));
+    h->hdr = static_cast<OutHeader*>(h->h_out);
+    h->ev = reinterpret_cast<lss_event*>(h->hdr + 1);
+    h->feat = reinterpret_cast<float*>(h->ev + h->ev_cap);
+    h->d_hdr = static_cast<OutHeader*>(dout);
+    h->d_ev = reinterpret_cast<lss_event*>(h->d_hdr + 1);
+    h->d_feat = reinterpret_cast<float*>(h->d_ev + h->ev_cap);
+
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (h->cfg.profile)
+        for (auto& e : h->evt) HIPCHK(hipEventCreate(&e));
+    launch_norms(h->d_U, K * C, T, h->d_nm, h->d_rnm, h->d_nm64, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->last_spike.assign(T, INT64_MIN / 2);
+}
+
+void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
+    std::vector<int> want(h->CP, 0);
+    for (int c = 0; c < h->C; ++c) {
+        const int v = chan_map ? chan_map[c] : h->cmap_default[c];
+        if (v < 0 || v >= ld) throw LssError(LSS_ERR_ARG, "channel map entry out of range of the row stride");
+        want[c] = v;
+    }
+    if (want != h->cmap_cur) {
+        HIPCHK(hipMemcpyAsync(h->d_cmap, want.data(), want.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        h->cmap_cur = want;
+    }
+}
+
+// Run one framed batch that is already on the device.
+void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t first_sample) {
+    if (W <= 0 || W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) +
+                                              " samples outside 1..max_batch " + std::to_string(h->Wmax));
+    hipStream_t s = h->stream;
+    const int M = h->M;
+    const bool prof = h->cfg.profile != 0;
+    if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
+    launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, s);
+    launch_whiten(h->d_Wr, h->CP, d_x, ld, W, h->d_cmap, h->C, h->d_kbw_ptr, h->d_kbw_idx, h->d_Xw, h->XS, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
+    launch_pcconv(h->d_Xw, h->XS, h->d_wpca, h->K, M, h->d_Bpc, h->BS, h->C, h->CP, W, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
+    launch_project(h->d_U2, h->KCP, h->TP / GEMM_BM, h->d_Bpc, h->BS, h->d_kbp_ptr, h->d_kbp_idx, h->d_B,
+                   h->BS, W, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
+    HIPCHK(hipGetLastError());
+    if (h->cfg.keep_stages)
+        HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
+
+    PeelParams p{};
+    int CW = 16;
+    const int gmax = std::min(h->peel_cap, h->max_groups);
+    while ((W + CW - 1) / CW > gmax && CW < PEEL_THREADS) CW *= 2;
+    const int G = (W + CW - 1) / CW;
+    if (G > gmax) throw LssError(LSS_ERR_ARG, "batch too large for one co-resident peel launch");
+    p.B = h->d_B; p.BS = h->BS;
+    p.T = h->T; p.W = W; p.M = M;
+    p.nm = h->d_nm; p.rnm = h->d_rnm;
+    p.ctc = h->d_ctc; p.tlo = h->d_tlo; p.thi = h->d_thi; p.orig = h->d_orig;
+    p.Th2 = h->Th * h->Th;
+    p.max_iter = h->max_iter;
+    p.CW = CW; p.G = G;
+    p.lss = h->sem == LSS_SEM_LSS;
+    p.Cfmax = h->d_Cfmax; p.imax = h->d_imax;
+    p.slots = h->d_slots; p.slot_cnt = h->d_slot_cnt; p.slot_cap = h->slot_cap;
+    p.list = h->d_list; p.list_cnt = h->d_list_cnt; p.cap_g = h->cap_g;
+    p.bar = h->d_bar; p.iter_cnt = h->d_iter_cnt; p.err = h->d_err;
+    p.Bpc = h->d_Bpc; p.CP = h->CP;
+    p.U = h->d_U; p.C = h->C; p.K = h->K; p.nC = h->nC;
+    p.iCC = h->d_iCC; p.iU = h->d_iU; p.Gt = h->d_Gt;
+    p.do_feat = 1;
+    p.hdr = h->d_hdr; p.ev = h->d_ev; p.feat = h->d_feat; p.ev_cap = h->ev_cap;
+    p.first_sample = first_sample; p.half = M / 2; p.nt0min = h->nt0min;
+    p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
+    p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
+    HIPCHK(launch_peel(p, s));
+    if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (prof) {
+        float ms = 0.f;
+        if (h->copy_timed) {
+            HIPCHK(hipEventElapsedTime(&ms, h->evt[0], h->evt[1]));
+            h->stage_ms[0] += ms;
+        }
+        for (int i = 1; i < LSS_NSTAGES; ++i) {
+            HIPCHK(hipEventElapsedTime(&ms, h->evt[i], h->evt[i + 1]));
+            h->stage_ms[i] += ms;
+        }
+        h->timed_batches += 1;
+        h->copy_timed = false;
+    }
+    const OutHeader hd = *h->hdr;
+    if (hd.err) {
+        std::string why;
+        if (hd.err & ERR_BARRIER) why += " barrier-timeout";
+        if (hd.err & ERR_LIST) why += " per-column-group spike list full";
+        if (hd.err & ERR_SLOT) why += " per-iteration slots full";
+        if (hd.err & ERR_STAGE) why += " neighbour staging full";
+        throw LssError(LSS_ERR_CAPACITY, "peel kernel reported:" + why);
+    }
+    h->last_W = W;
+    h->info.batches += 1;
+    h->info.last_iters = hd.n_iter;
+    h->info.last_spikes = (int32_t)hd.count;
+}
+
+// Copy the kernel's events (already sorted by sample) to the caller, applying
+// per-template duplicate suppression when enabled.
+int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample,
+             bool dedup) {
+    const int64_t n = std::min<int64_t>(h->hdr->count, h->ev_cap);
+    if (h->hdr->count > h->ev_cap) throw LssError(LSS_ERR_CAPACITY, "device event buffer overflow (max_events)");
+    const int64_t fw = (int64_t)h->nC * h->K;
+    int64_t out = 0, total = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const lss_event e = h->ev[i];
+        if (e.sample < lo_sample || e.sample >= hi_sample) continue;
+        if (dedup) {
+            int64_t& last = h->last_spike[e.template_id];
+            if (e.sample - last < h->dt) continue;
+            last = e.sample;
+        }
+        ++total;
+        if (out < cap) {
+            if (ev) ev[out] = e;
+            if (feat) std::memcpy(feat + out * fw, h->feat + i * fw, sizeof(float) * fw);
+            ++out;
+        }
+    }
+    h->info.events += out;
+    if (total > cap) {
+        throw LssError(LSS_ERR_CAPACITY, "event buffer holds " + std::to_string(cap) + " of " +
+                                             std::to_string(total) + " events");
+    }
+    return out;
+}
+
+int16_t* ensure_dx(lss_handle* h, size_t elems) {
+    if (elems > h->d_x_elems) {
+        if (h->d_x) HIPCHK(hipFree(h->d_x));
+        h->d_x = nullptr;
+        HIPCHK(hipMalloc(&h->d_x, elems * sizeof(int16_t)));
+        h->d_x_elems = elems;
+    }
+    return h->d_x;
+}
+
+void process_host_batch(lss_handle* h, const int16_t* samples, int64_t W, int32_t ld, const int32_t* chan_map,
+                        int64_t first_sample) {
+    if (!samples || W <= 0) throw LssError(LSS_ERR_ARG, "empty batch");
+    if (W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch longer than max_batch");
+    set_cmap(h, chan_map, ld);
+    int16_t* dx = ensure_dx(h, (size_t)W * ld);
+    if (h->cfg.profile) HIPCHK(hipEventRecord(h->evt[0], h->stream));
+    HIPCHK(hipMemcpyAsync(dx, samples, (size_t)W * ld * sizeof(int16_t), hipMemcpyHostToDevice, h->stream));
+    h->copy_timed = h->cfg.profile != 0;
+    run_batch(h, dx, (int)W, ld, first_sample);
+}
+
+bool dedup_on(const lss_handle* h) {
+    return h->cfg.dedup < 0 ? h->sem == LSS_SEM_LSS : h->cfg.dedup != 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lss_config_init(lss_config* cfg) {
+    if (!cfg) return LSS_ERR_ARG;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->device = 0;
+    cfg->chunk = 3000;
+    cfg->max_batch = 0;
+    cfg->max_iter = 0;
+    cfg->semantics = LSS_SEM_KILOSORT;
+    cfg->precision = LSS_PREC_FP32;
+    cfg->dedup = -1;
+    cfg->max_events = 65536;
+    cfg->keep_stages = 0;
+    cfg->sparse = 1;
+    cfg->profile = 0;
+    return LSS_OK;
+}
+
+int lss_create(const char* oss_input_dir, const lss_config* cfg, lss_handle** out) {
+    return guarded([&] {
+        if (!oss_input_dir || !out) throw LssError(LSS_ERR_ARG, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<lss_handle>();
+        if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
+        if (h->cfg.precision != LSS_PREC_FP32)
+            throw LssError(LSS_ERR_ARG, "precision mode not available in this build");
+        if (h->cfg.semantics != LSS_SEM_KILOSORT && h->cfg.semantics != LSS_SEM_LSS)
+            throw LssError(LSS_ERR_ARG, "unknown semantics");
+        if (h->cfg.chunk <= 0) throw LssError(LSS_ERR_ARG, "chunk must be positive");
+        if (h->cfg.max_events <= 0) throw LssError(LSS_ERR_ARG, "max_events must be positive");
+        h->dev = h->cfg.device;
+        h->sem = h->cfg.semantics;
+        h->max_iter = h->cfg.max_iter > 0 ? h->cfg.max_iter : (h->sem == LSS_SEM_LSS ? 50 : 100);
+        if (h->max_iter > 4096) throw LssError(LSS_ERR_ARG, "max_iter too large");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (h->dev < 0 || h->dev >= ndev) throw LssError(LSS_ERR_ARG, "no such HIP device");
+        HIPCHK(hipSetDevice(h->dev));
+        // default batch: one chunk plus two margins of the longest supported template
+        h->Wmax = h->cfg.max_batch > 0 ? h->cfg.max_batch : h->cfg.chunk + 2 * MAX_M;
+        load(h.get(), oss_input_dir);
+        h->hist.clear();
+        *out = h.release();
+    });
+}
+
+int lss_process_batch(lss_handle* h, const int16_t* samples, int64_t n_samples, int32_t ld,
+                      const int32_t* chan_map, int64_t first_sample, lss_event* ev, int64_t ev_cap,
+                      int64_t* n_ev, float* feat) {
+    return guarded([&] {
+        if (!h || !n_ev || ev_cap < 0) throw LssError(LSS_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(h->dev));
+        auto t0 = std::chrono::steady_clock::now();
+        *n_ev = 0;
+        process_host_batch(h, samples, n_samples, ld, chan_map, first_sample);
+        *n_ev = emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false);
+        h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int lss_process_batch_device(lss_handle* h, const int16_t* d_samples, int64_t n_samples, int32_t ld,
+                             const int32_t* chan_map, int64_t first_sample, lss_event* ev, int64_t ev_cap,
+                             int64_t* n_ev, float* feat) {
+    return guarded([&] {
+        if (!h || !n_ev || !d_samples || ev_cap < 0) throw LssError(LSS_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(h->dev));
+        auto t0 = std::chrono::steady_clock::now();
+        *n_ev = 0;
+        set_cmap(h, chan_map, ld);
+        run_batch(h, d_samples, (int)n_samples, ld, first_sample);
+        *n_ev = emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false);
+        h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int lss_process_chunk(lss_handle* h, const int16_t* samples, int64_t n_samples, int32_t ld,
+                      const int32_t* chan_map, int64_t first_sample, lss_event* ev, int64_t ev_cap,
+                      int64_t* n_ev, float* feat) {
+    return guarded([&] {
+        if (!h || !n_ev || !samples || n_samples <= 0 || ev_cap < 0) throw LssError(LSS_ERR_ARG, "bad argument");
+        if (ld < h->C && !chan_map) throw LssError(LSS_ERR_ARG, "row stride smaller than the channel count");
+        HIPCHK(hipSetDevice(h->dev));
+        auto t0 = std::chrono::steady_clock::now();
+        *n_ev = 0;
+        const int C = h->C, M = h->M;
+        if (h->flushed) throw LssError(LSS_ERR_STATE, "stream already flushed");
+        if (h->started && first_sample != h->next_sample)
+            throw LssError(LSS_ERR_ARG, "chunks must be contiguous");
+        // gather the chunk's channels on the host into the frame [hist | chunk]
+        std::vector<int> cm(C);
+        for (int c = 0; c < C; ++c) {
+            cm[c] = chan_map ? chan_map[c] : h->cmap_default[c];
+            if (cm[c] < 0 || cm[c] >= ld) throw LssError(LSS_ERR_ARG, "channel map entry out of range");
+        }
+        int64_t hist_n;
+        if (!h->started) {
+            // first batch: M columns repeating the first sample (KS/io.py:616-619)
+            h->hist.assign((size_t)M * C, 0);
+            for (int i = 0; i < M; ++i)
+                for (int c = 0; c < C; ++c) h->hist[(size_t)i * C + c] = samples[cm[c]];
+            hist_n = M;
+        } else {
+            hist_n = (int64_t)h->hist.size() / C;
+        }
+        const int64_t W = hist_n + n_samples;
+        if (W > h->Wmax) throw LssError(LSS_ERR_ARG, "chunk longer than the handle's max_batch allows");
+        h->frame.resize((size_t)W * C);
+        std::memcpy(h->frame.data(), h->hist.data(), h->hist.size() * sizeof(int16_t));
+        for (int64_t i = 0; i < n_samples; ++i) {
+            const int16_t* src = samples + i * ld;
+            int16_t* dst = &h->frame[(size_t)(hist_n + i) * C];
+            for (int c = 0; c < C; ++c) dst[c] = src[cm[c]];
+        }
+        const int64_t col0 = first_sample - hist_n;       // absolute sample of column 0
+        std::vector<int32_t> ident(C);
+        std::iota(ident.begin(), ident.end(), 0);
+        process_host_batch(h, h->frame.data(), W, C, ident.data(), col0);
+        // accepted columns [M, W-M) -> samples [col0 + M, col0 + W - M); report events there
+        const int64_t shift = -(M / 2) + h->nt0min;
+        *n_ev = emit(h, ev, ev_cap, feat, col0 + M + shift, col0 + W - M + shift, dedup_on(h));
+        // keep the last 2M samples as the next frame's left context
+        const int64_t keep = std::min<int64_t>(2 * M, W);
+        h->hist.assign(h->frame.end() - keep * C, h->frame.end());
+        h->started = true;
+        h->next_sample = first_sample + n_samples;
+        h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int lss_flush(lss_handle* h, lss_event* ev, int64_t ev_cap, int64_t* n_ev, float* feat) {
+    return guarded([&] {
+        if (!h || !n_ev) throw LssError(LSS_ERR_ARG, "null argument");
+        *n_ev = 0;
+        if (!h->started) throw LssError(LSS_ERR_STATE, "nothing to flush");
+        if (h->flushed) throw LssError(LSS_ERR_STATE, "already flushed");
+        HIPCHK(hipSetDevice(h->dev));
+        const int C = h->C, M = h->M;
+        const int64_t hist_n = (int64_t)h->hist.size() / C;
+        // pad the end with M copies of the last sample (KS/io.py:620-623)
+        const int64_t W = hist_n + M;
+        h->frame.resize((size_t)W * C);
+        std::memcpy(h->frame.data(), h->hist.data(), h->hist.size() * sizeof(int16_t));
+        for (int i = 0; i < M; ++i)
+            std::memcpy(&h->frame[(size_t)(hist_n + i) * C], &h->hist[(size_t)(hist_n - 1) * C], C * sizeof(int16_t));
+        const int64_t col0 = h->next_sample - hist_n;
+        std::vector<int32_t> ident(C);
+        std::iota(ident.begin(), ident.end(), 0);
+        process_host_batch(h, h->frame.data(), W, C, ident.data(), col0);
+        const int64_t shift = -(M / 2) + h->nt0min;
+        *n_ev = emit(h, ev, ev_cap, feat, col0 + M + shift, col0 + W - M + shift, dedup_on(h));
+        h->flushed = true;
+    });
+}
+
+int lss_query(lss_handle* h, lss_info* info) {
+    return guarded([&] {
+        if (!h || !info) throw LssError(LSS_ERR_ARG, "null argument");
+        lss_info i = h->info;
+        i.C = h->C; i.T = h->T; i.M = h->M; i.K = h->K; i.nC = h->nC;
+        i.nt0min = h->nt0min; i.dt = h->dt; i.semantics = h->sem; i.Th = h->Th;
+        i.max_batch = h->Wmax; i.chunk = h->cfg.chunk; i.max_iter = h->max_iter;
+        *info = i;
+    });
+}
+
+int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
+    return guarded([&] {
+        if (!h || !out || !bytes) throw LssError(LSS_ERR_ARG, "null argument");
+        if (!h->cfg.keep_stages && stage >= 2 && stage <= 4) throw LssError(LSS_ERR_STATE, "keep_stages is off");
+        if (h->last_W <= 0) throw LssError(LSS_ERR_STATE, "no batch processed yet");
+        HIPCHK(hipSetDevice(h->dev));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const int W = h->last_W, C = h->C, T = h->T, K = h->K;
+        int64_t need = 0;
+        if (stage == 0) need = (int64_t)C * W * 4;
+        else if (stage == 1) need = (int64_t)K * C * W * 4;
+        else if (stage == 2) need = (int64_t)T * W * 4;
+        else if (stage == 3 || stage == 4) need = (int64_t)W * 4;
+        else if (stage == 5) need = (int64_t)T * 4;
+        else throw LssError(LSS_ERR_ARG, "unknown stage");
+        if (*bytes < need) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+        char* o = static_cast<char*>(out);
+        if (stage == 0) {
+            HIPCHK(hipMemcpy2D(o, (size_t)W * 4, h->d_Xw + XOFF, (size_t)h->XS * 4, (size_t)W * 4, C,
+                               hipMemcpyDeviceToHost));
+        } else if (stage == 1) {
+            for (int k = 0; k < K; ++k)
+                HIPCHK(hipMemcpy2D(o + (size_t)k * C * W * 4, (size_t)W * 4, h->d_Bpc + (size_t)k * h->CP * h->BS,
+                                   (size_t)h->BS * 4, (size_t)W * 4, C, hipMemcpyDeviceToHost));
+        } else if (stage == 2) {
+            // rows back in file template order
+            std::vector<float> tmp((size_t)T * W);
+            HIPCHK(hipMemcpy2D(tmp.data(), (size_t)W * 4, h->d_dbgB, (size_t)h->BS * 4, (size_t)W * 4, T,
+                               hipMemcpyDeviceToHost));
+            float* of = reinterpret_cast<float*>(o);
+            for (int tp = 0; tp < T; ++tp)
+                std::memcpy(of + (size_t)h->orig[tp] * W, &tmp[(size_t)tp * W], (size_t)W * 4);
+        } else if (stage == 3) {
+            HIPCHK(hipMemcpy(o, h->d_dbgCf, (size_t)W * 4, hipMemcpyDeviceToHost));
+        } else if (stage == 4) {
+            std::vector<int> tmp(W);
+            HIPCHK(hipMemcpy(tmp.data(), h->d_dbgIm, (size_t)W * 4, hipMemcpyDeviceToHost));
+            int* oi = reinterpret_cast<int*>(o);
+            for (int w = 0; w < W; ++w) oi[w] = h->orig[tmp[w]];
+        } else if (stage == 5) {
+            std::vector<float> tmp(T);
+            HIPCHK(hipMemcpy(tmp.data(), h->d_nm, (size_t)T * 4, hipMemcpyDeviceToHost));
+            float* of = reinterpret_cast<float*>(o);
+            for (int tp = 0; tp < T; ++tp) of[h->orig[tp]] = tmp[tp];
+        }
+        *bytes = need;
+    });
+}
+
+int lss_stage_times(lss_handle* h, double* ms, int64_t* batches, int32_t reset) {
+    return guarded([&] {
+        if (!h || !ms || !batches) throw LssError(LSS_ERR_ARG, "null argument");
+        if (!h->cfg.profile) throw LssError(LSS_ERR_STATE, "profile is off");
+        for (int i = 0; i < LSS_NSTAGES; ++i) ms[i] = h->stage_ms[i];
+        *batches = h->timed_batches;
+        if (reset) {
+            for (double& v : h->stage_ms) v = 0.0;
+            h->timed_batches = 0;
+        }
+    });
+}
+
+int lss_destroy(lss_handle* h) {
+    return guarded([&] {
+        if (!h) return;
+        delete h;
+    });
+}
+
+const char* lss_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Internal interface between the C-ABI host layer and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/lss.h"
+
+namespace lss {
+
+constexpr int GEMM_BM = 64;     // template / channel rows per GEMM tile
+constexpr int GEMM_BN = 64;     // sample columns per GEMM tile
+constexpr int GEMM_BK = 16;     // reduction block (the unit of block-sparse skipping)
+constexpr int XOFF = 32;        // zero columns left of column 0 in each whitened row
+constexpr int LP = 128;         // padded lag axis of the ctc table (2M+1 <= 127)
+constexpr int PEEL_THREADS = 256;
+constexpr int MAX_M = 63;
+constexpr int MAX_K = 8;
+constexpr int MAX_STAGE = 256;  // spikes of one iteration touching one workgroup's columns
+constexpr int MAX_LIST = 1024;  // spikes per workgroup over a whole batch
+
+// peel error flags
+constexpr int ERR_BARRIER = 1;
+constexpr int ERR_LIST = 2;
+constexpr int ERR_SLOT = 4;
+constexpr int ERR_STAGE = 8;
+
+struct Spike {      // one accepted spike inside the peel kernel
+    int w;          // batch column
+    int t;          // template, internal order
+    float amp;
+    int it;         // iteration that found it
+};
+
+struct OutHeader {  // written by the peel kernel into mapped host memory
+    int64_t count;
+    int32_t n_iter;
+    int32_t err;
+    int32_t pad[4];
+};
+
+struct PeelParams {
+    float* B; int BS;
+    int T, W, M;
+    const float* nm; const float* rnm;
+    const float* ctc;                  // [T][T][LP], ctc[s][t][l]: template t row of spike template s
+    const int* tlo; const int* thi;    // nonzero row range of ctc[s]
+    const int* orig;                   // internal -> file template index
+    float Th2;
+    int max_iter;
+    int CW, G;
+    int lss;
+    float* Cfmax; int* imax;
+    Spike* slots; int* slot_cnt; int slot_cap;
+    Spike* list; int* list_cnt; int cap_g;
+    unsigned* bar; int* iter_cnt; int* err;
+    // features
+    const float* Bpc; int CP;
+    const float* U; int C, K, nC;
+    const int* iCC; const int* iU;
+    const float* Gt;                   // [K][K][2M-1]
+    int do_feat;
+    // outputs
+    OutHeader* hdr; lss_event* ev; float* feat; int ev_cap;
+    int64_t first_sample; int half; int nt0min;
+    float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
+};
+
+// Clears the per-batch counters the peel kernel relies on.
+void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, hipStream_t s);
+
+void launch_whiten(const float* Wr, int CP, const int16_t* x, int64_t ld, int W, const int* cmap, int C,
+                   const int* kb_ptr, const int* kb_idx, float* Xw, int XS, hipStream_t s);
+
+void launch_pcconv(const float* Xw, int XS, const float* wpca, int K, int M, float* Bpc, int BS,
+                   int C, int CP, int W, hipStream_t s);
+
+void launch_project(const float* A, int lda, int mtiles, const float* Bm, int ldb,
+                    const int* kb_ptr, const int* kb_idx, float* Cout, int ldc, int W, hipStream_t s);
+
+void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
+
+hipError_t launch_peel(const PeelParams& p, hipStream_t s);
+
+int peel_max_blocks(int device);
+
+}  // namespace lss

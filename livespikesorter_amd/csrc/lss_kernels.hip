@@ -1,0 +1,528 @@
+// HIP kernels of the online template matcher, written for gfx950 (CDNA4).
+//
+// Stage map (reference file:line -> kernel here):
+//   whitening GEMM   OnlineSpikesV2.cu:797-801, myGPUhelpers.cu:722-738 -> k_whiten (FP32 MFMA,
+//                    int16 -> f32 conversion and channel gather fused into the operand load)
+//   6-PC conv        myGPUhelpers.cu:356-400, KS/template_matching.py:145 -> k_pcconv (LDS-staged)
+//   projection       myGPUhelpers.cu:263-337, KS/template_matching.py:146 -> k_project (FP32 MFMA)
+//   template norms   OnlineSpikesV2.cu:623-637 -> k_norms (FP64)
+//   peel loop        OnlineSpikesV2.cu:903-1001, KS/template_matching.py:157-198 -> k_peel
+//   PC features      OnlineSpikesV2.cu:113-188, KS/template_matching.py:50-51 -> k_peel epilogue
+// (KS/ = src/Python/run_online_spikes/Lib/site-packages/kilosort/)
+
+#include "lss_kernels.h"
+
+#include <cmath>
+
+namespace lss {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA GEMM core.  A 256-thread workgroup owns a 64x64 output tile; its
+// four waves each own a 32x32 quadrant computed with v_mfma_f32_32x32x2f32.
+// Operand layouts of that instruction (lane l of 64):
+//   A[i][k]: i = l % 32, k = l / 32      B[k][j]: j = l % 32, k = l / 32
+//   D item r: row = (r % 4) + 8 * (r / 4) + 4 * (l / 32), column = l % 32
+// Both operands sit in LDS as [k][m] / [k][n] rows, so the 32 lanes of one
+// half-wave read 32 consecutive words.  The reduction runs only over the
+// K-blocks listed for the tile (exact-zero blocks of A are skipped).
+// ---------------------------------------------------------------------------
+constexpr int LDS_PAD = 4;
+
+struct GemmTile {
+    float As[GEMM_BK][GEMM_BM + LDS_PAD];
+    float Bs[GEMM_BK][GEMM_BN + LDS_PAD];
+};
+
+__device__ __forceinline__ f32x16 mfma_tile(const GemmTile& s, f32x16 acc, int lane, int wm, int wn) {
+#pragma unroll
+    for (int kk = 0; kk < GEMM_BK / 2; ++kk) {
+        const float a = s.As[kk * 2 + (lane >> 5)][wm * 32 + (lane & 31)];
+        const float b = s.Bs[kk * 2 + (lane >> 5)][wn * 32 + (lane & 31)];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void store_tile(float* __restrict__ C, int64_t ldc, int m0, int n0,
+                                           const f32x16& acc, int lane, int wm, int wn) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+        const int j = lane & 31;
+        C[(int64_t)(m0 + wm * 32 + i) * ldc + n0 + wn * 32 + j] = acc[r];
+    }
+}
+
+// Xw[c][XOFF + w] = sum_c' Wr[c][c'] * x[w][cmap[c']]   (rows c < CP, columns w < gridDim.x*64)
+// Columns w >= W read zero input and so store zero: they are the right-hand
+// zero padding the PC convolution reads.
+__global__ __launch_bounds__(256) void k_whiten(const float* __restrict__ Wr, int CP,
+                                                const int16_t* __restrict__ x, int64_t ld, int W,
+                                                const int* __restrict__ cmap, int C,
+                                                const int* __restrict__ kb_ptr,
+                                                const int* __restrict__ kb_idx,
+                                                float* __restrict__ Xw, int XS) {
+    __shared__ GemmTile s;
+    const int mt = blockIdx.y;
+    const int m0 = mt * GEMM_BM, n0 = blockIdx.x * GEMM_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int ar = tid >> 2, ak = (tid & 3) * 4;   // A: row, 4 consecutive k
+    const int bn = tid >> 2, bk = (tid & 3) * 4;   // B: column (sample), 4 consecutive channels
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int p0 = kb_ptr[mt], p1 = kb_ptr[mt + 1];
+    float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
+    float rb[4] = {0.f, 0.f, 0.f, 0.f};
+    auto gload = [&](int kb) {
+        const int k0 = kb * GEMM_BK;
+        ra = *reinterpret_cast<const float4*>(Wr + (int64_t)(m0 + ar) * CP + k0 + ak);
+        const int n = n0 + bn;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = k0 + bk + i;
+            rb[i] = (n < W && c < C) ? (float)x[(int64_t)n * ld + cmap[c]] : 0.f;
+        }
+    };
+    if (p0 < p1) gload(kb_idx[p0]);
+    for (int p = p0; p < p1; ++p) {
+        __syncthreads();
+        s.As[ak + 0][ar] = ra.x;
+        s.As[ak + 1][ar] = ra.y;
+        s.As[ak + 2][ar] = ra.z;
+        s.As[ak + 3][ar] = ra.w;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s.Bs[bk + i][bn] = rb[i];
+        __syncthreads();
+        if (p + 1 < p1) gload(kb_idx[p + 1]);
+        acc = mfma_tile(s, acc, lane, wm, wn);
+    }
+    store_tile(Xw + XOFF, XS, m0, n0, acc, lane, wm, wn);
+}
+
+// B[t][w] = sum_kk A[t][kk] * Bm[kk][w]   (A = spatial templates [TP][K*CP], Bm = PC conv [K*CP][BS])
+__global__ __launch_bounds__(256) void k_project(const float* __restrict__ A, int lda,
+                                                 const float* __restrict__ Bm, int ldb,
+                                                 const int* __restrict__ kb_ptr,
+                                                 const int* __restrict__ kb_idx,
+                                                 float* __restrict__ Cout, int ldc) {
+    __shared__ GemmTile s;
+    const int mt = blockIdx.y;
+    const int m0 = mt * GEMM_BM, n0 = blockIdx.x * GEMM_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int ar = tid >> 2, ak = (tid & 3) * 4;     // A: row, 4 consecutive k
+    const int br = tid >> 4, bc = (tid & 15) * 4;    // B: k row, 4 consecutive columns
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int p0 = kb_ptr[mt], p1 = kb_ptr[mt + 1];
+    float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
+    auto gload = [&](int kb) {
+        const int k0 = kb * GEMM_BK;
+        ra = *reinterpret_cast<const float4*>(A + (int64_t)(m0 + ar) * lda + k0 + ak);
+        rb = *reinterpret_cast<const float4*>(Bm + (int64_t)(k0 + br) * ldb + n0 + bc);
+    };
+    if (p0 < p1) gload(kb_idx[p0]);
+    for (int p = p0; p < p1; ++p) {
+        __syncthreads();
+        s.As[ak + 0][ar] = ra.x;
+        s.As[ak + 1][ar] = ra.y;
+        s.As[ak + 2][ar] = ra.z;
+        s.As[ak + 3][ar] = ra.w;
+        *reinterpret_cast<float4*>(&s.Bs[br][bc]) = rb;
+        __syncthreads();
+        if (p + 1 < p1) gload(kb_idx[p + 1]);
+        acc = mfma_tile(s, acc, lane, wm, wn);
+    }
+    store_tile(Cout, ldc, m0, n0, acc, lane, wm, wn);
+}
+
+// ---------------------------------------------------------------------------
+// 6-PC temporal convolution, 'same' zero padding, cross-correlation (no flip):
+//   Bpc[k*CP + c][w] = sum_m Xw[c][w + m - M/2] * wPCA[k][m]
+// One workgroup stages 256 + 64 samples of one whitened channel row in LDS
+// (coalesced), each thread produces the K outputs of one column.
+// ---------------------------------------------------------------------------
+template <int KK>
+__global__ __launch_bounds__(256) void k_pcconv(const float* __restrict__ Xw, int XS,
+                                                const float* __restrict__ wpca, int M,
+                                                float* __restrict__ Bpc, int BS, int CP) {
+    __shared__ float xs[256 + 2 * XOFF];
+    const int c = blockIdx.y, w0 = blockIdx.x * 256, tid = threadIdx.x;
+    const float* row = Xw + (int64_t)c * XS + XOFF;      // row[w] valid for w >= -XOFF
+    for (int i = tid; i < 256 + 2 * XOFF; i += 256) xs[i] = row[w0 - XOFF + i];
+    __syncthreads();
+    const int h = M / 2;
+    float acc[KK];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) acc[k] = 0.f;
+    const float* xp = xs + tid + XOFF - h;
+    for (int m = 0; m < M; ++m) {
+        const float v = xp[m];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) acc[k] = fmaf(wpca[k * M + m], v, acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < KK; ++k) Bpc[(int64_t)(k * CP + c) * BS + w0 + tid] = acc[k];
+}
+
+// nm[t] = sum_{k,c} U[t,k,c]^2 accumulated in FP64, stored as FP64 and rounded FP32;
+// rnm = FP32 reciprocal (the LSS matcher multiplies by 1/nm, reciprocal_kernel).
+__global__ __launch_bounds__(256) void k_norms(const float* __restrict__ U, int KC,
+                                               float* __restrict__ nm, float* __restrict__ rnm,
+                                               double* __restrict__ nm64) {
+    __shared__ double red[256];
+    const int t = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < KC; i += 256) {
+        const double u = U[(int64_t)t * KC + i];
+        s += u * u;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        nm64[t] = red[0];
+        nm[t] = (float)red[0];
+        rnm[t] = 1.0f / (float)red[0];
+    }
+}
+
+__global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err) {
+    for (int i = threadIdx.x; i < n_iter; i += blockDim.x) iter_cnt[i] = 0;
+    if (threadIdx.x == 0) {
+        *bar = 0u;
+        *err = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grid barrier for the persistent peel kernel (all workgroups co-resident by
+// cooperative launch).  Agent-scope release before arrival and acquire after
+// the wait make every workgroup's global writes visible across XCDs.  The
+// wait is bounded: on timeout the error flag is raised and the kernel winds
+// down instead of hanging.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned nblocks, unsigned& target, int* err) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        target += nblocks;
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1u << 24)) {
+                __hip_atomic_fetch_or(err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+__device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
+    return v > bv || (v == bv && o < bo);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent peel kernel: one workgroup owns CW consecutive columns of the
+// score matrix B (all templates).  Per iteration:
+//   A  apply the previous iteration's spikes to the owned columns
+//      (B[t][w] -= a * ctc[s][t][w - w_s + M], spikes in sample order), then
+//      Cf = relu(B)^2 / nm (zero in the first and last M columns) and the
+//      column max / first-index argmax over templates;          -- barrier --
+//   B  Cmax = max over [w-M, w+M] of Cfmax; accept w when Cmax > Th^2 and
+//      Cmax == Cfmax[w] (|diff| < 1e-9); amplitude B[t][w] / nm[t];
+//                                                                -- barrier --
+//   stop when no workgroup accepted a spike.
+// Because each column has one owner and spikes are applied in sample order,
+// the result is deterministic (no atomics on data).  Afterwards each
+// workgroup sorts its spikes by (column, iteration), writes events, and
+// computes PC features on the final residual.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
+    __shared__ Spike s_sp[MAX_STAGE];
+    __shared__ float s_val[PEEL_THREADS];
+    __shared__ int s_idx[PEEL_THREADS];
+    __shared__ int s_org[PEEL_THREADS];
+    __shared__ float s_win[256 + 2 * MAX_M + 2];
+    __shared__ int s_flag[256];
+    __shared__ int s_rank[MAX_LIST];
+    __shared__ int s_nsp, s_listn, s_stop, s_off;
+
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int CW = p.CW, M = p.M, T = p.T, W = p.W;
+    const int c0 = g * CW, c1 = min(W, c0 + CW);
+    const int nrg = PEEL_THREADS / CW;
+    const int col = tid % CW, rg = tid / CW;
+    unsigned target = 0;
+    if (tid == 0) s_listn = 0;
+    int it_found = 0;
+
+    for (int it = 0; it < p.max_iter; ++it) {
+        // ---------------- phase A ----------------
+        if (tid == 0) {
+            int n = 0;
+            if (it > 0) {
+                const int hlo = max(0, (c0 - M) / CW), hhi = min(p.G - 1, (c1 - 1 + M) / CW);
+                for (int hh = hlo; hh <= hhi; ++hh) {
+                    const int cnt = min(p.slot_cnt[hh], p.slot_cap);
+                    for (int i = 0; i < cnt; ++i) {
+                        const Spike sp = p.slots[(int64_t)hh * p.slot_cap + i];
+                        if (sp.w + M >= c0 && sp.w - M < c1) {
+                            if (n < MAX_STAGE) s_sp[n++] = sp;
+                            else atomicOr(p.err, ERR_STAGE);
+                        }
+                    }
+                }
+            }
+            s_nsp = n;
+        }
+        __syncthreads();
+        const int nsp = s_nsp;
+        if (it == 0 || nsp > 0) {
+            const int w = c0 + col;
+            float best = -INFINITY;
+            int bi = 0, bo = 0x7fffffff;
+            if (w < c1) {
+                const bool edge = (w < M) || (w >= W - M);
+                for (int t = rg; t < T; t += nrg) {
+                    float* bp = p.B + (int64_t)t * p.BS + w;
+                    float b = *bp;
+                    bool changed = false;
+                    for (int q = 0; q < nsp; ++q) {
+                        const Spike sp = s_sp[q];
+                        const int l = w - sp.w + M;
+                        if (l >= 0 && l <= 2 * M && t >= p.tlo[sp.t] && t < p.thi[sp.t]) {
+                            b -= sp.amp * p.ctc[((int64_t)sp.t * T + t) * LP + l];
+                            changed = true;
+                        }
+                    }
+                    if (changed) *bp = b;
+                    float cf = 0.f;
+                    if (!edge) {
+                        const float r = fmaxf(b, 0.f);
+                        cf = p.lss ? (r * r) * p.rnm[t] : (r * r) / p.nm[t];
+                    }
+                    const int o = p.orig[t];
+                    if (better(cf, o, best, bo)) { best = cf; bi = t; bo = o; }
+                }
+            }
+            s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo;
+            __syncthreads();
+            if (tid < CW) {
+                float bv = s_val[tid];
+                int bI = s_idx[tid], bO = s_org[tid];
+                for (int r = 1; r < nrg; ++r) {
+                    const int j = r * CW + tid;
+                    if (better(s_val[j], s_org[j], bv, bO)) { bv = s_val[j]; bI = s_idx[j]; bO = s_org[j]; }
+                }
+                const int ww = c0 + tid;
+                if (ww < c1) {
+                    p.Cfmax[ww] = bv;
+                    p.imax[ww] = bI;
+                    if (it == 0 && p.dbg_cfmax) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
+                }
+            }
+        }
+        grid_sync(p.bar, p.G, target, p.err);
+
+        // ---------------- phase B ----------------
+        const int nwin = (c1 - c0) + 2 * M;
+        for (int i = tid; i < nwin; i += PEEL_THREADS) {
+            const int ww = c0 - M + i;
+            s_win[i] = (ww >= 0 && ww < W) ? p.Cfmax[ww] : -INFINITY;
+        }
+        __syncthreads();
+        if (tid < CW) {
+            int f = 0;
+            if (c0 + tid < c1) {
+                float cm = -INFINITY;
+                for (int i = 0; i <= 2 * M; ++i) cm = fmaxf(cm, s_win[tid + i]);
+                const float cfw = s_win[tid + M];
+                f = p.lss ? (cm >= p.Th2 && (cm - cfw) < 1e-4f) : (cm > p.Th2 && fabsf(cm - cfw) < 1e-9f);
+                if (f) s_val[tid] = cm;
+            }
+            s_flag[tid] = f;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int n = 0, ln = s_listn;
+            for (int i = 0; i < c1 - c0; ++i) {
+                if (!s_flag[i]) continue;
+                const int ww = c0 + i;
+                if (!p.lss) {
+                    const int t = p.imax[ww];
+                    const float a = p.B[(int64_t)t * p.BS + ww] / p.nm[t];
+                    const Spike sp{ww, t, a, it};
+                    if (n < p.slot_cap) p.slots[(int64_t)g * p.slot_cap + n] = sp;
+                    else atomicOr(p.err, ERR_SLOT);
+                    if (ln < p.cap_g) p.list[(int64_t)g * p.cap_g + ln] = sp;
+                    else atomicOr(p.err, ERR_LIST);
+                    ++n; ++ln;
+                } else {
+                    // every template within 1e-4 of the pooled max (myGPUhelpers.cu:465)
+                    const float cm = s_val[i];
+                    for (int t = 0; t < T; ++t) {
+                        const float b = p.B[(int64_t)t * p.BS + ww];
+                        const float r = fmaxf(b, 0.f);
+                        const float cf = (r * r) * p.rnm[t];
+                        if (fabsf(cm - cf) < 1e-4f) {
+                            const Spike sp{ww, t, b * p.rnm[t], it};
+                            if (n < p.slot_cap) p.slots[(int64_t)g * p.slot_cap + n] = sp;
+                            else atomicOr(p.err, ERR_SLOT);
+                            if (ln < p.cap_g) p.list[(int64_t)g * p.cap_g + ln] = sp;
+                            else atomicOr(p.err, ERR_LIST);
+                            ++n; ++ln;
+                        }
+                    }
+                }
+            }
+            p.slot_cnt[g] = n;
+            s_listn = ln;
+            if (n > 0) atomicAdd(&p.iter_cnt[it], n);
+        }
+        grid_sync(p.bar, p.G, target, p.err);
+        if (tid == 0) {
+            const int tot = __hip_atomic_load(&p.iter_cnt[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int e = __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_stop = (tot == 0) || (e & ERR_BARRIER);
+        }
+        __syncthreads();
+        if (s_stop) break;
+        it_found = it + 1;
+    }
+
+    // ---------------- events and features ----------------
+    const int nmine = min(s_listn, p.cap_g);
+    if (tid == 0) p.list_cnt[g] = nmine;
+    grid_sync(p.bar, p.G, target, p.err);
+    {
+        int part = 0;
+        for (int hh = tid; hh < g; hh += PEEL_THREADS) part += p.list_cnt[hh];
+        s_idx[tid] = part;
+        __syncthreads();
+        for (int o = PEEL_THREADS / 2; o > 0; o >>= 1) {
+            if (tid < o) s_idx[tid] += s_idx[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) s_off = s_idx[0];
+        __syncthreads();
+    }
+    const int off = s_off;
+    const Spike* mine = p.list + (int64_t)g * p.cap_g;
+    for (int i = tid; i < nmine; i += PEEL_THREADS) {
+        const Spike si = mine[i];
+        int r = 0;
+        for (int j = 0; j < nmine; ++j) {
+            const Spike sj = mine[j];
+            r += (sj.w < si.w) || (sj.w == si.w && (sj.it < si.it || (sj.it == si.it && j < i)));
+        }
+        s_rank[i] = r;
+        const int d = off + r;
+        if (d < p.ev_cap) {
+            lss_event e;
+            e.sample = p.first_sample + si.w - p.half + p.nt0min;
+            e.template_id = p.orig[si.t];
+            e.amplitude = si.amp;
+            p.ev[d] = e;
+        }
+    }
+    __syncthreads();
+    if (p.do_feat) {
+        const int lane = tid & 63, wv = tid >> 6;
+        const int K = p.K, nC = p.nC, C = p.C, L2 = 2 * M - 1;
+        for (int i = wv; i < nmine; i += PEEL_THREADS / 64) {
+            const int d = off + s_rank[i];
+            if (d >= p.ev_cap || lane >= nC * K) continue;
+            const Spike si = mine[i];
+            const int j = lane / K, k = lane % K;
+            const int ch = p.iCC[j * C + p.iU[si.t]];
+            float f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
+            const int hlo = max(0, (si.w - (M - 1)) / CW), hhi = min(p.G - 1, (si.w + M - 1) / CW);
+            for (int hh = hlo; hh <= hhi; ++hh) {
+                const int cnt = p.list_cnt[hh];
+                const Spike* lst = p.list + (int64_t)hh * p.cap_g;
+                for (int q = 0; q < cnt; ++q) {
+                    const Spike sq = lst[q];
+                    const int dd = si.w - sq.w;
+                    if (dd < -(M - 1) || dd > M - 1) continue;
+                    float gs = 0.f;
+                    for (int k2 = 0; k2 < K; ++k2)
+                        gs += p.U[((int64_t)sq.t * K + k2) * C + ch] * p.Gt[(k2 * K + k) * L2 + dd + M - 1];
+                    f -= sq.amp * gs;
+                }
+            }
+            p.feat[(int64_t)d * nC * K + lane] = f;
+        }
+    }
+    if (g == p.G - 1 && tid == 0) {
+        p.hdr->count = off + nmine;
+        p.hdr->n_iter = it_found;
+        p.hdr->err = __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err);
+}
+
+void launch_whiten(const float* Wr, int CP, const int16_t* x, int64_t ld, int W, const int* cmap, int C,
+                   const int* kb_ptr, const int* kb_idx, float* Xw, int XS, hipStream_t s) {
+    const int ncols = (W + XOFF + GEMM_BN - 1) / GEMM_BN;   // cover the right zero pad
+    hipLaunchKernelGGL(k_whiten, dim3(ncols, CP / GEMM_BM), dim3(256), 0, s, Wr, CP, x, ld, W, cmap, C,
+                       kb_ptr, kb_idx, Xw, XS);
+}
+
+void launch_pcconv(const float* Xw, int XS, const float* wpca, int K, int M, float* Bpc, int BS,
+                   int C, int CP, int W, hipStream_t s) {
+    const int wc = (W + GEMM_BN - 1) / GEMM_BN * GEMM_BN;
+    dim3 grid((wc + 255) / 256, C);
+    switch (K) {
+#define LSS_PC_CASE(KK) \
+    case KK: hipLaunchKernelGGL(k_pcconv<KK>, grid, dim3(256), 0, s, Xw, XS, wpca, M, Bpc, BS, CP); break;
+        LSS_PC_CASE(1) LSS_PC_CASE(2) LSS_PC_CASE(3) LSS_PC_CASE(4)
+        LSS_PC_CASE(5) LSS_PC_CASE(6) LSS_PC_CASE(7) LSS_PC_CASE(8)
+#undef LSS_PC_CASE
+        default: break;
+    }
+}
+
+void launch_project(const float* A, int lda, int mtiles, const float* Bm, int ldb,
+                    const int* kb_ptr, const int* kb_idx, float* Cout, int ldc, int W, hipStream_t s) {
+    const int ncols = (W + GEMM_BN - 1) / GEMM_BN;
+    hipLaunchKernelGGL(k_project, dim3(ncols, mtiles), dim3(256), 0, s, A, lda, Bm, ldb, kb_ptr, kb_idx,
+                       Cout, ldc);
+}
+
+void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s) {
+    hipLaunchKernelGGL(k_norms, dim3(T), dim3(256), 0, s, U, KC, nm, rnm, nm64);
+}
+
+hipError_t launch_peel(const PeelParams& p, hipStream_t s) {
+    PeelParams copy = p;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_peel), dim3(p.G), dim3(PEEL_THREADS),
+                                      args, 0, s);
+}
+
+int peel_max_blocks(int device) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_peel, PEEL_THREADS, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+}  // namespace lss

@@ -1,0 +1,144 @@
+"""Python mirror of the reference's sorter interface over the C ABI.
+
+``OnlineSpikesV2`` here keeps the reference's names and argument meaning for
+the matcher path (src/SpikeSorter/OnlineSpikesV2.h:29-56): it is built from an
+``oss_input`` directory and matches batches of samples, returning the
+``(sample, template, amplitude)`` events and their PC features.  All work is
+done by ``liblss.so``; errors come back as :class:`LssError` with the
+library's message (the reference throws ``std::runtime_error`` via
+``_RUN_ERROR``, src/Helpers/Utils.h:41-46).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_FP32, SEM_KILOSORT, SEM_LSS, check,
+                    default_config, lib)
+
+__all__ = ["OnlineSpikesV2", "LssError", "SEM_KILOSORT", "SEM_LSS"]
+
+STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM = range(6)
+
+
+class OnlineSpikesV2:
+    """One matcher instance (one probe stream) on one HIP device."""
+
+    def __init__(self, oss_input_dir: str | os.PathLike, *, device: int = 0, chunk: int = 3000,
+                 max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
+                 precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
+                 keep_stages: bool = False, sparse: bool = True, profile: bool = False):
+        self._h = ct.c_void_p()
+        cfg = default_config(device=device, chunk=chunk, max_batch=max_batch, max_iter=max_iter,
+                             semantics=semantics, precision=precision, dedup=dedup,
+                             max_events=max_events, keep_stages=int(keep_stages), sparse=int(sparse),
+                             profile=int(profile))
+        check(lib().lss_create(os.fsencode(str(oss_input_dir)), ct.byref(cfg), ct.byref(self._h)))
+        self.info = self.query()
+        self.last_W = 0
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            check(lib().lss_destroy(self._h))
+            self._h = ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- queries -----------------------------------------------------------
+    def query(self) -> Info:
+        info = Info()
+        check(lib().lss_query(self._h, ct.byref(info)))
+        return info
+
+    def _out(self, cap: int):
+        ev = np.zeros(cap, EVENT_DTYPE)
+        nC, K = self.info.nC, self.info.K
+        feat = np.zeros((cap, nC, K), np.float32)
+        return ev, feat
+
+    # -- matching ------------------------------------------------------------
+    def kilosortMatchingPursuit(self, batch: np.ndarray, first_sample: int = 0, *, chan_map=None,
+                                cap: int = 65536):
+        """Match one framed batch ``[W][ld]`` int16 (columns [M, W-M) are eligible).
+
+        Counterpart of ``long kilosortMatchingPursuit(float* d_batch, long W)``
+        (OnlineSpikesV2.h:56); returns ``(events, features)``.
+        """
+        x = np.ascontiguousarray(batch, dtype=np.int16)
+        W, ld = x.shape
+        cm = None if chan_map is None else np.ascontiguousarray(chan_map, dtype=np.int32)
+        ev, feat = self._out(cap)
+        n = ct.c_int64(0)
+        check(lib().lss_process_batch(self._h, x.ctypes.data, W, ld,
+                                      None if cm is None else cm.ctypes.data, int(first_sample),
+                                      ev.ctypes.data, cap, ct.byref(n), feat.ctypes.data))
+        self.last_W = W
+        return ev[:n.value], feat[:n.value]
+
+    process_batch = kilosortMatchingPursuit
+
+    def process_batch_device(self, dev_ptr: int, n_samples: int, ld: int, first_sample: int = 0, *,
+                             chan_map=None, cap: int = 65536, features: bool = True):
+        """Match a batch already resident in device memory (``dev_ptr`` from the caller)."""
+        cm = None if chan_map is None else np.ascontiguousarray(chan_map, dtype=np.int32)
+        ev, feat = self._out(cap)
+        n = ct.c_int64(0)
+        check(lib().lss_process_batch_device(self._h, ct.c_void_p(dev_ptr), n_samples, ld,
+                                             None if cm is None else cm.ctypes.data,
+                                             int(first_sample), ev.ctypes.data, cap, ct.byref(n),
+                                             feat.ctypes.data if features else None))
+        self.last_W = n_samples
+        return ev[:n.value], feat[:n.value]
+
+    def process_chunk(self, chunk: np.ndarray, first_sample: int, *, chan_map=None,
+                      cap: int = 65536):
+        """Streaming form: next contiguous chunk ``[n][ld]`` of the stream."""
+        x = np.ascontiguousarray(chunk, dtype=np.int16)
+        n_s, ld = x.shape
+        cm = None if chan_map is None else np.ascontiguousarray(chan_map, dtype=np.int32)
+        ev, feat = self._out(cap)
+        n = ct.c_int64(0)
+        check(lib().lss_process_chunk(self._h, x.ctypes.data, n_s, ld,
+                                      None if cm is None else cm.ctypes.data, int(first_sample),
+                                      ev.ctypes.data, cap, ct.byref(n), feat.ctypes.data))
+        return ev[:n.value], feat[:n.value]
+
+    def flush(self, *, cap: int = 65536):
+        ev, feat = self._out(cap)
+        n = ct.c_int64(0)
+        check(lib().lss_flush(self._h, ev.ctypes.data, cap, ct.byref(n), feat.ctypes.data))
+        return ev[:n.value], feat[:n.value]
+
+    def stage_times(self, reset: bool = False):
+        """Accumulated device ms per stage (needs ``profile=True``): (dict, batches)."""
+        from ._capi import NSTAGES, STAGE_NAMES
+        ms = (ct.c_double * NSTAGES)()
+        nb = ct.c_int64(0)
+        check(lib().lss_stage_times(self._h, ms, ct.byref(nb), int(reset)))
+        return dict(zip(STAGE_NAMES, list(ms))), nb.value
+
+    def debug_stage(self, stage: int) -> np.ndarray:
+        """Per-stage device buffers of the last batch (needs ``keep_stages=True``)."""
+        info = self.info
+        W = self.last_W
+        shapes = {STAGE_XW: ((info.C, W), np.float32), STAGE_BPC: ((info.K, info.C, W), np.float32),
+                  STAGE_B: ((info.T, W), np.float32), STAGE_CFMAX: ((W,), np.float32),
+                  STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32)}
+        shape, dt = shapes[stage]
+        out = np.zeros(shape, dt)
+        nb = ct.c_int64(out.nbytes)
+        check(lib().lss_debug_stage(self._h, stage, out.ctypes.data, ct.byref(nb)))
+        return out

