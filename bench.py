@@ -68,6 +68,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-steps", type=int, default=100, help="chunks timed with host-resident input")
     ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-sized buffer three times (a known-size kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the matcher runs one independent instance per probe "
@@ -156,7 +159,8 @@ def main() -> int:
         import oracle
         from tests.parity import agreement
         tab = oracle.load_tables(d)
-        threads = len(os.sched_getaffinity(0))
+        # the GPU box's CPU share is 16 threads per GPU (OMP_NUM_THREADS is set to it there)
+        threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
         done, cpu_t, ident_n, within_n, tot_n, amp_err = 0, 0.0, 0, 0, 0, 0.0
         j = 0
         while cpu_t < args.cpu_seconds and j < n_batches:
@@ -186,6 +190,14 @@ def main() -> int:
                   "within_1_sample": round(within_n / max(tot_n, 1), 6), "amp_rel_err_max": amp_err}
         log(f"cpu baseline {cpu['value']}x realtime over {done} chunks; parity {parity}")
     sp.close()
+    if args.copy_calibration:
+        src = torch.empty(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} MiB device copies")
 
     out = {
         "metric": METRIC,

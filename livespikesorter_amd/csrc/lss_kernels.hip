@@ -250,6 +250,9 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 // computes PC features on the final residual.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
+    // Every product here is rounded before it is added or compared, as in the
+    // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
+#pragma clang fp contract(off)
     __shared__ Spike s_sp[MAX_STAGE];
     __shared__ float s_val[PEEL_THREADS];
     __shared__ int s_idx[PEEL_THREADS];
