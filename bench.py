@@ -77,8 +77,8 @@ def main() -> int:
               "stream per GPU and this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         return 2
 
-    import torch
     from livespikesorter_amd import OnlineSpikesV2
+    from livespikesorter_amd._hip import DeviceBuffer, synchronize
 
     N, M = 3000, 61
     W = N + 2 * M
@@ -87,10 +87,9 @@ def main() -> int:
     n_batches = (len(x) - W) // N + 1
     log(f"data ready in {time.time() - t0:.1f}s: {len(x)} samples x {x.shape[1]} ch, {n_batches} chunks")
 
-    dev = torch.device("cuda:0")
-    dx = torch.from_numpy(x).to(dev)
-    torch.cuda.synchronize()
-    base = dx.data_ptr()
+    dx = DeviceBuffer.from_array(x)            # the whole recording resident in HBM
+    synchronize()
+    base = dx.ptr
     C = x.shape[1]
 
     sp = OnlineSpikesV2(d, chunk=N, profile=True)
@@ -104,7 +103,7 @@ def main() -> int:
         step(j)
     sp.stage_times(reset=True)
     lat, iters, spikes = [], [], []
-    torch.cuda.synchronize()
+    synchronize()
     t_start = time.perf_counter()
     for j in range(args.warmup, args.warmup + args.steps):
         t1 = time.perf_counter()
@@ -113,7 +112,7 @@ def main() -> int:
         q = sp.query()
         iters.append(q.last_iters)
         spikes.append(q.last_spikes)
-    torch.cuda.synchronize()
+    synchronize()
     elapsed = time.perf_counter() - t_start
     stage_ms, nb = sp.stage_times(reset=True)
     rtf = args.steps * N / FS / elapsed
@@ -191,13 +190,16 @@ def main() -> int:
         log(f"cpu baseline {cpu['value']}x realtime over {done} chunks; parity {parity}")
     sp.close()
     if args.copy_calibration:
-        src = torch.empty(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
+        nbytes = args.copy_calibration << 20
+        src = DeviceBuffer(nbytes)
+        dst = DeviceBuffer(nbytes)
         for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {args.copy_calibration} MiB device copies")
+            dst.copy_from(src, nbytes)
+        synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} MiB device-to-device copies")
+        src.free()
+        dst.free()
+    dx.free()
 
     out = {
         "metric": METRIC,

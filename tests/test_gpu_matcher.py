@@ -33,7 +33,7 @@ def check_events(ref, ev, feat, *, M=61, nt0min=20, first_sample=0, label=""):
     n = max(len(ref_s), len(ev))
     print(f"[{label}] oracle {len(ref_s)} gpu {len(ev)} identical {ident:.5f} within1 {within:.5f}")
     assert ident >= 0.995, f"{label}: identical fraction {ident}"
-    assert within >= 1.0 - 1.0 / max(n, 1) * 0.0 - 1e-12 or within >= 0.999, f"{label}: within-1 {within}"
+    assert within >= 0.999, f"{label}: within-1 fraction {within}"
     ri = np.array([p[0] for p in pairs])
     gi = np.array([p[1] for p in pairs])
     exact = ref_s[ri] == ev["sample"][gi]
@@ -177,14 +177,15 @@ def test_lss_semantics(small):
 
 
 def test_device_resident_input(small):
-    torch = pytest.importorskip("torch")
+    from livespikesorter_amd._hip import DeviceBuffer
     m, d, tab, x = small
     batch = np.ascontiguousarray(x[:3122])
-    dx = torch.from_numpy(batch).to("cuda:0")
-    torch.cuda.synchronize()
+    dx = DeviceBuffer.from_array(batch)
+    assert np.array_equal(dx.download(batch.shape, batch.dtype), batch)
     with OnlineSpikesV2(d) as sp:
         e1, f1 = sp.process_batch(batch)
-        e2, f2 = sp.process_batch_device(dx.data_ptr(), batch.shape[0], batch.shape[1])
+        e2, f2 = sp.process_batch_device(dx.ptr, batch.shape[0], batch.shape[1])
+    dx.free()
     assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
 
 
