@@ -473,7 +473,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_imax = dalloc<int>(Wmax);
     h->peel_cap = peel_max_blocks(h->dev);
     if (h->peel_cap <= 0) throw LssError(LSS_ERR_HIP, "cannot query peel kernel occupancy");
-    h->max_groups = (Wmax + 15) / 16;
+    h->max_groups = (Wmax + MIN_CW - 1) / MIN_CW;
     h->slot_cap = 256;
     h->cap_g = MAX_LIST;
     h->d_slots = dalloc<Spike>((size_t)h->max_groups * h->slot_cap);
@@ -546,9 +546,9 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
         HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
 
     PeelParams p{};
-    int CW = 16;
+    int CW = MIN_CW;
     const int gmax = std::min(h->peel_cap, h->max_groups);
-    while ((W + CW - 1) / CW > gmax && CW < PEEL_THREADS) CW *= 2;
+    while ((W + CW - 1) / CW > gmax && CW < MAX_CW) CW *= 2;
     const int G = (W + CW - 1) / CW;
     if (G > gmax) throw LssError(LSS_ERR_ARG, "batch too large for one co-resident peel launch");
     p.B = h->d_B; p.BS = h->BS;

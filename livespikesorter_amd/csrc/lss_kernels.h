@@ -13,10 +13,13 @@ constexpr int GEMM_BN = 64;     // sample columns per GEMM tile
 constexpr int GEMM_BK = 16;     // reduction block (the unit of block-sparse skipping)
 constexpr int XOFF = 32;        // zero columns left of column 0 in each whitened row
 constexpr int LP = 128;         // padded lag axis of the ctc table (2M+1 <= 127)
-constexpr int PEEL_THREADS = 256;
+constexpr int PEEL_THREADS = 1024;
+constexpr int MIN_CW = 64;      // peel columns per workgroup: >= M, so neighbours are one step away
+constexpr int MAX_CW = 1024;
+constexpr int MAX_NBR = 3;      // workgroups whose columns a +-M window can reach (CW >= M)
 constexpr int MAX_M = 63;
 constexpr int MAX_K = 8;
-constexpr int MAX_STAGE = 256;  // spikes of one iteration touching one workgroup's columns
+constexpr int MAX_STAGE = 512;  // spikes of one iteration in a workgroup's neighbourhood
 constexpr int MAX_LIST = 1024;  // spikes per workgroup over a whole batch
 
 // peel error flags

@@ -204,29 +204,59 @@ __global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err) {
 }
 
 // ---------------------------------------------------------------------------
-// Grid barrier for the persistent peel kernel (all workgroups co-resident by
-// cooperative launch).  Agent-scope release before arrival and acquire after
-// the wait make every workgroup's global writes visible across XCDs.  The
-// wait is bounded: on timeout the error flag is raised and the kernel winds
-// down instead of hanging.
+// Cross-workgroup communication of the persistent peel kernel.
+//
+// Every value one workgroup hands to another (column maxima, accepted spikes,
+// counters) is written and read with agent-scope relaxed atomics, which on
+// CDNA are plain vector loads/stores with the coherence bit set: they reach
+// and read the device-coherent level directly, so no workgroup needs the
+// whole-L2 write-back / invalidate that an agent-scope fence costs on a
+// multi-XCD part (and the owner-private score tiles stay resident in L2).
+// A barrier then only has to order those accesses: each wave waits for its
+// outstanding memory operations, the workgroup meets, one lane arrives on a
+// device counter and spins (bounded) until all workgroups have arrived.
 // ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T ld_c(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_c(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_spike(Spike* d, const Spike& s) {
+    int* q = reinterpret_cast<int*>(d);
+    st_c(q + 0, s.w);
+    st_c(q + 1, s.t);
+    st_c(q + 2, __float_as_int(s.amp));
+    st_c(q + 3, s.it);
+}
+__device__ __forceinline__ Spike ld_spike(const Spike* d) {
+    const int* q = reinterpret_cast<const int*>(d);
+    Spike s;
+    s.w = ld_c(q + 0);
+    s.t = ld_c(q + 1);
+    s.amp = __int_as_float(ld_c(q + 2));
+    s.it = ld_c(q + 3);
+    return s;
+}
+
 __device__ __forceinline__ void grid_sync(unsigned* bar, unsigned nblocks, unsigned& target, int* err) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);       // this wave's coherent stores have completed
     __syncthreads();
     if (threadIdx.x == 0) {
         target += nblocks;
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1u << 24)) {
+        while (ld_c(bar) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 26)) {
                 __hip_atomic_fetch_or(err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
         }
     }
     __syncthreads();
-    __threadfence();
 }
 
 __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
@@ -235,60 +265,84 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 
 // ---------------------------------------------------------------------------
 // Persistent peel kernel: one workgroup owns CW consecutive columns of the
-// score matrix B (all templates).  Per iteration:
-//   A  apply the previous iteration's spikes to the owned columns
+// score matrix B (all templates); only the owner reads or writes them.
+// Per iteration:
+//   A  apply the previous iteration's spikes that reach the owned columns
 //      (B[t][w] -= a * ctc[s][t][w - w_s + M], spikes in sample order), then
 //      Cf = relu(B)^2 / nm (zero in the first and last M columns) and the
-//      column max / first-index argmax over templates;          -- barrier --
-//   B  Cmax = max over [w-M, w+M] of Cfmax; accept w when Cmax > Th^2 and
+//      column max / first-index argmax over templates;        -- barrier --
+//   B  Cmax = max of Cfmax over [w-M, w+M]; accept w when Cmax > Th^2 and
 //      Cmax == Cfmax[w] (|diff| < 1e-9); amplitude B[t][w] / nm[t];
-//                                                                -- barrier --
+//                                                              -- barrier --
 //   stop when no workgroup accepted a spike.
-// Because each column has one owner and spikes are applied in sample order,
-// the result is deterministic (no atomics on data).  Afterwards each
-// workgroup sorts its spikes by (column, iteration), writes events, and
-// computes PC features on the final residual.
+// One owner per column and a fixed application order make the result
+// deterministic; no data atomics are used.  Afterwards each workgroup sorts
+// its spikes by (column, iteration), writes the events, and computes the PC
+// features of the final residual.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
 #pragma clang fp contract(off)
-    __shared__ Spike s_sp[MAX_STAGE];
+    __shared__ Spike s_sp[MAX_STAGE];          // previous iteration's spikes touching the owned columns
+    __shared__ Spike s_list[MAX_LIST];         // this workgroup's spikes, whole batch
     __shared__ float s_val[PEEL_THREADS];
     __shared__ int s_idx[PEEL_THREADS];
     __shared__ int s_org[PEEL_THREADS];
-    __shared__ float s_win[256 + 2 * MAX_M + 2];
-    __shared__ int s_flag[256];
+    __shared__ float s_win[MAX_CW + 2 * MAX_M + 2];
+    __shared__ float s_cf[MAX_CW];             // own column maxima
+    __shared__ int s_im[MAX_CW];
+    __shared__ int s_flag[MAX_CW];
     __shared__ int s_rank[MAX_LIST];
-    __shared__ int s_nsp, s_listn, s_stop, s_off;
+    __shared__ int s_hcnt[MAX_NBR + 1];        // neighbour slot counts, then their prefix
+    __shared__ int s_nsp, s_listn, s_stop, s_off, s_n;
 
     const int g = blockIdx.x, tid = threadIdx.x;
     const int CW = p.CW, M = p.M, T = p.T, W = p.W;
     const int c0 = g * CW, c1 = min(W, c0 + CW);
     const int nrg = PEEL_THREADS / CW;
     const int col = tid % CW, rg = tid / CW;
+    const int hlo = max(0, (c0 - M) / CW), hhi = min(p.G - 1, (c1 - 1 + M) / CW);
+    const int nh = hhi - hlo + 1;             // <= MAX_NBR by the choice CW >= M
     unsigned target = 0;
     if (tid == 0) s_listn = 0;
     int it_found = 0;
 
     for (int it = 0; it < p.max_iter; ++it) {
         // ---------------- phase A ----------------
-        if (tid == 0) {
-            int n = 0;
-            if (it > 0) {
-                const int hlo = max(0, (c0 - M) / CW), hhi = min(p.G - 1, (c1 - 1 + M) / CW);
-                for (int hh = hlo; hh <= hhi; ++hh) {
-                    const int cnt = min(p.slot_cnt[hh], p.slot_cap);
-                    for (int i = 0; i < cnt; ++i) {
-                        const Spike sp = p.slots[(int64_t)hh * p.slot_cap + i];
-                        if (sp.w + M >= c0 && sp.w - M < c1) {
-                            if (n < MAX_STAGE) s_sp[n++] = sp;
-                            else atomicOr(p.err, ERR_STAGE);
-                        }
-                    }
+        if (it > 0) {
+            // stage the neighbours' spikes of the previous iteration (two coherent round trips)
+            if (tid < nh) s_hcnt[tid] = min(ld_c(&p.slot_cnt[hlo + tid]), p.slot_cap);
+            __syncthreads();
+            if (tid == 0) {
+                int acc = 0;
+                for (int i = 0; i < nh; ++i) {
+                    const int c = s_hcnt[i];
+                    s_hcnt[i] = acc;
+                    acc += c;
                 }
+                s_hcnt[nh] = acc;
             }
-            s_nsp = n;
+            __syncthreads();
+            const int tot = s_hcnt[nh];
+            for (int i = tid; i < tot && i < MAX_STAGE; i += PEEL_THREADS) {
+                int hh = 0;
+                while (hh + 1 < nh && s_hcnt[hh + 1] <= i) ++hh;
+                s_sp[i] = ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + (i - s_hcnt[hh])]);
+            }
+            if (tid == 0 && tot > MAX_STAGE) atomicOr(p.err, ERR_STAGE);
+            __syncthreads();
+            // keep those whose window reaches the owned columns, in order
+            if (tid == 0) {
+                int n = 0;
+                for (int i = 0; i < min(tot, MAX_STAGE); ++i) {
+                    const Spike sp = s_sp[i];
+                    if (sp.w + M >= c0 && sp.w - M < c1) s_sp[n++] = sp;
+                }
+                s_nsp = n;
+            }
+        } else if (tid == 0) {
+            s_nsp = 0;
         }
         __syncthreads();
         const int nsp = s_nsp;
@@ -331,8 +385,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                 }
                 const int ww = c0 + tid;
                 if (ww < c1) {
-                    p.Cfmax[ww] = bv;
-                    p.imax[ww] = bI;
+                    s_cf[tid] = bv;
+                    s_im[tid] = bI;
+                    st_c(&p.Cfmax[ww], bv);
                     if (it == 0 && p.dbg_cfmax) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
                 }
             }
@@ -343,7 +398,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
         const int nwin = (c1 - c0) + 2 * M;
         for (int i = tid; i < nwin; i += PEEL_THREADS) {
             const int ww = c0 - M + i;
-            s_win[i] = (ww >= 0 && ww < W) ? p.Cfmax[ww] : -INFINITY;
+            float v = -INFINITY;
+            if (ww >= c0 && ww < c1) v = s_cf[ww - c0];
+            else if (ww >= 0 && ww < W) v = ld_c(&p.Cfmax[ww]);
+            s_win[i] = v;
         }
         __syncthreads();
         if (tid < CW) {
@@ -364,12 +422,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                 if (!s_flag[i]) continue;
                 const int ww = c0 + i;
                 if (!p.lss) {
-                    const int t = p.imax[ww];
+                    const int t = s_im[i];
                     const float a = p.B[(int64_t)t * p.BS + ww] / p.nm[t];
                     const Spike sp{ww, t, a, it};
-                    if (n < p.slot_cap) p.slots[(int64_t)g * p.slot_cap + n] = sp;
+                    if (n < p.slot_cap) s_sp[n] = sp;
                     else atomicOr(p.err, ERR_SLOT);
-                    if (ln < p.cap_g) p.list[(int64_t)g * p.cap_g + ln] = sp;
+                    if (ln < MAX_LIST) s_list[ln] = sp;
                     else atomicOr(p.err, ERR_LIST);
                     ++n; ++ln;
                 } else {
@@ -381,23 +439,29 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                         const float cf = (r * r) * p.rnm[t];
                         if (fabsf(cm - cf) < 1e-4f) {
                             const Spike sp{ww, t, b * p.rnm[t], it};
-                            if (n < p.slot_cap) p.slots[(int64_t)g * p.slot_cap + n] = sp;
+                            if (n < p.slot_cap) s_sp[n] = sp;
                             else atomicOr(p.err, ERR_SLOT);
-                            if (ln < p.cap_g) p.list[(int64_t)g * p.cap_g + ln] = sp;
+                            if (ln < MAX_LIST) s_list[ln] = sp;
                             else atomicOr(p.err, ERR_LIST);
                             ++n; ++ln;
                         }
                     }
                 }
             }
-            p.slot_cnt[g] = n;
+            s_n = min(n, p.slot_cap);
             s_listn = ln;
-            if (n > 0) atomicAdd(&p.iter_cnt[it], n);
+            if (n > 0) __hip_atomic_fetch_add(&p.iter_cnt[it], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        {
+            const int n = s_n;
+            for (int i = tid; i < n; i += PEEL_THREADS) st_spike(&p.slots[(int64_t)g * p.slot_cap + i], s_sp[i]);
+            if (tid == 0) st_c(&p.slot_cnt[g], n);
         }
         grid_sync(p.bar, p.G, target, p.err);
         if (tid == 0) {
-            const int tot = __hip_atomic_load(&p.iter_cnt[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int e = __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int tot = ld_c(&p.iter_cnt[it]);
+            const int e = ld_c(p.err);
             s_stop = (tot == 0) || (e & ERR_BARRIER);
         }
         __syncthreads();
@@ -406,12 +470,13 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     }
 
     // ---------------- events and features ----------------
-    const int nmine = min(s_listn, p.cap_g);
-    if (tid == 0) p.list_cnt[g] = nmine;
+    const int nmine = min(s_listn, MAX_LIST);
+    for (int i = tid; i < nmine; i += PEEL_THREADS) st_spike(&p.list[(int64_t)g * p.cap_g + i], s_list[i]);
+    if (tid == 0) st_c(&p.list_cnt[g], nmine);
     grid_sync(p.bar, p.G, target, p.err);
     {
         int part = 0;
-        for (int hh = tid; hh < g; hh += PEEL_THREADS) part += p.list_cnt[hh];
+        for (int hh = tid; hh < g; hh += PEEL_THREADS) part += ld_c(&p.list_cnt[hh]);
         s_idx[tid] = part;
         __syncthreads();
         for (int o = PEEL_THREADS / 2; o > 0; o >>= 1) {
@@ -422,12 +487,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
         __syncthreads();
     }
     const int off = s_off;
-    const Spike* mine = p.list + (int64_t)g * p.cap_g;
     for (int i = tid; i < nmine; i += PEEL_THREADS) {
-        const Spike si = mine[i];
+        const Spike si = s_list[i];
         int r = 0;
         for (int j = 0; j < nmine; ++j) {
-            const Spike sj = mine[j];
+            const Spike sj = s_list[j];
             r += (sj.w < si.w) || (sj.w == si.w && (sj.it < si.it || (sj.it == si.it && j < i)));
         }
         s_rank[i] = r;
@@ -444,19 +508,19 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     if (p.do_feat) {
         const int lane = tid & 63, wv = tid >> 6;
         const int K = p.K, nC = p.nC, C = p.C, L2 = 2 * M - 1;
+        const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.G - 1, (c1 - 1 + M - 1) / CW);
         for (int i = wv; i < nmine; i += PEEL_THREADS / 64) {
             const int d = off + s_rank[i];
             if (d >= p.ev_cap || lane >= nC * K) continue;
-            const Spike si = mine[i];
+            const Spike si = s_list[i];
             const int j = lane / K, k = lane % K;
             const int ch = p.iCC[j * C + p.iU[si.t]];
             float f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
-            const int hlo = max(0, (si.w - (M - 1)) / CW), hhi = min(p.G - 1, (si.w + M - 1) / CW);
-            for (int hh = hlo; hh <= hhi; ++hh) {
-                const int cnt = p.list_cnt[hh];
+            for (int hh = flo; hh <= fhi; ++hh) {
+                const int cnt = ld_c(&p.list_cnt[hh]);
                 const Spike* lst = p.list + (int64_t)hh * p.cap_g;
                 for (int q = 0; q < cnt; ++q) {
-                    const Spike sq = lst[q];
+                    const Spike sq = hh == g ? s_list[q] : ld_spike(&lst[q]);
                     const int dd = si.w - sq.w;
                     if (dd < -(M - 1) || dd > M - 1) continue;
                     float gs = 0.f;
@@ -471,7 +535,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     if (g == p.G - 1 && tid == 0) {
         p.hdr->count = off + nmine;
         p.hdr->n_iter = it_found;
-        p.hdr->err = __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p.hdr->err = ld_c(p.err);
     }
 }
 
