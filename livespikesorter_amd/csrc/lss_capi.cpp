@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -114,7 +115,9 @@ struct lss_handle {
     float Th = 0.f;
     int sem = 0, max_iter = 0;
     int Wmax = 0, XS = 0, BS = 0;
-    int peel_cap = 0;     // co-resident peel workgroups
+    int peel_cap[5] = {};    // co-resident peel workgroups for R = 0, 8, 16, 32, 64
+    int ctc_rows = 0, t_first = 0;
+    bool coop = true;        // cooperative launch of the peel kernel
 
     // static device tables (internal template order)
     float *d_Wr = nullptr, *d_U2 = nullptr, *d_U = nullptr, *d_wpca = nullptr;
@@ -352,8 +355,13 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->info.proj_kblocks = kbp_ptr.back();
     h->info.proj_kblocks_dense = (int64_t)(TP / GEMM_BM) * (KCP / GEMM_BK);
 
-    // ctc (KS/template_matching.py:110-121) re-laid as ctc_i[s'][t'][l] (LP-padded) in internal order
-    std::vector<float> ctc_i((size_t)T * T * LP, 0.f);
+    // ctc (KS/template_matching.py:110-121) re-laid as ctc_i[s'][t'][l] (LP-padded) in internal
+    // order, rows padded with zeros to the register tile height of the peel kernel
+    h->ctc_rows = T;
+    for (int R : {8, 16, 32, 64})
+        if (T <= REG_MAX_T && (T + REG_NRG - 1) / REG_NRG <= R) { h->ctc_rows = REG_NRG * R; break; }
+    const int TS = h->ctc_rows;
+    std::vector<float> ctc_i((size_t)T * TS * LP, 0.f);
     std::vector<int> tlo(T, 0), thi(T, 0);
     {
         const int L = 2 * M + 1;
@@ -406,7 +414,7 @@ void load(lss_handle* h, const std::string& dir_in) {
             for (int tp = 0; tp < T; ++tp) {
                 const int t = h->orig[tp];
                 const float* src = &ctc[((size_t)t * T + s) * L];
-                float* dst = &ctc_i[((size_t)sp * T + tp) * LP];
+                float* dst = &ctc_i[((size_t)sp * TS + tp) * LP];
                 bool nz = false;
                 for (int l = 0; l < L; ++l) {
                     dst[l] = src[l];
@@ -455,7 +463,13 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_tlo = dalloc<int>(T); upload(h->d_tlo, tlo);
     h->d_thi = dalloc<int>(T); upload(h->d_thi, thi);
     h->d_Gt = dalloc<float>(Gt.size()); upload(h->d_Gt, Gt);
-    h->d_orig = dalloc<int>(T); upload(h->d_orig, h->orig);
+    {
+        std::vector<int> orig_p(TS, INT32_MAX);
+        std::copy(h->orig.begin(), h->orig.end(), orig_p.begin());
+        h->d_orig = dalloc<int>(TS); upload(h->d_orig, orig_p);
+        for (int tp = 0; tp < T; ++tp)
+            if (h->orig[tp] == 0) h->t_first = tp;
+    }
     h->d_iCC = dalloc<int>(iCCi.size()); upload(h->d_iCC, iCCi);
     h->d_iU = dalloc<int>(T); upload(h->d_iU, iUi);
     h->d_cmap = dalloc<int>(CP);
@@ -463,16 +477,26 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_kbw_idx = dalloc<int>(kbw_idx.size()); upload(h->d_kbw_idx, kbw_idx);
     h->d_kbp_ptr = dalloc<int>(kbp_ptr.size()); upload(h->d_kbp_ptr, kbp_ptr);
     h->d_kbp_idx = dalloc<int>(kbp_idx.size()); upload(h->d_kbp_idx, kbp_idx);
-    h->d_nm = dalloc<float>(T);
-    h->d_rnm = dalloc<float>(T);
+    // norms padded past T: +inf (Cf = 0) and reciprocal 0, filled in below T by k_norms
+    h->d_nm = dalloc<float>(TS); upload(h->d_nm, std::vector<float>(TS, INFINITY));
+    h->d_rnm = dalloc<float>(TS);
     h->d_nm64 = dalloc<double>(T);
     h->d_Xw = dalloc<float>((size_t)CP * h->XS);
     h->d_Bpc = dalloc<float>((size_t)KCP * h->BS);
     h->d_B = dalloc<float>((size_t)TP * h->BS);
     h->d_Cfmax = dalloc<float>(Wmax);
     h->d_imax = dalloc<int>(Wmax);
-    h->peel_cap = peel_max_blocks(h->dev);
-    if (h->peel_cap <= 0) throw LssError(LSS_ERR_HIP, "cannot query peel kernel occupancy");
+    {
+        const int Rs[5] = {0, 8, 16, 32, 64};
+        for (int i = 0; i < 5; ++i) {
+            h->peel_cap[i] = peel_max_blocks(h->dev, Rs[i]);
+            if (h->peel_cap[i] <= 0) throw LssError(LSS_ERR_HIP, "cannot query peel kernel occupancy");
+        }
+        // A profiler that cannot follow cooperative dispatches can ask for a plain
+        // launch; the grid is still sized to be co-resident on an idle device.
+        const char* lm = std::getenv("LSS_PEEL_LAUNCH");
+        h->coop = !(lm && std::string(lm) == "plain");
+    }
     h->max_groups = (Wmax + MIN_CW - 1) / MIN_CW;
     h->slot_cap = 256;
     h->cap_g = MAX_LIST;
@@ -546,12 +570,19 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
         HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
 
     PeelParams p{};
-    int CW = MIN_CW;
-    const int gmax = std::min(h->peel_cap, h->max_groups);
-    while ((W + CW - 1) / CW > gmax && CW < MAX_CW) CW *= 2;
-    const int G = (W + CW - 1) / CW;
-    if (G > gmax) throw LssError(LSS_ERR_ARG, "batch too large for one co-resident peel launch");
+    int CW = MIN_CW, R = 0, G = 0;
+    for (;; CW *= 2) {
+        G = (W + CW - 1) / CW;
+        R = peel_rows(h->T, CW);
+        const int cap = h->peel_cap[R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 : R == 32 ? 3 : 4];
+        if (G <= std::min(cap, h->max_groups)) break;
+        if (CW >= MAX_CW) throw LssError(LSS_ERR_ARG, "batch too large for one co-resident peel launch");
+    }
     p.B = h->d_B; p.BS = h->BS;
+    p.B_bytes = (int)std::min<int64_t>((int64_t)h->TP * h->BS * 4, INT32_MAX);
+    p.ctc_bytes = (int)std::min<int64_t>((int64_t)h->T * h->ctc_rows * LP * 4, INT32_MAX);
+    p.ctc_rows = h->ctc_rows;
+    p.t_first = h->t_first;
     p.T = h->T; p.W = W; p.M = M;
     p.nm = h->d_nm; p.rnm = h->d_rnm;
     p.ctc = h->d_ctc; p.tlo = h->d_tlo; p.thi = h->d_thi; p.orig = h->d_orig;
@@ -571,7 +602,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     p.first_sample = first_sample; p.half = M / 2; p.nt0min = h->nt0min;
     p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
-    HIPCHK(launch_peel(p, s));
+    HIPCHK(launch_peel(p, R, h->coop, s));
     if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
     HIPCHK(hipStreamSynchronize(s));
     if (prof) {

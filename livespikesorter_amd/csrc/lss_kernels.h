@@ -15,12 +15,14 @@ constexpr int XOFF = 32;        // zero columns left of column 0 in each whitene
 constexpr int LP = 128;         // padded lag axis of the ctc table (2M+1 <= 127)
 constexpr int PEEL_THREADS = 1024;
 constexpr int MIN_CW = 64;      // peel columns per workgroup: >= M, so neighbours are one step away
-constexpr int MAX_CW = 1024;
+constexpr int MAX_CW = 512;
 constexpr int MAX_NBR = 3;      // workgroups whose columns a +-M window can reach (CW >= M)
+constexpr int REG_NRG = 16;     // row groups of a register-resident peel tile (64 columns)
+constexpr int REG_MAX_T = 1024; // templates a register-resident tile holds (64 rows per thread)
 constexpr int MAX_M = 63;
 constexpr int MAX_K = 8;
 constexpr int MAX_STAGE = 512;  // spikes of one iteration in a workgroup's neighbourhood
-constexpr int MAX_LIST = 1024;  // spikes per workgroup over a whole batch
+constexpr int MAX_LIST = 512;   // spikes per workgroup over a whole batch
 
 // peel error flags
 constexpr int ERR_BARRIER = 1;
@@ -42,13 +44,19 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
     int32_t pad[4];
 };
 
+// word 3 of a raw buffer descriptor on gfx9-family parts (gfx950): 32-bit data format
+constexpr int BUF_DWORD3 = 0x00020000;
+
 struct PeelParams {
-    float* B; int BS;
+    float* B; int BS; int B_bytes;
     int T, W, M;
     const float* nm; const float* rnm;
-    const float* ctc;                  // [T][T][LP], ctc[s][t][l]: template t row of spike template s
+    const float* ctc;                  // [T][ctc_rows][LP], ctc[s][t][l]: template t row of spike template s
+    int ctc_rows;                      // row stride: T padded to the register tile height (zero rows)
+    int ctc_bytes;                     // size for the buffer descriptor (register path: < 2^31)
     const int* tlo; const int* thi;    // nonzero row range of ctc[s]
-    const int* orig;                   // internal -> file template index
+    const int* orig;                   // internal -> file template index (INT_MAX past T)
+    int t_first;                       // internal index of file template 0
     float Th2;
     int max_iter;
     int CW, G;
@@ -83,8 +91,11 @@ void launch_project(const float* A, int lda, int mtiles, const float* Bm, int ld
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
 
-hipError_t launch_peel(const PeelParams& p, hipStream_t s);
+// Rows of the score tile each peel thread keeps in registers (0: tile stays in global memory).
+int peel_rows(int T, int CW);
 
-int peel_max_blocks(int device);
+hipError_t launch_peel(const PeelParams& p, int R, bool cooperative, hipStream_t s);
+
+int peel_max_blocks(int device, int R);
 
 }  // namespace lss

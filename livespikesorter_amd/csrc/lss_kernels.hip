@@ -265,8 +265,10 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 
 // ---------------------------------------------------------------------------
 // Persistent peel kernel: one workgroup owns CW consecutive columns of the
-// score matrix B (all templates); only the owner reads or writes them.
-// Per iteration:
+// score matrix B (all templates); only the owner reads or writes them.  With
+// R > 0 the owned tile lives in registers for the whole batch (thread (col,
+// rg) holds rows rg, rg + nrg, ...); R == 0 keeps it in global memory (large
+// template counts).  Per iteration:
 //   A  apply the previous iteration's spikes that reach the owned columns
 //      (B[t][w] -= a * ctc[s][t][w - w_s + M], spikes in sample order), then
 //      Cf = relu(B)^2 / nm (zero in the first and last M columns) and the
@@ -274,119 +276,198 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 //   B  Cmax = max of Cfmax over [w-M, w+M]; accept w when Cmax > Th^2 and
 //      Cmax == Cfmax[w] (|diff| < 1e-9); amplitude B[t][w] / nm[t];
 //                                                              -- barrier --
-//   stop when no workgroup accepted a spike.
+//   the next iteration starts with one coherent round trip that reads the
+//   spike total (stop when zero) and the neighbours' new spikes.
 // One owner per column and a fixed application order make the result
 // deterministic; no data atomics are used.  Afterwards each workgroup sorts
 // its spikes by (column, iteration), writes the events, and computes the PC
 // features of the final residual.
 // ---------------------------------------------------------------------------
+constexpr int PRE = 4;   // neighbour spikes fetched speculatively with the counts
+
+template <int R>
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
 #pragma clang fp contract(off)
-    __shared__ Spike s_sp[MAX_STAGE];          // previous iteration's spikes touching the owned columns
+    // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
+    // immediate offset (no per-row address registers); the norm table comes first.
+    __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template norm (or its reciprocal, LSS)
+    __shared__ Spike s_sp[MAX_STAGE];          // staged / newly accepted spikes
     __shared__ Spike s_list[MAX_LIST];         // this workgroup's spikes, whole batch
-    __shared__ float s_val[PEEL_THREADS];
+    __shared__ Spike s_pre[MAX_NBR * PRE];
+    __shared__ float s_val[PEEL_THREADS];      // also the rank table of the epilogue
     __shared__ int s_idx[PEEL_THREADS];
     __shared__ int s_org[PEEL_THREADS];
+    __shared__ float s_bvv[PEEL_THREADS];
     __shared__ float s_win[MAX_CW + 2 * MAX_M + 2];
-    __shared__ float s_cf[MAX_CW];             // own column maxima
+    __shared__ float s_cf[MAX_CW];             // own column maxima, their argmax and its score
     __shared__ int s_im[MAX_CW];
+    __shared__ float s_bv[MAX_CW];
+    __shared__ float s_cm[MAX_CW];
     __shared__ int s_flag[MAX_CW];
-    __shared__ int s_rank[MAX_LIST];
-    __shared__ int s_hcnt[MAX_NBR + 1];        // neighbour slot counts, then their prefix
-    __shared__ int s_nsp, s_listn, s_stop, s_off, s_n;
+    __shared__ int s_hcnt[MAX_NBR + 1];
+    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand;
+    int* const s_rank = reinterpret_cast<int*>(s_val);
+    static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
 
     const int g = blockIdx.x, tid = threadIdx.x;
-    const int CW = p.CW, M = p.M, T = p.T, W = p.W;
+    // register tiles use a fixed 64-column width, so column and row strides are
+    // compile-time constants (LDS offsets fold into the instructions)
+    const int CW = R > 0 ? PEEL_THREADS / REG_NRG : p.CW;
+    const int M = p.M, T = p.T, W = p.W;
     const int c0 = g * CW, c1 = min(W, c0 + CW);
     const int nrg = PEEL_THREADS / CW;
     const int col = tid % CW, rg = tid / CW;
     const int hlo = max(0, (c0 - M) / CW), hhi = min(p.G - 1, (c1 - 1 + M) / CW);
-    const int nh = hhi - hlo + 1;             // <= MAX_NBR by the choice CW >= M
+    const int nh = hhi - hlo + 1;             // <= MAX_NBR because CW >= M
+    const int w = c0 + col;
+    const bool own = w < c1;
+    const bool edge = (w < M) || (w >= W - M);
     unsigned target = 0;
     if (tid == 0) s_listn = 0;
-    int it_found = 0;
 
-    for (int it = 0; it < p.max_iter; ++it) {
-        // ---------------- phase A ----------------
+    // Buffer descriptors: one scalar base per table plus a per-thread offset, so
+    // the R rows of a tile need no address registers (row strides go in SGPRs).
+    const __amdgpu_buffer_rsrc_t ctc_rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p.ctc), 0, p.ctc_bytes, BUF_DWORD3);
+    float breg[R > 0 ? R : 1];
+    if (R > 0) {
+        for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) s_nm[t] = p.lss ? p.rnm[t] : p.nm[t];
+        const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
+        const int voff = (rg * p.BS + w) * 4;
+        const int rstride = REG_NRG * p.BS * 4;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            breg[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(b_rs, voff, i * rstride, 0));
+        __syncthreads();
+    }
+
+    int it_found = 0;
+    for (int it = 0; it <= p.max_iter; ++it) {
+        int nsp = 0;
         if (it > 0) {
-            // stage the neighbours' spikes of the previous iteration (two coherent round trips)
-            if (tid < nh) s_hcnt[tid] = min(ld_c(&p.slot_cnt[hlo + tid]), p.slot_cap);
-            __syncthreads();
-            if (tid == 0) {
-                int acc = 0;
-                for (int i = 0; i < nh; ++i) {
-                    const int c = s_hcnt[i];
-                    s_hcnt[i] = acc;
-                    acc += c;
-                }
-                s_hcnt[nh] = acc;
+            // one coherent round trip: spike total of the last iteration, the
+            // neighbours' counts, and their first PRE spikes
+            if (tid == 0) { s_tot = ld_c(&p.iter_cnt[it - 1]); s_err = ld_c(p.err); }
+            if (tid >= 64 && tid < 64 + nh) s_hcnt[tid - 64] = min(ld_c(&p.slot_cnt[hlo + tid - 64]), p.slot_cap);
+            if (tid >= 128 && tid < 128 + nh * PRE) {
+                const int hh = (tid - 128) / PRE, k = (tid - 128) % PRE;
+                s_pre[hh * PRE + k] = ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + k]);
             }
             __syncthreads();
-            const int tot = s_hcnt[nh];
-            for (int i = tid; i < tot && i < MAX_STAGE; i += PEEL_THREADS) {
-                int hh = 0;
-                while (hh + 1 < nh && s_hcnt[hh + 1] <= i) ++hh;
-                s_sp[i] = ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + (i - s_hcnt[hh])]);
-            }
-            if (tid == 0 && tot > MAX_STAGE) atomicOr(p.err, ERR_STAGE);
-            __syncthreads();
-            // keep those whose window reaches the owned columns, in order
+            if (s_tot == 0 || (s_err & ERR_BARRIER)) break;
+            it_found = it;
+            if (it == p.max_iter) break;
             if (tid == 0) {
                 int n = 0;
-                for (int i = 0; i < min(tot, MAX_STAGE); ++i) {
-                    const Spike sp = s_sp[i];
-                    if (sp.w + M >= c0 && sp.w - M < c1) s_sp[n++] = sp;
+                for (int hh = 0; hh < nh; ++hh) {
+                    const int cnt = s_hcnt[hh];
+                    for (int k = 0; k < cnt; ++k) {
+                        const Spike sp = k < PRE ? s_pre[hh * PRE + k]
+                                                 : ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + k]);
+                        if (sp.w + M >= c0 && sp.w - M < c1) {
+                            if (n < MAX_STAGE) s_sp[n++] = sp;
+                            else atomicOr(p.err, ERR_STAGE);
+                        }
+                    }
                 }
                 s_nsp = n;
             }
-        } else if (tid == 0) {
-            s_nsp = 0;
+            __syncthreads();
+            nsp = s_nsp;
         }
-        __syncthreads();
-        const int nsp = s_nsp;
+
+        // ---------------- phase A ----------------
         if (it == 0 || nsp > 0) {
-            const int w = c0 + col;
-            float best = -INFINITY;
+            float best = -INFINITY, bval = 0.f;
             int bi = 0, bo = 0x7fffffff;
-            if (w < c1) {
-                const bool edge = (w < M) || (w >= W - M);
-                for (int t = rg; t < T; t += nrg) {
-                    float* bp = p.B + (int64_t)t * p.BS + w;
-                    float b = *bp;
-                    bool changed = false;
+            if (own && edge) {
+                // Cf is zero for every template: first file index wins the tie
+                best = 0.f; bi = p.t_first; bo = 0;
+            } else if (own) {
+                if (R > 0) {
+                    // The tables are padded to the tile height (zero ctc rows, +inf norms,
+                    // INT_MAX file indices), so no row needs a bounds check.
                     for (int q = 0; q < nsp; ++q) {
                         const Spike sp = s_sp[q];
                         const int l = w - sp.w + M;
-                        if (l >= 0 && l <= 2 * M && t >= p.tlo[sp.t] && t < p.thi[sp.t]) {
-                            b -= sp.amp * p.ctc[((int64_t)sp.t * T + t) * LP + l];
-                            changed = true;
+                        if (l < 0 || l > 2 * M) continue;
+                        // rows outside the template's nonzero band hold exact zeros, so
+                        // the update is applied to every row without a branch
+                        const int voff = ((sp.t * p.ctc_rows + rg) * LP + l) * 4;
+#pragma unroll
+                        for (int i = 0; i < R; ++i)
+                            breg[i] -= sp.amp * __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                                                    ctc_rs, voff, i * REG_NRG * LP * 4, 0));
+                    }
+                    int vb = rg;
+                    asm volatile("" : "+v"(vb));
+                    const float* nmp = s_nm + vb;
+                    int bslot = 0;          // winning row slot; its row is vb + REG_NRG * bslot
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const float r = fmaxf(breg[i], 0.f);
+                        const float cf = p.lss ? (r * r) * nmp[REG_NRG * i] : (r * r) / nmp[REG_NRG * i];
+                        // rows are scanned in increasing internal order; the file index
+                        // decides only exact ties (first index wins, as torch.max)
+                        if (cf > best) {
+                            best = cf; bslot = i; bo = -1; bval = breg[i];
+                        } else if (cf == best) {
+                            if (bo < 0) bo = p.orig[vb + REG_NRG * bslot];
+                            const int o = p.orig[vb + REG_NRG * i];
+                            if (o < bo) { bslot = i; bo = o; bval = breg[i]; }
                         }
                     }
-                    if (changed) *bp = b;
-                    float cf = 0.f;
-                    if (!edge) {
-                        const float r = fmaxf(b, 0.f);
-                        cf = p.lss ? (r * r) * p.rnm[t] : (r * r) / p.nm[t];
+                    bi = vb + REG_NRG * bslot;
+                    if (bo < 0) bo = p.orig[bi];
+                } else {
+                    for (int t = rg; t < T; t += nrg) {
+                        float* bp = p.B + (int64_t)t * p.BS + w;
+                        float b = *bp;
+                        bool changed = false;
+                        for (int q = 0; q < nsp; ++q) {
+                            const Spike sp = s_sp[q];
+                            const int l = w - sp.w + M;
+                            if (l >= 0 && l <= 2 * M && t >= p.tlo[sp.t] && t < p.thi[sp.t]) {
+                                b -= sp.amp * p.ctc[((int64_t)sp.t * p.ctc_rows + t) * LP + l];
+                                changed = true;
+                            }
+                        }
+                        if (changed) *bp = b;
+                        float cf = 0.f;
+                        if (!edge) {
+                            const float r = fmaxf(b, 0.f);
+                            cf = p.lss ? (r * r) * p.rnm[t] : (r * r) / p.nm[t];
+                        }
+                        const int o = p.orig[t];
+                        if (better(cf, o, best, bo)) { best = cf; bi = t; bo = o; bval = b; }
                     }
-                    const int o = p.orig[t];
-                    if (better(cf, o, best, bo)) { best = cf; bi = t; bo = o; }
                 }
             }
-            s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo;
+            s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo; s_bvv[tid] = bval;
             __syncthreads();
             if (tid < CW) {
-                float bv = s_val[tid];
-                int bI = s_idx[tid], bO = s_org[tid];
+                // per-thread base pointers: the row-group strides become immediate
+                // offsets.  The empty asm keeps the base from being treated as loop
+                // invariant, which would hoist one address register per row group.
+                int vb = tid;
+                asm volatile("" : "+v"(vb));
+                const float* pv = s_val + vb;
+                const int* pi = s_idx + vb;
+                const int* po = s_org + vb;
+                const float* pb = s_bvv + vb;
+                float bv = pv[0], bb = pb[0];
+                int bI = pi[0], bO = po[0];
                 for (int r = 1; r < nrg; ++r) {
-                    const int j = r * CW + tid;
-                    if (better(s_val[j], s_org[j], bv, bO)) { bv = s_val[j]; bI = s_idx[j]; bO = s_org[j]; }
+                    const int j = r * CW;
+                    if (better(pv[j], po[j], bv, bO)) { bv = pv[j]; bI = pi[j]; bO = po[j]; bb = pb[j]; }
                 }
                 const int ww = c0 + tid;
                 if (ww < c1) {
                     s_cf[tid] = bv;
                     s_im[tid] = bI;
+                    s_bv[tid] = bb;
                     st_c(&p.Cfmax[ww], bv);
                     if (it == 0 && p.dbg_cfmax) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
                 }
@@ -403,6 +484,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             else if (ww >= 0 && ww < W) v = ld_c(&p.Cfmax[ww]);
             s_win[i] = v;
         }
+        if (tid == 0) s_ncand = 0;
         __syncthreads();
         if (tid < CW) {
             int f = 0;
@@ -411,44 +493,75 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                 for (int i = 0; i <= 2 * M; ++i) cm = fmaxf(cm, s_win[tid + i]);
                 const float cfw = s_win[tid + M];
                 f = p.lss ? (cm >= p.Th2 && (cm - cfw) < 1e-4f) : (cm > p.Th2 && fabsf(cm - cfw) < 1e-9f);
-                if (f) s_val[tid] = cm;
+                s_cm[tid] = cm;
             }
             s_flag[tid] = f;
         }
         __syncthreads();
-        if (tid == 0) {
-            int n = 0, ln = s_listn;
-            for (int i = 0; i < c1 - c0; ++i) {
-                if (!s_flag[i]) continue;
-                const int ww = c0 + i;
-                if (!p.lss) {
-                    const int t = s_im[i];
-                    const float a = p.B[(int64_t)t * p.BS + ww] / p.nm[t];
-                    const Spike sp{ww, t, a, it};
-                    if (n < p.slot_cap) s_sp[n] = sp;
-                    else atomicOr(p.err, ERR_SLOT);
-                    if (ln < MAX_LIST) s_list[ln] = sp;
-                    else atomicOr(p.err, ERR_LIST);
-                    ++n; ++ln;
+        if (p.lss) {
+            // every template within 1e-4 of the pooled max (myGPUhelpers.cu:465); the
+            // candidates are collected, then ordered by (column, template)
+            if (own && s_flag[col]) {
+                const float cm = s_cm[col];
+                if (R > 0) {
+                    int vb = rg;
+                    asm volatile("" : "+v"(vb));
+#pragma unroll
+                    for (int i = 0; i < (R > 0 ? R : 1); ++i) {
+                        const int t = vb + REG_NRG * i;
+                        const float b = breg[i];
+                        const float r = fmaxf(b, 0.f);
+                        const float cf = (r * r) * s_nm[t];      // padded rows: rnm = 0
+                        if (fabsf(cm - cf) < 1e-4f) {
+                            const int k = atomicAdd(&s_ncand, 1);
+                            if (k < MAX_STAGE) s_sp[k] = Spike{w, t, b * s_nm[t], it};
+                        }
+                    }
                 } else {
-                    // every template within 1e-4 of the pooled max (myGPUhelpers.cu:465)
-                    const float cm = s_val[i];
-                    for (int t = 0; t < T; ++t) {
-                        const float b = p.B[(int64_t)t * p.BS + ww];
+                    for (int t = rg; t < T; t += nrg) {
+                        const float b = p.B[(int64_t)t * p.BS + w];
                         const float r = fmaxf(b, 0.f);
                         const float cf = (r * r) * p.rnm[t];
                         if (fabsf(cm - cf) < 1e-4f) {
-                            const Spike sp{ww, t, b * p.rnm[t], it};
-                            if (n < p.slot_cap) s_sp[n] = sp;
-                            else atomicOr(p.err, ERR_SLOT);
-                            if (ln < MAX_LIST) s_list[ln] = sp;
-                            else atomicOr(p.err, ERR_LIST);
-                            ++n; ++ln;
+                            const int k = atomicAdd(&s_ncand, 1);
+                            if (k < MAX_STAGE) s_sp[k] = Spike{w, t, b * p.rnm[t], it};
                         }
                     }
                 }
             }
-            s_n = min(n, p.slot_cap);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            int n = 0, ln = s_listn;
+            if (!p.lss) {
+                for (int i = 0; i < c1 - c0; ++i) {
+                    if (!s_flag[i]) continue;
+                    const int t = s_im[i];
+                    const Spike sp{c0 + i, t, s_bv[i] / p.nm[t], it};
+                    if (n < p.slot_cap) s_sp[n] = sp;
+                    else atomicOr(p.err, ERR_SLOT);
+                    ++n;
+                }
+            } else {
+                n = min(s_ncand, MAX_STAGE);
+                if (s_ncand > MAX_STAGE) atomicOr(p.err, ERR_SLOT);
+                for (int i = 1; i < n; ++i) {          // insertion sort by (column, template)
+                    const Spike x = s_sp[i];
+                    int j = i - 1;
+                    while (j >= 0 && (s_sp[j].w > x.w || (s_sp[j].w == x.w && s_sp[j].t > x.t))) {
+                        s_sp[j + 1] = s_sp[j];
+                        --j;
+                    }
+                    s_sp[j + 1] = x;
+                }
+                if (n > p.slot_cap) { atomicOr(p.err, ERR_SLOT); n = p.slot_cap; }
+            }
+            for (int i = 0; i < n; ++i) {
+                if (ln < MAX_LIST) s_list[ln] = s_sp[i];
+                else atomicOr(p.err, ERR_LIST);
+                ++ln;
+            }
+            s_n = n;
             s_listn = ln;
             if (n > 0) __hip_atomic_fetch_add(&p.iter_cnt[it], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -459,14 +572,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             if (tid == 0) st_c(&p.slot_cnt[g], n);
         }
         grid_sync(p.bar, p.G, target, p.err);
-        if (tid == 0) {
-            const int tot = ld_c(&p.iter_cnt[it]);
-            const int e = ld_c(p.err);
-            s_stop = (tot == 0) || (e & ERR_BARRIER);
-        }
-        __syncthreads();
-        if (s_stop) break;
-        it_found = it + 1;
     }
 
     // ---------------- events and features ----------------
@@ -517,7 +622,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             const int ch = p.iCC[j * C + p.iU[si.t]];
             float f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
             for (int hh = flo; hh <= fhi; ++hh) {
-                const int cnt = ld_c(&p.list_cnt[hh]);
+                const int cnt = hh == g ? nmine : ld_c(&p.list_cnt[hh]);
                 const Spike* lst = p.list + (int64_t)hh * p.cap_g;
                 for (int q = 0; q < cnt; ++q) {
                     const Spike sq = hh == g ? s_list[q] : ld_spike(&lst[q]);
@@ -578,16 +683,35 @@ void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* 
     hipLaunchKernelGGL(k_norms, dim3(T), dim3(256), 0, s, U, KC, nm, rnm, nm64);
 }
 
-hipError_t launch_peel(const PeelParams& p, hipStream_t s) {
-    PeelParams copy = p;
-    void* args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_peel), dim3(p.G), dim3(PEEL_THREADS),
-                                      args, 0, s);
+static const void* peel_fn(int R) {
+    switch (R) {
+        case 8: return reinterpret_cast<const void*>(k_peel<8>);
+        case 16: return reinterpret_cast<const void*>(k_peel<16>);
+        case 32: return reinterpret_cast<const void*>(k_peel<32>);
+        case 64: return reinterpret_cast<const void*>(k_peel<64>);
+        default: return reinterpret_cast<const void*>(k_peel<0>);
+    }
 }
 
-int peel_max_blocks(int device) {
+int peel_rows(int T, int CW) {
+    if (CW != PEEL_THREADS / REG_NRG || T > REG_MAX_T) return 0;
+    const int per = (T + REG_NRG - 1) / REG_NRG;
+    for (int R : {8, 16, 32, 64})
+        if (per <= R) return R;
+    return 0;
+}
+
+hipError_t launch_peel(const PeelParams& p, int R, bool cooperative, hipStream_t s) {
+    PeelParams copy = p;
+    void* args[] = {&copy};
+    if (cooperative)
+        return hipLaunchCooperativeKernel(peel_fn(R), dim3(p.G), dim3(PEEL_THREADS), args, 0, s);
+    return hipLaunchKernel(peel_fn(R), dim3(p.G), dim3(PEEL_THREADS), args, 0, s);
+}
+
+int peel_max_blocks(int device, int R) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_peel, PEEL_THREADS, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_fn(R), PEEL_THREADS, 0) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return per_cu * cus;
 }
