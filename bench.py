@@ -79,6 +79,7 @@ def main() -> int:
 
     from livespikesorter_amd import OnlineSpikesV2
     from livespikesorter_amd._hip import DeviceBuffer, synchronize
+    from livespikesorter_amd.matcher import STAGE_PHASES
 
     N, M = 3000, 61
     W = N + 2 * M
@@ -103,6 +104,7 @@ def main() -> int:
         step(j)
     sp.stage_times(reset=True)
     lat, iters, spikes = [], [], []
+    phases = np.zeros(6)
     synchronize()
     t_start = time.perf_counter()
     for j in range(args.warmup, args.warmup + args.steps):
@@ -112,6 +114,7 @@ def main() -> int:
         q = sp.query()
         iters.append(q.last_iters)
         spikes.append(q.last_spikes)
+        phases += sp.debug_stage(STAGE_PHASES)
     synchronize()
     elapsed = time.perf_counter() - t_start
     stage_ms, nb = sp.stage_times(reset=True)
@@ -120,6 +123,10 @@ def main() -> int:
         f"p99 {pct(lat, 99):.3f} ms; mean iterations {np.mean(iters):.1f}, spikes {np.mean(spikes):.1f}")
     avg = {k: v / max(nb, 1) for k, v in stage_ms.items()}
     log("device ms per chunk by stage: " + ", ".join(f"{k} {v:.4f}" for k, v in avg.items()))
+    # the peel kernel's own clock (100 MHz ticks), workgroup 0, per chunk
+    phase_names = ("exchange", "scan", "barrier1", "accept", "barrier2", "epilogue")
+    peel_phases = {k: round(v / args.steps / 100.0, 3) for k, v in zip(phase_names, phases)}
+    log(f"peel phases, us per chunk: {peel_phases}")
 
     # host-resident input: chunk latency from int16 on the host to events on the host
     host_lat = []
@@ -225,6 +232,7 @@ def main() -> int:
         "mean_iterations": round(I, 2),
         "mean_spikes_per_chunk": round(S, 2),
         "stages": stages,
+        "peel_phases_us": peel_phases,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,

@@ -862,6 +862,12 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
     return guarded([&] {
         if (!h || !out || !bytes) throw LssError(LSS_ERR_ARG, "null argument");
         if (!h->cfg.keep_stages && stage >= 2 && stage <= 4) throw LssError(LSS_ERR_STATE, "keep_stages is off");
+        if (stage == 6) {   // host-resident: no device access needed
+            if (*bytes < (int64_t)NPHASE * 8) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+            std::memcpy(out, h->hdr->phase_ticks, NPHASE * 8);
+            *bytes = NPHASE * 8;
+            return;
+        }
         if (h->last_W <= 0) throw LssError(LSS_ERR_STATE, "no batch processed yet");
         HIPCHK(hipSetDevice(h->dev));
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -872,6 +878,7 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
         else if (stage == 2) need = (int64_t)T * W * 4;
         else if (stage == 3 || stage == 4) need = (int64_t)W * 4;
         else if (stage == 5) need = (int64_t)T * 4;
+        else if (stage == 6) need = (int64_t)NPHASE * 8;
         else throw LssError(LSS_ERR_ARG, "unknown stage");
         if (*bytes < need) throw LssError(LSS_ERR_ARG, "stage buffer too small");
         char* o = static_cast<char*>(out);

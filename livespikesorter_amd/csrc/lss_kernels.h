@@ -42,7 +42,11 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
     int32_t n_iter;
     int32_t err;
     int32_t pad[4];
+    // workgroup 0's time per phase, in 100 MHz realtime ticks: top-of-iteration
+    // round trip, phase A, barrier 1, phase B, barrier 2, epilogue
+    uint64_t phase_ticks[8];
 };
+constexpr int NPHASE = 6;
 
 // word 3 of a raw buffer descriptor on gfx9-family parts (gfx950): 32-bit data format
 constexpr int BUF_DWORD3 = 0x00020000;

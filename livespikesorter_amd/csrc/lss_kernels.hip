@@ -285,6 +285,8 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 // ---------------------------------------------------------------------------
 constexpr int PRE = 4;   // neighbour spikes fetched speculatively with the counts
 
+__device__ __forceinline__ uint64_t rt_now() { return __builtin_amdgcn_s_memrealtime(); }
+
 template <int R>
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     // Every product here is rounded before it is added or compared, as in the
@@ -293,6 +295,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
     // immediate offset (no per-row address registers); the norm table comes first.
     __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template norm (or its reciprocal, LSS)
+    __shared__ int s_or[R > 0 ? REG_MAX_T : 1];      // per-template file index (INT_MAX past T)
     __shared__ Spike s_sp[MAX_STAGE];          // staged / newly accepted spikes
     __shared__ Spike s_list[MAX_LIST];         // this workgroup's spikes, whole batch
     __shared__ Spike s_pre[MAX_NBR * PRE];
@@ -307,6 +310,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     __shared__ float s_cm[MAX_CW];
     __shared__ int s_flag[MAX_CW];
     __shared__ int s_hcnt[MAX_NBR + 1];
+    __shared__ int s_hcnt_w[MAX_CW / 64];
     __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand;
     int* const s_rank = reinterpret_cast<int*>(s_val);
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
@@ -333,7 +337,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p.ctc), 0, p.ctc_bytes, BUF_DWORD3);
     float breg[R > 0 ? R : 1];
     if (R > 0) {
-        for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) s_nm[t] = p.lss ? p.rnm[t] : p.nm[t];
+        for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) {
+            s_nm[t] = p.lss ? p.rnm[t] : p.nm[t];
+            s_or[t] = p.orig[t];
+        }
         const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
         const int voff = (rg * p.BS + w) * 4;
         const int rstride = REG_NRG * p.BS * 4;
@@ -344,6 +351,16 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     }
 
     int it_found = 0;
+    const bool timing = (g == 0 && tid == 0);
+    uint64_t ph[NPHASE] = {0, 0, 0, 0, 0, 0};
+    uint64_t t_prev = timing ? rt_now() : 0;
+    auto mark = [&](int k) {
+        if (timing) {
+            const uint64_t t = rt_now();
+            ph[k] += t - t_prev;
+            t_prev = t;
+        }
+    };
     for (int it = 0; it <= p.max_iter; ++it) {
         int nsp = 0;
         if (it > 0) {
@@ -377,6 +394,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             __syncthreads();
             nsp = s_nsp;
         }
+        mark(0);
 
         // ---------------- phase A ----------------
         if (it == 0 || nsp > 0) {
@@ -404,23 +422,29 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                     int vb = rg;
                     asm volatile("" : "+v"(vb));
                     const float* nmp = s_nm + vb;
+                    const int* orp = s_or + vb;
+                    // branch-free argmax; on exact ties the smaller file index wins (first
+                    // index, as torch.max).  Rows go in groups of 8 with a compiler barrier
+                    // between groups so the table loads are not all hoisted at once.
+                    best = -1.f;
+                    bo = 0x7fffffff;
                     int bslot = 0;          // winning row slot; its row is vb + REG_NRG * bslot
 #pragma unroll
-                    for (int i = 0; i < R; ++i) {
-                        const float r = fmaxf(breg[i], 0.f);
-                        const float cf = p.lss ? (r * r) * nmp[REG_NRG * i] : (r * r) / nmp[REG_NRG * i];
-                        // rows are scanned in increasing internal order; the file index
-                        // decides only exact ties (first index wins, as torch.max)
-                        if (cf > best) {
-                            best = cf; bslot = i; bo = -1; bval = breg[i];
-                        } else if (cf == best) {
-                            if (bo < 0) bo = p.orig[vb + REG_NRG * bslot];
-                            const int o = p.orig[vb + REG_NRG * i];
-                            if (o < bo) { bslot = i; bo = o; bval = breg[i]; }
+                    for (int i0 = 0; i0 < R; i0 += 8) {
+#pragma unroll
+                        for (int i = i0; i < i0 + 8; ++i) {
+                            const float r = fmaxf(breg[i], 0.f);
+                            const float cf = p.lss ? (r * r) * nmp[REG_NRG * i] : (r * r) / nmp[REG_NRG * i];
+                            const int o = orp[REG_NRG * i];
+                            const bool gt = cf > best || (cf == best && o < bo);
+                            best = gt ? cf : best;
+                            bo = gt ? o : bo;
+                            bslot = gt ? i : bslot;
+                            bval = gt ? breg[i] : bval;
                         }
+                        asm volatile("" ::: "memory");
                     }
                     bi = vb + REG_NRG * bslot;
-                    if (bo < 0) bo = p.orig[bi];
                 } else {
                     for (int t = rg; t < T; t += nrg) {
                         float* bp = p.B + (int64_t)t * p.BS + w;
@@ -473,7 +497,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                 }
             }
         }
+        mark(1);
         grid_sync(p.bar, p.G, target, p.err);
+        mark(2);
 
         // ---------------- phase B ----------------
         const int nwin = (c1 - c0) + 2 * M;
@@ -531,38 +557,56 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             }
             __syncthreads();
         }
-        if (tid == 0) {
-            int n = 0, ln = s_listn;
-            if (!p.lss) {
-                for (int i = 0; i < c1 - c0; ++i) {
-                    if (!s_flag[i]) continue;
-                    const int t = s_im[i];
-                    const Spike sp{c0 + i, t, s_bv[i] / p.nm[t], it};
-                    if (n < p.slot_cap) s_sp[n] = sp;
-                    else atomicOr(p.err, ERR_SLOT);
-                    ++n;
+        if (!p.lss) {
+            // one spike per accepted column: compact the flags with per-wave ballots
+            const int lane = tid & 63, wv = tid >> 6;
+            const int nwv = (CW + 63) / 64;
+            const bool f = tid < CW && s_flag[tid];
+            const uint64_t bal = __ballot(f);
+            if (tid < CW && lane == 0) s_hcnt_w[wv] = __popcll(bal);
+            __syncthreads();
+            if (tid < CW) {
+                int base = 0;
+                for (int k = 0; k < wv; ++k) base += s_hcnt_w[k];
+                if (f) {
+                    const int idx = base + __popcll(bal & ((1ull << lane) - 1ull));
+                    const int t = s_im[tid];
+                    const Spike sp{c0 + tid, t, s_bv[tid] / p.nm[t], it};
+                    if (idx < p.slot_cap) s_sp[idx] = sp;
+                    const int ln = s_listn + idx;
+                    if (ln < MAX_LIST) s_list[ln] = sp;
                 }
-            } else {
-                n = min(s_ncand, MAX_STAGE);
-                if (s_ncand > MAX_STAGE) atomicOr(p.err, ERR_SLOT);
-                for (int i = 1; i < n; ++i) {          // insertion sort by (column, template)
-                    const Spike x = s_sp[i];
-                    int j = i - 1;
-                    while (j >= 0 && (s_sp[j].w > x.w || (s_sp[j].w == x.w && s_sp[j].t > x.t))) {
-                        s_sp[j + 1] = s_sp[j];
-                        --j;
-                    }
-                    s_sp[j + 1] = x;
-                }
-                if (n > p.slot_cap) { atomicOr(p.err, ERR_SLOT); n = p.slot_cap; }
             }
+            __syncthreads();
+            if (tid == 0) {
+                int n = 0;
+                for (int k = 0; k < nwv; ++k) n += s_hcnt_w[k];
+                if (n > p.slot_cap) atomicOr(p.err, ERR_SLOT);
+                if (s_listn + n > MAX_LIST) atomicOr(p.err, ERR_LIST);
+                s_n = min(n, p.slot_cap);
+                s_listn = min(s_listn + n, MAX_LIST);
+                if (n > 0) __hip_atomic_fetch_add(&p.iter_cnt[it], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (tid == 0) {
+            int n = min(s_ncand, MAX_STAGE), ln = s_listn;
+            if (s_ncand > MAX_STAGE) atomicOr(p.err, ERR_SLOT);
+            for (int i = 1; i < n; ++i) {          // insertion sort by (column, template)
+                const Spike x = s_sp[i];
+                int j = i - 1;
+                while (j >= 0 && (s_sp[j].w > x.w || (s_sp[j].w == x.w && s_sp[j].t > x.t))) {
+                    s_sp[j + 1] = s_sp[j];
+                    --j;
+                }
+                s_sp[j + 1] = x;
+            }
+            if (n > p.slot_cap) { atomicOr(p.err, ERR_SLOT); n = p.slot_cap; }
             for (int i = 0; i < n; ++i) {
                 if (ln < MAX_LIST) s_list[ln] = s_sp[i];
                 else atomicOr(p.err, ERR_LIST);
                 ++ln;
             }
             s_n = n;
-            s_listn = ln;
+            s_listn = min(ln, MAX_LIST);
             if (n > 0) __hip_atomic_fetch_add(&p.iter_cnt[it], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
@@ -571,8 +615,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             for (int i = tid; i < n; i += PEEL_THREADS) st_spike(&p.slots[(int64_t)g * p.slot_cap + i], s_sp[i]);
             if (tid == 0) st_c(&p.slot_cnt[g], n);
         }
+        mark(3);
         grid_sync(p.bar, p.G, target, p.err);
+        mark(4);
     }
+    mark(0);
 
     // ---------------- events and features ----------------
     const int nmine = min(s_listn, MAX_LIST);
@@ -637,6 +684,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             p.feat[(int64_t)d * nC * K + lane] = f;
         }
     }
+    mark(5);
+    if (timing)
+        for (int k = 0; k < NPHASE; ++k) p.hdr->phase_ticks[k] = ph[k];
     if (g == p.G - 1 && tid == 0) {
         p.hdr->count = off + nmine;
         p.hdr->n_iter = it_found;

@@ -20,7 +20,7 @@ from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_FP32, SEM_KILOSORT
 
 __all__ = ["OnlineSpikesV2", "LssError", "SEM_KILOSORT", "SEM_LSS"]
 
-STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM = range(6)
+STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES = range(7)
 
 
 class OnlineSpikesV2:
@@ -136,7 +136,8 @@ class OnlineSpikesV2:
         W = self.last_W
         shapes = {STAGE_XW: ((info.C, W), np.float32), STAGE_BPC: ((info.K, info.C, W), np.float32),
                   STAGE_B: ((info.T, W), np.float32), STAGE_CFMAX: ((W,), np.float32),
-                  STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32)}
+                  STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32),
+                  STAGE_PHASES: ((6,), np.uint64)}
         shape, dt = shapes[stage]
         out = np.zeros(shape, dt)
         nb = ct.c_int64(out.nbytes)
