@@ -125,9 +125,11 @@ int lss_query(struct lss_handle* h, lss_info* info);
 /* Copy a device stage of the last batch to host memory (test support; needs
  * keep_stages).  stage: 0 Xw [C][W] f32, 1 Bpc [K][C][W] f32, 2 B before the
  * peel [T][W] f32, 3 Cfmax of the first iteration [W] f32, 4 imax [W] i32,
- * 5 template norms [T] f32, 6 peel-kernel time per phase of the last batch
- * [6] u64 ticks of the 100 MHz device clock (top-of-iteration exchange,
- * scan, barrier, acceptance, barrier, events and features).
+ * 5 template norms [T] f32, 6 peel-kernel time per phase of the last batch,
+ * u64 ticks of the 100 MHz device clock: [0..6] summed over workgroups and
+ * [7..13] the largest workgroup's (top-of-iteration exchange, scan, barrier,
+ * acceptance, barrier, events and features, subtraction), [14] scan passes
+ * that did work, [15] workgroups.
  * *bytes: in = capacity, out = size written. */
 int lss_debug_stage(struct lss_handle* h, int32_t stage, void* out, int64_t* bytes);
 

@@ -133,6 +133,7 @@ struct lss_handle {
     Spike *d_slots = nullptr, *d_list = nullptr;
     int *d_slot_cnt = nullptr, *d_list_cnt = nullptr, *d_iter_cnt = nullptr, *d_err = nullptr;
     unsigned* d_bar = nullptr;
+    unsigned long long* d_prof = nullptr;
     int slot_cap = 0, cap_g = 0, max_groups = 0;
     // debug copies
     float *d_dbgB = nullptr, *d_dbgCf = nullptr;
@@ -175,7 +176,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x};
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof};
         (void)hipSetDevice(dev);
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -507,6 +508,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_iter_cnt = dalloc<int>(std::max(1, h->max_iter));
     h->d_err = dalloc<int>(1);
     h->d_bar = dalloc<unsigned>(1);
+    h->d_prof = dalloc<unsigned long long>(PROF_WORDS);
     if (h->cfg.keep_stages) {
         h->d_dbgB = dalloc<float>((size_t)TP * h->BS);
         h->d_dbgCf = dalloc<float>(Wmax);
@@ -557,7 +559,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     const int M = h->M;
     const bool prof = h->cfg.profile != 0;
     if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
-    launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, s);
+    launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, h->d_prof, s);
     launch_whiten(h->d_Wr, h->CP, d_x, ld, W, h->d_cmap, h->C, h->d_kbw_ptr, h->d_kbw_idx, h->d_Xw, h->XS, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
     launch_pcconv(h->d_Xw, h->XS, h->d_wpca, h->K, M, h->d_Bpc, h->BS, h->C, h->CP, W, s);
@@ -602,6 +604,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     p.first_sample = first_sample; p.half = M / 2; p.nt0min = h->nt0min;
     p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
+    p.prof = h->d_prof;
     HIPCHK(launch_peel(p, R, h->coop, s));
     if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
     HIPCHK(hipStreamSynchronize(s));
@@ -862,10 +865,11 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
     return guarded([&] {
         if (!h || !out || !bytes) throw LssError(LSS_ERR_ARG, "null argument");
         if (!h->cfg.keep_stages && stage >= 2 && stage <= 4) throw LssError(LSS_ERR_STATE, "keep_stages is off");
-        if (stage == 6) {   // host-resident: no device access needed
-            if (*bytes < (int64_t)NPHASE * 8) throw LssError(LSS_ERR_ARG, "stage buffer too small");
-            std::memcpy(out, h->hdr->phase_ticks, NPHASE * 8);
-            *bytes = NPHASE * 8;
+        if (stage == 6) {
+            if (*bytes < (int64_t)PROF_WORDS * 8) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+            HIPCHK(hipSetDevice(h->dev));
+            HIPCHK(hipMemcpy(out, h->d_prof, PROF_WORDS * 8, hipMemcpyDeviceToHost));
+            *bytes = PROF_WORDS * 8;
             return;
         }
         if (h->last_W <= 0) throw LssError(LSS_ERR_STATE, "no batch processed yet");
@@ -878,7 +882,7 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
         else if (stage == 2) need = (int64_t)T * W * 4;
         else if (stage == 3 || stage == 4) need = (int64_t)W * 4;
         else if (stage == 5) need = (int64_t)T * 4;
-        else if (stage == 6) need = (int64_t)NPHASE * 8;
+        else if (stage == 6) need = (int64_t)PROF_WORDS * 8;
         else throw LssError(LSS_ERR_ARG, "unknown stage");
         if (*bytes < need) throw LssError(LSS_ERR_ARG, "stage buffer too small");
         char* o = static_cast<char*>(out);

@@ -42,11 +42,14 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
     int32_t n_iter;
     int32_t err;
     int32_t pad[4];
-    // workgroup 0's time per phase, in 100 MHz realtime ticks: top-of-iteration
-    // round trip, phase A, barrier 1, phase B, barrier 2, epilogue
-    uint64_t phase_ticks[8];
+    int32_t pad2[4];
 };
-constexpr int NPHASE = 6;
+// Peel-kernel time per phase in 100 MHz realtime ticks, summed and maxed over
+// workgroups: top-of-iteration exchange, phase A scan, barrier 1, phase B,
+// barrier 2, epilogue, phase A subtraction; then the number of phase-A passes
+// that did work and the number of workgroups (summed).
+constexpr int NPHASE = 7;
+constexpr int PROF_WORDS = 2 * NPHASE + 2;
 
 // word 3 of a raw buffer descriptor on gfx9-family parts (gfx950): 32-bit data format
 constexpr int BUF_DWORD3 = 0x00020000;
@@ -79,10 +82,11 @@ struct PeelParams {
     OutHeader* hdr; lss_event* ev; float* feat; int ev_cap;
     int64_t first_sample; int half; int nt0min;
     float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
+    unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch
 };
 
 // Clears the per-batch counters the peel kernel relies on.
-void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, hipStream_t s);
+void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, hipStream_t s);
 
 void launch_whiten(const float* Wr, int CP, const int16_t* x, int64_t ld, int W, const int* cmap, int C,
                    const int* kb_ptr, const int* kb_idx, float* Xw, int XS, hipStream_t s);

@@ -195,8 +195,9 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ U, int 
     }
 }
 
-__global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err) {
+__global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof) {
     for (int i = threadIdx.x; i < n_iter; i += blockDim.x) iter_cnt[i] = 0;
+    if ((int)threadIdx.x < PROF_WORDS) prof[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
         *bar = 0u;
         *err = 0;
@@ -311,6 +312,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     __shared__ int s_flag[MAX_CW];
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ int s_hcnt_w[MAX_CW / 64];
+    __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
     __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand;
     int* const s_rank = reinterpret_cast<int*>(s_val);
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
@@ -342,8 +344,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             s_or[t] = p.orig[t];
         }
         const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
-        const int voff = (rg * p.BS + w) * 4;
-        const int rstride = REG_NRG * p.BS * 4;
+        // wave rg holds the contiguous row block [rg*R, rg*R + R) of its 64 columns
+        const int voff = (rg * R * p.BS + w) * 4;
+        const int rstride = p.BS * 4;
 #pragma unroll
         for (int i = 0; i < R; ++i)
             breg[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(b_rs, voff, i * rstride, 0));
@@ -351,8 +354,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     }
 
     int it_found = 0;
-    const bool timing = (g == 0 && tid == 0);
-    uint64_t ph[NPHASE] = {0, 0, 0, 0, 0, 0};
+    const bool timing = (tid == 0);
+    int a_passes = 0;
+    uint64_t ph[NPHASE] = {0, 0, 0, 0, 0, 0, 0};
     uint64_t t_prev = timing ? rt_now() : 0;
     auto mark = [&](int k) {
         if (timing) {
@@ -398,28 +402,74 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
 
         // ---------------- phase A ----------------
         if (it == 0 || nsp > 0) {
+            ++a_passes;
             float best = -INFINITY, bval = 0.f;
             int bi = 0, bo = 0x7fffffff;
+            if (R > 0) {
+                // Runs with the whole wave active (every lane tests spikes for the ballot);
+                // edge columns and columns past W are updated too but never read.
+                // The tables are padded to the tile height (zero ctc rows, +inf norms,
+                // INT_MAX file indices), so no row needs a bounds check.  Templates are
+                // ordered by main channel, so a spike's nonzero ctc rows form a band
+                // [tlo, thi): a wave whose row block misses it skips the spike (a
+                // wave-uniform branch).  Otherwise the R loads of the spike are all in
+                // flight before any is consumed.  A spike whose window misses this
+                // column contributes with amplitude 0 (b - 0*c is b), so lanes never
+                // diverge.  Spikes are applied in sample order.
+                const int r0 = rg * R;
+                const int lane = tid & 63;
+                int* whit = s_whit + rg * 64;
+                for (int q0 = 0; q0 < nsp; q0 += 64) {
+                    // this wave's spikes whose band reaches its row block, compacted with a ballot
+                    const int qq = q0 + lane;
+                    bool bh = false;
+                    if (qq < nsp) {
+                        const int st = s_sp[qq].t;
+                        bh = r0 < p.thi[st] && r0 + R > p.tlo[st];
+                    }
+                    const uint64_t bal = __ballot(bh);
+                    if (bh) whit[__popcll(bal & ((1ull << lane) - 1ull))] = qq;
+                    const int nhit = __popcll(bal);
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_s_waitcnt(0xc07f);     // lgkmcnt(0): the list is in LDS
+                    for (int h0 = 0; h0 < nhit; h0 += 4) {
+                        float am[4];
+                        int vo[4];
+    #pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const bool in = h0 + j < nhit;
+                            const Spike sp = s_sp[whit[in ? h0 + j : h0]];
+                            const int l = w - sp.w + M;
+                            const bool hit = in && l >= 0 && l <= 2 * M;
+                            am[j] = hit ? sp.amp : 0.f;
+                            vo[j] = ((sp.t * p.ctc_rows + r0) * LP + (hit ? l : 0)) * 4;
+                        }
+                        constexpr int RC = R < 8 ? R : 8;    // rows per load batch
+    #pragma unroll
+                        for (int i0 = 0; i0 < R; i0 += RC) {
+                            float c[4][RC];
+    #pragma unroll
+                            for (int j = 0; j < 4; ++j)
+    #pragma unroll
+                                for (int i = 0; i < RC; ++i)
+                                    c[j][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                                        ctc_rs, vo[j], (i0 + i) * LP * 4, 0));
+    #pragma unroll
+                            for (int i = 0; i < RC; ++i)
+    #pragma unroll
+                                for (int j = 0; j < 4; ++j) breg[i0 + i] -= am[j] * c[j][i];
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                mark(6);
+            }
             if (own && edge) {
                 // Cf is zero for every template: first file index wins the tie
                 best = 0.f; bi = p.t_first; bo = 0;
             } else if (own) {
                 if (R > 0) {
-                    // The tables are padded to the tile height (zero ctc rows, +inf norms,
-                    // INT_MAX file indices), so no row needs a bounds check.
-                    for (int q = 0; q < nsp; ++q) {
-                        const Spike sp = s_sp[q];
-                        const int l = w - sp.w + M;
-                        if (l < 0 || l > 2 * M) continue;
-                        // rows outside the template's nonzero band hold exact zeros, so
-                        // the update is applied to every row without a branch
-                        const int voff = ((sp.t * p.ctc_rows + rg) * LP + l) * 4;
-#pragma unroll
-                        for (int i = 0; i < R; ++i)
-                            breg[i] -= sp.amp * __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                                                    ctc_rs, voff, i * REG_NRG * LP * 4, 0));
-                    }
-                    int vb = rg;
+                    int vb = rg * R;
                     asm volatile("" : "+v"(vb));
                     const float* nmp = s_nm + vb;
                     const int* orp = s_or + vb;
@@ -428,14 +478,14 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                     // between groups so the table loads are not all hoisted at once.
                     best = -1.f;
                     bo = 0x7fffffff;
-                    int bslot = 0;          // winning row slot; its row is vb + REG_NRG * bslot
+                    int bslot = 0;          // winning row slot; its row is vb + bslot
 #pragma unroll
                     for (int i0 = 0; i0 < R; i0 += 8) {
 #pragma unroll
                         for (int i = i0; i < i0 + 8; ++i) {
                             const float r = fmaxf(breg[i], 0.f);
-                            const float cf = p.lss ? (r * r) * nmp[REG_NRG * i] : (r * r) / nmp[REG_NRG * i];
-                            const int o = orp[REG_NRG * i];
+                            const float cf = p.lss ? (r * r) * nmp[i] : (r * r) / nmp[i];
+                            const int o = orp[i];
                             const bool gt = cf > best || (cf == best && o < bo);
                             best = gt ? cf : best;
                             bo = gt ? o : bo;
@@ -444,7 +494,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
                         }
                         asm volatile("" ::: "memory");
                     }
-                    bi = vb + REG_NRG * bslot;
+                    bi = vb + bslot;
                 } else {
                     for (int t = rg; t < T; t += nrg) {
                         float* bp = p.B + (int64_t)t * p.BS + w;
@@ -530,11 +580,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
             if (own && s_flag[col]) {
                 const float cm = s_cm[col];
                 if (R > 0) {
-                    int vb = rg;
+                    int vb = rg * R;
                     asm volatile("" : "+v"(vb));
 #pragma unroll
                     for (int i = 0; i < (R > 0 ? R : 1); ++i) {
-                        const int t = vb + REG_NRG * i;
+                        const int t = vb + i;
                         const float b = breg[i];
                         const float r = fmaxf(b, 0.f);
                         const float cf = (r * r) * s_nm[t];      // padded rows: rnm = 0
@@ -685,8 +735,14 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
         }
     }
     mark(5);
-    if (timing)
-        for (int k = 0; k < NPHASE; ++k) p.hdr->phase_ticks[k] = ph[k];
+    if (timing) {
+        for (int k = 0; k < NPHASE; ++k) {
+            atomicAdd(&p.prof[k], (unsigned long long)ph[k]);
+            atomicMax(&p.prof[NPHASE + k], (unsigned long long)ph[k]);
+        }
+        atomicAdd(&p.prof[2 * NPHASE], (unsigned long long)a_passes);
+        atomicAdd(&p.prof[2 * NPHASE + 1], 1ull);
+    }
     if (g == p.G - 1 && tid == 0) {
         p.hdr->count = off + nmine;
         p.hdr->n_iter = it_found;
@@ -697,8 +753,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err);
+void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err, prof);
 }
 
 void launch_whiten(const float* Wr, int CP, const int16_t* x, int64_t ld, int W, const int* cmap, int C,

@@ -137,7 +137,7 @@ class OnlineSpikesV2:
         shapes = {STAGE_XW: ((info.C, W), np.float32), STAGE_BPC: ((info.K, info.C, W), np.float32),
                   STAGE_B: ((info.T, W), np.float32), STAGE_CFMAX: ((W,), np.float32),
                   STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32),
-                  STAGE_PHASES: ((6,), np.uint64)}
+                  STAGE_PHASES: ((16,), np.uint64)}
         shape, dt = shapes[stage]
         out = np.zeros(shape, dt)
         nb = ct.c_int64(out.nbytes)
