@@ -156,9 +156,20 @@ def main() -> int:
     dom = max(work, key=lambda k: avg[k])
     bound, amount, unit, peak = work[dom]
     achieved = amount / (avg[dom] / 1e3)
+    # measured HBM bytes per launch of that kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes over this same workload (scaled by a copy kernel)
+    traffic = None
+    tf = ROOT / "profiles" / "cfg2_plain_fp32_traffic.json"
+    if tf.exists():
+        kname = {"whiten": "lss::k_whiten", "pcconv": "lss::k_pcconv", "project": "lss::k_project",
+                 "peel": "lss::k_peel"}[dom]
+        for k, v in json.loads(tf.read_text())["bytes_per_launch"].items():
+            if k.startswith(kname):
+                traffic = round(v)
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 5), "traffic": None, "kernel": dom,
-                "ms_per_launch": round(avg[dom], 5)}
+                "frac": round(achieved / peak, 5), "traffic": traffic, "kernel": dom,
+                "ms_per_launch": round(avg[dom], 5),
+                "algorithmic_bytes" if bound == "hbm" else "algorithmic_tflop": round(amount * (1e9 if bound == "hbm" else 1.0), 3)}
     stages = {k: {"ms": round(avg[k], 5), "bound": work[k][0] if k in work else "pcie",
                   "achieved": round(work[k][1] / (avg[k] / 1e3), 3) if k in work and avg[k] > 0 else None,
                   "unit": work[k][2] if k in work else None} for k in avg if k != "h2d_copy"}
