@@ -143,6 +143,26 @@ int lss_stage_times(struct lss_handle* h, double* ms, int64_t* batches, int32_t 
 
 int lss_destroy(struct lss_handle* h);
 
+/* ---- groups: several probe streams per launch ----
+ * No reference counterpart (the reference runs one independent sorter thread
+ * per probe, main.cpp:77-85).  A group matches one framed batch of every
+ * stream per call with a single hipGraph replay (one batched launch per
+ * stage; the peel kernel is shared by all streams).  Streams may have
+ * different template counts; they share K, M and cfg.  cfg->max_batch is the
+ * framed batch width W used for every stream. */
+struct lss_group;
+int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const lss_config* cfg,
+                     struct lss_group** out);
+/* samples[s]: stream s's batch, [W][ld[s]] int16 (device pointers when
+ * device_input != 0); first_sample[s]: absolute index of its column 0.
+ * Results per stream as lss_process_batch (ev[s], ev_cap[s], n_ev[s], feat[s]
+ * nullable). */
+int lss_group_process(struct lss_group* g, const int16_t* const* samples, const int32_t* ld,
+                      int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
+                      const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
+int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
+int lss_group_destroy(struct lss_group* g);
+
 /* Message of the last error on this thread ("" if none). */
 const char* lss_last_error(void);
 

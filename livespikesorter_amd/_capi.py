@@ -29,7 +29,8 @@ PREC_BF16_RESCORE = 1
 
 ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_process_batch_device",
                  "lss_process_chunk", "lss_flush", "lss_query", "lss_debug_stage", "lss_stage_times",
-                 "lss_destroy", "lss_last_error")
+                 "lss_destroy", "lss_last_error", "lss_group_create", "lss_group_process",
+                 "lss_group_query", "lss_group_destroy")
 NSTAGES = 5
 STAGE_NAMES = ("h2d_copy", "whiten", "pcconv", "project", "peel")
 
@@ -89,6 +90,13 @@ def lib() -> ct.CDLL:
         L.lss_debug_stage.argtypes = [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.POINTER(ct.c_int64)]
         L.lss_stage_times.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64), ct.c_int32]
         L.lss_destroy.argtypes = [ct.c_void_p]
+        L.lss_group_create.argtypes = [ct.c_int32, ct.POINTER(ct.c_char_p), ct.POINTER(Config),
+                                       ct.POINTER(ct.c_void_p)]
+        L.lss_group_process.argtypes = [ct.c_void_p, ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_int32), ct.c_int32,
+                                        ct.POINTER(ct.c_int64), ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_int64),
+                                        ct.POINTER(ct.c_int64), ct.POINTER(ct.c_void_p)]
+        L.lss_group_query.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(Info)]
+        L.lss_group_destroy.argtypes = [ct.c_void_p]
         _LIB = L
     return _LIB
 

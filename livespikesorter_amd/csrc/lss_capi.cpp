@@ -117,6 +117,8 @@ struct lss_handle {
     int Wmax = 0, XS = 0, BS = 0;
     int peel_cap[5] = {};    // co-resident peel workgroups for R = 0, 8, 16, 32, 64
     int ctc_rows = 0, t_first = 0;
+    int min_rows = 0;        // force the padded ctc row count up to this (streams sharing one peel launch)
+    int64_t cur_first = 0;   // absolute sample of column 0 of the last batch
     bool coop = true;        // cooperative launch of the peel kernel
 
     // static device tables (internal template order)
@@ -361,6 +363,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->ctc_rows = T;
     for (int R : {8, 16, 32, 64})
         if (T <= REG_MAX_T && (T + REG_NRG - 1) / REG_NRG <= R) { h->ctc_rows = REG_NRG * R; break; }
+    h->ctc_rows = std::max(h->ctc_rows, h->min_rows);
     const int TS = h->ctc_rows;
     std::vector<float> ctc_i((size_t)T * TS * LP, 0.f);
     std::vector<int> tlo(T, 0), thi(T, 0);
@@ -546,40 +549,51 @@ void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
         want[c] = v;
     }
     if (want != h->cmap_cur) {
-        HIPCHK(hipMemcpyAsync(h->d_cmap, want.data(), want.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(h->d_cmap, want.data(), want.size() * sizeof(int), hipMemcpyHostToDevice));
         h->cmap_cur = want;
     }
 }
 
 // Run one framed batch that is already on the device.
-void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t first_sample) {
-    if (W <= 0 || W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) +
-                                              " samples outside 1..max_batch " + std::to_string(h->Wmax));
-    hipStream_t s = h->stream;
-    const int M = h->M;
-    const bool prof = h->cfg.profile != 0;
-    if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
-    launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, h->d_prof, s);
-    launch_whiten(h->d_Wr, h->CP, d_x, ld, W, h->d_cmap, h->C, h->d_kbw_ptr, h->d_kbw_idx, h->d_Xw, h->XS, s);
-    if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
-    launch_pcconv(h->d_Xw, h->XS, h->d_wpca, h->K, M, h->d_Bpc, h->BS, h->C, h->CP, W, s);
-    if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
-    launch_project(h->d_U2, h->KCP, h->TP / GEMM_BM, h->d_Bpc, h->BS, h->d_kbp_ptr, h->d_kbp_idx, h->d_B,
-                   h->BS, W, s);
-    if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
-    HIPCHK(hipGetLastError());
-    if (h->cfg.keep_stages)
-        HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
+}  // namespace
 
-    PeelParams p{};
-    int CW = MIN_CW, R = 0, G = 0;
-    for (;; CW *= 2) {
-        G = (W + CW - 1) / CW;
-        R = peel_rows(h->T, CW);
-        const int cap = h->peel_cap[R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 : R == 32 ? 3 : 4];
-        if (G <= std::min(cap, h->max_groups)) break;
-        if (CW >= MAX_CW) throw LssError(LSS_ERR_ARG, "batch too large for one co-resident peel launch");
-    }
+struct BatchPlan {
+    WhitenJob wj;
+    PcconvJob cj;
+    ProjectJob pj;
+    PeelParams pp;
+    int R = 0, CW = 0, G = 0;
+};
+
+inline int peel_cap_index(int R) { return R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 : R == 32 ? 3 : 4; }
+
+// Job descriptors of one handle for one batch of W columns.  The input is read
+// from d_x, or, when xptr is set, from the device pointer stored at *xptr.
+void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, int W, int64_t ld, int CW, int R,
+                BatchPlan& bp) {
+    const int M = h->M;
+    WhitenJob& wj = bp.wj;
+    wj = WhitenJob{};
+    wj.Wr = h->d_Wr; wj.x = d_x; wj.xptr = xptr; wj.cmap = h->d_cmap;
+    wj.kb_ptr = h->d_kbw_ptr; wj.kb_idx = h->d_kbw_idx; wj.Xw = h->d_Xw;
+    wj.ld = ld; wj.CP = h->CP; wj.W = W; wj.C = h->C; wj.XS = h->XS;
+    wj.mtiles = h->CP / GEMM_BM;
+    wj.ntiles = (W + XOFF + GEMM_BN - 1) / GEMM_BN;          // cover the right zero pad
+    PcconvJob& cj = bp.cj;
+    cj = PcconvJob{};
+    cj.Xw = h->d_Xw; cj.wpca = h->d_wpca; cj.Bpc = h->d_Bpc;
+    cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP;
+    cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
+    ProjectJob& pj = bp.pj;
+    pj = ProjectJob{};
+    pj.A = h->d_U2; pj.Bm = h->d_Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = h->d_B;
+    pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
+    pj.mtiles = h->TP / GEMM_BM;
+    pj.ntiles = (W + GEMM_BN - 1) / GEMM_BN;
+    PeelParams& p = bp.pp;
+    p = PeelParams{};
+    const int G = (W + CW - 1) / CW;
     p.B = h->d_B; p.BS = h->BS;
     p.B_bytes = (int)std::min<int64_t>((int64_t)h->TP * h->BS * 4, INT32_MAX);
     p.ctc_bytes = (int)std::min<int64_t>((int64_t)h->T * h->ctc_rows * LP * 4, INT32_MAX);
@@ -601,26 +615,27 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     p.iCC = h->d_iCC; p.iU = h->d_iU; p.Gt = h->d_Gt;
     p.do_feat = 1;
     p.hdr = h->d_hdr; p.ev = h->d_ev; p.feat = h->d_feat; p.ev_cap = h->ev_cap;
-    p.first_sample = first_sample; p.half = M / 2; p.nt0min = h->nt0min;
+    p.half = M / 2; p.nt0min = h->nt0min;
     p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
     p.prof = h->d_prof;
-    HIPCHK(launch_peel(p, R, h->coop, s));
-    if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (prof) {
-        float ms = 0.f;
-        if (h->copy_timed) {
-            HIPCHK(hipEventElapsedTime(&ms, h->evt[0], h->evt[1]));
-            h->stage_ms[0] += ms;
-        }
-        for (int i = 1; i < LSS_NSTAGES; ++i) {
-            HIPCHK(hipEventElapsedTime(&ms, h->evt[i], h->evt[i + 1]));
-            h->stage_ms[i] += ms;
-        }
-        h->timed_batches += 1;
-        h->copy_timed = false;
+    bp.R = R; bp.CW = CW; bp.G = G;
+    (void)R;
+}
+
+// Peel geometry for one handle: the narrowest column block whose grid is co-resident.
+void peel_geometry(const lss_handle* h, int W, int& CW, int& R) {
+    for (CW = MIN_CW;; CW *= 2) {
+        const int G = (W + CW - 1) / CW;
+        R = peel_rows(h->T, CW);
+        const int cap = h->peel_cap[peel_cap_index(R)];
+        if (G <= std::min(cap, h->max_groups)) return;
+        if (CW >= MAX_CW) throw LssError(LSS_ERR_ARG, "batch too large for one co-resident peel launch");
     }
+}
+
+// Check the peel kernel's error flags of one handle after a batch.
+void check_peel(lss_handle* h, int W) {
     const OutHeader hd = *h->hdr;
     if (hd.err) {
         std::string why;
@@ -636,6 +651,57 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     h->info.last_spikes = (int32_t)hd.count;
 }
 
+namespace {
+
+// Run one framed batch that is already on the device.
+void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t first_sample) {
+    if (W <= 0 || W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) +
+                                              " samples outside 1..max_batch " + std::to_string(h->Wmax));
+    hipStream_t s = h->stream;
+    const bool prof = h->cfg.profile != 0;
+    int CW = 0, R = 0;
+    peel_geometry(h, W, CW, R);
+    BatchPlan bp;
+    plan_batch(h, d_x, nullptr, W, ld, CW, R, bp);
+    WhitenBatch wb{};
+    wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
+    PcconvBatch cb{};
+    cb.n = 1; cb.start[0] = 0; cb.start[1] = bp.cj.wblocks * h->C; cb.job[0] = bp.cj;
+    ProjectBatch pb{};
+    pb.n = 1; pb.start[0] = 0; pb.start[1] = gemm_blocks(bp.pj.mtiles, bp.pj.ntiles); pb.job[0] = bp.pj;
+    PeelBatch kb{};
+    kb.n = 1; kb.g_start[0] = 0; kb.g_start[1] = bp.G; kb.p[0] = bp.pp;
+    if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
+    launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, h->d_prof, s);
+    launch_whiten(wb, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
+    launch_pcconv(cb, h->K, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
+    launch_project(pb, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
+    HIPCHK(hipGetLastError());
+    if (h->cfg.keep_stages)
+        HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
+    HIPCHK(launch_peel(kb, R, h->coop, s));
+    if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (prof) {
+        float ms = 0.f;
+        if (h->copy_timed) {
+            HIPCHK(hipEventElapsedTime(&ms, h->evt[0], h->evt[1]));
+            h->stage_ms[0] += ms;
+        }
+        for (int i = 1; i < LSS_NSTAGES; ++i) {
+            HIPCHK(hipEventElapsedTime(&ms, h->evt[i], h->evt[i + 1]));
+            h->stage_ms[i] += ms;
+        }
+        h->timed_batches += 1;
+        h->copy_timed = false;
+    }
+    h->cur_first = first_sample;
+    check_peel(h, W);
+}
+
 // Copy the kernel's events (already sorted by sample) to the caller, applying
 // per-template duplicate suppression when enabled.
 int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample,
@@ -645,7 +711,8 @@ int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_
     const int64_t fw = (int64_t)h->nC * h->K;
     int64_t out = 0, total = 0;
     for (int64_t i = 0; i < n; ++i) {
-        const lss_event e = h->ev[i];
+        lss_event e = h->ev[i];
+        e.sample += h->cur_first;           // the kernel writes samples relative to column 0
         if (e.sample < lo_sample || e.sample >= hi_sample) continue;
         if (dedup) {
             int64_t& last = h->last_spike[e.template_id];
@@ -693,6 +760,32 @@ bool dedup_on(const lss_handle* h) {
     return h->cfg.dedup < 0 ? h->sem == LSS_SEM_LSS : h->cfg.dedup != 0;
 }
 
+// Build a handle from an oss_input directory (lss_create; also each stream of a group).
+std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_config* cfg, int min_rows) {
+    auto h = std::make_unique<lss_handle>();
+    if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
+    if (h->cfg.precision != LSS_PREC_FP32)
+        throw LssError(LSS_ERR_ARG, "precision mode not available in this build");
+    if (h->cfg.semantics != LSS_SEM_KILOSORT && h->cfg.semantics != LSS_SEM_LSS)
+        throw LssError(LSS_ERR_ARG, "unknown semantics");
+    if (h->cfg.chunk <= 0) throw LssError(LSS_ERR_ARG, "chunk must be positive");
+    if (h->cfg.max_events <= 0) throw LssError(LSS_ERR_ARG, "max_events must be positive");
+    h->dev = h->cfg.device;
+    h->sem = h->cfg.semantics;
+    h->max_iter = h->cfg.max_iter > 0 ? h->cfg.max_iter : (h->sem == LSS_SEM_LSS ? 50 : 100);
+    if (h->max_iter > 4096) throw LssError(LSS_ERR_ARG, "max_iter too large");
+    h->min_rows = min_rows;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (h->dev < 0 || h->dev >= ndev) throw LssError(LSS_ERR_ARG, "no such HIP device");
+    HIPCHK(hipSetDevice(h->dev));
+    // default batch: one chunk plus two margins of the longest supported template
+    h->Wmax = h->cfg.max_batch > 0 ? h->cfg.max_batch : h->cfg.chunk + 2 * MAX_M;
+    load(h.get(), oss_input_dir);
+    h->hist.clear();
+    return h;
+}
+
 }  // namespace
 
 extern "C" {
@@ -718,27 +811,7 @@ int lss_create(const char* oss_input_dir, const lss_config* cfg, lss_handle** ou
     return guarded([&] {
         if (!oss_input_dir || !out) throw LssError(LSS_ERR_ARG, "null argument");
         *out = nullptr;
-        auto h = std::make_unique<lss_handle>();
-        if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
-        if (h->cfg.precision != LSS_PREC_FP32)
-            throw LssError(LSS_ERR_ARG, "precision mode not available in this build");
-        if (h->cfg.semantics != LSS_SEM_KILOSORT && h->cfg.semantics != LSS_SEM_LSS)
-            throw LssError(LSS_ERR_ARG, "unknown semantics");
-        if (h->cfg.chunk <= 0) throw LssError(LSS_ERR_ARG, "chunk must be positive");
-        if (h->cfg.max_events <= 0) throw LssError(LSS_ERR_ARG, "max_events must be positive");
-        h->dev = h->cfg.device;
-        h->sem = h->cfg.semantics;
-        h->max_iter = h->cfg.max_iter > 0 ? h->cfg.max_iter : (h->sem == LSS_SEM_LSS ? 50 : 100);
-        if (h->max_iter > 4096) throw LssError(LSS_ERR_ARG, "max_iter too large");
-        int ndev = 0;
-        HIPCHK(hipGetDeviceCount(&ndev));
-        if (h->dev < 0 || h->dev >= ndev) throw LssError(LSS_ERR_ARG, "no such HIP device");
-        HIPCHK(hipSetDevice(h->dev));
-        // default batch: one chunk plus two margins of the longest supported template
-        h->Wmax = h->cfg.max_batch > 0 ? h->cfg.max_batch : h->cfg.chunk + 2 * MAX_M;
-        load(h.get(), oss_input_dir);
-        h->hist.clear();
-        *out = h.release();
+        *out = create_handle(oss_input_dir, cfg, 0).release();
     });
 }
 
@@ -939,5 +1012,176 @@ int lss_destroy(lss_handle* h) {
 }
 
 const char* lss_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Groups: several probe streams matched by one launch sequence per tick,
+// captured once into a hipGraph (north_star form (2)).  Each stream keeps its
+// own tables and buffers (an internal handle); the kernels are the batched
+// forms of the single-stream ones, the peel partitions its workgroups among
+// the streams (a barrier counter per stream), and every table is padded to the
+// group's register tile height so one kernel instance serves all template counts.
+// ---------------------------------------------------------------------------
+struct lss_group {
+    std::vector<std::unique_ptr<lss_handle>> hs;
+    int n = 0, W = 0, R = 0, CW = 0, dev = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const int16_t** h_xptr = nullptr;    // pinned: this tick's input pointers
+    const int16_t** d_xptr = nullptr;    // device copy the whiten jobs read
+    std::vector<int32_t> ld_captured;
+    WhitenBatch wb{};
+    PcconvBatch cb{};
+    ProjectBatch pb{};
+    PeelBatch kb{};
+    ResetBatch rb{};
+    ~lss_group() {
+        (void)hipSetDevice(dev);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (h_xptr) (void)hipHostFree(h_xptr);
+        if (d_xptr) (void)hipFree(d_xptr);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int64_t peek_templates(const std::string& dir_in) {
+    std::string dir = dir_in;
+    if (!dir.empty() && dir.back() != '/') dir += '/';
+    auto a = npy_load(dir + "Wall3.npy");
+    if (a.shape.size() != 3) throw LssError(LSS_ERR_IO, "Wall3.npy must be [T][K][C]");
+    return a.shape[0];
+}
+
+// (Re)build the job tables and capture the tick's launch sequence into a graph.
+void group_capture(lss_group* g, const int32_t* ld) {
+    const int n = g->n;
+    g->wb = WhitenBatch{}; g->cb = PcconvBatch{}; g->pb = ProjectBatch{}; g->kb = PeelBatch{}; g->rb = ResetBatch{};
+    g->wb.n = g->cb.n = g->pb.n = g->kb.n = g->rb.n = n;
+    for (int s2 = 0; s2 < n; ++s2) {
+        lss_handle* h = g->hs[s2].get();
+        BatchPlan bp;
+        plan_batch(h, nullptr, &g->d_xptr[s2], g->W, ld[s2], g->CW, g->R, bp);
+        g->wb.job[s2] = bp.wj;
+        g->wb.start[s2 + 1] = g->wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
+        g->cb.job[s2] = bp.cj;
+        g->cb.start[s2 + 1] = g->cb.start[s2] + bp.cj.wblocks * h->C;
+        g->pb.job[s2] = bp.pj;
+        g->pb.start[s2 + 1] = g->pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
+        g->kb.p[s2] = bp.pp;
+        g->kb.g_start[s2 + 1] = g->kb.g_start[s2] + bp.G;
+        g->rb.bar[s2] = h->d_bar; g->rb.iter_cnt[s2] = h->d_iter_cnt; g->rb.n_iter[s2] = h->max_iter;
+        g->rb.err[s2] = h->d_err; g->rb.prof[s2] = h->d_prof;
+    }
+    if (g->exec) { HIPCHK(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
+    if (g->graph) { HIPCHK(hipGraphDestroy(g->graph)); g->graph = nullptr; }
+    hipStream_t st = g->stream;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    HIPCHK(hipMemcpyAsync(g->d_xptr, g->h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
+    launch_reset_batch(g->rb, st);
+    launch_whiten(g->wb, st);
+    launch_pcconv(g->cb, g->hs[0]->K, st);
+    launch_project(g->pb, st);
+    HIPCHK(launch_peel(g->kb, g->R, false, st));
+    HIPCHK(hipStreamEndCapture(st, &g->graph));
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->ld_captured.assign(ld, ld + n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const lss_config* cfg, lss_group** out) {
+    return guarded([&] {
+        if (!out || !oss_input_dirs || !cfg) throw LssError(LSS_ERR_ARG, "null argument");
+        *out = nullptr;
+        if (n_streams < 1 || n_streams > MAX_JOBS)
+            throw LssError(LSS_ERR_ARG, "a group holds 1.." + std::to_string(MAX_JOBS) + " streams");
+        if (cfg->max_batch <= 0) throw LssError(LSS_ERR_ARG, "a group needs max_batch (the framed batch width)");
+        auto g = std::make_unique<lss_group>();
+        g->n = n_streams;
+        g->W = cfg->max_batch;
+        g->dev = cfg->device;
+        int64_t tmax = 0;
+        for (int s2 = 0; s2 < n_streams; ++s2) {
+            if (!oss_input_dirs[s2]) throw LssError(LSS_ERR_ARG, "null directory");
+            tmax = std::max(tmax, peek_templates(oss_input_dirs[s2]));
+        }
+        // one peel kernel instance for all streams: the register tile of the largest T
+        g->CW = MIN_CW;
+        g->R = peel_rows((int)tmax, g->CW);
+        const int rows = g->R > 0 ? REG_NRG * g->R : 0;
+        for (int s2 = 0; s2 < n_streams; ++s2) {
+            g->hs.push_back(create_handle(oss_input_dirs[s2], cfg, rows));
+            if (g->hs.back()->K != g->hs[0]->K || g->hs.back()->M != g->hs[0]->M)
+                throw LssError(LSS_ERR_ARG, "streams of a group must share K and M");
+        }
+        if (g->R == 0) g->CW = 2 * MIN_CW;
+        int gsum = 0;
+        for (size_t s2 = 0; s2 < g->hs.size(); ++s2) gsum += (g->W + g->CW - 1) / g->CW;
+        if (gsum > g->hs[0]->peel_cap[peel_cap_index(g->R)])
+            throw LssError(LSS_ERR_ARG, "the group's peel grid (" + std::to_string(gsum) +
+                                            " workgroups) is not co-resident on this device");
+        HIPCHK(hipSetDevice(g->dev));
+        HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        HIPCHK(hipHostMalloc((void**)&g->h_xptr, sizeof(void*) * MAX_JOBS, hipHostMallocDefault));
+        HIPCHK(hipMalloc((void**)&g->d_xptr, sizeof(void*) * MAX_JOBS));
+        *out = g.release();
+    });
+}
+
+int lss_group_process(lss_group* g, const int16_t* const* samples, const int32_t* ld, int32_t device_input,
+                      const int64_t* first_sample, lss_event* const* ev, const int64_t* ev_cap, int64_t* n_ev,
+                      float* const* feat) {
+    return guarded([&] {
+        if (!g || !samples || !ld || !first_sample || !ev || !ev_cap || !n_ev)
+            throw LssError(LSS_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(g->dev));
+        auto t0 = std::chrono::steady_clock::now();
+        const int n = g->n;
+        for (int s2 = 0; s2 < n; ++s2) {
+            lss_handle* h = g->hs[s2].get();
+            n_ev[s2] = 0;
+            if (!samples[s2]) throw LssError(LSS_ERR_ARG, "null samples");
+            set_cmap(h, nullptr, ld[s2]);
+            if (device_input) {
+                g->h_xptr[s2] = samples[s2];
+            } else {
+                int16_t* dx = ensure_dx(h, (size_t)g->W * ld[s2]);
+                HIPCHK(hipMemcpyAsync(dx, samples[s2], (size_t)g->W * ld[s2] * sizeof(int16_t),
+                                      hipMemcpyHostToDevice, g->stream));
+                g->h_xptr[s2] = dx;
+            }
+        }
+        if (!g->exec || !std::equal(ld, ld + n, g->ld_captured.begin())) group_capture(g, ld);
+        HIPCHK(hipGraphLaunch(g->exec, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int s2 = 0; s2 < n; ++s2) {
+            lss_handle* h = g->hs[s2].get();
+            h->cur_first = first_sample[s2];
+            check_peel(h, g->W);
+            n_ev[s2] = emit(h, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr, INT64_MIN, INT64_MAX, false);
+            h->info.last_ms = ms;
+        }
+    });
+}
+
+int lss_group_query(lss_group* g, int32_t stream, lss_info* info) {
+    return guarded([&] {
+        if (!g || !info || stream < 0 || stream >= g->n) throw LssError(LSS_ERR_ARG, "bad argument");
+        const int rc = lss_query(g->hs[stream].get(), info);
+        if (rc != LSS_OK) throw LssError(rc, g_last_error);
+    });
+}
+
+int lss_group_destroy(lss_group* g) {
+    return guarded([&] { delete g; });
+}
 
 }  // extern "C"

@@ -80,29 +80,71 @@ struct PeelParams {
     int do_feat;
     // outputs
     OutHeader* hdr; lss_event* ev; float* feat; int ev_cap;
-    int64_t first_sample; int half; int nt0min;
+    int half; int nt0min;
     float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
     unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch
 };
 
+// ---- batched launch descriptors (kernel arguments) ----
+constexpr int MAX_JOBS = 8;
+
+struct WhitenJob {
+    const float* Wr; const int16_t* x;
+    const int16_t* const* xptr;        // when set, the input pointer is read from device memory
+    const int* cmap; const int* kb_ptr; const int* kb_idx; float* Xw;
+    int64_t ld; int CP, W, C, XS, mtiles, ntiles;
+};
+struct ProjectJob {
+    const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
+    int lda, ldb, ldc, mtiles, ntiles;
+};
+struct PcconvJob {
+    const float* Xw; const float* wpca; float* Bpc;
+    int XS, M, BS, CP, wblocks;
+};
+template <class J> struct JobBatch {
+    int n;
+    int start[MAX_JOBS + 1];       // first block of each job; start[n] = blocks in the launch
+    J job[MAX_JOBS];
+};
+using WhitenBatch = JobBatch<WhitenJob>;
+using ProjectBatch = JobBatch<ProjectJob>;
+using PcconvBatch = JobBatch<PcconvJob>;
+
+// Blocks a GEMM job occupies: M-tiles x N-tiles padded to a multiple of 8 (XCD order).
+inline int gemm_blocks(int mtiles, int ntiles) { return mtiles * ((ntiles + 7) / 8 * 8); }
+
 // Clears the per-batch counters the peel kernel relies on.
 void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, hipStream_t s);
 
-void launch_whiten(const float* Wr, int CP, const int16_t* x, int64_t ld, int W, const int* cmap, int C,
-                   const int* kb_ptr, const int* kb_idx, float* Xw, int XS, hipStream_t s);
-
-void launch_pcconv(const float* Xw, int XS, const float* wpca, int K, int M, float* Bpc, int BS,
-                   int C, int CP, int W, hipStream_t s);
-
-void launch_project(const float* A, int lda, int mtiles, const float* Bm, int ldb,
-                    const int* kb_ptr, const int* kb_idx, float* Cout, int ldc, int W, hipStream_t s);
+void launch_whiten(const WhitenBatch& b, hipStream_t s);
+void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s);
+void launch_project(const ProjectBatch& b, hipStream_t s);
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
 
 // Rows of the score tile each peel thread keeps in registers (0: tile stays in global memory).
 int peel_rows(int T, int CW);
 
-hipError_t launch_peel(const PeelParams& p, int R, bool cooperative, hipStream_t s);
+struct PeelBatch {
+    int n;
+    int g_start[MAX_JOBS + 1];     // first workgroup of each stream; g_start[n] = grid size
+    PeelParams p[MAX_JOBS];
+};
+
+static_assert(sizeof(PeelBatch) <= 4096, "peel batch must fit the kernel-argument segment");
+
+hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t s);
+
+struct ResetBatch {
+    int n;
+    unsigned* bar[MAX_JOBS];
+    int* iter_cnt[MAX_JOBS];
+    int n_iter[MAX_JOBS];
+    int* err[MAX_JOBS];
+    unsigned long long* prof[MAX_JOBS];
+};
+void launch_reset_batch(const ResetBatch& b, hipStream_t s);
 
 int peel_max_blocks(int device, int R);
 

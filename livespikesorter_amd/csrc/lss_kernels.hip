@@ -55,18 +55,46 @@ __device__ __forceinline__ void store_tile(float* __restrict__ C, int64_t ldc, i
     }
 }
 
-// Xw[c][XOFF + w] = sum_c' Wr[c][c'] * x[w][cmap[c']]   (rows c < CP, columns w < gridDim.x*64)
+// ---------------------------------------------------------------------------
+// Batched launches: one launch serves up to MAX_JOBS independent jobs (probe
+// streams).  Block ids are dealt to jobs by the start offsets in the batch
+// descriptor (kernel arguments).
+//
+// XCD-aware tile order: the GPU deals consecutive workgroup ids round-robin to
+// its 8 XCDs, each with its own L2.  Block k of a GEMM job goes to XCD k % 8;
+// the job's tiles are ordered so that all M-tiles of one 64-column N-tile sit
+// on the same XCD, so the shared B-operand column block is fetched from HBM
+// once into that XCD's L2 instead of once per M-tile.  Jobs start at multiples
+// of 8 and pad their N-tile count to a multiple of 8 (padding blocks exit).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int job_of(const int* start, int n, int b) {
+    int j = 0;
+    while (j + 1 < n && b >= start[j + 1]) ++j;
+    return j;
+}
+
+__device__ __forceinline__ bool xcd_tile(int k, int mtiles, int ntiles, int& m, int& nt) {
+    const int slot = k >> 3;
+    nt = (k & 7) + 8 * (slot / mtiles);
+    m = slot % mtiles;
+    return nt < ntiles;
+}
+
+// Xw[c][XOFF + w] = sum_c' Wr[c][c'] * x[w][cmap[c']]   (rows c < CP)
 // Columns w >= W read zero input and so store zero: they are the right-hand
 // zero padding the PC convolution reads.
-__global__ __launch_bounds__(256) void k_whiten(const float* __restrict__ Wr, int CP,
-                                                const int16_t* __restrict__ x, int64_t ld, int W,
-                                                const int* __restrict__ cmap, int C,
-                                                const int* __restrict__ kb_ptr,
-                                                const int* __restrict__ kb_idx,
-                                                float* __restrict__ Xw, int XS) {
+__global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
     __shared__ GemmTile s;
-    const int mt = blockIdx.y;
-    const int m0 = mt * GEMM_BM, n0 = blockIdx.x * GEMM_BN;
+    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    const WhitenJob& jb = bt.job[j];
+    int mt, nt;
+    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    const float* __restrict__ Wr = jb.Wr;
+    const int16_t* __restrict__ x = jb.xptr ? *jb.xptr : jb.x;
+    const int* __restrict__ cmap = jb.cmap;
+    const int CP = jb.CP, W = jb.W, C = jb.C;
+    const int64_t ld = jb.ld;
+    const int m0 = mt * GEMM_BM, n0 = nt * GEMM_BN;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const int ar = tid >> 2, ak = (tid & 3) * 4;   // A: row, 4 consecutive k
@@ -74,7 +102,7 @@ __global__ __launch_bounds__(256) void k_whiten(const float* __restrict__ Wr, in
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    const int p0 = kb_ptr[mt], p1 = kb_ptr[mt + 1];
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
     float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
     float rb[4] = {0.f, 0.f, 0.f, 0.f};
     auto gload = [&](int kb) {
@@ -87,7 +115,7 @@ __global__ __launch_bounds__(256) void k_whiten(const float* __restrict__ Wr, in
             rb[i] = (n < W && c < C) ? (float)x[(int64_t)n * ld + cmap[c]] : 0.f;
         }
     };
-    if (p0 < p1) gload(kb_idx[p0]);
+    if (p0 < p1) gload(jb.kb_idx[p0]);
     for (int p = p0; p < p1; ++p) {
         __syncthreads();
         s.As[ak + 0][ar] = ra.x;
@@ -97,21 +125,23 @@ __global__ __launch_bounds__(256) void k_whiten(const float* __restrict__ Wr, in
 #pragma unroll
         for (int i = 0; i < 4; ++i) s.Bs[bk + i][bn] = rb[i];
         __syncthreads();
-        if (p + 1 < p1) gload(kb_idx[p + 1]);
+        if (p + 1 < p1) gload(jb.kb_idx[p + 1]);
         acc = mfma_tile(s, acc, lane, wm, wn);
     }
-    store_tile(Xw + XOFF, XS, m0, n0, acc, lane, wm, wn);
+    store_tile(jb.Xw + XOFF, jb.XS, m0, n0, acc, lane, wm, wn);
 }
 
 // B[t][w] = sum_kk A[t][kk] * Bm[kk][w]   (A = spatial templates [TP][K*CP], Bm = PC conv [K*CP][BS])
-__global__ __launch_bounds__(256) void k_project(const float* __restrict__ A, int lda,
-                                                 const float* __restrict__ Bm, int ldb,
-                                                 const int* __restrict__ kb_ptr,
-                                                 const int* __restrict__ kb_idx,
-                                                 float* __restrict__ Cout, int ldc) {
+__global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     __shared__ GemmTile s;
-    const int mt = blockIdx.y;
-    const int m0 = mt * GEMM_BM, n0 = blockIdx.x * GEMM_BN;
+    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    const ProjectJob& jb = bt.job[j];
+    int mt, nt;
+    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    const float* __restrict__ A = jb.A;
+    const float* __restrict__ Bm = jb.Bm;
+    const int lda = jb.lda, ldb = jb.ldb;
+    const int m0 = mt * GEMM_BM, n0 = nt * GEMM_BN;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const int ar = tid >> 2, ak = (tid & 3) * 4;     // A: row, 4 consecutive k
@@ -119,14 +149,14 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ A, in
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    const int p0 = kb_ptr[mt], p1 = kb_ptr[mt + 1];
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
     float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
     auto gload = [&](int kb) {
         const int k0 = kb * GEMM_BK;
         ra = *reinterpret_cast<const float4*>(A + (int64_t)(m0 + ar) * lda + k0 + ak);
         rb = *reinterpret_cast<const float4*>(Bm + (int64_t)(k0 + br) * ldb + n0 + bc);
     };
-    if (p0 < p1) gload(kb_idx[p0]);
+    if (p0 < p1) gload(jb.kb_idx[p0]);
     for (int p = p0; p < p1; ++p) {
         __syncthreads();
         s.As[ak + 0][ar] = ra.x;
@@ -135,10 +165,10 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ A, in
         s.As[ak + 3][ar] = ra.w;
         *reinterpret_cast<float4*>(&s.Bs[br][bc]) = rb;
         __syncthreads();
-        if (p + 1 < p1) gload(kb_idx[p + 1]);
+        if (p + 1 < p1) gload(jb.kb_idx[p + 1]);
         acc = mfma_tile(s, acc, lane, wm, wn);
     }
-    store_tile(Cout, ldc, m0, n0, acc, lane, wm, wn);
+    store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -148,26 +178,29 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ A, in
 // (coalesced), each thread produces the K outputs of one column.
 // ---------------------------------------------------------------------------
 template <int KK>
-__global__ __launch_bounds__(256) void k_pcconv(const float* __restrict__ Xw, int XS,
-                                                const float* __restrict__ wpca, int M,
-                                                float* __restrict__ Bpc, int BS, int CP) {
+__global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     __shared__ float xs[256 + 2 * XOFF];
-    const int c = blockIdx.y, w0 = blockIdx.x * 256, tid = threadIdx.x;
-    const float* row = Xw + (int64_t)c * XS + XOFF;      // row[w] valid for w >= -XOFF
+    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    const PcconvJob& jb = bt.job[j];
+    const int k = blockIdx.x - bt.start[j];
+    const int c = k / jb.wblocks, w0 = (k % jb.wblocks) * 256, tid = threadIdx.x;
+    const int M = jb.M;
+    const float* __restrict__ wpca = jb.wpca;
+    const float* row = jb.Xw + (int64_t)c * jb.XS + XOFF;      // row[w] valid for w >= -XOFF
     for (int i = tid; i < 256 + 2 * XOFF; i += 256) xs[i] = row[w0 - XOFF + i];
     __syncthreads();
     const int h = M / 2;
     float acc[KK];
 #pragma unroll
-    for (int k = 0; k < KK; ++k) acc[k] = 0.f;
+    for (int q = 0; q < KK; ++q) acc[q] = 0.f;
     const float* xp = xs + tid + XOFF - h;
     for (int m = 0; m < M; ++m) {
         const float v = xp[m];
 #pragma unroll
-        for (int k = 0; k < KK; ++k) acc[k] = fmaf(wpca[k * M + m], v, acc[k]);
+        for (int q = 0; q < KK; ++q) acc[q] = fmaf(wpca[q * M + m], v, acc[q]);
     }
 #pragma unroll
-    for (int k = 0; k < KK; ++k) Bpc[(int64_t)(k * CP + c) * BS + w0 + tid] = acc[k];
+    for (int q = 0; q < KK; ++q) jb.Bpc[(int64_t)(q * jb.CP + c) * jb.BS + w0 + tid] = acc[q];
 }
 
 // nm[t] = sum_{k,c} U[t,k,c]^2 accumulated in FP64, stored as FP64 and rounded FP32;
@@ -289,10 +322,12 @@ constexpr int PRE = 4;   // neighbour spikes fetched speculatively with the coun
 __device__ __forceinline__ uint64_t rt_now() { return __builtin_amdgcn_s_memrealtime(); }
 
 template <int R>
-__global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
+__global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
 #pragma clang fp contract(off)
+    const int sj = job_of(bt.g_start, bt.n, blockIdx.x);
+    const PeelParams& p = bt.p[sj];
     // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
     // immediate offset (no per-row address registers); the norm table comes first.
     __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template norm (or its reciprocal, LSS)
@@ -317,7 +352,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     int* const s_rank = reinterpret_cast<int*>(s_val);
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
 
-    const int g = blockIdx.x, tid = threadIdx.x;
+    const int g = blockIdx.x - bt.g_start[sj], tid = threadIdx.x;
     // register tiles use a fixed 64-column width, so column and row strides are
     // compile-time constants (LDS offsets fold into the instructions)
     const int CW = R > 0 ? PEEL_THREADS / REG_NRG : p.CW;
@@ -700,7 +735,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
         const int d = off + r;
         if (d < p.ev_cap) {
             lss_event e;
-            e.sample = p.first_sample + si.w - p.half + p.nt0min;
+            e.sample = si.w - p.half + p.nt0min;     // the host adds the batch's first sample
             e.template_id = p.orig[si.t];
             e.amplitude = si.amp;
             p.ev[d] = e;
@@ -750,6 +785,18 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelParams p) {
     }
 }
 
+// One block per stream: the per-batch counters of every stream of a group.
+__global__ void k_reset_batch(ResetBatch b) {
+    const int j = blockIdx.x;
+    if (j >= b.n) return;
+    for (int i = threadIdx.x; i < b.n_iter[j]; i += blockDim.x) b.iter_cnt[j][i] = 0;
+    if ((int)threadIdx.x < PROF_WORDS) b.prof[j][threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        *b.bar[j] = 0u;
+        *b.err[j] = 0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -757,20 +804,18 @@ void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned l
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err, prof);
 }
 
-void launch_whiten(const float* Wr, int CP, const int16_t* x, int64_t ld, int W, const int* cmap, int C,
-                   const int* kb_ptr, const int* kb_idx, float* Xw, int XS, hipStream_t s) {
-    const int ncols = (W + XOFF + GEMM_BN - 1) / GEMM_BN;   // cover the right zero pad
-    hipLaunchKernelGGL(k_whiten, dim3(ncols, CP / GEMM_BM), dim3(256), 0, s, Wr, CP, x, ld, W, cmap, C,
-                       kb_ptr, kb_idx, Xw, XS);
+void launch_reset_batch(const ResetBatch& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_batch, dim3(b.n), dim3(256), 0, s, b);
 }
 
-void launch_pcconv(const float* Xw, int XS, const float* wpca, int K, int M, float* Bpc, int BS,
-                   int C, int CP, int W, hipStream_t s) {
-    const int wc = (W + GEMM_BN - 1) / GEMM_BN * GEMM_BN;
-    dim3 grid((wc + 255) / 256, C);
+void launch_whiten(const WhitenBatch& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_whiten, dim3(b.start[b.n]), dim3(256), 0, s, b);
+}
+
+void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {
     switch (K) {
 #define LSS_PC_CASE(KK) \
-    case KK: hipLaunchKernelGGL(k_pcconv<KK>, grid, dim3(256), 0, s, Xw, XS, wpca, M, Bpc, BS, CP); break;
+    case KK: hipLaunchKernelGGL(k_pcconv<KK>, dim3(b.start[b.n]), dim3(256), 0, s, b); break;
         LSS_PC_CASE(1) LSS_PC_CASE(2) LSS_PC_CASE(3) LSS_PC_CASE(4)
         LSS_PC_CASE(5) LSS_PC_CASE(6) LSS_PC_CASE(7) LSS_PC_CASE(8)
 #undef LSS_PC_CASE
@@ -778,11 +823,8 @@ void launch_pcconv(const float* Xw, int XS, const float* wpca, int K, int M, flo
     }
 }
 
-void launch_project(const float* A, int lda, int mtiles, const float* Bm, int ldb,
-                    const int* kb_ptr, const int* kb_idx, float* Cout, int ldc, int W, hipStream_t s) {
-    const int ncols = (W + GEMM_BN - 1) / GEMM_BN;
-    hipLaunchKernelGGL(k_project, dim3(ncols, mtiles), dim3(256), 0, s, A, lda, Bm, ldb, kb_ptr, kb_idx,
-                       Cout, ldc);
+void launch_project(const ProjectBatch& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_project, dim3(b.start[b.n]), dim3(256), 0, s, b);
 }
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s) {
@@ -807,12 +849,13 @@ int peel_rows(int T, int CW) {
     return 0;
 }
 
-hipError_t launch_peel(const PeelParams& p, int R, bool cooperative, hipStream_t s) {
-    PeelParams copy = p;
+hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t s) {
+    PeelBatch copy = b;
     void* args[] = {&copy};
+    const int G = b.g_start[b.n];
     if (cooperative)
-        return hipLaunchCooperativeKernel(peel_fn(R), dim3(p.G), dim3(PEEL_THREADS), args, 0, s);
-    return hipLaunchKernel(peel_fn(R), dim3(p.G), dim3(PEEL_THREADS), args, 0, s);
+        return hipLaunchCooperativeKernel(peel_fn(R), dim3(G), dim3(PEEL_THREADS), args, 0, s);
+    return hipLaunchKernel(peel_fn(R), dim3(G), dim3(PEEL_THREADS), args, 0, s);
 }
 
 int peel_max_blocks(int device, int R) {

@@ -18,7 +18,7 @@ import numpy as np
 from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_FP32, SEM_KILOSORT, SEM_LSS, check,
                     default_config, lib)
 
-__all__ = ["OnlineSpikesV2", "LssError", "SEM_KILOSORT", "SEM_LSS"]
+__all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS"]
 
 STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES = range(7)
 
@@ -143,3 +143,73 @@ class OnlineSpikesV2:
         nb = ct.c_int64(out.nbytes)
         check(lib().lss_debug_stage(self._h, stage, out.ctypes.data, ct.byref(nb)))
         return out
+
+
+class StreamGroup:
+    """Several probe streams matched by one graph launch per tick (lss_group_*).
+
+    No counterpart in the reference, which runs one sorter thread per probe
+    (main.cpp:77-85).  Each call to :meth:`process` matches one framed batch
+    of ``batch`` columns of every stream.
+    """
+
+    def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
+                 semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True):
+        self._g = ct.c_void_p()
+        self.n = len(oss_input_dirs)
+        cfg = default_config(device=device, chunk=batch, max_batch=batch, max_iter=max_iter,
+                             semantics=semantics, max_events=max_events, sparse=int(sparse))
+        dirs = (ct.c_char_p * self.n)(*[os.fsencode(str(d)) for d in oss_input_dirs])
+        check(lib().lss_group_create(self.n, dirs, ct.byref(cfg), ct.byref(self._g)))
+        self.batch = batch
+        self.info = [self.query(s) for s in range(self.n)]
+
+    def query(self, stream: int) -> Info:
+        info = Info()
+        check(lib().lss_group_query(self._g, stream, ct.byref(info)))
+        return info
+
+    def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536):
+        """``batches``: per stream an int16 array ``[batch][ld]`` (or, with
+        ``device=True``, a device pointer and ``lds`` the row strides)."""
+        n = self.n
+        if device:
+            ptrs = [int(b) for b in batches]
+            ld = list(lds)
+            keep = None
+        else:
+            keep = [np.ascontiguousarray(b, dtype=np.int16) for b in batches]
+            for b in keep:
+                if b.shape[0] != self.batch:
+                    raise ValueError("every batch must have the group's width")
+            ptrs = [b.ctypes.data for b in keep]
+            ld = [b.shape[1] for b in keep]
+        evs = [np.zeros(cap, EVENT_DTYPE) for _ in range(n)]
+        feats = [np.zeros((cap, self.info[s].nC, self.info[s].K), np.float32) for s in range(n)]
+        c_ptrs = (ct.c_void_p * n)(*ptrs)
+        c_ld = (ct.c_int32 * n)(*ld)
+        c_first = (ct.c_int64 * n)(*[int(f) for f in first_samples])
+        c_ev = (ct.c_void_p * n)(*[e.ctypes.data for e in evs])
+        c_cap = (ct.c_int64 * n)(*([cap] * n))
+        c_n = (ct.c_int64 * n)()
+        c_feat = (ct.c_void_p * n)(*[f.ctypes.data for f in feats])
+        check(lib().lss_group_process(self._g, c_ptrs, c_ld, int(device), c_first, c_ev, c_cap, c_n, c_feat))
+        del keep
+        return [(evs[s][:c_n[s]], feats[s][:c_n[s]]) for s in range(n)]
+
+    def close(self) -> None:
+        if self._g:
+            check(lib().lss_group_destroy(self._g))
+            self._g = ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

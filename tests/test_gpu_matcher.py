@@ -203,3 +203,30 @@ def test_errors(small):
     with pytest.raises(LssError) as ei:
         OnlineSpikesV2(str(d) + "_missing")
     assert ei.value.code == -2
+
+
+def test_stream_group_matches_single_handles(model_dirs):
+    """Streams of different template counts in one graph launch give, bit for
+    bit, the events and features of separate single-stream handles."""
+    from livespikesorter_amd import StreamGroup
+    specs = [dict(C=64, T=32, seed=31), dict(C=64, T=96, seed=32), dict(C=96, T=160, seed=33)]
+    dirs, recs = [], []
+    for k, sp in enumerate(specs):
+        m, d = model_dirs("grp%d" % k, C=sp["C"], T=sp["T"], seed=sp["seed"], whitening="banded",
+                          rate=(2.0, 20.0), amp=(11.0, 3.0))
+        x, _ = synth.make_recording(m, 3 * 1622 + 200, seed=sp["seed"] + 100)
+        dirs.append(d)
+        recs.append(x)
+    W = 1622
+    singles = [OnlineSpikesV2(d, max_batch=W) for d in dirs]
+    with StreamGroup(dirs, batch=W) as grp:
+        for tick in range(3):
+            bs = [x[tick * 1500:tick * 1500 + W] for x in recs]
+            out = grp.process(bs, [tick * 1500] * len(bs))
+            for k, (ev, feat) in enumerate(out):
+                e1, f1 = singles[k].process_batch(bs[k], first_sample=tick * 1500)
+                assert len(e1) > 0
+                assert np.array_equal(ev, e1), f"stream {k} tick {tick}"
+                assert np.array_equal(feat, f1)
+    for sp in singles:
+        sp.close()
