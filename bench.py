@@ -32,6 +32,7 @@ METRIC = "realtime factor, p99 chunk latency (384ch); BF16/gated vs FP32 @99% sp
 FS = 30000.0
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def log(*a):
@@ -168,6 +169,171 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
+def stage_work(info, C, W, M, iters, spikes, bf16=False):
+    """Algorithmic work per launch of each device stage: (bound, amount, unit, peak).
+
+    whiten / project: MFMA flops over the listed (nonzero) 64x16 operand blocks;
+    pcconv: int16-converted input read once plus the PC projection written once
+    (f32; BF16 mode also writes the BF16 copy); peel: the score tile read per
+    iteration, Cfmax exchange, ctc rows and score updates per spike, and the
+    feature reads per spike."""
+    T, K = info.T, info.K
+    Wc = (W + 63) // 64 * 64
+    I, S = float(np.mean(iters)), float(np.mean(spikes))
+    pk = BF16_MFMA_PEAK_TFLOPS if bf16 else FP32_MFMA_PEAK_TFLOPS
+    return {
+        "whiten": ("mfma", 2.0 * info.white_kblocks * 64 * 16 * Wc / 1e12, "TFLOP/s", FP32_MFMA_PEAK_TFLOPS),
+        "pcconv": ("hbm", (4.0 * C * W + (6.0 if bf16 else 4.0) * K * C * W) / 1e9, "GB/s", HBM_PEAK_GBS),
+        "project": ("mfma", 2.0 * info.proj_kblocks * 64 * 16 * Wc / 1e12, "TFLOP/s", pk),
+        "peel": ("hbm", (I * 4.0 * T * W + I * 8.0 * W + S * T * (2 * M + 1) * 12.0
+                         + S * 10 * M * 4.0) / 1e9, "GB/s", HBM_PEAK_GBS),
+    }
+
+
+def make_bf16_data(cache: Path, seconds: float):
+    """384 ch, 1024 templates, bursty gamma(0.5) ISIs (the BF16 precision workload)."""
+    from livespikesorter_amd import synth
+    cache.mkdir(parents=True, exist_ok=True)
+    d = cache / "cfg3_oss_input"
+    m = synth.make_model(**synth.CONFIGS["cfg3"])
+    if not (d / "ctc.npy").exists():
+        synth.write_oss_input(m, d)
+    rp = cache / f"cfg3_rec_{int(seconds)}s.npy"
+    if rp.exists():
+        x = np.load(rp)
+    else:
+        x, _ = synth.make_recording(m, int(seconds * FS))
+        np.save(rp, x)
+    return m, d, x
+
+
+def bench_bf16(args) -> dict:
+    """BF16 projection + FP32 rescoring on 60000-sample batches, against the FP32 GPU
+    matcher on the same batches and the FP64 oracle on a bounded sample."""
+    import oracle
+    from livespikesorter_amd import OnlineSpikesV2
+    from livespikesorter_amd._hip import DeviceBuffer, synchronize
+    from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_FP32
+    from tests.parity import agreement
+    N, M = 60000, 61
+    W = N + 2 * M
+    t0 = time.time()
+    m, d, x = make_bf16_data(Path(args.cache), args.bf16_seconds)
+    n_batches = (len(x) - W) // N + 1
+    log(f"bf16 data ready in {time.time() - t0:.1f}s: {len(x)} samples, {n_batches} batches of {W}")
+    dx = DeviceBuffer.from_array(x)
+    C = x.shape[1]
+    synchronize()
+    steps = min(args.steps, n_batches)
+
+    def run(prec):
+        sp = OnlineSpikesV2(d, chunk=N, max_batch=W, precision=prec, profile=True)
+        for j in range(min(args.warmup, 2)):
+            sp.process_batch_device(dx.ptr + (j % n_batches) * N * C * 2, W, C, first_sample=j * N)
+        sp.stage_times(reset=True)
+        lat, evs, iters, spikes = [], [], [], []
+        synchronize()
+        ts = time.perf_counter()
+        for j in range(steps):
+            t1 = time.perf_counter()
+            ev, _ = sp.process_batch_device(dx.ptr + j * N * C * 2, W, C, first_sample=j * N)
+            lat.append((time.perf_counter() - t1) * 1e3)
+            q = sp.query()
+            iters.append(q.last_iters)
+            spikes.append(q.last_spikes)
+            evs.append(ev)
+        synchronize()
+        el = time.perf_counter() - ts
+        st, nb = sp.stage_times(reset=True)
+        return sp, el, lat, evs, iters, spikes, {k: v / max(nb, 1) for k, v in st.items()}
+
+    sb, el_b, lat_b, ev_b, it_b, sk_b, st_b = run(PREC_BF16_RESCORE)
+    sf, el_f, lat_f, ev_f, it_f, sk_f, st_f = run(PREC_FP32)
+    rtf_b = steps * N / FS / el_b
+    rtf_f = steps * N / FS / el_f
+    log(f"bf16: {rtf_b:.1f}x realtime, p99 {pct(lat_b, 99):.2f} ms; fp32: {rtf_f:.1f}x, p99 {pct(lat_f, 99):.2f} ms")
+    log("bf16 stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st_b.items()))
+    log("fp32 stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st_f.items()))
+    # spike agreement with the plain FP32 GPU matcher over the compared batches
+    n_cmp = min(steps, args.bf16_compare)
+    tot = ident_n = within_n = 0
+    for j in range(n_cmp):
+        a, b = ev_f[j], ev_b[j]
+        i, w, _ = agreement(a["sample"], a["template"], b["sample"], b["template"])
+        n = max(len(a), len(b))
+        ident_n += i * n
+        within_n += w * n
+        tot += n
+    vs_fp32 = {"batches": n_cmp, "events": tot, "identical": round(ident_n / max(tot, 1), 6),
+               "within_1_sample": round(within_n / max(tot, 1), 6)}
+    log(f"bf16 vs fp32 GPU: {vs_fp32}")
+    info = sb.query()
+    # the FP64 oracle (and the FP32 CPU restatement, the CPU baseline) on the first batch
+    tab = oracle.load_tables(d)
+    threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    batch = x[:W]
+    vs_fp64 = cpu = None
+    if not args.no_cpu:
+        t1 = time.perf_counter()
+        ref = oracle.run_model_batch(tab, batch, bits=64, threads=threads, features=False)
+        t64 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        oracle.run_model_batch(tab, batch, bits=32, threads=threads, features=False)
+        t32 = time.perf_counter() - t1
+        rs = ref.w - M // 2 + m.nt0min
+        res = {}
+        for name, ev in (("bf16", ev_b[0]), ("fp32", ev_f[0])):
+            i, w, pairs = agreement(rs, ref.t, ev["sample"], ev["template"])
+            ri = np.array([p[0] for p in pairs])
+            gi = np.array([p[1] for p in pairs])
+            ex = rs[ri] == ev["sample"][gi]
+            err = np.abs(ev["amplitude"][gi[ex]] - ref.amp[ri[ex]]) / np.abs(ref.amp[ri[ex]])
+            res[name] = {"identical": round(i, 6), "within_1_sample": round(w, 6),
+                         "amp_rel_err_max": float(err.max()) if len(err) else None,
+                         "amp_rel_err_median": float(np.median(err)) if len(err) else None}
+        vs_fp64 = {"batches": 1, "events_fp64": int(len(rs)), **res, "oracle_seconds": round(t64, 2)}
+        log(f"vs FP64 oracle: {vs_fp64}")
+        cpu = {"value": round(N / FS / t32, 4), "unit": "x realtime", "cores": threads, "kind": "port",
+               "sample": f"first 60000-sample batch (W={W}), FP32 C restatement without features, "
+                         f"OpenMP {threads} threads, {t32:.1f}s"}
+    sb.close()
+    sf.close()
+    dx.free()
+    work = stage_work(info, C, W, M, it_b, sk_b, bf16=True)
+    dom = max(work, key=lambda k: st_b[k])
+    bound, amount, unit, peak = work[dom]
+    achieved = amount / (st_b[dom] / 1e3)
+    proj = work["project"]
+    return {
+        "metric": METRIC + " [BF16 workload]",
+        "value": round(rtf_b, 3),
+        "unit": "x realtime",
+        "n_gpus": 1, "steps": steps, "warmup": min(args.warmup, 2),
+        "ms_per_step": round(el_b / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (projection), f32",
+        "data": "synthetic (seeded generator: 384 ch, 1024 templates, gamma(0.5) ISIs, banded whitening)",
+        "config": {"workload": "BF16 MFMA projection with FP32 accumulation, local maxima with Cf in "
+                               "[0.9, 1.1] Th^2 rescored in FP32; 384 channels, 1024 templates, "
+                               f"60000-sample batches + 61-sample margins (W={W}), "
+                               f"{args.bf16_seconds:.0f} s recording resident in HBM",
+                   "channels": C, "templates": info.T, "batch_columns": W},
+        "p50_ms": round(pct(lat_b, 50), 3), "p99_ms": round(pct(lat_b, 99), 3),
+        "fp32_x_realtime": round(rtf_f, 3), "fp32_p99_ms": round(pct(lat_f, 99), 3),
+        "speedup_over_fp32": round(rtf_b / rtf_f, 4),
+        "project_ms": {"bf16": round(st_b["project"], 4), "fp32": round(st_f["project"], 4)},
+        "project_tflops": {"bf16": round(proj[1] / (st_b["project"] / 1e3), 2),
+                           "fp32": round(proj[1] / (st_f["project"] / 1e3), 2)},
+        "stages_ms": {k: round(v, 4) for k, v in st_b.items()},
+        "mean_iterations": round(float(np.mean(it_b)), 2), "mean_spikes_per_batch": round(float(np.mean(sk_b)), 1),
+        "rescored": int(info.rescored), "rescore_flips": int(info.rescore_flips),
+        "vs_fp32_gpu": vs_fp32, "vs_fp64_oracle": vs_fp64,
+        "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                     "frac": round(achieved / peak, 5), "traffic": None, "kernel": dom,
+                     "ms_per_launch": round(st_b[dom], 5)},
+        "cpu_baseline": cpu,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,8 +344,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-steps", type=int, default=100, help="chunks timed with host-resident input")
     ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
-    ap.add_argument("--workload", choices=("cfg2", "group"), default="cfg2",
-                    help="cfg2: the headline plain FP32 matcher; group: 8 streams per graph launch")
+    ap.add_argument("--workload", choices=("cfg2", "group", "bf16"), default="cfg2",
+                    help="cfg2: the headline plain FP32 matcher; group: 8 streams per graph launch; "
+                         "bf16: BF16 projection with FP32 rescoring, 1024 templates, 60000-sample batches")
+    ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
+    ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized buffer three times (a known-size kernel "
@@ -192,6 +361,9 @@ def main() -> int:
 
     if args.workload == "group":
         print(json.dumps(bench_group(args)), flush=True)
+        return 0
+    if args.workload == "bf16":
+        print(json.dumps(bench_bf16(args)), flush=True)
         return 0
 
     from livespikesorter_amd import OnlineSpikesV2
@@ -259,17 +431,10 @@ def main() -> int:
     sp.stage_times(reset=True)
 
     # roofline of the dominant device stage (algorithmic work per launch / measured time)
-    T, K = info.T, info.K
-    Wc = (W + 63) // 64 * 64
+    T = info.T
     I = float(np.mean(iters))
     S = float(np.mean(spikes))
-    work = {
-        "whiten": ("mfma", 2.0 * info.white_kblocks * 64 * 16 * Wc / 1e12, "TFLOP/s", FP32_MFMA_PEAK_TFLOPS),
-        "pcconv": ("hbm", (4.0 * C * W + 4.0 * K * C * W) / 1e9, "GB/s", HBM_PEAK_GBS),
-        "project": ("mfma", 2.0 * info.proj_kblocks * 64 * 16 * Wc / 1e12, "TFLOP/s", FP32_MFMA_PEAK_TFLOPS),
-        "peel": ("hbm", (I * 4.0 * T * W + I * 8.0 * W + S * T * (2 * M + 1) * 12.0
-                         + S * 10 * M * 4.0) / 1e9, "GB/s", HBM_PEAK_GBS),
-    }
+    work = stage_work(info, C, W, M, iters, spikes)
     dom = max(work, key=lambda k: avg[k])
     bound, amount, unit, peak = work[dom]
     achieved = amount / (avg[dom] / 1e3)

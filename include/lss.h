@@ -78,6 +78,10 @@ typedef struct lss_info {
     int64_t white_kblocks;  /* 64x16 whitening operand blocks computed per 64-sample column tile */
     int64_t proj_kblocks;   /* 64x16 projection operand blocks computed per 64-sample column tile */
     int64_t proj_kblocks_dense;  /* the same without exact-zero skipping */
+    int32_t precision;      /* LSS_PREC_FP32 or LSS_PREC_BF16_RESCORE */
+    int32_t reserved0;
+    int64_t rescored;       /* BF16 mode: local maxima rescored in FP32 (running total) */
+    int64_t rescore_flips;  /* BF16 mode: of those, accept/reject decisions the rescore changed */
 } lss_info;
 
 /* Fill *cfg with defaults. */

@@ -57,7 +57,8 @@ class Info(ct.Structure):
                 ("chunk", ct.c_int32), ("max_iter", ct.c_int32), ("batches", ct.c_int64),
                 ("events", ct.c_int64), ("last_iters", ct.c_int32), ("last_spikes", ct.c_int32),
                 ("last_ms", ct.c_double), ("white_kblocks", ct.c_int64), ("proj_kblocks", ct.c_int64),
-                ("proj_kblocks_dense", ct.c_int64)]
+                ("proj_kblocks_dense", ct.c_int64), ("precision", ct.c_int32), ("reserved0", ct.c_int32),
+                ("rescored", ct.c_int64), ("rescore_flips", ct.c_int64)]
 
 
 class LssError(RuntimeError):

@@ -42,6 +42,14 @@ struct LssError : std::runtime_error {
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// float -> bfloat16 bit pattern, round to nearest even (inputs are finite)
+inline uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 template <class F>
 int guarded(F&& f) {
     try {
@@ -120,6 +128,7 @@ struct lss_handle {
     int min_rows = 0;        // force the padded ctc row count up to this (streams sharing one peel launch)
     int64_t cur_first = 0;   // absolute sample of column 0 of the last batch
     bool coop = true;        // cooperative launch of the peel kernel
+    bool bf16 = false;       // BF16 projection with FP32 rescoring (LSS_PREC_BF16_RESCORE)
 
     // static device tables (internal template order)
     float *d_Wr = nullptr, *d_U2 = nullptr, *d_U = nullptr, *d_wpca = nullptr;
@@ -136,6 +145,10 @@ struct lss_handle {
     int *d_slot_cnt = nullptr, *d_list_cnt = nullptr, *d_iter_cnt = nullptr, *d_err = nullptr;
     unsigned* d_bar = nullptr;
     unsigned long long* d_prof = nullptr;
+    // BF16 projection operands, its unsubtracted result, and the rescore counters
+    uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
+    float* d_Binit = nullptr;
+    unsigned long long* d_rescnt = nullptr;
     int slot_cap = 0, cap_g = 0, max_groups = 0;
     // debug copies
     float *d_dbgB = nullptr, *d_dbgCf = nullptr;
@@ -178,7 +191,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof};
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt};
         (void)hipSetDevice(dev);
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -461,6 +474,11 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->XS = h->BS + 2 * XOFF + 64;
     h->d_Wr = dalloc<float>(Wrp.size()); upload(h->d_Wr, Wrp);
     h->d_U2 = dalloc<float>(U2.size()); upload(h->d_U2, U2);
+    if (h->bf16) {
+        std::vector<uint16_t> U2b(U2.size());
+        for (size_t i = 0; i < U2.size(); ++i) U2b[i] = bf16_bits(U2[i]);
+        h->d_U2b = dalloc<uint16_t>(U2b.size()); upload(h->d_U2b, U2b);
+    }
     h->d_U = dalloc<float>(Ui.size()); upload(h->d_U, Ui);
     h->d_wpca = dalloc<float>(wpf.size()); upload(h->d_wpca, wpf);
     h->d_ctc = dalloc<float>(ctc_i.size()); upload(h->d_ctc, ctc_i);
@@ -488,6 +506,12 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_Xw = dalloc<float>((size_t)CP * h->XS);
     h->d_Bpc = dalloc<float>((size_t)KCP * h->BS);
     h->d_B = dalloc<float>((size_t)TP * h->BS);
+    if (h->bf16) {
+        h->d_Bpcb = dalloc<uint16_t>((size_t)KCP * h->BS);
+        h->d_Binit = dalloc<float>((size_t)TP * h->BS);
+        h->d_rescnt = dalloc<unsigned long long>(2);
+        HIPCHK(hipMemset(h->d_rescnt, 0, 2 * sizeof(unsigned long long)));
+    }
     h->d_Cfmax = dalloc<float>(Wmax);
     h->d_imax = dalloc<int>(Wmax);
     {
@@ -583,12 +607,18 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     PcconvJob& cj = bp.cj;
     cj = PcconvJob{};
     cj.Xw = h->d_Xw; cj.wpca = h->d_wpca; cj.Bpc = h->d_Bpc;
+    cj.Bpcb = reinterpret_cast<__bf16*>(h->d_Bpcb);
     cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP;
     cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
     ProjectJob& pj = bp.pj;
     pj = ProjectJob{};
     pj.A = h->d_U2; pj.Bm = h->d_Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = h->d_B;
     pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
+    if (h->bf16) {
+        pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
+        pj.Bmb = reinterpret_cast<const __bf16*>(h->d_Bpcb);
+        pj.C2 = h->d_Binit;
+    }
     pj.mtiles = h->TP / GEMM_BM;
     pj.ntiles = (W + GEMM_BN - 1) / GEMM_BN;
     PeelParams& p = bp.pp;
@@ -619,6 +649,16 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
     p.prof = h->d_prof;
+    if (h->bf16) {
+        // candidates within 10% of threshold on the score scale are decided in FP32
+        p.rescore = 1;
+        p.lo2 = 0.9f * p.Th2;
+        p.hi2 = 1.1f * p.Th2;
+        p.U2 = h->d_U2; p.ldu = h->KCP;
+        p.kb_ptr = h->d_kbp_ptr; p.kb_idx = h->d_kbp_idx;
+        p.Binit = h->d_Binit;
+        p.n_rescored = h->d_rescnt;
+    }
     bp.R = R; bp.CW = CW; bp.G = G;
     (void)R;
 }
@@ -677,7 +717,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
     launch_pcconv(cb, h->K, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
-    launch_project(pb, s);
+    launch_project(pb, h->bf16, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
     HIPCHK(hipGetLastError());
     if (h->cfg.keep_stages)
@@ -764,8 +804,11 @@ bool dedup_on(const lss_handle* h) {
 std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_config* cfg, int min_rows) {
     auto h = std::make_unique<lss_handle>();
     if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
-    if (h->cfg.precision != LSS_PREC_FP32)
-        throw LssError(LSS_ERR_ARG, "precision mode not available in this build");
+    if (h->cfg.precision != LSS_PREC_FP32 && h->cfg.precision != LSS_PREC_BF16_RESCORE)
+        throw LssError(LSS_ERR_ARG, "unknown precision mode");
+    h->bf16 = h->cfg.precision == LSS_PREC_BF16_RESCORE;
+    if (h->bf16 && h->cfg.semantics != LSS_SEM_KILOSORT)
+        throw LssError(LSS_ERR_ARG, "BF16 projection with FP32 rescoring is defined for Kilosort semantics");
     if (h->cfg.semantics != LSS_SEM_KILOSORT && h->cfg.semantics != LSS_SEM_LSS)
         throw LssError(LSS_ERR_ARG, "unknown semantics");
     if (h->cfg.chunk <= 0) throw LssError(LSS_ERR_ARG, "chunk must be positive");
@@ -930,6 +973,15 @@ int lss_query(lss_handle* h, lss_info* info) {
         i.C = h->C; i.T = h->T; i.M = h->M; i.K = h->K; i.nC = h->nC;
         i.nt0min = h->nt0min; i.dt = h->dt; i.semantics = h->sem; i.Th = h->Th;
         i.max_batch = h->Wmax; i.chunk = h->cfg.chunk; i.max_iter = h->max_iter;
+        i.precision = h->bf16 ? LSS_PREC_BF16_RESCORE : LSS_PREC_FP32;
+        if (h->bf16) {
+            unsigned long long c[2];
+            HIPCHK(hipSetDevice(h->dev));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipMemcpy(c, h->d_rescnt, sizeof(c), hipMemcpyDeviceToHost));
+            i.rescored = (int64_t)c[0];
+            i.rescore_flips = (int64_t)c[1];
+        }
         *info = i;
     });
 }
@@ -1085,7 +1137,7 @@ void group_capture(lss_group* g, const int32_t* ld) {
     launch_reset_batch(g->rb, st);
     launch_whiten(g->wb, st);
     launch_pcconv(g->cb, g->hs[0]->K, st);
-    launch_project(g->pb, st);
+    launch_project(g->pb, g->hs[0]->bf16, st);
     HIPCHK(launch_peel(g->kb, g->R, false, st));
     HIPCHK(hipStreamEndCapture(st, &g->graph));
     HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));

@@ -30,12 +30,13 @@ constexpr int ERR_LIST = 2;
 constexpr int ERR_SLOT = 4;
 constexpr int ERR_STAGE = 8;
 
-struct Spike {      // one accepted spike inside the peel kernel
+struct alignas(16) Spike {   // one accepted spike inside the peel kernel
     int w;          // batch column
     int t;          // template, internal order
     float amp;
     int it;         // iteration that found it
 };
+static_assert(sizeof(Spike) == 16, "spikes are moved as one 16-byte word");
 
 struct OutHeader {  // written by the peel kernel into mapped host memory
     int64_t count;
@@ -83,6 +84,17 @@ struct PeelParams {
     int half; int nt0min;
     float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
     unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch
+    // FP32 rescoring of a BF16 projection (rescore != 0): a column whose best
+    // score reaches lo2 gets that score recomputed as the FP32 dot product of U2
+    // row t with Bpc column w (over the listed K-blocks of t's 64-row tile) minus
+    // the BF16 value Binit[t][w], added to its current score before it is
+    // published; hi2 is the upper end of the band reported as near-threshold.
+    int rescore;
+    float lo2, hi2;
+    const float* U2; int ldu;          // [TP][K*CP], FP32
+    const int* kb_ptr; const int* kb_idx;
+    const float* Binit;                // the BF16 projection before any subtraction
+    unsigned long long* n_rescored;    // running totals: FP32 rescores, threshold decisions flipped
 };
 
 // ---- batched launch descriptors (kernel arguments) ----
@@ -96,10 +108,13 @@ struct WhitenJob {
 };
 struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
+    const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
+    float* C2;                            // optional second copy of the result
     int lda, ldb, ldc, mtiles, ntiles;
 };
 struct PcconvJob {
     const float* Xw; const float* wpca; float* Bpc;
+    __bf16* Bpcb;                         // optional BF16 copy of the PC projection
     int XS, M, BS, CP, wblocks;
 };
 template <class J> struct JobBatch {
@@ -119,7 +134,7 @@ void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned l
 
 void launch_whiten(const WhitenBatch& b, hipStream_t s);
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s);
-void launch_project(const ProjectBatch& b, hipStream_t s);
+void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s);
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
 

@@ -201,6 +201,81 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     }
 #pragma unroll
     for (int q = 0; q < KK; ++q) jb.Bpc[(int64_t)(q * jb.CP + c) * jb.BS + w0 + tid] = acc[q];
+    if (jb.Bpcb) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) jb.Bpcb[(int64_t)(q * jb.CP + c) * jb.BS + w0 + tid] = (__bf16)acc[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 projection with FP32 accumulation (v_mfma_f32_32x32x16_bf16; lane l holds
+// A[l % 32][8 * (l / 32) + e] and B[8 * (l / 32) + e][l % 32], e = 0..7).  Each
+// pipeline step gathers four listed 16-wide K-blocks into a 64-deep LDS tile
+// (rows k-contiguous, padded), so the block-sparse skipping keeps its 16-row
+// granularity.  Optionally writes a second copy of the result (the initial
+// scores that the peel's FP32 rescoring compares against).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int BK16 = 64;           // K depth of one BF16 step (four 16-blocks)
+constexpr int BPAD = 8;
+
+__global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[GEMM_BM][BK16 + BPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[GEMM_BN][BK16 + BPAD];
+    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    const ProjectJob& jb = bt.job[j];
+    int mt, nt;
+    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    const __bf16* __restrict__ A = jb.Ab;
+    const __bf16* __restrict__ Bm = jb.Bmb;
+    const int lda = jb.lda, ldb = jb.ldb;
+    const int m0 = mt * GEMM_BM, n0 = nt * GEMM_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int ar = tid >> 2, aq = tid & 3;                  // A: row, sub-block
+    const int bq = tid >> 6, brow = (tid & 63) >> 2, bc = (tid & 3) * 16;   // B: sub-block, k row, 16 columns
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    for (int p = p0; p < p1; p += 4) {
+        // gather: sub-block q of this step is K-block kb_idx[p + q] (zeros past the list end)
+        {
+            const int pa = p + aq;
+            uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0;
+            if (pa < p1) {
+                const __bf16* src = A + (int64_t)(m0 + ar) * lda + jb.kb_idx[pa] * GEMM_BK;
+                v0 = *reinterpret_cast<const uint4*>(src);
+                v1 = *reinterpret_cast<const uint4*>(src + 8);
+            }
+            __syncthreads();
+            *reinterpret_cast<uint4*>(&As[ar][aq * 16]) = v0;
+            *reinterpret_cast<uint4*>(&As[ar][aq * 16 + 8]) = v1;
+            const int pb = p + bq;
+            __bf16 tmp[16];
+            if (pb < p1) {
+                const __bf16* src = Bm + (int64_t)(jb.kb_idx[pb] * GEMM_BK + brow) * ldb + n0 + bc;
+                const uint4 u0 = *reinterpret_cast<const uint4*>(src);
+                const uint4 u1 = *reinterpret_cast<const uint4*>(src + 8);
+                __builtin_memcpy(tmp, &u0, 16);
+                __builtin_memcpy(tmp + 8, &u1, 16);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) tmp[e] = (__bf16)0.f;
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e) Bs[bc + e][bq * 16 + brow] = tmp[e];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK16 / 16; ++kk) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + (lane & 31)][kk * 16 + 8 * (lane >> 5)]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + (lane & 31)][kk * 16 + 8 * (lane >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+    }
+    store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
+    if (jb.C2) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
 }
 
 // nm[t] = sum_{k,c} U[t,k,c]^2 accumulated in FP64, stored as FP64 and rounded FP32;
@@ -345,6 +420,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ float s_bv[MAX_CW];
     __shared__ float s_cm[MAX_CW];
     __shared__ int s_flag[MAX_CW];
+    __shared__ int s_rt[MAX_CW];               // FP32 rescoring: template whose correction is cached
+    __shared__ float s_rd[MAX_CW];             //   and the correction (FP32 score - BF16 score)
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ int s_hcnt_w[MAX_CW / 64];
     __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
@@ -367,6 +444,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     const bool edge = (w < M) || (w >= W - M);
     unsigned target = 0;
     if (tid == 0) s_listn = 0;
+    for (int i = tid; i < MAX_CW; i += PEEL_THREADS) s_rt[i] = -1;
 
     // Buffer descriptors: one scalar base per table plus a per-thread offset, so
     // the R rows of a tile need no address registers (row strides go in SGPRs).
@@ -577,9 +655,71 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     s_cf[tid] = bv;
                     s_im[tid] = bI;
                     s_bv[tid] = bb;
-                    st_c(&p.Cfmax[ww], bv);
+                    if (!p.rescore) st_c(&p.Cfmax[ww], bv);
                     if (it == 0 && p.dbg_cfmax) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
                 }
+            }
+            if (p.rescore) {
+                // BF16 projection: every column whose best score reaches lo2 (10% under
+                // threshold) is rescored in FP32 before it is published, so pooling,
+                // acceptance and amplitudes see FP32 scores wherever they can matter
+                // (a column under lo2 in BF16 cannot beat an accepted one in FP32).
+                // The correction of (t, w) is fixed (subtractions are applied to both
+                // alike), so it is computed once per (column, template) and cached.
+                __syncthreads();
+                const int lane = tid & 63, wv = tid >> 6;
+                bool need = false;
+                if (tid < CW && c0 + tid < c1) need = s_cf[tid] >= p.lo2 && s_rt[tid] != s_im[tid];
+                const uint64_t bal = __ballot(need);
+                if (tid < CW && lane == 0) s_hcnt_w[wv] = __popcll(bal);
+                __syncthreads();
+                int nres = 0, base = 0;
+                for (int k = 0; k < (CW + 63) / 64; ++k) {
+                    if (k == wv) base = nres;
+                    nres += s_hcnt_w[k];
+                }
+                if (need) s_org[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
+                __syncthreads();
+                for (int q = wv; q < nres; q += PEEL_THREADS / 64) {
+                    const int cl = s_org[q];
+                    const int t = s_im[cl], ww = c0 + cl;
+                    // FP32 dot of U2 row t with Bpc column ww over the K-blocks listed for
+                    // t's 64-row tile (the others are exact zeros)
+                    const int mt = t / GEMM_BM;
+                    const int e0 = p.kb_ptr[mt] * GEMM_BK, e1 = p.kb_ptr[mt + 1] * GEMM_BK;
+                    const float* urow = p.U2 + (int64_t)t * p.ldu;
+                    float acc = 0.f;
+                    for (int e = e0 + lane; e < e1; e += 64) {
+                        const int kc = p.kb_idx[e / GEMM_BK] * GEMM_BK + e % GEMM_BK;
+                        acc += urow[kc] * p.Bpc[(int64_t)kc * p.BS + ww];
+                    }
+                    // butterfly: every lane ends with the same, order-fixed sum
+                    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+                    if (lane == 0) {
+                        s_rd[cl] = acc - p.Binit[(int64_t)t * p.BS + ww];
+                        s_rt[cl] = t;
+                    }
+                }
+                __syncthreads();
+                int flip = 0;
+                if (tid < CW && c0 + tid < c1) {
+                    float cf = s_cf[tid];
+                    if (cf >= p.lo2) {
+                        const int t = s_im[tid];
+                        const float b = s_bv[tid] + s_rd[tid];
+                        const float r = fmaxf(b, 0.f);
+                        const float c32 = (r * r) / p.nm[t];
+                        flip = (cf > p.Th2) != (c32 > p.Th2);
+                        s_bv[tid] = b;
+                        s_cf[tid] = cf = c32;
+                    }
+                    st_c(&p.Cfmax[c0 + tid], cf);
+                }
+                const uint64_t fb = __ballot(flip);
+                if (lane == 0 && wv == 0) {
+                    if (nres > 0) atomicAdd(&p.n_rescored[0], (unsigned long long)nres);
+                }
+                if (lane == 0 && fb) atomicAdd(&p.n_rescored[1], (unsigned long long)__popcll(fb));
             }
         }
         mark(1);
@@ -823,8 +963,9 @@ void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {
     }
 }
 
-void launch_project(const ProjectBatch& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_project, dim3(b.start[b.n]), dim3(256), 0, s, b);
+void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s) {
+    if (bf16) hipLaunchKernelGGL(k_project_bf16, dim3(b.start[b.n]), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_project, dim3(b.start[b.n]), dim3(256), 0, s, b);
 }
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s) {

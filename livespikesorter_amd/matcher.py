@@ -15,10 +15,11 @@ import os
 
 import numpy as np
 
-from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_FP32, SEM_KILOSORT, SEM_LSS, check,
+from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_BF16_RESCORE, PREC_FP32, SEM_KILOSORT, SEM_LSS, check,
                     default_config, lib)
 
-__all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS"]
+__all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS", "PREC_FP32",
+           "PREC_BF16_RESCORE"]
 
 STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES = range(7)
 
@@ -154,11 +155,13 @@ class StreamGroup:
     """
 
     def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
-                 semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True):
+                 semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True,
+                 precision: int = PREC_FP32):
         self._g = ct.c_void_p()
         self.n = len(oss_input_dirs)
         cfg = default_config(device=device, chunk=batch, max_batch=batch, max_iter=max_iter,
-                             semantics=semantics, max_events=max_events, sparse=int(sparse))
+                             semantics=semantics, max_events=max_events, sparse=int(sparse),
+                             precision=precision)
         dirs = (ct.c_char_p * self.n)(*[os.fsencode(str(d)) for d in oss_input_dirs])
         check(lib().lss_group_create(self.n, dirs, ct.byref(cfg), ct.byref(self._g)))
         self.batch = batch
