@@ -405,7 +405,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     const PeelParams& p = bt.p[sj];
     // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
     // immediate offset (no per-row address registers); the norm table comes first.
-    __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template norm (or its reciprocal, LSS)
+    __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template reciprocal norm (0 past T)
     __shared__ int s_or[R > 0 ? REG_MAX_T : 1];      // per-template file index (INT_MAX past T)
     __shared__ Spike s_sp[MAX_STAGE];          // staged / newly accepted spikes
     __shared__ Spike s_list[MAX_LIST];         // this workgroup's spikes, whole batch
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     float breg[R > 0 ? R : 1];
     if (R > 0) {
         for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) {
-            s_nm[t] = p.lss ? p.rnm[t] : p.nm[t];
+            s_nm[t] = p.rnm[t];
             s_or[t] = p.orig[t];
         }
         const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
@@ -467,15 +467,17 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     }
 
     int it_found = 0;
+    // phase clock: thread 0 accumulates in LDS (no registers held across the loop)
     const bool timing = (tid == 0);
-    int a_passes = 0;
-    uint64_t ph[NPHASE] = {0, 0, 0, 0, 0, 0, 0};
-    uint64_t t_prev = timing ? rt_now() : 0;
+    __shared__ uint64_t s_ph[NPHASE + 2];      // phases, then the previous mark and the pass count
+    if (tid < NPHASE + 2) s_ph[tid] = 0;
+    __syncthreads();
+    if (timing) s_ph[NPHASE] = rt_now();
     auto mark = [&](int k) {
         if (timing) {
             const uint64_t t = rt_now();
-            ph[k] += t - t_prev;
-            t_prev = t;
+            s_ph[k] += t - s_ph[NPHASE];
+            s_ph[NPHASE] = t;
         }
     };
     for (int it = 0; it <= p.max_iter; ++it) {
@@ -515,9 +517,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
 
         // ---------------- phase A ----------------
         if (it == 0 || nsp > 0) {
-            ++a_passes;
+            if (timing) s_ph[NPHASE + 1] += 1;
             float best = -INFINITY, bval = 0.f;
             int bi = 0, bo = 0x7fffffff;
+            // A wave whose rows no spike touched keeps last pass's per-thread
+            // maxima in LDS (its registers did not change): it skips the scan.
+            bool wdirty = it == 0 || R == 0;
             if (R > 0) {
                 // Runs with the whole wave active (every lane tests spikes for the ballot);
                 // edge columns and columns past W are updated too but never read.
@@ -543,6 +548,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     const uint64_t bal = __ballot(bh);
                     if (bh) whit[__popcll(bal & ((1ull << lane) - 1ull))] = qq;
                     const int nhit = __popcll(bal);
+                    wdirty = wdirty || nhit > 0;
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_s_waitcnt(0xc07f);     // lgkmcnt(0): the list is in LDS
                     for (int h0 = 0; h0 < nhit; h0 += 4) {
@@ -557,7 +563,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             am[j] = hit ? sp.amp : 0.f;
                             vo[j] = ((sp.t * p.ctc_rows + r0) * LP + (hit ? l : 0)) * 4;
                         }
-                        constexpr int RC = R < 8 ? R : 8;    // rows per load batch
+                        constexpr int RC = R < 8 ? R : (R >= 64 ? 4 : 8);    // rows per load batch
     #pragma unroll
                         for (int i0 = 0; i0 < R; i0 += RC) {
                             float c[4][RC];
@@ -577,35 +583,69 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 }
                 mark(6);
             }
-            if (own && edge) {
+            if (!wdirty) {
+                // nothing to rescan
+            } else if (own && edge) {
                 // Cf is zero for every template: first file index wins the tie
                 best = 0.f; bi = p.t_first; bo = 0;
             } else if (own) {
                 if (R > 0) {
                     int vb = rg * R;
                     asm volatile("" : "+v"(vb));
-                    const float* nmp = s_nm + vb;
+                    const float* rnp = s_nm + vb;
                     const int* orp = s_or + vb;
-                    // branch-free argmax; on exact ties the smaller file index wins (first
-                    // index, as torch.max).  Rows go in groups of 8 with a compiler barrier
+                    // Argmax in two steps.  a = r^2 * (1/nm) is within 3 * 2^-24 (relative)
+                    // of Cf = r^2 / nm (Kilosort; LSS multiplies, so a is Cf exactly).  One
+                    // branch-free pass keeps the largest a, its row, and the second largest;
+                    // when the runner-up is within 2^-20 of the winner (or ties it, LSS) the
+                    // rows in that band are decided on exact Cf, smaller file index first on
+                    // ties, as torch.max.  Rows go in groups of 8 with a compiler barrier
                     // between groups so the table loads are not all hoisted at once.
-                    best = -1.f;
-                    bo = 0x7fffffff;
+                    float amax = -1.f, a2 = -1.f;
                     int bslot = 0;          // winning row slot; its row is vb + bslot
 #pragma unroll
                     for (int i0 = 0; i0 < R; i0 += 8) {
 #pragma unroll
                         for (int i = i0; i < i0 + 8; ++i) {
-                            const float r = fmaxf(breg[i], 0.f);
-                            const float cf = p.lss ? (r * r) * nmp[i] : (r * r) / nmp[i];
-                            const int o = orp[i];
-                            const bool gt = cf > best || (cf == best && o < bo);
-                            best = gt ? cf : best;
-                            bo = gt ? o : bo;
+                            // selects rather than fmaxf: no NaN canonicalisation (scores are finite)
+                            const float r = breg[i] > 0.f ? breg[i] : 0.f;
+                            const float a = (r * r) * rnp[i];
+                            const bool gt = a > amax;
+                            a2 = gt ? amax : (a > a2 ? a : a2);
+                            amax = gt ? a : amax;
                             bslot = gt ? i : bslot;
                             bval = gt ? breg[i] : bval;
                         }
                         asm volatile("" ::: "memory");
+                    }
+                    const float thr = p.lss ? amax : amax * (1.f - 0x1p-20f);
+                    if (a2 < thr) {
+                        const float r = fmaxf(bval, 0.f);
+                        best = p.lss ? amax : (r * r) / p.nm[vb + bslot];
+                        bo = orp[bslot];
+                    } else {
+                        best = -1.f;
+                        bo = 0x7fffffff;
+                        const float* nmp = p.nm + vb;
+                        // recompute the squares here instead of keeping 64 of them live
+#pragma unroll
+                        for (int i = 0; i < R; ++i) asm volatile("" : "+v"(breg[i]));
+#pragma unroll
+                        for (int i0 = 0; i0 < R; i0 += 8) {
+#pragma unroll
+                            for (int i = i0; i < i0 + 8; ++i) {
+                                const float r = fmaxf(breg[i], 0.f);
+                                const float a = (r * r) * rnp[i];
+                                const float cf = p.lss ? a : (r * r) / nmp[i];
+                                const int o = orp[i];
+                                const bool gt = a >= thr && (cf > best || (cf == best && o < bo));
+                                best = gt ? cf : best;
+                                bo = gt ? o : bo;
+                                bslot = gt ? i : bslot;
+                                bval = gt ? breg[i] : bval;
+                            }
+                            asm volatile("" ::: "memory");
+                        }
                     }
                     bi = vb + bslot;
                 } else {
@@ -632,7 +672,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     }
                 }
             }
-            s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo; s_bvv[tid] = bval;
+            if (wdirty) { s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo; s_bvv[tid] = bval; }
             __syncthreads();
             if (tid < CW) {
                 // per-thread base pointers: the row-group strides become immediate
@@ -678,10 +718,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     if (k == wv) base = nres;
                     nres += s_hcnt_w[k];
                 }
-                if (need) s_org[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
+                if (need) s_flag[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
                 __syncthreads();
                 for (int q = wv; q < nres; q += PEEL_THREADS / 64) {
-                    const int cl = s_org[q];
+                    const int cl = s_flag[q];
                     const int t = s_im[cl], ww = c0 + cl;
                     // FP32 dot of U2 row t with Bpc column ww over the K-blocks listed for
                     // t's 64-row tile (the others are exact zeros)
@@ -912,10 +952,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     mark(5);
     if (timing) {
         for (int k = 0; k < NPHASE; ++k) {
-            atomicAdd(&p.prof[k], (unsigned long long)ph[k]);
-            atomicMax(&p.prof[NPHASE + k], (unsigned long long)ph[k]);
+            atomicAdd(&p.prof[k], (unsigned long long)s_ph[k]);
+            atomicMax(&p.prof[NPHASE + k], (unsigned long long)s_ph[k]);
         }
-        atomicAdd(&p.prof[2 * NPHASE], (unsigned long long)a_passes);
+        atomicAdd(&p.prof[2 * NPHASE], (unsigned long long)s_ph[NPHASE + 1]);
         atomicAdd(&p.prof[2 * NPHASE + 1], 1ull);
     }
     if (g == p.G - 1 && tid == 0) {
