@@ -52,7 +52,9 @@ typedef struct lss_config {
     int32_t keep_stages;   /* nonzero: keep per-stage device copies for lss_debug_stage */
     int32_t sparse;        /* nonzero (default): skip exact-zero template/channel blocks */
     int32_t profile;       /* nonzero: time each device stage with HIP events (lss_stage_times) */
-    int32_t reserved[5];
+    int32_t peel_split;    /* template subsets per peel column block: 0 = automatic,
+                              1 = none, n > 1 = at most n (Kilosort semantics only) */
+    int32_t reserved[4];
 } lss_config;
 
 /* One spike: (sample, template, amplitude), as OnlineSpikesPayload carries them
@@ -82,6 +84,10 @@ typedef struct lss_info {
     int32_t reserved0;
     int64_t rescored;       /* BF16 mode: local maxima rescored in FP32 (running total) */
     int64_t rescore_flips;  /* BF16 mode: of those, accept/reject decisions the rescore changed */
+    int32_t peel_cw;        /* last batch's peel geometry: columns per workgroup, */
+    int32_t peel_rows;      /*   score rows per thread held in registers (0: in memory), */
+    int32_t peel_split;     /*   template subsets per column block, */
+    int32_t peel_groups;    /*   and workgroups */
 } lss_info;
 
 /* Fill *cfg with defaults. */

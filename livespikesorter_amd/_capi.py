@@ -39,7 +39,8 @@ class Config(ct.Structure):
     _fields_ = [("device", ct.c_int32), ("chunk", ct.c_int32), ("max_batch", ct.c_int32),
                 ("max_iter", ct.c_int32), ("semantics", ct.c_int32), ("precision", ct.c_int32),
                 ("dedup", ct.c_int32), ("max_events", ct.c_int32), ("keep_stages", ct.c_int32),
-                ("sparse", ct.c_int32), ("profile", ct.c_int32), ("reserved", ct.c_int32 * 5)]
+                ("sparse", ct.c_int32), ("profile", ct.c_int32), ("peel_split", ct.c_int32),
+                ("reserved", ct.c_int32 * 4)]
 
 
 class Event(ct.Structure):
@@ -58,7 +59,8 @@ class Info(ct.Structure):
                 ("events", ct.c_int64), ("last_iters", ct.c_int32), ("last_spikes", ct.c_int32),
                 ("last_ms", ct.c_double), ("white_kblocks", ct.c_int64), ("proj_kblocks", ct.c_int64),
                 ("proj_kblocks_dense", ct.c_int64), ("precision", ct.c_int32), ("reserved0", ct.c_int32),
-                ("rescored", ct.c_int64), ("rescore_flips", ct.c_int64)]
+                ("rescored", ct.c_int64), ("rescore_flips", ct.c_int64), ("peel_cw", ct.c_int32),
+                ("peel_rows", ct.c_int32), ("peel_split", ct.c_int32), ("peel_groups", ct.c_int32)]
 
 
 class LssError(RuntimeError):

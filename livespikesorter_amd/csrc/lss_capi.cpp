@@ -150,6 +150,10 @@ struct lss_handle {
     float* d_Binit = nullptr;
     unsigned long long* d_rescnt = nullptr;
     int slot_cap = 0, cap_g = 0, max_groups = 0;
+    // template split of the peel: per-subset column maxima [max_split][Wmax]
+    int max_split = 1;
+    float *d_Pcf = nullptr, *d_Pbv = nullptr;
+    int* d_Pim = nullptr;
     // debug copies
     float *d_dbgB = nullptr, *d_dbgCf = nullptr;
     int* d_dbgIm = nullptr;
@@ -191,7 +195,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt};
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_Pcf, d_Pbv, d_Pim};
         (void)hipSetDevice(dev);
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -531,10 +535,20 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_slots = dalloc<Spike>((size_t)h->max_groups * h->slot_cap);
     h->d_list = dalloc<Spike>((size_t)h->max_groups * h->cap_g);
     h->d_slot_cnt = dalloc<int>(h->max_groups);
+    // split the templates of a column block over up to 16 workgroups, 128 rows each at most
+    if (h->T <= REG_MAX_T && h->sem == LSS_SEM_KILOSORT && h->min_rows == 0) {
+        h->max_split = std::min<int>(16, (h->T + 127) / 128);
+        if (h->cfg.peel_split > 0) h->max_split = std::min(h->max_split, (int)h->cfg.peel_split);
+    }
+    if (h->max_split > 1) {
+        h->d_Pcf = dalloc<float>((size_t)h->max_split * Wmax);
+        h->d_Pim = dalloc<int>((size_t)h->max_split * Wmax);
+        h->d_Pbv = dalloc<float>((size_t)h->max_split * Wmax);
+    }
     h->d_list_cnt = dalloc<int>(h->max_groups);
     h->d_iter_cnt = dalloc<int>(std::max(1, h->max_iter));
     h->d_err = dalloc<int>(1);
-    h->d_bar = dalloc<unsigned>(1);
+    h->d_bar = dalloc<unsigned>(BAR_WORDS);
     h->d_prof = dalloc<unsigned long long>(PROF_WORDS);
     if (h->cfg.keep_stages) {
         h->d_dbgB = dalloc<float>((size_t)TP * h->BS);
@@ -595,7 +609,7 @@ inline int peel_cap_index(int R) { return R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 
 // Job descriptors of one handle for one batch of W columns.  The input is read
 // from d_x, or, when xptr is set, from the device pointer stored at *xptr.
 void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, int W, int64_t ld, int CW, int R,
-                BatchPlan& bp) {
+                int S, BatchPlan& bp) {
     const int M = h->M;
     WhitenJob& wj = bp.wj;
     wj = WhitenJob{};
@@ -623,7 +637,8 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     pj.ntiles = (W + GEMM_BN - 1) / GEMM_BN;
     PeelParams& p = bp.pp;
     p = PeelParams{};
-    const int G = (W + CW - 1) / CW;
+    const int NB = (W + CW - 1) / CW;
+    const int G = S * NB;
     p.B = h->d_B; p.BS = h->BS;
     p.B_bytes = (int)std::min<int64_t>((int64_t)h->TP * h->BS * 4, INT32_MAX);
     p.ctc_bytes = (int)std::min<int64_t>((int64_t)h->T * h->ctc_rows * LP * 4, INT32_MAX);
@@ -635,6 +650,8 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.Th2 = h->Th * h->Th;
     p.max_iter = h->max_iter;
     p.CW = CW; p.G = G;
+    p.S = S; p.NB = NB;
+    p.Pcf = h->d_Pcf; p.Pim = h->d_Pim; p.Pbv = h->d_Pbv;
     p.lss = h->sem == LSS_SEM_LSS;
     p.Cfmax = h->d_Cfmax; p.imax = h->d_imax;
     p.slots = h->d_slots; p.slot_cnt = h->d_slot_cnt; p.slot_cap = h->slot_cap;
@@ -663,8 +680,22 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     (void)R;
 }
 
-// Peel geometry for one handle: the narrowest column block whose grid is co-resident.
-void peel_geometry(const lss_handle* h, int W, int& CW, int& R) {
+// Peel geometry for one handle.  Register tiles (64 columns) first, with the
+// templates split over S workgroups per column block when the co-resident grid
+// has room: the fewest rows per thread that fit (fewer rows, shorter scans).
+// Otherwise the narrowest column block whose grid is co-resident.
+void peel_geometry(const lss_handle* h, int W, int& CW, int& R, int& S) {
+    S = 1;
+    if (h->max_split > 1) {
+        const int NB = (W + MIN_CW - 1) / MIN_CW;
+        for (int r : {8, 16, 32}) {
+            const int s2 = std::min(h->max_split, (h->T + REG_NRG * r - 1) / (REG_NRG * r));
+            if (s2 < 2 || s2 * REG_NRG * r < h->T || s2 * REG_NRG * r > h->ctc_rows) continue;
+            if (NB > h->max_groups || s2 * NB > h->peel_cap[peel_cap_index(r)]) continue;
+            CW = MIN_CW; R = r; S = s2;
+            return;
+        }
+    }
     for (CW = MIN_CW;; CW *= 2) {
         const int G = (W + CW - 1) / CW;
         R = peel_rows(h->T, CW);
@@ -699,10 +730,11 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
                                               " samples outside 1..max_batch " + std::to_string(h->Wmax));
     hipStream_t s = h->stream;
     const bool prof = h->cfg.profile != 0;
-    int CW = 0, R = 0;
-    peel_geometry(h, W, CW, R);
+    int CW = 0, R = 0, S = 1;
+    peel_geometry(h, W, CW, R, S);
     BatchPlan bp;
-    plan_batch(h, d_x, nullptr, W, ld, CW, R, bp);
+    plan_batch(h, d_x, nullptr, W, ld, CW, R, S, bp);
+    h->info.peel_cw = CW; h->info.peel_rows = R; h->info.peel_split = S; h->info.peel_groups = bp.G;
     WhitenBatch wb{};
     wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
     PcconvBatch cb{};
@@ -1117,7 +1149,7 @@ void group_capture(lss_group* g, const int32_t* ld) {
     for (int s2 = 0; s2 < n; ++s2) {
         lss_handle* h = g->hs[s2].get();
         BatchPlan bp;
-        plan_batch(h, nullptr, &g->d_xptr[s2], g->W, ld[s2], g->CW, g->R, bp);
+        plan_batch(h, nullptr, &g->d_xptr[s2], g->W, ld[s2], g->CW, g->R, 1, bp);
         g->wb.job[s2] = bp.wj;
         g->wb.start[s2 + 1] = g->wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
         g->cb.job[s2] = bp.cj;

@@ -24,6 +24,17 @@ constexpr int MAX_K = 8;
 constexpr int MAX_STAGE = 512;  // spikes of one iteration in a workgroup's neighbourhood
 constexpr int MAX_LIST = 512;   // spikes per workgroup over a whole batch
 
+// Grid barrier words: a top counter, per-group arrival counters and per-group
+// release words, each on its own 256-byte line.  Grids of at least BAR_HIER_MIN
+// workgroups arrive in groups of BAR_GROUP (one arrival per group on the top
+// counter) and wait on their group's release word; smaller grids use the top
+// counter alone.
+constexpr int BAR_LINE = 64;
+constexpr int BAR_GROUP = 32;
+constexpr int BAR_MAX_GROUPS = 32;
+constexpr int BAR_HIER_MIN = 96;
+constexpr int BAR_WORDS = BAR_LINE * (1 + 2 * BAR_MAX_GROUPS);
+
 // peel error flags
 constexpr int ERR_BARRIER = 1;
 constexpr int ERR_LIST = 2;
@@ -68,6 +79,8 @@ struct PeelParams {
     float Th2;
     int max_iter;
     int CW, G;
+    int S, NB;                         // template subsets per column block, column blocks (G = S * NB)
+    float* Pcf; int* Pim; float* Pbv;  // per-subset column maxima [S][W] (S > 1)
     int lss;
     float* Cfmax; int* imax;
     Spike* slots; int* slot_cnt; int slot_cap;

@@ -306,10 +306,8 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ U, int 
 __global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof) {
     for (int i = threadIdx.x; i < n_iter; i += blockDim.x) iter_cnt[i] = 0;
     if ((int)threadIdx.x < PROF_WORDS) prof[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) {
-        *bar = 0u;
-        *err = 0;
-    }
+    if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) bar[threadIdx.x * BAR_LINE] = 0u;
+    if (threadIdx.x == 0) *err = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -350,14 +348,33 @@ __device__ __forceinline__ Spike ld_spike(const Spike* d) {
     return s;
 }
 
-__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned nblocks, unsigned& target, int* err) {
+// Barrier number `gen` (1, 2, ...) of a grid of G workgroups; g is this one.
+__device__ __forceinline__ void grid_sync(unsigned* bar, int G, int g, unsigned& gen, int* err) {
     __builtin_amdgcn_s_waitcnt(0);       // this wave's coherent stores have completed
     __syncthreads();
     if (threadIdx.x == 0) {
-        target += nblocks;
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        gen += 1;
+        const unsigned* wait_on = bar;
+        unsigned target = gen * (unsigned)G;
+        if (G >= BAR_HIER_MIN) {
+            // arrive on the group counter; the group's last arriver arrives on the
+            // top counter, and the grid's last arriver releases every group
+            const int ng = (G + BAR_GROUP - 1) / BAR_GROUP, grp = g / BAR_GROUP;
+            const unsigned gsize = (unsigned)min(BAR_GROUP, G - grp * BAR_GROUP);
+            const unsigned old = __hip_atomic_fetch_add(bar + BAR_LINE * (1 + grp), 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old == gen * gsize - 1u) {
+                const unsigned o2 = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (o2 == gen * (unsigned)ng - 1u)
+                    for (int k = 0; k < ng; ++k) st_c(bar + BAR_LINE * (1 + BAR_MAX_GROUPS + k), gen);
+            }
+            wait_on = bar + BAR_LINE * (1 + BAR_MAX_GROUPS + grp);
+            target = gen;
+        } else {
+            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         unsigned spins = 0;
-        while (ld_c(bar) < target) {
+        while (ld_c(wait_on) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins > (1u << 26)) {
                 __hip_atomic_fetch_or(err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -430,19 +447,27 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
 
     const int g = blockIdx.x - bt.g_start[sj], tid = threadIdx.x;
+    // Workgroup g owns column block cb = g / S and template subset sb = g % S:
+    // rows [sb * REG_NRG * R, (sb + 1) * REG_NRG * R) of the block's columns.
+    // Subset 0 is the block's owner: it accepts spikes, publishes them and
+    // writes the events; the other subsets publish per-column partial maxima.
+    const int S = p.S;
+    const int cb = g / S, sb = g - cb * S;
+    const int rbase = R > 0 ? sb * REG_NRG * R : 0;
+    const bool owner = sb == 0;
     // register tiles use a fixed 64-column width, so column and row strides are
     // compile-time constants (LDS offsets fold into the instructions)
     const int CW = R > 0 ? PEEL_THREADS / REG_NRG : p.CW;
     const int M = p.M, T = p.T, W = p.W;
-    const int c0 = g * CW, c1 = min(W, c0 + CW);
+    const int c0 = cb * CW, c1 = min(W, c0 + CW);
     const int nrg = PEEL_THREADS / CW;
     const int col = tid % CW, rg = tid / CW;
-    const int hlo = max(0, (c0 - M) / CW), hhi = min(p.G - 1, (c1 - 1 + M) / CW);
+    const int hlo = max(0, (c0 - M) / CW), hhi = min(p.NB - 1, (c1 - 1 + M) / CW);
     const int nh = hhi - hlo + 1;             // <= MAX_NBR because CW >= M
     const int w = c0 + col;
     const bool own = w < c1;
     const bool edge = (w < M) || (w >= W - M);
-    unsigned target = 0;
+    unsigned bar_gen = 0;
     if (tid == 0) s_listn = 0;
     for (int i = tid; i < MAX_CW; i += PEEL_THREADS) s_rt[i] = -1;
 
@@ -453,18 +478,47 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     float breg[R > 0 ? R : 1];
     if (R > 0) {
         for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) {
-            s_nm[t] = p.rnm[t];
-            s_or[t] = p.orig[t];
+            s_nm[t] = p.rnm[rbase + t];
+            s_or[t] = p.orig[rbase + t];
         }
+        // rows past the score matrix read as zero (buffer loads beyond the range)
         const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
-        // wave rg holds the contiguous row block [rg*R, rg*R + R) of its 64 columns
-        const int voff = (rg * R * p.BS + w) * 4;
+        // wave rg holds the contiguous row block rbase + [rg*R, rg*R + R) of its 64 columns
+        const int voff = ((rbase + rg * R) * p.BS + w) * 4;
         const int rstride = p.BS * 4;
 #pragma unroll
         for (int i = 0; i < R; ++i)
             breg[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(b_rs, voff, i * rstride, 0));
         __syncthreads();
     }
+
+    // Publish own column cl's maximum: the column maximum itself (one subset) or
+    // this subset's partial (score, template, B value).
+    auto publish = [&](int cl) {
+        const int ww = c0 + cl;
+        if (S == 1) {
+            st_c(&p.Cfmax[ww], s_cf[cl]);
+        } else {
+            const int64_t k = (int64_t)sb * W + ww;
+            st_c(&p.Pcf[k], s_cf[cl]);
+            st_c(&p.Pim[k], s_im[cl]);
+            st_c(&p.Pbv[k], s_bv[cl]);
+        }
+    };
+
+    // The winning partial of column ww over the subsets: largest score, smaller
+    // file index on ties (the argmax over all templates).
+    auto subset_winner = [&](int ww, float& v, int& t, float& b) {
+        v = -1.f;
+        int bo2 = 0x7fffffff;
+        for (int s2 = 0; s2 < S; ++s2) {
+            const int64_t k = (int64_t)s2 * W + ww;
+            const float c = ld_c(&p.Pcf[k]);
+            const int tt = ld_c(&p.Pim[k]);
+            const int o = p.orig[tt];
+            if (c > v || (c == v && o < bo2)) { v = c; bo2 = o; t = tt; b = ld_c(&p.Pbv[k]); }
+        }
+    };
 
     int it_found = 0;
     // phase clock: thread 0 accumulates in LDS (no registers held across the loop)
@@ -534,7 +588,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 // flight before any is consumed.  A spike whose window misses this
                 // column contributes with amplitude 0 (b - 0*c is b), so lanes never
                 // diverge.  Spikes are applied in sample order.
-                const int r0 = rg * R;
+                const int r0 = rbase + rg * R;
                 const int lane = tid & 63;
                 int* whit = s_whit + rg * 64;
                 for (int q0 = 0; q0 < nsp; q0 += 64) {
@@ -621,12 +675,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     const float thr = p.lss ? amax : amax * (1.f - 0x1p-20f);
                     if (a2 < thr) {
                         const float r = fmaxf(bval, 0.f);
-                        best = p.lss ? amax : (r * r) / p.nm[vb + bslot];
+                        best = p.lss ? amax : (r * r) / p.nm[rbase + vb + bslot];
                         bo = orp[bslot];
                     } else {
                         best = -1.f;
                         bo = 0x7fffffff;
-                        const float* nmp = p.nm + vb;
+                        const float* nmp = p.nm + rbase + vb;
                         // recompute the squares here instead of keeping 64 of them live
 #pragma unroll
                         for (int i = 0; i < R; ++i) asm volatile("" : "+v"(breg[i]));
@@ -647,7 +701,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             asm volatile("" ::: "memory");
                         }
                     }
-                    bi = vb + bslot;
+                    bi = rbase + vb + bslot;
                 } else {
                     for (int t = rg; t < T; t += nrg) {
                         float* bp = p.B + (int64_t)t * p.BS + w;
@@ -695,8 +749,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     s_cf[tid] = bv;
                     s_im[tid] = bI;
                     s_bv[tid] = bb;
-                    if (!p.rescore) st_c(&p.Cfmax[ww], bv);
-                    if (it == 0 && p.dbg_cfmax) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
+                    if (!p.rescore) publish(tid);
+                    if (it == 0 && p.dbg_cfmax && S == 1) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
                 }
             }
             if (p.rescore) {
@@ -751,9 +805,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         const float c32 = (r * r) / p.nm[t];
                         flip = (cf > p.Th2) != (c32 > p.Th2);
                         s_bv[tid] = b;
-                        s_cf[tid] = cf = c32;
+                        s_cf[tid] = c32;
                     }
-                    st_c(&p.Cfmax[c0 + tid], cf);
+                    publish(tid);
                 }
                 const uint64_t fb = __ballot(flip);
                 if (lane == 0 && wv == 0) {
@@ -763,18 +817,33 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
         }
         mark(1);
-        grid_sync(p.bar, p.G, target, p.err);
+        grid_sync(p.bar, p.G, g, bar_gen, p.err);
         mark(2);
 
-        // ---------------- phase B ----------------
+        // ---------------- phase B (the column block's owner) ----------------
         const int nwin = (c1 - c0) + 2 * M;
-        for (int i = tid; i < nwin; i += PEEL_THREADS) {
-            const int ww = c0 - M + i;
-            float v = -INFINITY;
-            if (ww >= c0 && ww < c1) v = s_cf[ww - c0];
-            else if (ww >= 0 && ww < W) v = ld_c(&p.Cfmax[ww]);
-            s_win[i] = v;
+        if (S == 1) {
+            for (int i = tid; i < nwin; i += PEEL_THREADS) {
+                const int ww = c0 - M + i;
+                float v = -INFINITY;
+                if (ww >= c0 && ww < c1) v = s_cf[ww - c0];
+                else if (ww >= 0 && ww < W) v = ld_c(&p.Cfmax[ww]);
+                s_win[i] = v;
+            }
+        } else if (owner) {
+            // column maxima = max over the subsets' partials; scores are >= 0, so their
+            // bit patterns order as integers (columns outside the batch stay at 0)
+            int* const s_wi = reinterpret_cast<int*>(s_win);
+            for (int i = tid; i < nwin; i += PEEL_THREADS) s_wi[i] = 0;
+            __syncthreads();
+            for (int i = tid; i < nwin * S; i += PEEL_THREADS) {
+                const int s2 = i / nwin, iw = i - s2 * nwin, ww = c0 - M + iw;
+                if (ww >= 0 && ww < W) atomicMax(&s_wi[iw], __float_as_int(ld_c(&p.Pcf[(int64_t)s2 * W + ww])));
+            }
         }
+        if (!owner) {
+            // the owner accepts and publishes; nothing to do until the barrier
+        } else {
         if (tid == 0) s_ncand = 0;
         __syncthreads();
         if (tid < CW) {
@@ -785,6 +854,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 const float cfw = s_win[tid + M];
                 f = p.lss ? (cm >= p.Th2 && (cm - cfw) < 1e-4f) : (cm > p.Th2 && fabsf(cm - cfw) < 1e-9f);
                 s_cm[tid] = cm;
+                if (S > 1 && it == 0 && p.dbg_cfmax) {
+                    float v; int t; float b;
+                    subset_winner(c0 + tid, v, t, b);
+                    p.dbg_cfmax[c0 + tid] = v;
+                    p.dbg_imax[c0 + tid] = t;
+                }
             }
             s_flag[tid] = f;
         }
@@ -835,14 +910,50 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 for (int k = 0; k < wv; ++k) base += s_hcnt_w[k];
                 if (f) {
                     const int idx = base + __popcll(bal & ((1ull << lane) - 1ull));
-                    const int t = s_im[tid];
-                    const Spike sp{c0 + tid, t, s_bv[tid] / p.nm[t], it};
-                    if (idx < p.slot_cap) s_sp[idx] = sp;
-                    const int ln = s_listn + idx;
-                    if (ln < MAX_LIST) s_list[ln] = sp;
+                    if (S == 1) {
+                        const int t = s_im[tid];
+                        const Spike sp{c0 + tid, t, s_bv[tid] / p.nm[t], it};
+                        if (idx < p.slot_cap) s_sp[idx] = sp;
+                        const int ln = s_listn + idx;
+                        if (ln < MAX_LIST) s_list[ln] = sp;
+                    } else {
+                        s_sp[idx] = Spike{c0 + tid, 0, 0.f, it};   // idx < CW <= MAX_STAGE
+                    }
                 }
             }
             __syncthreads();
+            if (S > 1) {
+                // template and amplitude of each accepted column from the subsets'
+                // partials: one wave per column, one lane per subset, then a
+                // butterfly argmax (largest score, smaller file index on ties)
+                int n = 0;
+                for (int k = 0; k < nwv; ++k) n += s_hcnt_w[k];
+                for (int q = wv; q < n; q += PEEL_THREADS / 64) {
+                    const int ww = s_sp[q].w;
+                    float v = -1.f, b = 0.f;
+                    int t = 0, o = 0x7fffffff;
+                    if (lane < S) {
+                        const int64_t k = (int64_t)lane * W + ww;
+                        v = ld_c(&p.Pcf[k]);
+                        t = ld_c(&p.Pim[k]);
+                        b = ld_c(&p.Pbv[k]);
+                        o = p.orig[t];
+                    }
+                    for (int m = 32; m > 0; m >>= 1) {
+                        const float v2 = __shfl_xor(v, m), b2 = __shfl_xor(b, m);
+                        const int t2 = __shfl_xor(t, m), o2 = __shfl_xor(o, m);
+                        if (v2 > v || (v2 == v && o2 < o)) { v = v2; b = b2; t = t2; o = o2; }
+                    }
+                    if (lane == 0) {
+                        s_sp[q].t = t;
+                        s_sp[q].amp = b / p.nm[t];
+                    }
+                }
+                __syncthreads();
+                for (int i = tid; i < min(n, p.slot_cap); i += PEEL_THREADS)
+                    if (s_listn + i < MAX_LIST) s_list[s_listn + i] = s_sp[i];
+                __syncthreads();        // s_listn is advanced below
+            }
             if (tid == 0) {
                 int n = 0;
                 for (int k = 0; k < nwv; ++k) n += s_hcnt_w[k];
@@ -877,23 +988,24 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         __syncthreads();
         {
             const int n = s_n;
-            for (int i = tid; i < n; i += PEEL_THREADS) st_spike(&p.slots[(int64_t)g * p.slot_cap + i], s_sp[i]);
-            if (tid == 0) st_c(&p.slot_cnt[g], n);
+            for (int i = tid; i < n; i += PEEL_THREADS) st_spike(&p.slots[(int64_t)cb * p.slot_cap + i], s_sp[i]);
+            if (tid == 0) st_c(&p.slot_cnt[cb], n);
         }
+        }   // owner
         mark(3);
-        grid_sync(p.bar, p.G, target, p.err);
+        grid_sync(p.bar, p.G, g, bar_gen, p.err);
         mark(4);
     }
     mark(0);
 
     // ---------------- events and features ----------------
-    const int nmine = min(s_listn, MAX_LIST);
-    for (int i = tid; i < nmine; i += PEEL_THREADS) st_spike(&p.list[(int64_t)g * p.cap_g + i], s_list[i]);
-    if (tid == 0) st_c(&p.list_cnt[g], nmine);
-    grid_sync(p.bar, p.G, target, p.err);
-    {
+    const int nmine = owner ? min(s_listn, MAX_LIST) : 0;
+    for (int i = tid; i < nmine; i += PEEL_THREADS) st_spike(&p.list[(int64_t)cb * p.cap_g + i], s_list[i]);
+    if (tid == 0 && owner) st_c(&p.list_cnt[cb], nmine);
+    grid_sync(p.bar, p.G, g, bar_gen, p.err);
+    if (owner) {
         int part = 0;
-        for (int hh = tid; hh < g; hh += PEEL_THREADS) part += ld_c(&p.list_cnt[hh]);
+        for (int hh = tid; hh < cb; hh += PEEL_THREADS) part += ld_c(&p.list_cnt[hh]);
         s_idx[tid] = part;
         __syncthreads();
         for (int o = PEEL_THREADS / 2; o > 0; o >>= 1) {
@@ -925,7 +1037,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     if (p.do_feat) {
         const int lane = tid & 63, wv = tid >> 6;
         const int K = p.K, nC = p.nC, C = p.C, L2 = 2 * M - 1;
-        const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.G - 1, (c1 - 1 + M - 1) / CW);
+        const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.NB - 1, (c1 - 1 + M - 1) / CW);
         for (int i = wv; i < nmine; i += PEEL_THREADS / 64) {
             const int d = off + s_rank[i];
             if (d >= p.ev_cap || lane >= nC * K) continue;
@@ -934,10 +1046,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             const int ch = p.iCC[j * C + p.iU[si.t]];
             float f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
             for (int hh = flo; hh <= fhi; ++hh) {
-                const int cnt = hh == g ? nmine : ld_c(&p.list_cnt[hh]);
+                const int cnt = hh == cb ? nmine : ld_c(&p.list_cnt[hh]);
                 const Spike* lst = p.list + (int64_t)hh * p.cap_g;
                 for (int q = 0; q < cnt; ++q) {
-                    const Spike sq = hh == g ? s_list[q] : ld_spike(&lst[q]);
+                    const Spike sq = hh == cb ? s_list[q] : ld_spike(&lst[q]);
                     const int dd = si.w - sq.w;
                     if (dd < -(M - 1) || dd > M - 1) continue;
                     float gs = 0.f;
@@ -958,7 +1070,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         atomicAdd(&p.prof[2 * NPHASE], (unsigned long long)s_ph[NPHASE + 1]);
         atomicAdd(&p.prof[2 * NPHASE + 1], 1ull);
     }
-    if (g == p.G - 1 && tid == 0) {
+    if (cb == p.NB - 1 && owner && tid == 0) {
         p.hdr->count = off + nmine;
         p.hdr->n_iter = it_found;
         p.hdr->err = ld_c(p.err);
@@ -971,10 +1083,8 @@ __global__ void k_reset_batch(ResetBatch b) {
     if (j >= b.n) return;
     for (int i = threadIdx.x; i < b.n_iter[j]; i += blockDim.x) b.iter_cnt[j][i] = 0;
     if ((int)threadIdx.x < PROF_WORDS) b.prof[j][threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) {
-        *b.bar[j] = 0u;
-        *b.err[j] = 0;
-    }
+    if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) b.bar[j][threadIdx.x * BAR_LINE] = 0u;
+    if (threadIdx.x == 0) *b.err[j] = 0;
 }
 
 // ---------------------------------------------------------------------------

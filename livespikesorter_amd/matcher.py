@@ -30,12 +30,13 @@ class OnlineSpikesV2:
     def __init__(self, oss_input_dir: str | os.PathLike, *, device: int = 0, chunk: int = 3000,
                  max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
                  precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
-                 keep_stages: bool = False, sparse: bool = True, profile: bool = False):
+                 keep_stages: bool = False, sparse: bool = True, profile: bool = False,
+                 peel_split: int = 0):
         self._h = ct.c_void_p()
         cfg = default_config(device=device, chunk=chunk, max_batch=max_batch, max_iter=max_iter,
                              semantics=semantics, precision=precision, dedup=dedup,
                              max_events=max_events, keep_stages=int(keep_stages), sparse=int(sparse),
-                             profile=int(profile))
+                             profile=int(profile), peel_split=peel_split)
         check(lib().lss_create(os.fsencode(str(oss_input_dir)), ct.byref(cfg), ct.byref(self._h)))
         self.info = self.query()
         self.last_W = 0

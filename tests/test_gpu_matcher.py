@@ -121,6 +121,38 @@ def test_sparse_skipping_is_exact(cfg2):
     assert np.array_equal(fa, fb)
 
 
+def test_template_split_is_exact(cfg2, model_dirs):
+    """Splitting a column block's templates over several workgroups changes no bit."""
+    m, d, tab, x = cfg2
+    batch = x[:3122]
+    with OnlineSpikesV2(d, peel_split=1) as a, OnlineSpikesV2(d) as b, \
+            OnlineSpikesV2(d, peel_split=2, keep_stages=True) as c:
+        ea, fa = a.process_batch(batch)
+        eb, fb = b.process_batch(batch)
+        ec, fc = c.process_batch(batch)
+        qa, qb, qc = a.query(), b.query(), c.query()
+        cf_c = c.debug_stage(STAGE_CFMAX)
+        im_c = c.debug_stage(STAGE_IMAX)
+    print(f"split {qa.peel_split}/{qb.peel_split}/{qc.peel_split}, rows {qa.peel_rows}/{qb.peel_rows}/"
+          f"{qc.peel_rows}, groups {qa.peel_groups}/{qb.peel_groups}/{qc.peel_groups}")
+    assert qa.peel_split == 1 and qb.peel_split > 1 and qc.peel_split == 2
+    assert np.array_equal(ea, eb) and np.array_equal(fa, fb)
+    assert np.array_equal(ea, ec) and np.array_equal(fa, fc)
+    # first-iteration column maxima assembled from the subsets' partials
+    ref = _oracle(tab, batch, stages=True, features=False)
+    assert np.abs(cf_c - ref.Cfmax).max() <= 1e-4 * np.abs(ref.Cfmax).max()
+    assert np.mean(im_c == ref.imax) >= 0.999
+    # 1024 templates, 1500-sample chunks: eight subsets of 128 rows
+    m5, d5 = model_dirs("cfg2", T=1024)
+    x5, _ = synth.make_recording(m5, 1622 * 2, seed=synth.TEST_SEED)
+    with OnlineSpikesV2(d5, max_batch=1622, peel_split=1) as a, OnlineSpikesV2(d5, max_batch=1622) as b:
+        for j in range(2):
+            ea, fa = a.process_batch(x5[j * 1500:j * 1500 + 1622])
+            eb, fb = b.process_batch(x5[j * 1500:j * 1500 + 1622])
+            assert np.array_equal(ea, eb) and np.array_equal(fa, fb)
+        assert b.query().peel_split == 8
+
+
 def test_deterministic(cfg2):
     m, d, tab, x = cfg2
     batch = x[3000:6122]
