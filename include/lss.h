@@ -88,6 +88,10 @@ typedef struct lss_info {
     int32_t peel_rows;      /*   score rows per thread held in registers (0: in memory), */
     int32_t peel_split;     /*   template subsets per column block, */
     int32_t peel_groups;    /*   and workgroups */
+    int64_t gate_tiles_scored;   /* proposal gate, running totals: score tiles computed, */
+    int64_t gate_tiles;          /*   score tiles in the batches, */
+    int64_t gate_fallback_blocks; /*  column blocks scored in full by the fallback, */
+    int64_t gate_blocks;         /*   and column blocks */
 } lss_info;
 
 /* Fill *cfg with defaults. */
@@ -144,11 +148,12 @@ int lss_query(struct lss_handle* h, lss_info* info);
 int lss_debug_stage(struct lss_handle* h, int32_t stage, void* out, int64_t* bytes);
 
 /* Device time per stage accumulated over the batches run with cfg.profile set:
- * ms[0] input copy, ms[1] whitening, ms[2] PC conv, ms[3] projection, ms[4] peel
- * loop with events and features (LSS_NSTAGES entries); *batches = batches timed.
+ * ms[0] input copy, ms[1] whitening, ms[2] proposal network and gate (0 without
+ * one), ms[3] PC conv, ms[4] projection, ms[5] peel loop with events and
+ * features (LSS_NSTAGES entries); *batches = batches timed.
  * reset != 0 clears the accumulators after reading.  No reference counterpart:
  * the reference's per-stage Timer scopes print nothing (src/Helpers/Timer.h:12-16). */
-#define LSS_NSTAGES 5
+#define LSS_NSTAGES 6
 int lss_stage_times(struct lss_handle* h, double* ms, int64_t* batches, int32_t reset);
 
 int lss_destroy(struct lss_handle* h);
@@ -172,6 +177,39 @@ int lss_group_process(struct lss_group* g, const int16_t* const* samples, const 
                       const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
 int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
 int lss_group_destroy(struct lss_group* g);
+
+/* ---- spike-proposal network (a learned gate; no reference counterpart) ----
+ * A 4-layer 1-D convolution (32 channels, kernel 9, BF16 weights and
+ * activations, FP32 accumulation) run on the whitened batch per 16-channel
+ * group of the probe (channels 16g-4 .. 16g+19), giving P(spike) per (group,
+ * sample).  Attached to a handle, it gates which 64-template x 64-column
+ * score tiles are computed: per column block, a group whose largest P within
+ * +-M columns is >= hi marks the template tiles of its own and adjacent
+ * groups; a block where any group lies strictly between lo and hi is scored
+ * in full (the fallback).  Trained on the GPU by lss_proposal_train_step. */
+struct lss_proposal;
+int32_t lss_proposal_n_params(void);
+/* channels: probe channels of the inputs given to _forward/_grad; max_batch:
+ * their widest batch.  Weights are He-initialised from seed. */
+int lss_proposal_create(int32_t device, int32_t channels, int32_t max_batch, uint64_t seed,
+                        struct lss_proposal** out);
+int lss_proposal_set_params(struct lss_proposal* net, const float* params, int64_t n);
+int lss_proposal_get_params(struct lss_proposal* net, float* params, int64_t n);
+/* xw: host whitened batch [channels][n]; prob: host [groups][n]. */
+int lss_proposal_forward(struct lss_proposal* net, const float* xw, int32_t n, float* prob);
+/* Loss (weighted binary cross-entropy with logits, positives weighted by
+ * pos_weight, mean over groups x samples) and its gradient [n_params] for
+ * labels [groups][n] (0/1 bytes). */
+int lss_proposal_grad(struct lss_proposal* net, const float* xw, const uint8_t* labels, int32_t n,
+                      float pos_weight, float* loss, float* grad);
+/* One Adam step (beta 0.9 / 0.999) on a batch that handle h whitens on the
+ * device (d_samples: framed int16 batch [n][ld] in device memory). */
+int lss_proposal_train_step(struct lss_proposal* net, struct lss_handle* h, const int16_t* d_samples, int64_t n,
+                            int32_t ld, const uint8_t* labels, float pos_weight, float lr, float* loss);
+int lss_proposal_destroy(struct lss_proposal* net);
+/* Attach (net != NULL) or detach the gate of a handle (Kilosort semantics).
+ * The handle keeps a reference: destroy the network after the handle. */
+int lss_set_proposal(struct lss_handle* h, struct lss_proposal* net, float lo, float hi);
 
 /* Message of the last error on this thread ("" if none). */
 const char* lss_last_error(void);

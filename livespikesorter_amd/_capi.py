@@ -30,9 +30,11 @@ PREC_BF16_RESCORE = 1
 ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_process_batch_device",
                  "lss_process_chunk", "lss_flush", "lss_query", "lss_debug_stage", "lss_stage_times",
                  "lss_destroy", "lss_last_error", "lss_group_create", "lss_group_process",
-                 "lss_group_query", "lss_group_destroy")
-NSTAGES = 5
-STAGE_NAMES = ("h2d_copy", "whiten", "pcconv", "project", "peel")
+                 "lss_group_query", "lss_group_destroy", "lss_proposal_n_params", "lss_proposal_create",
+                 "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
+                 "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal")
+NSTAGES = 6
+STAGE_NAMES = ("h2d_copy", "whiten", "proposal", "pcconv", "project", "peel")
 
 
 class Config(ct.Structure):
@@ -60,7 +62,9 @@ class Info(ct.Structure):
                 ("last_ms", ct.c_double), ("white_kblocks", ct.c_int64), ("proj_kblocks", ct.c_int64),
                 ("proj_kblocks_dense", ct.c_int64), ("precision", ct.c_int32), ("reserved0", ct.c_int32),
                 ("rescored", ct.c_int64), ("rescore_flips", ct.c_int64), ("peel_cw", ct.c_int32),
-                ("peel_rows", ct.c_int32), ("peel_split", ct.c_int32), ("peel_groups", ct.c_int32)]
+                ("peel_rows", ct.c_int32), ("peel_split", ct.c_int32), ("peel_groups", ct.c_int32),
+                ("gate_tiles_scored", ct.c_int64), ("gate_tiles", ct.c_int64),
+                ("gate_fallback_blocks", ct.c_int64), ("gate_blocks", ct.c_int64)]
 
 
 class LssError(RuntimeError):
@@ -100,6 +104,18 @@ def lib() -> ct.CDLL:
                                         ct.POINTER(ct.c_int64), ct.POINTER(ct.c_void_p)]
         L.lss_group_query.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(Info)]
         L.lss_group_destroy.argtypes = [ct.c_void_p]
+        L.lss_proposal_n_params.restype = ct.c_int32
+        L.lss_proposal_n_params.argtypes = []
+        L.lss_proposal_create.argtypes = [ct.c_int32, ct.c_int32, ct.c_int32, ct.c_uint64, ct.POINTER(ct.c_void_p)]
+        L.lss_proposal_set_params.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
+        L.lss_proposal_get_params.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
+        L.lss_proposal_forward.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_void_p]
+        L.lss_proposal_grad.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_float,
+                                        ct.POINTER(ct.c_float), ct.c_void_p]
+        L.lss_proposal_train_step.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32,
+                                              ct.c_void_p, ct.c_float, ct.c_float, ct.POINTER(ct.c_float)]
+        L.lss_proposal_destroy.argtypes = [ct.c_void_p]
+        L.lss_set_proposal.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_float]
         _LIB = L
     return _LIB
 

@@ -15,11 +15,13 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "lss_kernels.h"
+#include "lss_proposal.h"
 #include "npy.hpp"
 
 using namespace lss;
@@ -114,6 +116,31 @@ void expect_shape(const NpyArray& a, std::vector<int64_t> want, const std::strin
 
 }  // namespace
 
+namespace {
+// device activations of the proposal network for batches up to max_W columns
+void alloc_pn_buffers(PnBuffers& b, int G, int max_W);
+void free_pn_buffers(PnBuffers& b);
+}  // namespace
+
+struct lss_proposal {
+    int dev = 0, C = 0, G = 0, max_W = 0;
+    hipStream_t stream = nullptr;
+    PnWeights wt{};
+    float *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr;
+    float* d_xw = nullptr;          // host-input staging [C][max_W]
+    uint8_t* d_y = nullptr;         // labels [G][Wp]
+    PnBuffers buf{};
+    int step = 0;
+    ~lss_proposal() {
+        (void)hipSetDevice(dev);
+        void* ptrs[] = {wt.params, wt.wf1, wt.wf2, wt.wf3, wt.wb2, wt.wb3, d_m, d_v, d_grad, d_xw, d_y};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        free_pn_buffers(buf);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 struct lss_handle {
     lss_config cfg{};
     int dev = 0;
@@ -149,6 +176,14 @@ struct lss_handle {
     uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
     float* d_Binit = nullptr;
     unsigned long long* d_rescnt = nullptr;
+    // proposal gate (lss_set_proposal): the network, this handle's activations and the tile mask
+    lss_proposal* prop = nullptr;
+    PnBuffers pnb{};
+    PnGate gate{};
+    uint8_t* d_gate = nullptr;
+    int* d_mt_glo = nullptr;
+    int* d_mt_ghi = nullptr;
+    unsigned long long* d_gate_stats = nullptr;
     int slot_cap = 0, cap_g = 0, max_groups = 0;
     // template split of the peel: per-subset column maxima [max_split][Wmax]
     int max_split = 1;
@@ -171,6 +206,7 @@ struct lss_handle {
     int ev_cap = 0;
 
     std::vector<int> orig, cmap_default, cmap_cur;
+    std::vector<int> iU_int;         // main channel of each template, internal order
     int last_W = 0;
 
     // streaming state
@@ -195,10 +231,12 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_Pcf, d_Pbv, d_Pim};
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_Pcf, d_Pbv, d_Pim,
+                        d_gate, d_mt_glo, d_mt_ghi, d_gate_stats};
         (void)hipSetDevice(dev);
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
+        free_pn_buffers(pnb);
         if (h_out) (void)hipHostFree(h_out);
         for (auto& e : evt)
             if (e) (void)hipEventDestroy(e);
@@ -209,6 +247,33 @@ struct lss_handle {
 };
 
 namespace {
+
+void alloc_pn_buffers(PnBuffers& b, int G, int max_W) {
+    b.G = G;
+    b.max_W = max_W;
+    b.Wp = (int)round_up(max_W, PN_SLAB);
+    const size_t n1 = (size_t)G * b.Wp;
+    b.a0 = dalloc<__bf16>(n1 * PN_CIN);
+    b.a1 = dalloc<__bf16>(n1 * PN_CH);
+    b.a2 = dalloc<__bf16>(n1 * PN_CH);
+    b.a3 = dalloc<__bf16>(n1 * PN_CH);
+    b.z = dalloc<float>(n1);
+    b.p = dalloc<float>(n1);
+    b.dz = dalloc<float>(n1);
+    b.d3 = dalloc<__bf16>(n1 * PN_CH);
+    b.d2 = dalloc<__bf16>(n1 * PN_CH);
+    b.d1 = dalloc<__bf16>(n1 * PN_CH);
+    b.part = dalloc<float>(pn_part_floats(G, b.Wp));
+    b.bpart = dalloc<float>((size_t)G * (b.Wp / PN_SLAB) * PN_CH);
+    b.lpart = dalloc<float>((n1 + 255) / 256);
+}
+
+void free_pn_buffers(PnBuffers& b) {
+    void* ptrs[] = {b.a0, b.a1, b.a2, b.a3, b.z, b.p, b.dz, b.d3, b.d2, b.d1, b.part, b.bpart, b.lpart};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    b = PnBuffers{};
+}
 
 // K-block lists for block-sparse GEMMs: for each 64-row tile of A [rows][cols]
 // (row-major, cols a multiple of 16), the 16-column blocks holding any nonzero.
@@ -498,6 +563,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     }
     h->d_iCC = dalloc<int>(iCCi.size()); upload(h->d_iCC, iCCi);
     h->d_iU = dalloc<int>(T); upload(h->d_iU, iUi);
+    h->iU_int = iUi;
     h->d_cmap = dalloc<int>(CP);
     h->d_kbw_ptr = dalloc<int>(kbw_ptr.size()); upload(h->d_kbw_ptr, kbw_ptr);
     h->d_kbw_idx = dalloc<int>(kbw_idx.size()); upload(h->d_kbw_idx, kbw_idx);
@@ -628,6 +694,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     pj = ProjectJob{};
     pj.A = h->d_U2; pj.Bm = h->d_Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = h->d_B;
     pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
+    if (h->prop) { pj.gate = h->d_gate; pj.gate_mt = h->TP / GEMM_BM; }
     if (h->bf16) {
         pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
         pj.Bmb = reinterpret_cast<const __bf16*>(h->d_Bpcb);
@@ -666,6 +733,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
     p.prof = h->d_prof;
+    if (h->prop) { p.gate = h->d_gate; p.gate_mt = h->TP / GEMM_BM; }
     if (h->bf16) {
         // candidates within 10% of threshold on the score scale are decided in FP32
         p.rescore = 1;
@@ -724,6 +792,12 @@ void check_peel(lss_handle* h, int W) {
 
 namespace {
 
+// Proposal network on the whitened batch, then the tile mask.
+void run_gate(lss_handle* h, int W, hipStream_t s) {
+    pn_forward(h->prop->wt, h->pnb, h->d_Xw, h->XS, XOFF, h->C, W, s);
+    pn_gate(h->pnb, h->gate, W, h->M, s);
+}
+
 // Run one framed batch that is already on the device.
 void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t first_sample) {
     if (W <= 0 || W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) +
@@ -747,15 +821,17 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, h->d_prof, s);
     launch_whiten(wb, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
-    launch_pcconv(cb, h->K, s);
+    if (h->prop) run_gate(h, W, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
-    launch_project(pb, h->bf16, s);
+    launch_pcconv(cb, h->K, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
+    launch_project(pb, h->bf16, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
     HIPCHK(hipGetLastError());
     if (h->cfg.keep_stages)
         HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
     HIPCHK(launch_peel(kb, R, h->coop, s));
-    if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
+    if (prof) HIPCHK(hipEventRecord(h->evt[6], s));
     HIPCHK(hipStreamSynchronize(s));
     if (prof) {
         float ms = 0.f;
@@ -1006,6 +1082,16 @@ int lss_query(lss_handle* h, lss_info* info) {
         i.nt0min = h->nt0min; i.dt = h->dt; i.semantics = h->sem; i.Th = h->Th;
         i.max_batch = h->Wmax; i.chunk = h->cfg.chunk; i.max_iter = h->max_iter;
         i.precision = h->bf16 ? LSS_PREC_BF16_RESCORE : LSS_PREC_FP32;
+        if (h->d_gate_stats) {
+            unsigned long long c[4];
+            HIPCHK(hipSetDevice(h->dev));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipMemcpy(c, h->d_gate_stats, sizeof(c), hipMemcpyDeviceToHost));
+            i.gate_tiles_scored = (int64_t)c[0];
+            i.gate_tiles = (int64_t)c[1];
+            i.gate_fallback_blocks = (int64_t)c[2];
+            i.gate_blocks = (int64_t)c[3];
+        }
         if (h->bf16) {
             unsigned long long c[2];
             HIPCHK(hipSetDevice(h->dev));
@@ -1168,6 +1254,8 @@ void group_capture(lss_group* g, const int32_t* ld) {
     HIPCHK(hipMemcpyAsync(g->d_xptr, g->h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
     launch_reset_batch(g->rb, st);
     launch_whiten(g->wb, st);
+    for (int s2 = 0; s2 < n; ++s2)
+        if (g->hs[s2]->prop) run_gate(g->hs[s2].get(), g->W, st);
     launch_pcconv(g->cb, g->hs[0]->K, st);
     launch_project(g->pb, g->hs[0]->bf16, st);
     HIPCHK(launch_peel(g->kb, g->R, false, st));
@@ -1266,6 +1354,201 @@ int lss_group_query(lss_group* g, int32_t stream, lss_info* info) {
 
 int lss_group_destroy(lss_group* g) {
     return guarded([&] { delete g; });
+}
+
+// ---- spike-proposal network ----
+
+int32_t lss_proposal_n_params(void) { return PN_NPARAMS; }
+
+int lss_proposal_create(int32_t device, int32_t channels, int32_t max_batch, uint64_t seed, lss_proposal** out) {
+    return guarded([&] {
+        if (!out || channels <= 0 || max_batch <= 0) throw LssError(LSS_ERR_ARG, "bad argument");
+        if (pn_groups(channels) > 64) throw LssError(LSS_ERR_ARG, "at most 1024 channels");
+        *out = nullptr;
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw LssError(LSS_ERR_ARG, "no such HIP device");
+        HIPCHK(hipSetDevice(device));
+        auto net = std::make_unique<lss_proposal>();
+        net->dev = device;
+        net->C = channels;
+        net->G = pn_groups(channels);
+        net->max_W = max_batch;
+        HIPCHK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
+        net->wt.params = dalloc<float>(PN_NPARAMS);
+        net->wt.wf1 = dalloc<__bf16>(PN_CH * PN_KP1);
+        net->wt.wf2 = dalloc<__bf16>(PN_CH * PN_KP);
+        net->wt.wf3 = dalloc<__bf16>(PN_CH * PN_KP);
+        net->wt.wb2 = dalloc<__bf16>(PN_CH * PN_KP);
+        net->wt.wb3 = dalloc<__bf16>(PN_CH * PN_KP);
+        net->d_m = dalloc<float>(PN_NPARAMS);
+        net->d_v = dalloc<float>(PN_NPARAMS);
+        net->d_grad = dalloc<float>(PN_NPARAMS);
+        net->d_xw = dalloc<float>((size_t)channels * max_batch);
+        alloc_pn_buffers(net->buf, net->G, max_batch);
+        net->d_y = dalloc<uint8_t>((size_t)net->G * net->buf.Wp);
+        // He initialisation (normal, std sqrt(2 / fan_in)), zero biases
+        std::vector<float> w(PN_NPARAMS, 0.f);
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<float> nd(0.f, 1.f);
+        auto fill = [&](int off, int n, int fan_in) {
+            const float sd = std::sqrt(2.f / (float)fan_in);
+            for (int i = 0; i < n; ++i) w[off + i] = sd * nd(rng);
+        };
+        fill(PN_W1, PN_B1 - PN_W1, PN_CIN * PN_KW);
+        fill(PN_W2, PN_B2 - PN_W2, PN_CH * PN_KW);
+        fill(PN_W3, PN_B3 - PN_W3, PN_CH * PN_KW);
+        fill(PN_W4, PN_B4 - PN_W4, PN_CH * PN_KW);
+        HIPCHK(hipMemcpy(net->wt.params, w.data(), sizeof(float) * PN_NPARAMS, hipMemcpyHostToDevice));
+        pn_pack(net->wt, net->stream);
+        HIPCHK(hipStreamSynchronize(net->stream));
+        *out = net.release();
+    });
+}
+
+int lss_proposal_set_params(lss_proposal* net, const float* params, int64_t n) {
+    return guarded([&] {
+        if (!net || !params || n != PN_NPARAMS) throw LssError(LSS_ERR_ARG, "expected lss_proposal_n_params() values");
+        HIPCHK(hipSetDevice(net->dev));
+        HIPCHK(hipMemcpy(net->wt.params, params, sizeof(float) * PN_NPARAMS, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(net->d_m, 0, sizeof(float) * PN_NPARAMS));
+        HIPCHK(hipMemset(net->d_v, 0, sizeof(float) * PN_NPARAMS));
+        net->step = 0;
+        pn_pack(net->wt, net->stream);
+        HIPCHK(hipStreamSynchronize(net->stream));
+    });
+}
+
+int lss_proposal_get_params(lss_proposal* net, float* params, int64_t n) {
+    return guarded([&] {
+        if (!net || !params || n != PN_NPARAMS) throw LssError(LSS_ERR_ARG, "expected lss_proposal_n_params() values");
+        HIPCHK(hipSetDevice(net->dev));
+        HIPCHK(hipStreamSynchronize(net->stream));
+        HIPCHK(hipMemcpy(params, net->wt.params, sizeof(float) * PN_NPARAMS, hipMemcpyDeviceToHost));
+    });
+}
+
+namespace {
+void pn_check_width(const lss_proposal* net, int32_t n) {
+    if (n <= 0 || n > net->max_W) throw LssError(LSS_ERR_ARG, "batch width outside 1..max_batch");
+}
+void pn_upload_xw(lss_proposal* net, const float* xw, int32_t n) {
+    HIPCHK(hipMemcpyAsync(net->d_xw, xw, sizeof(float) * (size_t)net->C * n, hipMemcpyHostToDevice, net->stream));
+}
+void pn_upload_labels(lss_proposal* net, const uint8_t* y, int32_t n) {
+    HIPCHK(hipMemsetAsync(net->d_y, 0, (size_t)net->G * net->buf.Wp, net->stream));
+    HIPCHK(hipMemcpy2DAsync(net->d_y, net->buf.Wp, y, n, n, net->G, hipMemcpyHostToDevice, net->stream));
+}
+float pn_loss(lss_proposal* net) {
+    const int nb = (net->G * net->buf.Wp + 255) / 256;
+    std::vector<float> lp(nb);
+    HIPCHK(hipMemcpyAsync(lp.data(), net->buf.lpart, sizeof(float) * nb, hipMemcpyDeviceToHost, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    double l = 0.0;
+    for (float v : lp) l += v;
+    return (float)l;
+}
+}  // namespace
+
+int lss_proposal_forward(lss_proposal* net, const float* xw, int32_t n, float* prob) {
+    return guarded([&] {
+        if (!net || !xw || !prob) throw LssError(LSS_ERR_ARG, "null argument");
+        pn_check_width(net, n);
+        HIPCHK(hipSetDevice(net->dev));
+        pn_upload_xw(net, xw, n);
+        pn_forward(net->wt, net->buf, net->d_xw, n, 0, net->C, n, net->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy2DAsync(prob, sizeof(float) * n, net->buf.p, sizeof(float) * net->buf.Wp, sizeof(float) * n,
+                                net->G, hipMemcpyDeviceToHost, net->stream));
+        HIPCHK(hipStreamSynchronize(net->stream));
+    });
+}
+
+int lss_proposal_grad(lss_proposal* net, const float* xw, const uint8_t* labels, int32_t n, float pos_weight,
+                      float* loss, float* grad) {
+    return guarded([&] {
+        if (!net || !xw || !labels || !grad) throw LssError(LSS_ERR_ARG, "null argument");
+        pn_check_width(net, n);
+        HIPCHK(hipSetDevice(net->dev));
+        pn_upload_xw(net, xw, n);
+        pn_upload_labels(net, labels, n);
+        pn_forward(net->wt, net->buf, net->d_xw, n, 0, net->C, n, net->stream);
+        pn_backward(net->wt, net->buf, net->d_y, n, pos_weight, net->d_grad, net->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grad, net->d_grad, sizeof(float) * PN_NPARAMS, hipMemcpyDeviceToHost, net->stream));
+        const float l = pn_loss(net);
+        if (loss) *loss = l;
+    });
+}
+
+int lss_proposal_train_step(lss_proposal* net, lss_handle* h, const int16_t* d_samples, int64_t n, int32_t ld,
+                            const uint8_t* labels, float pos_weight, float lr, float* loss) {
+    return guarded([&] {
+        if (!net || !h || !d_samples || !labels) throw LssError(LSS_ERR_ARG, "null argument");
+        if (n <= 0 || n > std::min<int64_t>(net->max_W, h->Wmax)) throw LssError(LSS_ERR_ARG, "batch too wide");
+        if (h->dev != net->dev) throw LssError(LSS_ERR_ARG, "network and handle on different devices");
+        if (pn_groups(h->C) != net->G) throw LssError(LSS_ERR_ARG, "network built for another channel count");
+        HIPCHK(hipSetDevice(net->dev));
+        set_cmap(h, nullptr, ld);
+        BatchPlan bp;
+        plan_batch(h, d_samples, nullptr, (int)n, ld, MIN_CW, 0, 1, bp);
+        WhitenBatch wb{};
+        wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
+        launch_whiten(wb, h->stream);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        pn_upload_labels(net, labels, (int32_t)n);
+        pn_forward(net->wt, net->buf, h->d_Xw, h->XS, XOFF, h->C, (int)n, net->stream);
+        pn_backward(net->wt, net->buf, net->d_y, (int)n, pos_weight, net->d_grad, net->stream);
+        net->step += 1;
+        pn_adam(net->wt, net->d_grad, net->d_m, net->d_v, lr, 0.9f, 0.999f, 1e-8f, net->step, net->stream);
+        pn_pack(net->wt, net->stream);
+        HIPCHK(hipGetLastError());
+        const float l = pn_loss(net);
+        if (loss) *loss = l;
+    });
+}
+
+int lss_proposal_destroy(lss_proposal* net) {
+    return guarded([&] { delete net; });
+}
+
+int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi) {
+    return guarded([&] {
+        if (!h) throw LssError(LSS_ERR_ARG, "null handle");
+        HIPCHK(hipSetDevice(h->dev));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (!net) { h->prop = nullptr; return; }
+        if (h->sem != LSS_SEM_KILOSORT) throw LssError(LSS_ERR_ARG, "the proposal gate is defined for Kilosort semantics");
+        if (net->dev != h->dev) throw LssError(LSS_ERR_ARG, "network and handle on different devices");
+        if (!(lo >= 0.f && lo < hi && hi <= 1.f)) throw LssError(LSS_ERR_ARG, "need 0 <= lo < hi <= 1");
+        const int G = pn_groups(h->C);
+        if (G > 64) throw LssError(LSS_ERR_ARG, "at most 1024 channels");
+        if (!h->pnb.a0) alloc_pn_buffers(h->pnb, G, h->Wmax);
+        const int MT = h->TP / GEMM_BM;
+        if (!h->d_gate) {
+            const int NBmax = (h->Wmax + 63) / 64;
+            h->d_gate = dalloc<uint8_t>((size_t)NBmax * MT);
+            // group span of each 64-template tile (internal order, sorted by main channel)
+            std::vector<int> glo(MT, 1 << 20), ghi(MT, -(1 << 20));
+            for (int tp = 0; tp < h->T; ++tp) {
+                const int m = tp / GEMM_BM;
+                const int g = h->iU_int[tp] / PN_GROUP;
+                glo[m] = std::min(glo[m], g);
+                ghi[m] = std::max(ghi[m], g);
+            }
+            h->d_mt_glo = dalloc<int>(MT); upload(h->d_mt_glo, glo);
+            h->d_mt_ghi = dalloc<int>(MT); upload(h->d_mt_ghi, ghi);
+            h->d_gate_stats = dalloc<unsigned long long>(4);
+        }
+        h->gate.mask = h->d_gate;
+        h->gate.mt_glo = h->d_mt_glo;
+        h->gate.mt_ghi = h->d_mt_ghi;
+        h->gate.MT = MT;
+        h->gate.lo = lo;
+        h->gate.hi = hi;
+        h->gate.stats = h->d_gate_stats;
+        h->prop = net;
+    });
 }
 
 }  // extern "C"

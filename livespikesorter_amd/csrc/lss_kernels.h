@@ -108,6 +108,8 @@ struct PeelParams {
     const int* kb_ptr; const int* kb_idx;
     const float* Binit;                // the BF16 projection before any subtraction
     unsigned long long* n_rescored;    // running totals: FP32 rescores, threshold decisions flipped
+    // proposal gate: score tile (64 columns, 64 templates) scored iff gate[(w/64) * gate_mt + t/64]
+    const uint8_t* gate; int gate_mt;
 };
 
 // ---- batched launch descriptors (kernel arguments) ----
@@ -123,6 +125,7 @@ struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
     const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
     float* C2;                            // optional second copy of the result
+    const uint8_t* gate; int gate_mt;     // optional tile mask [ntile][gate_mt]: 0 = not computed
     int lda, ldb, ldc, mtiles, ntiles;
 };
 struct PcconvJob {

@@ -138,6 +138,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     const ProjectJob& jb = bt.job[j];
     int mt, nt;
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
     const float* __restrict__ A = jb.A;
     const float* __restrict__ Bm = jb.Bm;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     const ProjectJob& jb = bt.job[j];
     int mt, nt;
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
     const __bf16* __restrict__ A = jb.Ab;
     const __bf16* __restrict__ Bm = jb.Bmb;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -577,6 +579,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             // A wave whose rows no spike touched keeps last pass's per-thread
             // maxima in LDS (its registers did not change): it skips the scan.
             bool wdirty = it == 0 || R == 0;
+            // a wave whose 64-template tile the gate did not propose for this column
+            // block scores zero everywhere (its stale rows are neither updated nor read)
+            const bool woff = R > 0 && p.gate && !p.gate[cb * p.gate_mt + (rbase + rg * R) / GEMM_BM];
             if (R > 0) {
                 // Runs with the whole wave active (every lane tests spikes for the ballot);
                 // edge columns and columns past W are updated too but never read.
@@ -595,7 +600,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     // this wave's spikes whose band reaches its row block, compacted with a ballot
                     const int qq = q0 + lane;
                     bool bh = false;
-                    if (qq < nsp) {
+                    if (qq < nsp && !woff) {
                         const int st = s_sp[qq].t;
                         bh = r0 < p.thi[st] && r0 + R > p.tlo[st];
                     }
@@ -639,6 +644,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
             if (!wdirty) {
                 // nothing to rescan
+            } else if (own && woff) {
+                best = 0.f; bi = rbase + rg * R; bo = 0x7fffffff; bval = 0.f;
             } else if (own && edge) {
                 // Cf is zero for every template: first file index wins the tie
                 best = 0.f; bi = p.t_first; bo = 0;
@@ -704,6 +711,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     bi = rbase + vb + bslot;
                 } else {
                     for (int t = rg; t < T; t += nrg) {
+                        if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) {
+                            if (better(0.f, p.orig[t], best, bo)) { best = 0.f; bi = t; bo = p.orig[t]; bval = 0.f; }
+                            continue;
+                        }
                         float* bp = p.B + (int64_t)t * p.BS + w;
                         float b = *bp;
                         bool changed = false;

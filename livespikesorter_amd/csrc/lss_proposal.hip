@@ -1,0 +1,449 @@
+// Spike-proposal network kernels for gfx950: BF16 activations and weights, FP32
+// accumulation on MFMA (v_mfma_f32_32x32x16_bf16), FP32 master weights.
+//
+// A layer is a GEMM: out[co][w] = sum_{k' < CI*9} Wt[co][k'] * X[k'][w] with the
+// im2col operand X[ci*9 + k][w] = in[ci][w + k - 4] gathered on the fly from an
+// LDS tile of the input (zero outside the batch).  One wave computes a 32 x 32
+// (channel x column) tile; lane l holds A[l % 32][8 (l / 32) + e] and
+// B[8 (l / 32) + e][l % 32], e = 0..7, and D element r of lane l is
+// (row (r % 4) + 8 (r / 4) + 4 (l / 32), column l % 32).
+// The data gradient of a layer is the same GEMM with the weights transposed and
+// the taps reversed; the weight gradient is a GEMM over columns with per-slab
+// partials summed in a fixed order (deterministic).
+#include "lss_proposal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace lss {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TW = PN_TILE + PN_KW - 1;      // input tile columns of a convolution workgroup
+
+// ---------------------------------------------------------------------------
+// input: group g, row ci = whitened channel 16 g - 4 + ci (zero off the probe)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pn_input(const float* __restrict__ xw, int64_t xs, int xoff, int C,
+                                                  int W, int Wp, __bf16* __restrict__ a0) {
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    const int row = blockIdx.y;                       // g * PN_CIN + ci
+    if (w >= Wp) return;
+    const int g = row / PN_CIN, ci = row - g * PN_CIN;
+    const int ch = g * PN_GROUP - PN_HALO + ci;
+    float v = 0.f;
+    if (w < W && ch >= 0 && ch < C) v = xw[(int64_t)ch * xs + xoff + w];
+    a0[(int64_t)row * Wp + w] = (__bf16)v;
+}
+
+// ---------------------------------------------------------------------------
+// convolution layer, 32 output channels.  MODE 0: + bias, ReLU (forward).
+// MODE 1: multiplied by (mask > 0) (data gradient through the previous ReLU).
+// ---------------------------------------------------------------------------
+template <int CI, int MODE>
+__global__ __launch_bounds__(256) void k_pn_conv(const __bf16* __restrict__ in, const __bf16* __restrict__ wt,
+                                                 const float* __restrict__ bias, const __bf16* __restrict__ mask,
+                                                 __bf16* __restrict__ out, int W, int Wp) {
+    constexpr int KR = CI * PN_KW;
+    constexpr int KP = (KR + 15) / 16 * 16;
+    constexpr int XR = TW + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 xs[CI * XR];
+    __shared__ __attribute__((aligned(16))) __bf16 ws[PN_CH][KP + 8];
+    const int g = blockIdx.y, w0 = blockIdx.x * PN_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const __bf16* src = in + (int64_t)g * CI * Wp;
+    for (int i = tid; i < CI * TW; i += 256) {
+        const int ci = i / TW, c = i - ci * TW, w = w0 - PN_PAD + c;
+        xs[ci * XR + c] = (w >= 0 && w < W) ? src[(int64_t)ci * Wp + w] : (__bf16)0.f;
+    }
+    for (int i = tid; i < PN_CH * (KP / 8); i += 256) {
+        const int r = i / (KP / 8), c8 = i - r * (KP / 8);
+        *reinterpret_cast<uint4*>(&ws[r][c8 * 8]) = *reinterpret_cast<const uint4*>(wt + r * KP + c8 * 8);
+    }
+    __syncthreads();
+    const int h = lane >> 5, j = lane & 31;
+    const int jc = wv * 32 + j;                  // this lane's output column within the tile
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int s = 0; s < KP / 16; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&ws[j][16 * s + 8 * h]);
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            // reduction index of this element for either half of the wave (compile-time)
+            const int k0 = 16 * s + e, k1 = 16 * s + 8 + e;
+            const int o0 = (k0 / PN_KW) * XR + k0 % PN_KW, o1 = (k1 / PN_KW) * XR + k1 % PN_KW;
+            const bool v0 = k0 < KR, v1 = k1 < KR;
+            const bool valid = h ? v1 : v0;
+            const int off = h ? (v1 ? o1 : 0) : (v0 ? o0 : 0);
+            const __bf16 x = xs[off + jc];
+            b[e] = valid ? x : (__bf16)0.f;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int w = w0 + jc;
+    if (w >= Wp) return;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int co = (r & 3) + 8 * (r >> 2) + 4 * h;
+        const int64_t o = ((int64_t)g * PN_CH + co) * Wp + w;
+        float v = acc[r];
+        if (MODE == 0) {
+            v = fmaxf(v + bias[co], 0.f);
+        } else {
+            v = (float)mask[o] > 0.f ? v : 0.f;
+        }
+        out[o] = w < W ? (__bf16)v : (__bf16)0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// head: logit z = b4 + sum_{ci,k} W4[ci][k] a3[ci][w + k - 4]; p = sigmoid(z)
+// (HIP's device expf)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pn_head(const __bf16* __restrict__ a3, const float* __restrict__ params,
+                                                 int W, int Wp, float* __restrict__ z, float* __restrict__ p) {
+    __shared__ float w4[PN_CH * PN_KW + 1];
+    for (int i = threadIdx.x; i < PN_CH * PN_KW + 1; i += 256) w4[i] = params[PN_W4 + i];
+    __syncthreads();
+    const int w = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
+    if (w >= Wp) return;
+    float acc = w4[PN_CH * PN_KW];
+    if (w < W) {
+        for (int ci = 0; ci < PN_CH; ++ci) {
+            const __bf16* row = a3 + ((int64_t)g * PN_CH + ci) * Wp;
+#pragma unroll
+            for (int k = 0; k < PN_KW; ++k) {
+                const int j = w + k - PN_PAD;
+                if (j >= 0 && j < W) acc += w4[ci * PN_KW + k] * (float)row[j];
+            }
+        }
+    }
+    const int64_t o = (int64_t)g * Wp + w;
+    z[o] = w < W ? acc : 0.f;
+    p[o] = w < W ? 1.f / (1.f + expf(-acc)) : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// weighted BCE with logits: dz = wgt (sigmoid(z) - y) / n, loss partial per block
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pn_loss(const float* __restrict__ z, const uint8_t* __restrict__ y,
+                                                 int G, int W, int Wp, float pos_weight, float* __restrict__ dz,
+                                                 float* __restrict__ lpart) {
+    __shared__ float red[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int n = G * W;
+    float l = 0.f;
+    if (i < G * Wp) {
+        const int g = i / Wp, w = i - g * Wp;
+        float d = 0.f;
+        if (w < W) {
+            const float zz = z[i];
+            const float yy = y[(int64_t)g * Wp + w] ? 1.f : 0.f;
+            const float sp_pos = fmaxf(zz, 0.f) + log1pf(expf(-fabsf(zz)));   // softplus(z)
+            const float sp_neg = sp_pos - zz;                                  // softplus(-z)
+            const float wgt = yy > 0.5f ? pos_weight : 1.f;
+            l = pos_weight * yy * sp_neg + (1.f - yy) * sp_pos;
+            d = wgt * (1.f / (1.f + expf(-zz)) - yy) / (float)n;
+        }
+        dz[i] = d;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0] / (float)n;
+}
+
+// data gradient of the head through layer 3's ReLU
+__global__ __launch_bounds__(256) void k_pn_head_back(const float* __restrict__ dz, const float* __restrict__ params,
+                                                      const __bf16* __restrict__ a3, int W, int Wp,
+                                                      __bf16* __restrict__ d3) {
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    const int row = blockIdx.y;                       // g * 32 + ci
+    if (w >= Wp) return;
+    const int g = row / PN_CH, ci = row - g * PN_CH;
+    float v = 0.f;
+    const int64_t o = (int64_t)row * Wp + w;
+    if (w < W && (float)a3[o] > 0.f) {
+        for (int k = 0; k < PN_KW; ++k) {
+            const int j = w - k + PN_PAD;
+            if (j >= 0 && j < W) v += params[PN_W4 + ci * PN_KW + k] * dz[(int64_t)g * Wp + j];
+        }
+    }
+    d3[o] = (__bf16)v;
+}
+
+// weight and bias gradient of the head: one workgroup per input channel (and one for the bias)
+__global__ __launch_bounds__(256) void k_pn_head_wgrad(const float* __restrict__ dz, const __bf16* __restrict__ a3,
+                                                       int G, int W, int Wp, float* __restrict__ grad) {
+    __shared__ float red[PN_KW][256];
+    const int ci = blockIdx.x;                        // PN_CH: the bias
+    float acc[PN_KW];
+#pragma unroll
+    for (int k = 0; k < PN_KW; ++k) acc[k] = 0.f;
+    for (int i = threadIdx.x; i < G * W; i += 256) {
+        const int g = i / W, w = i - g * W;
+        const float d = dz[(int64_t)g * Wp + w];
+        if (ci == PN_CH) { acc[0] += d; continue; }
+        const __bf16* row = a3 + ((int64_t)g * PN_CH + ci) * Wp;
+#pragma unroll
+        for (int k = 0; k < PN_KW; ++k) {
+            const int j = w + k - PN_PAD;
+            if (j >= 0 && j < W) acc[k] += d * (float)row[j];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PN_KW; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < PN_KW; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (ci == PN_CH) grad[PN_B4] = red[0][0];
+        else for (int k = 0; k < PN_KW; ++k) grad[PN_W4 + ci * PN_KW + k] = red[k][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient of a convolution layer over one slab of PN_SLAB columns:
+// part[co][ci*9 + k] = sum_w dy[co][w] x[ci][w + k - 4]; bpart[co] = sum_w dy[co][w]
+// ---------------------------------------------------------------------------
+template <int CI>
+__global__ __launch_bounds__(256) void k_pn_wgrad(const __bf16* __restrict__ dy, const __bf16* __restrict__ x,
+                                                  int W, int Wp, float* __restrict__ part,
+                                                  float* __restrict__ bpart) {
+    constexpr int KR = CI * PN_KW;
+    constexpr int NCH = (KR + 31) / 32;
+    constexpr int NP = NCH * 32;
+    constexpr int YR = PN_SLAB + 8;
+    constexpr int XR = PN_SLAB + PN_KW - 1 + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 ys[PN_CH * YR];
+    __shared__ __attribute__((aligned(16))) __bf16 xs[CI * XR];
+    __shared__ float bred[PN_CH][8];
+    const int g = blockIdx.y, slab = blockIdx.x, w0 = slab * PN_SLAB;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i = tid; i < PN_CH * PN_SLAB; i += 256) {
+        const int co = i / PN_SLAB, c = i - co * PN_SLAB, w = w0 + c;
+        ys[co * YR + c] = w < W ? dy[((int64_t)g * PN_CH + co) * Wp + w] : (__bf16)0.f;
+    }
+    for (int i = tid; i < CI * (PN_SLAB + PN_KW - 1); i += 256) {
+        const int ci = i / (PN_SLAB + PN_KW - 1), c = i - ci * (PN_SLAB + PN_KW - 1), w = w0 - PN_PAD + c;
+        xs[ci * XR + c] = (w >= 0 && w < W) ? x[((int64_t)g * CI + ci) * Wp + w] : (__bf16)0.f;
+    }
+    __syncthreads();
+    const int pid = g * gridDim.x + slab;
+    {   // bias partial: 8 threads per channel, fixed order
+        const int co = tid >> 3, q = tid & 7;
+        float s = 0.f;
+        for (int c = q; c < PN_SLAB; c += 8) s += (float)ys[co * YR + c];
+        bred[co][q] = s;
+        __syncthreads();
+        if (tid < PN_CH) {
+            float t = 0.f;
+            for (int k = 0; k < 8; ++k) t += bred[tid][k];
+            bpart[(int64_t)pid * PN_CH + tid] = t;
+        }
+    }
+    const int h = lane >> 5, j = lane & 31;
+    for (int ch = wv; ch < NCH; ch += 4) {
+        const int n = ch * 32 + j;                 // weight index ci*9 + k of this lane's column
+        const bool nvalid = n < KR;
+        const int ci = nvalid ? n / PN_KW : 0, tap = nvalid ? n % PN_KW : 0;
+        const __bf16* xrow = xs + ci * XR + tap;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll 4
+        for (int s = 0; s < PN_SLAB / 16; ++s) {
+            const int c0 = 16 * s + 8 * h;
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&ys[j * YR + c0]);   // A[co = j][w]
+            bf16x8 b;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) b[e] = nvalid ? xrow[c0 + e] : (__bf16)0.f;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int co = (r & 3) + 8 * (r >> 2) + 4 * h;
+            part[((int64_t)pid * PN_CH + co) * NP + n] = acc[r];
+        }
+    }
+}
+
+// sum of the weight-gradient partials in partial order (deterministic)
+__global__ __launch_bounds__(256) void k_pn_reduce(const float* __restrict__ part, const float* __restrict__ bpart,
+                                                   int nparts, int KR, int NP, float* __restrict__ gw,
+                                                   float* __restrict__ gb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < PN_CH * KR) {
+        const int co = i / KR, n = i - co * KR;
+        float s = 0.f;
+        for (int q = 0; q < nparts; ++q) s += part[((int64_t)q * PN_CH + co) * NP + n];
+        gw[i] = s;
+    } else if (i < PN_CH * KR + PN_CH) {
+        const int co = i - PN_CH * KR;
+        float s = 0.f;
+        for (int q = 0; q < nparts; ++q) s += bpart[(int64_t)q * PN_CH + co];
+        gb[co] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adam on the FP32 master weights
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pn_adam(float* __restrict__ w, const float* __restrict__ g,
+                                                 float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                                 float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= PN_NPARAMS) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 operand copies of the weights (forward and data-gradient layouts)
+__global__ __launch_bounds__(256) void k_pn_pack(PnWeights wt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const float* P = wt.params;
+    // forward: [co][KP] with k' = ci*9 + k; zero past CI*9
+    if (i < PN_CH * PN_KP1) {
+        const int co = i / PN_KP1, kk = i - co * PN_KP1;
+        wt.wf1[i] = (__bf16)(kk < PN_CIN * PN_KW ? P[PN_W1 + co * PN_CIN * PN_KW + kk] : 0.f);
+    }
+    if (i < PN_CH * PN_KP) {
+        const int r = i / PN_KP, kk = i - r * PN_KP;
+        wt.wf2[i] = (__bf16)P[PN_W2 + r * PN_CH * PN_KW + kk];
+        wt.wf3[i] = (__bf16)P[PN_W3 + r * PN_CH * PN_KW + kk];
+        // data gradient: row r = ci, k' = co*9 + k holds W[co][ci][8 - k]
+        const int co = kk / PN_KW, k = kk - co * PN_KW;
+        const int src = (co * PN_CH + r) * PN_KW + (PN_KW - 1 - k);
+        wt.wb2[i] = (__bf16)P[PN_W2 + src];
+        wt.wb3[i] = (__bf16)P[PN_W3 + src];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gate: per 64-column block and group, the largest P within +-M columns of the
+// block; a block with any group strictly between lo and hi falls back to full
+// scoring, otherwise a template tile is scored when a group within one of its
+// group span has P >= hi.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pn_gate(const float* __restrict__ p, int G, int W, int Wp, int M,
+                                                 PnGate gt) {
+    __shared__ int s_state[64];                       // per group: 0 off, 1 uncertain, 2 on
+    __shared__ int s_unc;
+    const int n = blockIdx.x;
+    const int lo = max(0, n * 64 - M), hi = min(W, n * 64 + 64 + M);
+    if (threadIdx.x == 0) s_unc = 0;
+    __syncthreads();
+    for (int g = threadIdx.x >> 6; g < G; g += 4) {  // one wave per group
+        float mx = 0.f;
+        for (int w = lo + (threadIdx.x & 63); w < hi; w += 64) mx = fmaxf(mx, p[(int64_t)g * Wp + w]);
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if ((threadIdx.x & 63) == 0) {
+            const int st = mx >= gt.hi ? 2 : (mx > gt.lo ? 1 : 0);
+            s_state[g] = st;
+            if (st == 1) atomicOr(&s_unc, 1);
+        }
+    }
+    __syncthreads();
+    int scored = 0;
+    for (int m = threadIdx.x; m < gt.MT; m += 256) {
+        bool on = s_unc != 0;
+        if (!on) {
+            const int a = max(0, gt.mt_glo[m] - 1), b = min(G - 1, gt.mt_ghi[m] + 1);
+            for (int g = a; g <= b && !on; ++g) on = s_state[g] == 2;
+        }
+        gt.mask[(int64_t)n * gt.MT + m] = on ? 1 : 0;
+        scored += on;
+    }
+    for (int o = 32; o > 0; o >>= 1) scored += __shfl_xor(scored, o);
+    if ((threadIdx.x & 63) == 0 && scored) atomicAdd(&gt.stats[0], (unsigned long long)scored);
+    if (threadIdx.x == 0) {
+        atomicAdd(&gt.stats[1], (unsigned long long)gt.MT);
+        if (s_unc) atomicAdd(&gt.stats[2], 1ull);
+        atomicAdd(&gt.stats[3], 1ull);
+    }
+}
+
+}  // namespace
+
+int pn_groups(int C) { return (C + PN_GROUP - 1) / PN_GROUP; }
+
+size_t pn_part_floats(int G, int Wp) {
+    const int slabs = (Wp + PN_SLAB - 1) / PN_SLAB;
+    return (size_t)G * slabs * PN_CH * ((PN_CH * PN_KW + 31) / 32 * 32);
+}
+
+void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
+                hipStream_t s) {
+    const int G = b.G, Wp = b.Wp;
+    hipLaunchKernelGGL(k_pn_input, dim3((Wp + 255) / 256, G * PN_CIN), dim3(256), 0, s, xw, xs, xoff, C, W, Wp, b.a0);
+    const dim3 grid((W + PN_TILE - 1) / PN_TILE, G);
+    hipLaunchKernelGGL((k_pn_conv<PN_CIN, 0>), grid, dim3(256), 0, s, b.a0, wt.wf1, wt.params + PN_B1,
+                       (const __bf16*)nullptr, b.a1, W, Wp);
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, b.a1, wt.wf2, wt.params + PN_B2,
+                       (const __bf16*)nullptr, b.a2, W, Wp);
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, b.a2, wt.wf3, wt.params + PN_B3,
+                       (const __bf16*)nullptr, b.a3, W, Wp);
+    hipLaunchKernelGGL(k_pn_head, dim3((Wp + 255) / 256, G), dim3(256), 0, s, b.a3, wt.params, W, Wp, b.z, b.p);
+}
+
+void pn_gate(const PnBuffers& b, const PnGate& gt, int W, int M, hipStream_t s) {
+    hipLaunchKernelGGL(k_pn_gate, dim3((W + 63) / 64), dim3(256), 0, s, b.p, b.G, W, b.Wp, M, gt);
+}
+
+void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight, float* grad,
+                 hipStream_t s) {
+    const int G = b.G, Wp = b.Wp;
+    const int nb = (G * Wp + 255) / 256;
+    hipLaunchKernelGGL(k_pn_loss, dim3(nb), dim3(256), 0, s, b.z, y, G, W, Wp, pos_weight, b.dz, b.lpart);
+    hipLaunchKernelGGL(k_pn_head_wgrad, dim3(PN_CH + 1), dim3(256), 0, s, b.dz, b.a3, G, W, Wp, grad);
+    hipLaunchKernelGGL(k_pn_head_back, dim3((Wp + 255) / 256, G * PN_CH), dim3(256), 0, s, b.dz, wt.params, b.a3,
+                       W, Wp, b.d3);
+    const dim3 grid((W + PN_TILE - 1) / PN_TILE, G);
+    // data gradients through layers 3 and 2 (masked by the ReLU of the layer below)
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, b.d3, wt.wb3, (const float*)nullptr, b.a2,
+                       b.d2, W, Wp);
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, b.d2, wt.wb2, (const float*)nullptr, b.a1,
+                       b.d1, W, Wp);
+    // weight gradients of layers 3, 2, 1
+    const int slabs = (W + PN_SLAB - 1) / PN_SLAB;
+    const dim3 sg(slabs, G);
+    const int nparts = slabs * G;
+    const int NP32 = (PN_CH * PN_KW + 31) / 32 * 32, NP24 = (PN_CIN * PN_KW + 31) / 32 * 32;
+    hipLaunchKernelGGL(k_pn_wgrad<PN_CH>, sg, dim3(256), 0, s, b.d3, b.a2, W, Wp, b.part, b.bpart);
+    hipLaunchKernelGGL(k_pn_reduce, dim3((PN_CH * PN_CH * PN_KW + PN_CH + 255) / 256), dim3(256), 0, s, b.part,
+                       b.bpart, nparts, PN_CH * PN_KW, NP32, grad + PN_W3, grad + PN_B3);
+    hipLaunchKernelGGL(k_pn_wgrad<PN_CH>, sg, dim3(256), 0, s, b.d2, b.a1, W, Wp, b.part, b.bpart);
+    hipLaunchKernelGGL(k_pn_reduce, dim3((PN_CH * PN_CH * PN_KW + PN_CH + 255) / 256), dim3(256), 0, s, b.part,
+                       b.bpart, nparts, PN_CH * PN_KW, NP32, grad + PN_W2, grad + PN_B2);
+    hipLaunchKernelGGL(k_pn_wgrad<PN_CIN>, sg, dim3(256), 0, s, b.d1, b.a0, W, Wp, b.part, b.bpart);
+    hipLaunchKernelGGL(k_pn_reduce, dim3((PN_CH * PN_CIN * PN_KW + PN_CH + 255) / 256), dim3(256), 0, s, b.part,
+                       b.bpart, nparts, PN_CIN * PN_KW, NP24, grad + PN_W1, grad + PN_B1);
+}
+
+void pn_adam(const PnWeights& wt, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+             float eps, int step, hipStream_t s) {
+    const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(k_pn_adam, dim3((PN_NPARAMS + 255) / 256), dim3(256), 0, s, wt.params, grad, m, v, lr, beta1,
+                       beta2, eps, c1, c2);
+}
+
+void pn_pack(const PnWeights& wt, hipStream_t s) {
+    hipLaunchKernelGGL(k_pn_pack, dim3((PN_CH * PN_KP + 255) / 256), dim3(256), 0, s, wt);
+}
+
+}  // namespace lss

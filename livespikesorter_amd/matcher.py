@@ -41,6 +41,12 @@ class OnlineSpikesV2:
         self.info = self.query()
         self.last_W = 0
 
+    def set_proposal(self, net, lo: float = 0.25, hi: float = 0.75) -> None:
+        """Gate score tiles with a :class:`~livespikesorter_amd.proposal.ProposalNet`
+        (``None`` detaches it).  Keep the network alive while it is attached."""
+        check(lib().lss_set_proposal(self._h, net._p if net is not None else None, lo, hi))
+        self._prop = net
+
     # -- lifecycle ---------------------------------------------------------
     def close(self) -> None:
         if self._h:

@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parent.parent
 def declared_functions():
     text = (ROOT / "include" / "lss.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(lss_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|const char\*)\s+(lss_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_and_binding_agree():
