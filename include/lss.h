@@ -184,7 +184,7 @@ int lss_group_destroy(struct lss_group* g);
  * group of the probe (channels 16g-4 .. 16g+19), giving P(spike) per (group,
  * sample).  Attached to a handle, it gates which 64-template x 64-column
  * score tiles are computed: per column block, a group whose largest P within
- * +-M columns is >= hi marks the template tiles of its own and adjacent
+ * the block (widened by `reach` columns) is >= hi marks the template tiles of its own and adjacent
  * groups; a block where any group lies strictly between lo and hi is scored
  * in full (the fallback).  Trained on the GPU by lss_proposal_train_step. */
 struct lss_proposal;
@@ -208,8 +208,9 @@ int lss_proposal_train_step(struct lss_proposal* net, struct lss_handle* h, cons
                             int32_t ld, const uint8_t* labels, float pos_weight, float lr, float* loss);
 int lss_proposal_destroy(struct lss_proposal* net);
 /* Attach (net != NULL) or detach the gate of a handle (Kilosort semantics).
- * The handle keeps a reference: destroy the network after the handle. */
-int lss_set_proposal(struct lss_handle* h, struct lss_proposal* net, float lo, float hi);
+ * reach: columns on each side of a 64-column block whose P counts for it
+ * (0..M).  The handle keeps a reference: destroy the network after the handle. */
+int lss_set_proposal(struct lss_handle* h, struct lss_proposal* net, float lo, float hi, int32_t reach);
 
 /* Message of the last error on this thread ("" if none). */
 const char* lss_last_error(void);

@@ -115,7 +115,7 @@ def lib() -> ct.CDLL:
         L.lss_proposal_train_step.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32,
                                               ct.c_void_p, ct.c_float, ct.c_float, ct.POINTER(ct.c_float)]
         L.lss_proposal_destroy.argtypes = [ct.c_void_p]
-        L.lss_set_proposal.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_float]
+        L.lss_set_proposal.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_float, ct.c_int32]
         _LIB = L
     return _LIB
 

@@ -795,7 +795,7 @@ namespace {
 // Proposal network on the whitened batch, then the tile mask.
 void run_gate(lss_handle* h, int W, hipStream_t s) {
     pn_forward(h->prop->wt, h->pnb, h->d_Xw, h->XS, XOFF, h->C, W, s);
-    pn_gate(h->pnb, h->gate, W, h->M, s);
+    pn_gate(h->pnb, h->gate, W, s);
 }
 
 // Run one framed batch that is already on the device.
@@ -1512,7 +1512,7 @@ int lss_proposal_destroy(lss_proposal* net) {
     return guarded([&] { delete net; });
 }
 
-int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi) {
+int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi, int32_t reach) {
     return guarded([&] {
         if (!h) throw LssError(LSS_ERR_ARG, "null handle");
         HIPCHK(hipSetDevice(h->dev));
@@ -1521,6 +1521,7 @@ int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi) {
         if (h->sem != LSS_SEM_KILOSORT) throw LssError(LSS_ERR_ARG, "the proposal gate is defined for Kilosort semantics");
         if (net->dev != h->dev) throw LssError(LSS_ERR_ARG, "network and handle on different devices");
         if (!(lo >= 0.f && lo < hi && hi <= 1.f)) throw LssError(LSS_ERR_ARG, "need 0 <= lo < hi <= 1");
+        if (reach < 0 || reach > h->M) throw LssError(LSS_ERR_ARG, "reach outside 0..M");
         const int G = pn_groups(h->C);
         if (G > 64) throw LssError(LSS_ERR_ARG, "at most 1024 channels");
         if (!h->pnb.a0) alloc_pn_buffers(h->pnb, G, h->Wmax);
@@ -1546,6 +1547,7 @@ int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi) {
         h->gate.MT = MT;
         h->gate.lo = lo;
         h->gate.hi = hi;
+        h->gate.reach = reach;
         h->gate.stats = h->d_gate_stats;
         h->prop = net;
     });

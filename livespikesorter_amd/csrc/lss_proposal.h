@@ -57,6 +57,7 @@ struct PnGate {
     const int* mt_glo; const int* mt_ghi;    // group span of each template tile
     int MT;
     float lo, hi;                            // P <= lo: no spike; P >= hi: spike; between: fallback
+    int reach;                               // columns each side of a block whose P counts for it
     unsigned long long* stats;               // tiles scored, tiles, fallback blocks, blocks (running)
 };
 
@@ -66,7 +67,7 @@ size_t pn_part_floats(int G, int Wp);
 // forward from the matcher's whitened buffer (rows = channels, data at column xoff)
 void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
                 hipStream_t s);
-void pn_gate(const PnBuffers& b, const PnGate& gt, int W, int M, hipStream_t s);
+void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s);
 // loss and gradients (into grad [PN_NPARAMS]) for labels y [G][Wp] (0/1 bytes)
 void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight,
                  float* grad, hipStream_t s);

@@ -335,17 +335,16 @@ __global__ __launch_bounds__(256) void k_pn_pack(PnWeights wt) {
 }
 
 // ---------------------------------------------------------------------------
-// gate: per 64-column block and group, the largest P within +-M columns of the
-// block; a block with any group strictly between lo and hi falls back to full
+// gate: per 64-column block and group, the largest P within the block widened
+// by `reach` columns on each side; a block with any group strictly between lo and hi falls back to full
 // scoring, otherwise a template tile is scored when a group within one of its
 // group span has P >= hi.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pn_gate(const float* __restrict__ p, int G, int W, int Wp, int M,
-                                                 PnGate gt) {
+__global__ __launch_bounds__(256) void k_pn_gate(const float* __restrict__ p, int G, int W, int Wp, PnGate gt) {
     __shared__ int s_state[64];                       // per group: 0 off, 1 uncertain, 2 on
     __shared__ int s_unc;
     const int n = blockIdx.x;
-    const int lo = max(0, n * 64 - M), hi = min(W, n * 64 + 64 + M);
+    const int lo = max(0, n * 64 - gt.reach), hi = min(W, n * 64 + 64 + gt.reach);
     if (threadIdx.x == 0) s_unc = 0;
     __syncthreads();
     for (int g = threadIdx.x >> 6; g < G; g += 4) {  // one wave per group
@@ -401,8 +400,8 @@ void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_
     hipLaunchKernelGGL(k_pn_head, dim3((Wp + 255) / 256, G), dim3(256), 0, s, b.a3, wt.params, W, Wp, b.z, b.p);
 }
 
-void pn_gate(const PnBuffers& b, const PnGate& gt, int W, int M, hipStream_t s) {
-    hipLaunchKernelGGL(k_pn_gate, dim3((W + 63) / 64), dim3(256), 0, s, b.p, b.G, W, b.Wp, M, gt);
+void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s) {
+    hipLaunchKernelGGL(k_pn_gate, dim3((W + 63) / 64), dim3(256), 0, s, b.p, b.G, W, b.Wp, gt);
 }
 
 void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight, float* grad,

@@ -41,10 +41,10 @@ class OnlineSpikesV2:
         self.info = self.query()
         self.last_W = 0
 
-    def set_proposal(self, net, lo: float = 0.25, hi: float = 0.75) -> None:
+    def set_proposal(self, net, lo: float = 0.25, hi: float = 0.75, reach: int = 4) -> None:
         """Gate score tiles with a :class:`~livespikesorter_amd.proposal.ProposalNet`
         (``None`` detaches it).  Keep the network alive while it is attached."""
-        check(lib().lss_set_proposal(self._h, net._p if net is not None else None, lo, hi))
+        check(lib().lss_set_proposal(self._h, net._p if net is not None else None, lo, hi, reach))
         self._prop = net
 
     # -- lifecycle ---------------------------------------------------------

@@ -246,36 +246,63 @@ def draw_spikes(model: ProbeModel, n_samples: int, seed: int) -> Truth:
     return Truth(c[order], t[order], a[order])
 
 
-def make_recording(model: ProbeModel, n_samples: int, seed: int | None = None,
-                   block: int = 30000, noise: float = 1.0):
-    """int16 recording ``[n_samples][C]`` (channel-fastest, as SpikeGLX streams it)."""
-    seed = model.seed if seed is None else seed
-    truth = draw_spikes(model, n_samples, seed)
+def _render_block(model: ProbeModel, truth: Truth, b0: int, b1: int, seed_seq, noise: float) -> np.ndarray:
+    """int16 samples [b0, b1) of a recording (one noise stream per block)."""
     C, M = model.C, model.M
     half = M // 2
     waves = model.waves                                   # [T][M][C]
-    out = np.empty((n_samples, C), dtype=np.int16)
+    rng = np.random.Generator(np.random.PCG64(seed_seq))
+    x = rng.standard_normal((b1 - b0, C), dtype=np.float32) * np.float32(noise)
+    lo = np.searchsorted(truth.centre, b0 - half, side="left")
+    hi = np.searchsorted(truth.centre, b1 + half, side="left")
+    for i in range(lo, hi):
+        s = int(truth.centre[i])
+        w0, w1 = s - half, s + half + 1
+        a0, a1 = max(w0, b0), min(w1, b1)
+        if a0 >= a1:
+            continue
+        x[a0 - b0:a1 - b0] += truth.amplitude[i] * waves[truth.template[i], a0 - w0:a1 - w0]
+    if model.mix is not None:
+        x = x @ model.mix.T.astype(np.float32)
+    q = np.rint(x * np.float32(model.gain))
+    np.clip(q, -32768, 32767, out=q)
+    return q.astype(np.int16)
+
+
+_POOL_STATE = {}
+
+
+def _render_job(args):
+    b0, b1, seed_seq = args
+    st = _POOL_STATE
+    return b0, _render_block(st["model"], st["truth"], b0, b1, seed_seq, st["noise"])
+
+
+def make_recording(model: ProbeModel, n_samples: int, seed: int | None = None,
+                   block: int = 30000, noise: float = 1.0, workers: int = 1):
+    """int16 recording ``[n_samples][C]`` (channel-fastest, as SpikeGLX streams it).
+
+    ``workers`` > 1 renders the blocks in forked processes; the result is the
+    same array (each block has its own noise stream)."""
+    seed = model.seed if seed is None else seed
+    truth = draw_spikes(model, n_samples, seed)
+    out = np.empty((n_samples, model.C), dtype=np.int16)
     noise_ss = np.random.SeedSequence([seed, _S_NOISE])
-    blocks = range(0, n_samples, block)
-    seeds = noise_ss.spawn(len(blocks))
-    for bi, b0 in enumerate(blocks):
-        b1 = min(n_samples, b0 + block)
-        rng = np.random.Generator(np.random.PCG64(seeds[bi]))
-        x = rng.standard_normal((b1 - b0, C), dtype=np.float32) * np.float32(noise)
-        lo = np.searchsorted(truth.centre, b0 - half, side="left")
-        hi = np.searchsorted(truth.centre, b1 + half, side="left")
-        for i in range(lo, hi):
-            s = int(truth.centre[i])
-            w0, w1 = s - half, s + half + 1
-            a0, a1 = max(w0, b0), min(w1, b1)
-            if a0 >= a1:
-                continue
-            x[a0 - b0:a1 - b0] += truth.amplitude[i] * waves[truth.template[i], a0 - w0:a1 - w0]
-        if model.mix is not None:
-            x = x @ model.mix.T.astype(np.float32)
-        q = np.rint(x * np.float32(model.gain))
-        np.clip(q, -32768, 32767, out=q)
-        out[b0:b1] = q.astype(np.int16)
+    starts = list(range(0, n_samples, block))
+    seeds = noise_ss.spawn(len(starts))
+    jobs = [(b0, min(n_samples, b0 + block), seeds[bi]) for bi, b0 in enumerate(starts)]
+    if workers > 1 and len(jobs) > 1:
+        import multiprocessing as mp
+        _POOL_STATE.update(model=model, truth=truth, noise=noise)
+        try:
+            with mp.get_context("fork").Pool(workers) as pool:
+                for b0, q in pool.imap_unordered(_render_job, jobs, chunksize=2):
+                    out[b0:b0 + len(q)] = q
+        finally:
+            _POOL_STATE.clear()
+    else:
+        for b0, b1, sq in jobs:
+            out[b0:b1] = _render_block(model, truth, b0, b1, sq, noise)
     return out, truth
 
 
@@ -296,4 +323,7 @@ CONFIGS = {
     # BF16 projection: 384 ch, 1024 templates, bursty gamma(0.5) ISIs.
     "cfg3": dict(C=384, T=1024, whitening="banded", rate=(0.5, 20.0), amp=(12.0, 3.0),
                  isi="gamma", gamma_shape=0.5),
+    # Learned proposal + BF16 matcher: 384 ch, 1024 templates, 1500-sample chunks
+    # (the probe model is drawn from the test seed; training recordings from TRAIN_SEED).
+    "cfg5": dict(C=384, T=1024, whitening="banded", rate=(0.5, 20.0), amp=(12.0, 3.0)),
 }

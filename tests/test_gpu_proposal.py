@@ -126,3 +126,54 @@ def test_gate_extremes(model_dirs):
                 assert (q.gate_fallback_blocks == q.gate_blocks) == (expect == "fallback")
             sp.set_proposal(None)
             net.close()
+
+
+def test_trained_gate_gain(tmp_path):
+    """What the trained network gains over the method without it, on a test-seed
+    recording of the 1024-template probe it was trained for (training used
+    TRAIN_SEED recordings): the fraction of score tiles left uncomputed at >= 99%
+    spike agreement with the plain FP32 matcher, against an untrained network
+    (same gate) and no network (every tile).  Timings are printed, not asserted."""
+    import time
+    from pathlib import Path
+    from livespikesorter_amd._hip import DeviceBuffer, synchronize
+    from livespikesorter_amd.matcher import PREC_BF16_RESCORE
+    from livespikesorter_amd.train_proposal import _agreement
+    weights = Path(__file__).resolve().parent.parent / "livespikesorter_amd" / "data" / "proposal_cfg5.npy"
+    m = synth.make_model(**synth.CONFIGS["cfg5"])
+    d = tmp_path / "cfg5"
+    synth.write_oss_input(m, d)
+    x, _ = synth.make_recording(m, 8 * 30000, seed=synth.TEST_SEED, workers=8)
+    N, W, C = 1500, 1622, 384
+    nb = (len(x) - W) // N + 1
+    dx = DeviceBuffer.from_array(x)
+    synchronize()
+
+    def run(net, prec):
+        with OnlineSpikesV2(d, chunk=N, max_batch=W, precision=prec) as sp:
+            if net is not None:
+                sp.set_proposal(net, 0.25, 0.75, 4)
+            evs = []
+            t0 = time.perf_counter()
+            for j in range(nb):
+                evs.append(sp.process_batch_device(dx.ptr + j * N * C * 2, W, C, first_sample=j * N)[0])
+            el = time.perf_counter() - t0
+            q = sp.query()
+            sp.set_proposal(None)
+        frac = q.gate_tiles_scored / q.gate_tiles if q.gate_tiles else 1.0
+        return np.concatenate(evs), frac, nb * N / 30000 / el
+
+    ref, _, rt_plain = run(None, 0)
+    res = {}
+    with ProposalNet(C, W, seed=1) as untrained, ProposalNet(C, W, seed=1) as trained:
+        trained.load(weights)
+        for name, net in (("none", None), ("untrained", untrained), ("trained", trained)):
+            e, frac, rt = run(net, PREC_BF16_RESCORE)
+            ident, within = _agreement(ref, e)
+            res[name] = (ident, within, frac, rt)
+            print(f"{name:9s}: agreement identical {ident:.5f} within1 {within:.5f}, tiles scored {frac:.4f}, "
+                  f"{rt:.1f}x realtime (plain FP32 {rt_plain:.1f}x)")
+    dx.free()
+    assert res["trained"][1] >= 0.99
+    assert res["untrained"][2] == 1.0 and res["none"][2] == 1.0
+    assert res["trained"][2] < 0.98
