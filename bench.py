@@ -137,13 +137,17 @@ def bench_group(args) -> dict:
     seq = time.perf_counter() - ts2
     for sp in singles:
         sp.close()
+    grp = StreamGroup(dirs, batch=W)
+    plat, pbusy, pticks = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205)
+    grp.close()
     for bf in bufs:
         bf.free()
     rtf = data_s / elapsed
     seq_rtf = seq_steps * len(dirs) * N / FS / seq
     log(f"group: {args.steps} ticks in {elapsed:.3f}s: {rtf:.1f}x realtime aggregate, tick p50 "
         f"{pct(wall, 50):.3f} ms p99 {pct(wall, 99):.3f} ms; chunk latency with arrival jitter p50 "
-        f"{pct(lat, 50):.3f} p99 {pct(lat, 99):.3f} ms; sequential single handles {seq_rtf:.1f}x")
+        f"{pct(lat, 50):.3f} p99 {pct(lat, 99):.3f} ms; sequential single handles {seq_rtf:.1f}x; "
+        f"paced (launch on arrival) latency p50 {pct(plat, 50):.3f} p99 {pct(plat, 99):.3f} ms")
     return {
         "metric": METRIC + " [batching workload]",
         "value": round(rtf, 3),
@@ -159,6 +163,8 @@ def bench_group(args) -> dict:
         "samples_per_s": round(args.steps * len(dirs) * N / elapsed, 1),
         "tick_p50_ms": round(pct(wall, 50), 4), "tick_p99_ms": round(pct(wall, 99), 4),
         "chunk_latency_p50_ms": round(pct(lat, 50), 4), "chunk_latency_p99_ms": round(pct(lat, 99), 4),
+        "paced_latency_p50_ms": round(pct(plat, 50), 4), "paced_latency_p99_ms": round(pct(plat, 99), 4),
+        "paced_gpu_busy_fraction": round(pbusy, 4), "paced_launches": pticks,
         "spikes_per_tick": round(spikes / args.steps, 1),
         "sequential_single_handles_x_realtime": round(seq_rtf, 3),
         "speedup_over_sequential": round(rtf / seq_rtf, 3),
@@ -167,6 +173,167 @@ def bench_group(args) -> dict:
 
 def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed):
+    """Stream the group's recordings in real time: chunk k of stream s is handed
+    over at k * N / FS plus a uniform +-5 ms jitter.  Whenever chunks are
+    waiting, one launch matches the oldest waiting chunk of every stream that
+    has one (the others idle).  Latency = events on the host - hand-over."""
+    S = grp.n
+    period = N / FS
+    rng = np.random.default_rng(seed)
+    n_chunks = min(n_ticks, int(seconds / period))
+    arrive = np.arange(n_chunks)[:, None] * period + 0.005 + rng.uniform(-0.005, 0.005, size=(n_chunks, S))
+    nxt = [0] * S
+    lat, busy, ticks = [], 0.0, 0
+    t0 = time.perf_counter()
+    while min(nxt) < n_chunks:
+        now = time.perf_counter() - t0
+        ready = [s for s in range(S) if nxt[s] < n_chunks and arrive[nxt[s], s] <= now]
+        if not ready:
+            wait = min(arrive[nxt[s], s] for s in range(S) if nxt[s] < n_chunks) - now
+            if wait > 0.0004:
+                time.sleep(wait - 0.0003)
+            continue
+        ptrs = [bufs[s].ptr + nxt[s] * N * C * 2 if s in ready else None for s in range(S)]
+        ts = time.perf_counter()
+        grp.process(ptrs, [nxt[s] * N for s in range(S)], device=True, lds=[C] * S)
+        te = time.perf_counter()
+        busy += te - ts
+        ticks += 1
+        for s in ready:
+            lat.append((te - t0 - arrive[nxt[s], s]) * 1e3)
+            nxt[s] += 1
+    wall = time.perf_counter() - t0
+    return lat, busy / wall, ticks
+
+
+def make_target_data(cache: Path, seconds: float, n_streams: int = 8):
+    """n_streams recordings of the 384-channel, 1024-template probe (the model the
+    proposal network was trained for), seeds TEST_SEED + 100 + k."""
+    from livespikesorter_amd import synth
+    cache.mkdir(parents=True, exist_ok=True)
+    d = cache / "cfg5_oss_input"
+    m = synth.make_model(**synth.CONFIGS["cfg5"])
+    if not (d / "ctc.npy").exists():
+        synth.write_oss_input(m, d)
+    recs = []
+    for k in range(n_streams):
+        rp = cache / f"cfg5_rec{k}_{int(seconds)}s.npy"
+        if rp.exists():
+            x = np.load(rp)
+        else:
+            x, _ = synth.make_recording(m, int(seconds * FS), seed=synth.TEST_SEED + 100 + k,
+                                        workers=min(16, os.cpu_count() or 1))
+            np.save(rp, x)
+        recs.append(x)
+    return m, d, recs
+
+
+def bench_target(args) -> dict:
+    """8 probe streams of 384 channels x 1024 templates, 1500-sample chunks: the
+    plain FP32 matcher one stream at a time, and one graph launch per tick for
+    the plain FP32, BF16 and gated BF16 (trained proposal network) matchers;
+    throughput, paced latency under +-5 ms arrival jitter, and spike agreement
+    with the plain FP32 matcher."""
+    from livespikesorter_amd import OnlineSpikesV2, StreamGroup
+    from livespikesorter_amd._hip import DeviceBuffer, synchronize
+    from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_FP32
+    from livespikesorter_amd.proposal import ProposalNet
+    from tests.parity import agreement
+    N, M = 1500, 61
+    W = N + 2 * M
+    t0 = time.time()
+    m, d, recs = make_target_data(Path(args.cache), args.group_seconds)
+    S = len(recs)
+    C = recs[0].shape[1]
+    n_ticks = min((len(x) - W) // N + 1 for x in recs)
+    log(f"target data ready in {time.time() - t0:.1f}s: {S} streams x {n_ticks} chunks")
+    bufs = [DeviceBuffer.from_array(x) for x in recs]
+    synchronize()
+    ptr = lambda s, j: bufs[s].ptr + j * N * C * 2
+    # plain FP32, one handle per stream, streams one after another (the reference's arrangement)
+    singles = [OnlineSpikesV2(d, chunk=N, max_batch=W) for _ in range(S)]
+    ref = [[None] * n_ticks for _ in range(S)]
+    for j in range(3):
+        for s in range(S):
+            singles[s].process_batch_device(ptr(s, j), W, C, first_sample=j * N)
+    synchronize()
+    ts = time.perf_counter()
+    for j in range(n_ticks):
+        for s in range(S):
+            ref[s][j] = singles[s].process_batch_device(ptr(s, j), W, C, first_sample=j * N)[0]
+    seq_s = time.perf_counter() - ts
+    for sp in singles:
+        sp.close()
+    seq_rtf = n_ticks * S * N / FS / seq_s
+    log(f"plain FP32, single handles in sequence: {seq_rtf:.1f}x realtime aggregate")
+    weights = ROOT / "livespikesorter_amd" / "data" / "proposal_cfg5.npy"
+    net = ProposalNet(C, W, seed=1)
+    net.load(weights)
+    modes = {}
+    for name, prec, gated in (("group_fp32", PREC_FP32, False), ("group_bf16", PREC_BF16_RESCORE, False),
+                              ("group_bf16_gated", PREC_BF16_RESCORE, True)):
+        grp = StreamGroup([d] * S, batch=W, precision=prec)
+        if gated:
+            for s in range(S):
+                grp.set_proposal(s, net, 0.25, 0.75, 4)
+        for j in range(3):
+            grp.process([ptr(s, j) for s in range(S)], [j * N] * S, device=True, lds=[C] * S)
+        synchronize()
+        wall, outs = [], []
+        ts = time.perf_counter()
+        for j in range(n_ticks):
+            t1 = time.perf_counter()
+            outs.append(grp.process([ptr(s, j) for s in range(S)], [j * N] * S, device=True, lds=[C] * S))
+            wall.append((time.perf_counter() - t1) * 1e3)
+        el = time.perf_counter() - ts
+        tot, ident_n, within_n = 0, 0.0, 0.0
+        for j, out in enumerate(outs):
+            for s in range(S):
+                e, r = out[s][0], ref[s][j]
+                i, w_, _ = agreement(r["sample"], r["template"], e["sample"], e["template"])
+                nn = max(len(r), len(e))
+                ident_n += i * nn
+                within_n += w_ * nn
+                tot += nn
+        lat, busy, nt = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205)
+        q = [grp.query(s) for s in range(S)]
+        grp.close()
+        rtf = n_ticks * S * N / FS / el
+        res = {"x_realtime": round(rtf, 2), "tick_p50_ms": round(pct(wall, 50), 3),
+               "tick_p99_ms": round(pct(wall, 99), 3),
+               "paced_latency_p50_ms": round(pct(lat, 50), 3), "paced_latency_p99_ms": round(pct(lat, 99), 3),
+               "paced_gpu_busy_fraction": round(busy, 4), "paced_launches": nt,
+               "agreement_vs_plain_fp32": {"identical": round(ident_n / max(tot, 1), 6),
+                                           "within_1_sample": round(within_n / max(tot, 1), 6), "events": tot},
+               "speedup_over_plain_fp32_sequential": round(rtf / seq_rtf, 3)}
+        if gated:
+            sc = sum(x.gate_tiles_scored for x in q)
+            tt = sum(x.gate_tiles for x in q)
+            res["tiles_scored_fraction"] = round(sc / max(tt, 1), 4)
+        modes[name] = res
+        log(f"{name}: {res}")
+    net.close()
+    for b in bufs:
+        b.free()
+    g = modes["group_bf16_gated"]
+    return {
+        "metric": METRIC + " [target: gated BF16 batched, 384 ch x 1024 templates]",
+        "value": g["x_realtime"],
+        "unit": "x realtime (aggregate over 8 streams)",
+        "n_gpus": 1, "steps": n_ticks, "warmup": 3,
+        "ms_per_step": g["tick_p50_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (projection, proposal), f32",
+        "data": "synthetic (seeded generator: 8 recordings of one 384-ch, 1024-template probe)",
+        "config": {"workload": "8 streams x 384 ch x 1024 templates, 1500-sample chunks + 61-sample margins "
+                               "(W=1622), one graph launch per tick, BF16 projection with FP32 rescoring, "
+                               "proposal-network gate; paced run with +-5 ms arrival jitter",
+                   "streams": S, "templates": 1024, "chunk": N, "batch_columns": W},
+        "plain_fp32_sequential_x_realtime": round(seq_rtf, 2),
+        "modes": modes,
+    }
 
 
 def stage_work(info, C, W, M, iters, spikes, bf16=False):
@@ -344,7 +511,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-steps", type=int, default=100, help="chunks timed with host-resident input")
     ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
-    ap.add_argument("--workload", choices=("cfg2", "group", "bf16"), default="cfg2",
+    ap.add_argument("--workload", choices=("cfg2", "group", "bf16", "target"), default="cfg2",
                     help="cfg2: the headline plain FP32 matcher; group: 8 streams per graph launch; "
                          "bf16: BF16 projection with FP32 rescoring, 1024 templates, 60000-sample batches")
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
@@ -364,6 +531,9 @@ def main() -> int:
         return 0
     if args.workload == "bf16":
         print(json.dumps(bench_bf16(args)), flush=True)
+        return 0
+    if args.workload == "target":
+        print(json.dumps(bench_target(args)), flush=True)
         return 0
 
     from livespikesorter_amd import OnlineSpikesV2

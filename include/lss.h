@@ -158,6 +158,23 @@ int lss_stage_times(struct lss_handle* h, double* ms, int64_t* batches, int32_t 
 
 int lss_destroy(struct lss_handle* h);
 
+/* Autotuning of the peel geometry for batches of n_samples columns: every
+ * candidate split of the templates over workgroups (1, and the fewest
+ * subsets reaching 32, 16 and 8 register rows per thread) is timed on the
+ * given device batch (median wall time of `reps` runs after one warm-up);
+ * the fastest is used for that batch width from then on.  Results do not
+ * depend on the split (bit-identical), so tuning on live data is safe.  No
+ * reference counterpart. */
+typedef struct lss_tune_result {
+    int32_t n;             /* candidates timed */
+    int32_t chosen;        /* index of the one kept */
+    int32_t split[8];      /* template subsets per column block */
+    int32_t rows[8];       /* register rows per thread */
+    double ms[8];          /* median wall time of one batch */
+} lss_tune_result;
+int lss_autotune(struct lss_handle* h, const int16_t* d_samples, int64_t n_samples, int32_t ld, int32_t reps,
+                 lss_tune_result* out);
+
 /* ---- groups: several probe streams per launch ----
  * No reference counterpart (the reference runs one independent sorter thread
  * per probe, main.cpp:77-85).  A group matches one framed batch of every
@@ -169,13 +186,18 @@ struct lss_group;
 int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const lss_config* cfg,
                      struct lss_group** out);
 /* samples[s]: stream s's batch, [W][ld[s]] int16 (device pointers when
- * device_input != 0); first_sample[s]: absolute index of its column 0.
+ * device_input != 0), or NULL when stream s has no batch this call (its
+ * kernels return at once; n_ev[s] = 0); first_sample[s]: absolute index of
+ * its column 0.
  * Results per stream as lss_process_batch (ev[s], ev_cap[s], n_ev[s], feat[s]
  * nullable). */
 int lss_group_process(struct lss_group* g, const int16_t* const* samples, const int32_t* ld,
                       int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
                       const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
 int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
+/* lss_set_proposal for one stream of a group (the launch graph is rebuilt). */
+int lss_group_set_proposal(struct lss_group* g, int32_t stream, struct lss_proposal* net, float lo, float hi,
+                           int32_t reach);
 int lss_group_destroy(struct lss_group* g);
 
 /* ---- spike-proposal network (a learned gate; no reference counterpart) ----

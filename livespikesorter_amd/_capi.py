@@ -32,7 +32,8 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_destroy", "lss_last_error", "lss_group_create", "lss_group_process",
                  "lss_group_query", "lss_group_destroy", "lss_proposal_n_params", "lss_proposal_create",
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
-                 "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal")
+                 "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
+                 "lss_autotune", "lss_group_set_proposal")
 NSTAGES = 6
 STAGE_NAMES = ("h2d_copy", "whiten", "proposal", "pcconv", "project", "peel")
 
@@ -65,6 +66,11 @@ class Info(ct.Structure):
                 ("peel_rows", ct.c_int32), ("peel_split", ct.c_int32), ("peel_groups", ct.c_int32),
                 ("gate_tiles_scored", ct.c_int64), ("gate_tiles", ct.c_int64),
                 ("gate_fallback_blocks", ct.c_int64), ("gate_blocks", ct.c_int64)]
+
+
+class TuneResult(ct.Structure):
+    _fields_ = [("n", ct.c_int32), ("chosen", ct.c_int32), ("split", ct.c_int32 * 8), ("rows", ct.c_int32 * 8),
+                ("ms", ct.c_double * 8)]
 
 
 class LssError(RuntimeError):
@@ -115,6 +121,10 @@ def lib() -> ct.CDLL:
         L.lss_proposal_train_step.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32,
                                               ct.c_void_p, ct.c_float, ct.c_float, ct.POINTER(ct.c_float)]
         L.lss_proposal_destroy.argtypes = [ct.c_void_p]
+        L.lss_autotune.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_int32,
+                                   ct.POINTER(TuneResult)]
+        L.lss_group_set_proposal.argtypes = [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_float, ct.c_float,
+                                             ct.c_int32]
         L.lss_set_proposal.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_float, ct.c_int32]
         _LIB = L
     return _LIB

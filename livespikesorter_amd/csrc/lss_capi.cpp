@@ -187,6 +187,8 @@ struct lss_handle {
     int slot_cap = 0, cap_g = 0, max_groups = 0;
     // template split of the peel: per-subset column maxima [max_split][Wmax]
     int max_split = 1;
+    int force_split = 0;                 // autotuning: the split being timed (0: none forced)
+    std::map<int, int> tuned_split;      // batch width -> split chosen by lss_autotune
     float *d_Pcf = nullptr, *d_Pbv = nullptr;
     int* d_Pim = nullptr;
     // debug copies
@@ -688,6 +690,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     cj = PcconvJob{};
     cj.Xw = h->d_Xw; cj.wpca = h->d_wpca; cj.Bpc = h->d_Bpc;
     cj.Bpcb = reinterpret_cast<__bf16*>(h->d_Bpcb);
+    cj.active = xptr;
     cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP;
     cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
     ProjectJob& pj = bp.pj;
@@ -695,6 +698,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     pj.A = h->d_U2; pj.Bm = h->d_Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = h->d_B;
     pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
     if (h->prop) { pj.gate = h->d_gate; pj.gate_mt = h->TP / GEMM_BM; }
+    pj.active = xptr;
     if (h->bf16) {
         pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
         pj.Bmb = reinterpret_cast<const __bf16*>(h->d_Bpcb);
@@ -734,6 +738,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
     p.prof = h->d_prof;
     if (h->prop) { p.gate = h->d_gate; p.gate_mt = h->TP / GEMM_BM; }
+    p.active = xptr;
     if (h->bf16) {
         // candidates within 10% of threshold on the score scale are decided in FP32
         p.rescore = 1;
@@ -752,17 +757,47 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
 // templates split over S workgroups per column block when the co-resident grid
 // has room: the fewest rows per thread that fit (fewer rows, shorter scans).
 // Otherwise the narrowest column block whose grid is co-resident.
+// Register rows per thread for S subsets of T templates (0: no register tile fits).
+int split_rows(int T, int S) {
+    for (int r : {8, 16, 32, 64})
+        if (S * REG_NRG * r >= T) return r;
+    return 0;
+}
+
+// Whether S subsets with R rows each form a valid co-resident split geometry.
+bool split_fits(const lss_handle* h, int W, int S, int R) {
+    const int NB = (W + MIN_CW - 1) / MIN_CW;
+    return S >= 2 && S <= h->max_split && R > 0 && R <= 32 && S * REG_NRG * R >= h->T &&
+           S * REG_NRG * R <= h->ctc_rows && NB <= h->max_groups && S * NB <= h->peel_cap[peel_cap_index(R)];
+}
+
+// The split sizes worth trying for a batch of W columns: none, then the fewest
+// subsets that reach 32, 16 and 8 rows per thread.
+std::vector<int> split_candidates(const lss_handle* h, int W) {
+    std::vector<int> out{1};
+    for (int r : {32, 16, 8}) {
+        const int s2 = (h->T + REG_NRG * r - 1) / (REG_NRG * r);
+        if (split_fits(h, W, s2, split_rows(h->T, s2)) && std::find(out.begin(), out.end(), s2) == out.end())
+            out.push_back(s2);
+    }
+    return out;
+}
+
 void peel_geometry(const lss_handle* h, int W, int& CW, int& R, int& S) {
     S = 1;
-    if (h->max_split > 1) {
-        const int NB = (W + MIN_CW - 1) / MIN_CW;
-        for (int r : {8, 16, 32}) {
-            const int s2 = std::min(h->max_split, (h->T + REG_NRG * r - 1) / (REG_NRG * r));
-            if (s2 < 2 || s2 * REG_NRG * r < h->T || s2 * REG_NRG * r > h->ctc_rows) continue;
-            if (NB > h->max_groups || s2 * NB > h->peel_cap[peel_cap_index(r)]) continue;
-            CW = MIN_CW; R = r; S = s2;
-            return;
-        }
+    int want = h->force_split;
+    if (want == 0) {
+        const auto it = h->tuned_split.find(W);
+        if (it != h->tuned_split.end()) want = it->second;
+    }
+    if (want > 1) {
+        const int r = split_rows(h->T, want);
+        if (split_fits(h, W, want, r)) { CW = MIN_CW; R = r; S = want; return; }
+    }
+    if (want == 0 && h->max_split > 1) {
+        // default: the fewest rows per thread that fit
+        const auto cand = split_candidates(h, W);
+        if (cand.size() > 1) { S = cand.back(); CW = MIN_CW; R = split_rows(h->T, S); return; }
     }
     for (CW = MIN_CW;; CW *= 2) {
         const int G = (W + CW - 1) / CW;
@@ -793,9 +828,9 @@ void check_peel(lss_handle* h, int W) {
 namespace {
 
 // Proposal network on the whitened batch, then the tile mask.
-void run_gate(lss_handle* h, int W, hipStream_t s) {
-    pn_forward(h->prop->wt, h->pnb, h->d_Xw, h->XS, XOFF, h->C, W, s);
-    pn_gate(h->pnb, h->gate, W, s);
+void run_gate(lss_handle* h, int W, hipStream_t s, const int16_t* const* active = nullptr) {
+    pn_forward(h->prop->wt, h->pnb, h->d_Xw, h->XS, XOFF, h->C, W, s, active);
+    pn_gate(h->pnb, h->gate, W, s, active);
 }
 
 // Run one framed batch that is already on the device.
@@ -1161,6 +1196,54 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
     });
 }
 
+int lss_autotune(lss_handle* h, const int16_t* d_samples, int64_t n, int32_t ld, int32_t reps, lss_tune_result* out) {
+    return guarded([&] {
+        if (!h || !d_samples || !out) throw LssError(LSS_ERR_ARG, "null argument");
+        if (n <= 0 || n > h->Wmax) throw LssError(LSS_ERR_ARG, "batch width outside 1..max_batch");
+        if (reps < 1 || reps > 1000) throw LssError(LSS_ERR_ARG, "reps outside 1..1000");
+        HIPCHK(hipSetDevice(h->dev));
+        set_cmap(h, nullptr, ld);
+        const int W = (int)n;
+        const lss_info saved = h->info;
+        const bool prof = h->cfg.profile != 0;
+        h->cfg.profile = 0;
+        *out = lss_tune_result{};
+        std::vector<int> cand = split_candidates(h, W);
+        if (cand.size() > 8) cand.resize(8);
+        int best = 0;
+        try {
+            for (size_t i = 0; i < cand.size(); ++i) {
+                h->force_split = cand[i];
+                int CW = 0, R = 0, S = 1;
+                peel_geometry(h, W, CW, R, S);
+                std::vector<double> t;
+                for (int r = 0; r <= reps; ++r) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    run_batch(h, d_samples, W, ld, 0);
+                    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    if (r > 0) t.push_back(ms);
+                }
+                std::sort(t.begin(), t.end());
+                out->split[i] = S;
+                out->rows[i] = R;
+                out->ms[i] = t[t.size() / 2];
+                if (out->ms[i] < out->ms[best]) best = (int)i;
+            }
+        } catch (...) {
+            h->force_split = 0;
+            h->cfg.profile = prof;
+            h->info = saved;
+            throw;
+        }
+        h->force_split = 0;
+        h->cfg.profile = prof;
+        h->info = saved;
+        out->n = (int32_t)cand.size();
+        out->chosen = best;
+        h->tuned_split[W] = out->split[best];
+    });
+}
+
 int lss_stage_times(lss_handle* h, double* ms, int64_t* batches, int32_t reset) {
     return guarded([&] {
         if (!h || !ms || !batches) throw LssError(LSS_ERR_ARG, "null argument");
@@ -1255,7 +1338,7 @@ void group_capture(lss_group* g, const int32_t* ld) {
     launch_reset_batch(g->rb, st);
     launch_whiten(g->wb, st);
     for (int s2 = 0; s2 < n; ++s2)
-        if (g->hs[s2]->prop) run_gate(g->hs[s2].get(), g->W, st);
+        if (g->hs[s2]->prop) run_gate(g->hs[s2].get(), g->W, st, &g->d_xptr[s2]);
     launch_pcconv(g->cb, g->hs[0]->K, st);
     launch_project(g->pb, g->hs[0]->bf16, st);
     HIPCHK(launch_peel(g->kb, g->R, false, st));
@@ -1316,10 +1399,15 @@ int lss_group_process(lss_group* g, const int16_t* const* samples, const int32_t
         HIPCHK(hipSetDevice(g->dev));
         auto t0 = std::chrono::steady_clock::now();
         const int n = g->n;
+        int active = 0;
         for (int s2 = 0; s2 < n; ++s2) {
             lss_handle* h = g->hs[s2].get();
             n_ev[s2] = 0;
-            if (!samples[s2]) throw LssError(LSS_ERR_ARG, "null samples");
+            if (!samples[s2]) {             // idle this tick: every kernel of the stream returns at once
+                g->h_xptr[s2] = nullptr;
+                continue;
+            }
+            ++active;
             set_cmap(h, nullptr, ld[s2]);
             if (device_input) {
                 g->h_xptr[s2] = samples[s2];
@@ -1330,11 +1418,13 @@ int lss_group_process(lss_group* g, const int16_t* const* samples, const int32_t
                 g->h_xptr[s2] = dx;
             }
         }
+        if (!active) return;
         if (!g->exec || !std::equal(ld, ld + n, g->ld_captured.begin())) group_capture(g, ld);
         HIPCHK(hipGraphLaunch(g->exec, g->stream));
         HIPCHK(hipStreamSynchronize(g->stream));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         for (int s2 = 0; s2 < n; ++s2) {
+            if (!samples[s2]) continue;
             lss_handle* h = g->hs[s2].get();
             h->cur_first = first_sample[s2];
             check_peel(h, g->W);
@@ -1349,6 +1439,18 @@ int lss_group_query(lss_group* g, int32_t stream, lss_info* info) {
         if (!g || !info || stream < 0 || stream >= g->n) throw LssError(LSS_ERR_ARG, "bad argument");
         const int rc = lss_query(g->hs[stream].get(), info);
         if (rc != LSS_OK) throw LssError(rc, g_last_error);
+    });
+}
+
+int lss_group_set_proposal(lss_group* g, int32_t stream, lss_proposal* net, float lo, float hi, int32_t reach) {
+    return guarded([&] {
+        if (!g || stream < 0 || stream >= g->n) throw LssError(LSS_ERR_ARG, "bad argument");
+        HIPCHK(hipSetDevice(g->dev));
+        HIPCHK(hipStreamSynchronize(g->stream));
+        const int rc = lss_set_proposal(g->hs[stream].get(), net, lo, hi, reach);
+        if (rc != LSS_OK) throw LssError(rc, g_last_error);
+        if (g->exec) { HIPCHK(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
+        if (g->graph) { HIPCHK(hipGraphDestroy(g->graph)); g->graph = nullptr; }
     });
 }
 

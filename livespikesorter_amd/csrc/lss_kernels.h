@@ -110,6 +110,8 @@ struct PeelParams {
     unsigned long long* n_rescored;    // running totals: FP32 rescores, threshold decisions flipped
     // proposal gate: score tile (64 columns, 64 templates) scored iff gate[(w/64) * gate_mt + t/64]
     const uint8_t* gate; int gate_mt;
+    // optional: the stream is idle this launch when *active is null (every workgroup returns)
+    const int16_t* const* active;
 };
 
 // ---- batched launch descriptors (kernel arguments) ----
@@ -126,11 +128,13 @@ struct ProjectJob {
     const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
     float* C2;                            // optional second copy of the result
     const uint8_t* gate; int gate_mt;     // optional tile mask [ntile][gate_mt]: 0 = not computed
+    const int16_t* const* active;         // optional: the job is skipped when *active is null
     int lda, ldb, ldc, mtiles, ntiles;
 };
 struct PcconvJob {
     const float* Xw; const float* wpca; float* Bpc;
     __bf16* Bpcb;                         // optional BF16 copy of the PC projection
+    const int16_t* const* active;         // optional: the job is skipped when *active is null
     int XS, M, BS, CP, wblocks;
 };
 template <class J> struct JobBatch {

@@ -91,6 +91,7 @@ __global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     const float* __restrict__ Wr = jb.Wr;
     const int16_t* __restrict__ x = jb.xptr ? *jb.xptr : jb.x;
+    if (!x) return;                                             // stream idle this launch
     const int* __restrict__ cmap = jb.cmap;
     const int CP = jb.CP, W = jb.W, C = jb.C;
     const int64_t ld = jb.ld;
@@ -139,6 +140,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     int mt, nt;
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
+    if (jb.active && !*jb.active) return;                      // stream idle this launch
     const float* __restrict__ A = jb.A;
     const float* __restrict__ Bm = jb.Bm;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -183,6 +185,7 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     __shared__ float xs[256 + 2 * XOFF];
     const int j = job_of(bt.start, bt.n, blockIdx.x);
     const PcconvJob& jb = bt.job[j];
+    if (jb.active && !*jb.active) return;                      // stream idle this launch
     const int k = blockIdx.x - bt.start[j];
     const int c = k / jb.wblocks, w0 = (k % jb.wblocks) * 256, tid = threadIdx.x;
     const int M = jb.M;
@@ -228,6 +231,7 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     int mt, nt;
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
+    if (jb.active && !*jb.active) return;                      // stream idle this launch
     const __bf16* __restrict__ A = jb.Ab;
     const __bf16* __restrict__ Bm = jb.Bmb;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -422,6 +426,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
 #pragma clang fp contract(off)
     const int sj = job_of(bt.g_start, bt.n, blockIdx.x);
     const PeelParams& p = bt.p[sj];
+    if (p.active && !*p.active) return;                         // stream idle: every workgroup leaves
     // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
     // immediate offset (no per-row address registers); the norm table comes first.
     __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template reciprocal norm (0 past T)

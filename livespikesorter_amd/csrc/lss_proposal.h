@@ -65,9 +65,10 @@ int pn_groups(int C);
 size_t pn_part_floats(int G, int Wp);
 
 // forward from the matcher's whitened buffer (rows = channels, data at column xoff)
+// active (nullable): the launches do nothing when *active is null (an idle stream of a group)
 void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
-                hipStream_t s);
-void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s);
+                hipStream_t s, const int16_t* const* active = nullptr);
+void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s, const int16_t* const* active = nullptr);
 // loss and gradients (into grad [PN_NPARAMS]) for labels y [G][Wp] (0/1 bytes)
 void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight,
                  float* grad, hipStream_t s);

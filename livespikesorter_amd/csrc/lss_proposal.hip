@@ -27,7 +27,9 @@ constexpr int TW = PN_TILE + PN_KW - 1;      // input tile columns of a convolut
 // input: group g, row ci = whitened channel 16 g - 4 + ci (zero off the probe)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_pn_input(const float* __restrict__ xw, int64_t xs, int xoff, int C,
-                                                  int W, int Wp, __bf16* __restrict__ a0) {
+                                                  int W, int Wp, __bf16* __restrict__ a0,
+                                                  const int16_t* const* active) {
+    if (active && !*active) return;
     const int w = blockIdx.x * 256 + threadIdx.x;
     const int row = blockIdx.y;                       // g * PN_CIN + ci
     if (w >= Wp) return;
@@ -45,7 +47,9 @@ __global__ __launch_bounds__(256) void k_pn_input(const float* __restrict__ xw, 
 template <int CI, int MODE>
 __global__ __launch_bounds__(256) void k_pn_conv(const __bf16* __restrict__ in, const __bf16* __restrict__ wt,
                                                  const float* __restrict__ bias, const __bf16* __restrict__ mask,
-                                                 __bf16* __restrict__ out, int W, int Wp) {
+                                                 __bf16* __restrict__ out, int W, int Wp,
+                                                 const int16_t* const* active) {
+    if (active && !*active) return;
     constexpr int KR = CI * PN_KW;
     constexpr int KP = (KR + 15) / 16 * 16;
     constexpr int XR = TW + 8;
@@ -106,7 +110,9 @@ __global__ __launch_bounds__(256) void k_pn_conv(const __bf16* __restrict__ in, 
 // (HIP's device expf)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_pn_head(const __bf16* __restrict__ a3, const float* __restrict__ params,
-                                                 int W, int Wp, float* __restrict__ z, float* __restrict__ p) {
+                                                 int W, int Wp, float* __restrict__ z, float* __restrict__ p,
+                                                 const int16_t* const* active) {
+    if (active && !*active) return;
     __shared__ float w4[PN_CH * PN_KW + 1];
     for (int i = threadIdx.x; i < PN_CH * PN_KW + 1; i += 256) w4[i] = params[PN_W4 + i];
     __syncthreads();
@@ -340,7 +346,9 @@ __global__ __launch_bounds__(256) void k_pn_pack(PnWeights wt) {
 // scoring, otherwise a template tile is scored when a group within one of its
 // group span has P >= hi.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pn_gate(const float* __restrict__ p, int G, int W, int Wp, PnGate gt) {
+__global__ __launch_bounds__(256) void k_pn_gate(const float* __restrict__ p, int G, int W, int Wp, PnGate gt,
+                                                 const int16_t* const* active) {
+    if (active && !*active) return;
     __shared__ int s_state[64];                       // per group: 0 off, 1 uncertain, 2 on
     __shared__ int s_unc;
     const int n = blockIdx.x;
@@ -387,21 +395,23 @@ size_t pn_part_floats(int G, int Wp) {
 }
 
 void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
-                hipStream_t s) {
+                hipStream_t s, const int16_t* const* active) {
     const int G = b.G, Wp = b.Wp;
-    hipLaunchKernelGGL(k_pn_input, dim3((Wp + 255) / 256, G * PN_CIN), dim3(256), 0, s, xw, xs, xoff, C, W, Wp, b.a0);
+    hipLaunchKernelGGL(k_pn_input, dim3((Wp + 255) / 256, G * PN_CIN), dim3(256), 0, s, xw, xs, xoff, C, W, Wp, b.a0,
+                       active);
     const dim3 grid((W + PN_TILE - 1) / PN_TILE, G);
     hipLaunchKernelGGL((k_pn_conv<PN_CIN, 0>), grid, dim3(256), 0, s, b.a0, wt.wf1, wt.params + PN_B1,
-                       (const __bf16*)nullptr, b.a1, W, Wp);
+                       (const __bf16*)nullptr, b.a1, W, Wp, active);
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, b.a1, wt.wf2, wt.params + PN_B2,
-                       (const __bf16*)nullptr, b.a2, W, Wp);
+                       (const __bf16*)nullptr, b.a2, W, Wp, active);
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, b.a2, wt.wf3, wt.params + PN_B3,
-                       (const __bf16*)nullptr, b.a3, W, Wp);
-    hipLaunchKernelGGL(k_pn_head, dim3((Wp + 255) / 256, G), dim3(256), 0, s, b.a3, wt.params, W, Wp, b.z, b.p);
+                       (const __bf16*)nullptr, b.a3, W, Wp, active);
+    hipLaunchKernelGGL(k_pn_head, dim3((Wp + 255) / 256, G), dim3(256), 0, s, b.a3, wt.params, W, Wp, b.z, b.p,
+                       active);
 }
 
-void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s) {
-    hipLaunchKernelGGL(k_pn_gate, dim3((W + 63) / 64), dim3(256), 0, s, b.p, b.G, W, b.Wp, gt);
+void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s, const int16_t* const* active) {
+    hipLaunchKernelGGL(k_pn_gate, dim3((W + 63) / 64), dim3(256), 0, s, b.p, b.G, W, b.Wp, gt, active);
 }
 
 void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight, float* grad,
@@ -415,9 +425,9 @@ void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int 
     const dim3 grid((W + PN_TILE - 1) / PN_TILE, G);
     // data gradients through layers 3 and 2 (masked by the ReLU of the layer below)
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, b.d3, wt.wb3, (const float*)nullptr, b.a2,
-                       b.d2, W, Wp);
+                       b.d2, W, Wp, (const int16_t* const*)nullptr);
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, b.d2, wt.wb2, (const float*)nullptr, b.a1,
-                       b.d1, W, Wp);
+                       b.d1, W, Wp, (const int16_t* const*)nullptr);
     // weight gradients of layers 3, 2, 1
     const int slabs = (W + PN_SLAB - 1) / PN_SLAB;
     const dim3 sg(slabs, G);

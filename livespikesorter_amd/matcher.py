@@ -41,6 +41,14 @@ class OnlineSpikesV2:
         self.info = self.query()
         self.last_W = 0
 
+    def autotune(self, dev_ptr: int, n_samples: int, ld: int, reps: int = 5) -> list:
+        """Time each peel split on a device batch and keep the fastest for this
+        width (lss_autotune).  Returns [(split, rows, ms, chosen)]."""
+        from ._capi import TuneResult
+        r = TuneResult()
+        check(lib().lss_autotune(self._h, ct.c_void_p(dev_ptr), n_samples, ld, reps, ct.byref(r)))
+        return [(r.split[i], r.rows[i], r.ms[i], i == r.chosen) for i in range(r.n)]
+
     def set_proposal(self, net, lo: float = 0.25, hi: float = 0.75, reach: int = 4) -> None:
         """Gate score tiles with a :class:`~livespikesorter_amd.proposal.ProposalNet`
         (``None`` detaches it).  Keep the network alive while it is attached."""
@@ -72,10 +80,13 @@ class OnlineSpikesV2:
         return info
 
     def _out(self, cap: int):
-        ev = np.zeros(cap, EVENT_DTYPE)
-        nC, K = self.info.nC, self.info.K
-        feat = np.zeros((cap, nC, K), np.float32)
-        return ev, feat
+        # output buffers are allocated once per capacity and reused (a fresh
+        # 16 MB feature buffer per call costs more than a whole chunk); results
+        # are returned as copies of the filled part
+        if getattr(self, "_bufs", None) is None or self._bufs[0] != cap:
+            nC, K = self.info.nC, self.info.K
+            self._bufs = (cap, np.zeros(cap, EVENT_DTYPE), np.zeros((cap, nC, K), np.float32))
+        return self._bufs[1], self._bufs[2]
 
     # -- matching ------------------------------------------------------------
     def kilosortMatchingPursuit(self, batch: np.ndarray, first_sample: int = 0, *, chan_map=None,
@@ -94,7 +105,7 @@ class OnlineSpikesV2:
                                       None if cm is None else cm.ctypes.data, int(first_sample),
                                       ev.ctypes.data, cap, ct.byref(n), feat.ctypes.data))
         self.last_W = W
-        return ev[:n.value], feat[:n.value]
+        return ev[:n.value].copy(), feat[:n.value].copy()
 
     process_batch = kilosortMatchingPursuit
 
@@ -109,7 +120,7 @@ class OnlineSpikesV2:
                                              int(first_sample), ev.ctypes.data, cap, ct.byref(n),
                                              feat.ctypes.data if features else None))
         self.last_W = n_samples
-        return ev[:n.value], feat[:n.value]
+        return ev[:n.value].copy(), feat[:n.value].copy()
 
     def process_chunk(self, chunk: np.ndarray, first_sample: int, *, chan_map=None,
                       cap: int = 65536):
@@ -122,13 +133,13 @@ class OnlineSpikesV2:
         check(lib().lss_process_chunk(self._h, x.ctypes.data, n_s, ld,
                                       None if cm is None else cm.ctypes.data, int(first_sample),
                                       ev.ctypes.data, cap, ct.byref(n), feat.ctypes.data))
-        return ev[:n.value], feat[:n.value]
+        return ev[:n.value].copy(), feat[:n.value].copy()
 
     def flush(self, *, cap: int = 65536):
         ev, feat = self._out(cap)
         n = ct.c_int64(0)
         check(lib().lss_flush(self._h, ev.ctypes.data, cap, ct.byref(n), feat.ctypes.data))
-        return ev[:n.value], feat[:n.value]
+        return ev[:n.value].copy(), feat[:n.value].copy()
 
     def stage_times(self, reset: bool = False):
         """Accumulated device ms per stage (needs ``profile=True``): (dict, batches)."""
@@ -179,23 +190,36 @@ class StreamGroup:
         check(lib().lss_group_query(self._g, stream, ct.byref(info)))
         return info
 
+    def set_proposal(self, stream: int, net, lo: float = 0.25, hi: float = 0.75, reach: int = 4) -> None:
+        """Gate one stream's score tiles with a proposal network (``None`` detaches)."""
+        check(lib().lss_group_set_proposal(self._g, stream, net._p if net is not None else None, lo, hi, reach))
+        self._props = getattr(self, "_props", {})
+        self._props[stream] = net
+
     def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536):
         """``batches``: per stream an int16 array ``[batch][ld]`` (or, with
-        ``device=True``, a device pointer and ``lds`` the row strides)."""
+        ``device=True``, a device pointer and ``lds`` the row strides); ``None``
+        for a stream with no batch this call (it returns no events)."""
         n = self.n
         if device:
-            ptrs = [int(b) for b in batches]
+            ptrs = [int(b) if b is not None else None for b in batches]
             ld = list(lds)
             keep = None
         else:
-            keep = [np.ascontiguousarray(b, dtype=np.int16) for b in batches]
+            keep = [np.ascontiguousarray(b, dtype=np.int16) if b is not None else None for b in batches]
             for b in keep:
-                if b.shape[0] != self.batch:
+                if b is not None and b.shape[0] != self.batch:
                     raise ValueError("every batch must have the group's width")
-            ptrs = [b.ctypes.data for b in keep]
-            ld = [b.shape[1] for b in keep]
-        evs = [np.zeros(cap, EVENT_DTYPE) for _ in range(n)]
-        feats = [np.zeros((cap, self.info[s].nC, self.info[s].K), np.float32) for s in range(n)]
+            ptrs = [b.ctypes.data if b is not None else None for b in keep]
+            # an idle stream keeps the row stride it was last given (no re-capture)
+            last = getattr(self, "_ld", None) or [next(b for b in keep if b is not None).shape[1]] * n
+            ld = list(lds) if lds is not None else [b.shape[1] if b is not None else last[s]
+                                                     for s, b in enumerate(keep)]
+            self._ld = ld
+        if getattr(self, "_bufs", None) is None or self._bufs[0] != cap:     # reused across calls
+            self._bufs = (cap, [np.zeros(cap, EVENT_DTYPE) for _ in range(n)],
+                          [np.zeros((cap, self.info[s].nC, self.info[s].K), np.float32) for s in range(n)])
+        evs, feats = self._bufs[1], self._bufs[2]
         c_ptrs = (ct.c_void_p * n)(*ptrs)
         c_ld = (ct.c_int32 * n)(*ld)
         c_first = (ct.c_int64 * n)(*[int(f) for f in first_samples])
@@ -205,7 +229,7 @@ class StreamGroup:
         c_feat = (ct.c_void_p * n)(*[f.ctypes.data for f in feats])
         check(lib().lss_group_process(self._g, c_ptrs, c_ld, int(device), c_first, c_ev, c_cap, c_n, c_feat))
         del keep
-        return [(evs[s][:c_n[s]], feats[s][:c_n[s]]) for s in range(n)]
+        return [(evs[s][:c_n[s]].copy(), feats[s][:c_n[s]].copy()) for s in range(n)]
 
     def close(self) -> None:
         if self._g:

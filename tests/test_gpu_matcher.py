@@ -153,6 +153,26 @@ def test_template_split_is_exact(cfg2, model_dirs):
         assert b.query().peel_split == 8
 
 
+def test_autotune_keeps_results(cfg2):
+    """lss_autotune times every split on a device batch, keeps the fastest for
+    that width, and the output stays bit-identical."""
+    from livespikesorter_amd._hip import DeviceBuffer, synchronize
+    m, d, tab, x = cfg2
+    batch = np.ascontiguousarray(x[:3122])
+    dx = DeviceBuffer.from_array(batch)
+    synchronize()
+    with OnlineSpikesV2(d) as a, OnlineSpikesV2(d) as b:
+        ea, fa = a.process_batch_device(dx.ptr, 3122, 384)
+        res = b.autotune(dx.ptr, 3122, 384, reps=3)
+        print("autotune:", res)
+        eb, fb = b.process_batch_device(dx.ptr, 3122, 384)
+        chosen = [r for r in res if r[3]][0]
+        assert b.query().peel_split == chosen[0]
+    dx.free()
+    assert len(res) >= 2 and res[0][0] == 1
+    assert np.array_equal(ea, eb) and np.array_equal(fa, fb)
+
+
 def test_deterministic(cfg2):
     m, d, tab, x = cfg2
     batch = x[3000:6122]
@@ -246,7 +266,7 @@ def test_stream_group_matches_single_handles(model_dirs):
     for k, sp in enumerate(specs):
         m, d = model_dirs("grp%d" % k, C=sp["C"], T=sp["T"], seed=sp["seed"], whitening="banded",
                           rate=(2.0, 20.0), amp=(11.0, 3.0))
-        x, _ = synth.make_recording(m, 3 * 1622 + 200, seed=sp["seed"] + 100)
+        x, _ = synth.make_recording(m, 4 * 1500 + 1622, seed=sp["seed"] + 100)
         dirs.append(d)
         recs.append(x)
     W = 1622
@@ -260,5 +280,17 @@ def test_stream_group_matches_single_handles(model_dirs):
                 assert len(e1) > 0
                 assert np.array_equal(ev, e1), f"stream {k} tick {tick}"
                 assert np.array_equal(feat, f1)
+        # a tick with stream 1 idle: the others are unchanged, stream 1 returns nothing
+        bs = [x[4 * 1500:4 * 1500 + W] for x in recs]
+        idle = [bs[0], None, bs[2]]
+        out = grp.process(idle, [4 * 1500] * 3)
+        assert len(out[1][0]) == 0
+        for k in (0, 2):
+            e1, f1 = singles[k].process_batch(bs[k], first_sample=4 * 1500)
+            assert np.array_equal(out[k][0], e1) and np.array_equal(out[k][1], f1)
+        # and the next full tick still matches (stream 1's state was untouched)
+        out = grp.process(bs, [4 * 1500] * 3)
+        e1, f1 = singles[1].process_batch(bs[1], first_sample=4 * 1500)
+        assert np.array_equal(out[1][0], e1)
     for sp in singles:
         sp.close()
