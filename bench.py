@@ -138,7 +138,8 @@ def bench_group(args) -> dict:
     for sp in singles:
         sp.close()
     grp = StreamGroup(dirs, batch=W)
-    plat, pbusy, pticks = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205)
+    plat, pbusy, pticks = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205,
+                                        lanes=args.lanes)
     grp.close()
     for bf in bufs:
         bf.free()
@@ -175,38 +176,57 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
-def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed):
+def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed, lanes=2):
     """Stream the group's recordings in real time: chunk k of stream s is handed
     over at k * N / FS plus a uniform +-5 ms jitter.  Whenever chunks are
-    waiting, one launch matches the oldest waiting chunk of every stream that
-    has one (the others idle).  Latency = events on the host - hand-over."""
+    waiting and a launch lane is free, one launch on that lane matches the
+    oldest waiting chunk of every stream that has one and is not in flight
+    (the others idle); with two lanes, a chunk arriving during a launch starts
+    at once on the other lane.  Latency = events on the host - hand-over."""
+    from concurrent.futures import ThreadPoolExecutor
     S = grp.n
     period = N / FS
     rng = np.random.default_rng(seed)
     n_chunks = min(n_ticks, int(seconds / period))
     arrive = np.arange(n_chunks)[:, None] * period + 0.005 + rng.uniform(-0.005, 0.005, size=(n_chunks, S))
     nxt = [0] * S
-    lat, busy, ticks = [], 0.0, 0
+    busy_s = [False] * S
+    lat, busy, launches = [], 0.0, 0
+    free = list(range(lanes))
+    inflight = {}
     t0 = time.perf_counter()
-    while min(nxt) < n_chunks:
-        now = time.perf_counter() - t0
-        ready = [s for s in range(S) if nxt[s] < n_chunks and arrive[nxt[s], s] <= now]
-        if not ready:
-            wait = min(arrive[nxt[s], s] for s in range(S) if nxt[s] < n_chunks) - now
-            if wait > 0.0004:
-                time.sleep(wait - 0.0003)
-            continue
-        ptrs = [bufs[s].ptr + nxt[s] * N * C * 2 if s in ready else None for s in range(S)]
+
+    def work(lane, ptrs, first):
         ts = time.perf_counter()
-        grp.process(ptrs, [nxt[s] * N for s in range(S)], device=True, lds=[C] * S)
-        te = time.perf_counter()
-        busy += te - ts
-        ticks += 1
-        for s in ready:
-            lat.append((te - t0 - arrive[nxt[s], s]) * 1e3)
-            nxt[s] += 1
+        grp.process(ptrs, first, device=True, lds=[C] * S, lane=lane)
+        return ts, time.perf_counter()
+
+    with ThreadPoolExecutor(lanes) as pool:
+        while min(nxt) < n_chunks or inflight:
+            for lane in [l for l, (f, _, _) in inflight.items() if f.done()]:
+                f, streams, ks = inflight.pop(lane)
+                ts, te = f.result()
+                busy += te - ts
+                for s, k in zip(streams, ks):
+                    lat.append((te - t0 - arrive[k, s]) * 1e3)
+                    busy_s[s] = False
+                free.append(lane)
+            now = time.perf_counter() - t0
+            ready = [s for s in range(S) if not busy_s[s] and nxt[s] < n_chunks and arrive[nxt[s], s] <= now]
+            if ready and free:
+                lane = free.pop(0)
+                ptrs = [bufs[s].ptr + nxt[s] * N * C * 2 if s in ready else None for s in range(S)]
+                first = [nxt[s] * N for s in range(S)]
+                ks = [nxt[s] for s in ready]
+                for s in ready:
+                    busy_s[s] = True
+                    nxt[s] += 1
+                inflight[lane] = (pool.submit(work, lane, ptrs, first), ready, ks)
+                launches += 1
+                continue
+            time.sleep(0.0002)
     wall = time.perf_counter() - t0
-    return lat, busy / wall, ticks
+    return lat, busy / wall, launches
 
 
 def make_target_data(cache: Path, seconds: float, n_streams: int = 8):
@@ -298,7 +318,8 @@ def bench_target(args) -> dict:
                 ident_n += i * nn
                 within_n += w_ * nn
                 tot += nn
-        lat, busy, nt = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205)
+        lat, busy, nt = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205,
+                                      lanes=args.lanes)
         q = [grp.query(s) for s in range(S)]
         grp.close()
         rtf = n_ticks * S * N / FS / el
@@ -517,6 +538,7 @@ def main() -> int:
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
+    ap.add_argument("--lanes", type=int, default=4, help="concurrent launch lanes of the paced group runs (1-4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized buffer three times (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")

@@ -194,6 +194,12 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
 int lss_group_process(struct lss_group* g, const int16_t* const* samples, const int32_t* ld,
                       int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
                       const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
+/* The same on launch lane `lane` (0..3; lss_group_process uses lane 0).
+ * Each lane has its own HIP stream and graph; lanes may be driven from
+ * several host threads at once, provided no stream has a batch in two. */
+int lss_group_process_lane(struct lss_group* g, int32_t lane, const int16_t* const* samples, const int32_t* ld,
+                           int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
+                           const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
 int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
 /* lss_set_proposal for one stream of a group (the launch graph is rebuilt). */
 int lss_group_set_proposal(struct lss_group* g, int32_t stream, struct lss_proposal* net, float lo, float hi,

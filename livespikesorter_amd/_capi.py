@@ -33,7 +33,7 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_group_query", "lss_group_destroy", "lss_proposal_n_params", "lss_proposal_create",
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
-                 "lss_autotune", "lss_group_set_proposal")
+                 "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane")
 NSTAGES = 6
 STAGE_NAMES = ("h2d_copy", "whiten", "proposal", "pcconv", "project", "peel")
 
@@ -108,6 +108,7 @@ def lib() -> ct.CDLL:
         L.lss_group_process.argtypes = [ct.c_void_p, ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_int32), ct.c_int32,
                                         ct.POINTER(ct.c_int64), ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_int64),
                                         ct.POINTER(ct.c_int64), ct.POINTER(ct.c_void_p)]
+        L.lss_group_process_lane.argtypes = [ct.c_void_p, ct.c_int32] + L.lss_group_process.argtypes[1:]
         L.lss_group_query.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(Info)]
         L.lss_group_destroy.argtypes = [ct.c_void_p]
         L.lss_proposal_n_params.restype = ct.c_int32

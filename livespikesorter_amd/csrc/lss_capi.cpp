@@ -1276,27 +1276,42 @@ const char* lss_last_error(void) { return g_last_error.c_str(); }
 // the streams (a barrier counter per stream), and every table is padded to the
 // group's register tile height so one kernel instance serves all template counts.
 // ---------------------------------------------------------------------------
-struct lss_group {
-    std::vector<std::unique_ptr<lss_handle>> hs;
-    int n = 0, W = 0, R = 0, CW = 0, dev = 0;
+// A launch lane: its own HIP stream, captured graph and input-pointer table.
+// Lanes run concurrently on disjoint sets of streams (a stream idle in a
+// lane's launch is untouched by it).
+struct GroupLane {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    const int16_t** h_xptr = nullptr;    // pinned: this tick's input pointers
-    const int16_t** d_xptr = nullptr;    // device copy the whiten jobs read
+    const int16_t** h_xptr = nullptr;    // pinned: this launch's input pointers (null = idle)
+    const int16_t** d_xptr = nullptr;    // device copy the kernels read
     std::vector<int32_t> ld_captured;
     WhitenBatch wb{};
     PcconvBatch cb{};
     ProjectBatch pb{};
     PeelBatch kb{};
     ResetBatch rb{};
-    ~lss_group() {
-        (void)hipSetDevice(dev);
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (h_xptr) (void)hipHostFree(h_xptr);
-        if (d_xptr) (void)hipFree(d_xptr);
-        if (stream) (void)hipStreamDestroy(stream);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+constexpr int GROUP_LANES = 4;
+
+struct lss_group {
+    std::vector<std::unique_ptr<lss_handle>> hs;
+    int n = 0, W = 0, R = 0, CW = 0, dev = 0;
+    GroupLane lanes[GROUP_LANES];
+    ~lss_group() {
+        (void)hipSetDevice(dev);
+        for (auto& L : lanes) {
+            L.drop_graph();
+            if (L.h_xptr) (void)hipHostFree(L.h_xptr);
+            if (L.d_xptr) (void)hipFree(L.d_xptr);
+            if (L.stream) (void)hipStreamDestroy(L.stream);
+        }
     }
 };
 
@@ -1311,40 +1326,51 @@ int64_t peek_templates(const std::string& dir_in) {
 }
 
 // (Re)build the job tables and capture the tick's launch sequence into a graph.
-void group_capture(lss_group* g, const int32_t* ld) {
+void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     const int n = g->n;
-    g->wb = WhitenBatch{}; g->cb = PcconvBatch{}; g->pb = ProjectBatch{}; g->kb = PeelBatch{}; g->rb = ResetBatch{};
-    g->wb.n = g->cb.n = g->pb.n = g->kb.n = g->rb.n = n;
+    L.wb = WhitenBatch{}; L.cb = PcconvBatch{}; L.pb = ProjectBatch{}; L.kb = PeelBatch{}; L.rb = ResetBatch{};
+    L.wb.n = L.cb.n = L.pb.n = L.kb.n = L.rb.n = n;
     for (int s2 = 0; s2 < n; ++s2) {
         lss_handle* h = g->hs[s2].get();
         BatchPlan bp;
-        plan_batch(h, nullptr, &g->d_xptr[s2], g->W, ld[s2], g->CW, g->R, 1, bp);
-        g->wb.job[s2] = bp.wj;
-        g->wb.start[s2 + 1] = g->wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
-        g->cb.job[s2] = bp.cj;
-        g->cb.start[s2 + 1] = g->cb.start[s2] + bp.cj.wblocks * h->C;
-        g->pb.job[s2] = bp.pj;
-        g->pb.start[s2 + 1] = g->pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
-        g->kb.p[s2] = bp.pp;
-        g->kb.g_start[s2 + 1] = g->kb.g_start[s2] + bp.G;
-        g->rb.bar[s2] = h->d_bar; g->rb.iter_cnt[s2] = h->d_iter_cnt; g->rb.n_iter[s2] = h->max_iter;
-        g->rb.err[s2] = h->d_err; g->rb.prof[s2] = h->d_prof;
+        plan_batch(h, nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, 1, bp);
+        L.wb.job[s2] = bp.wj;
+        L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
+        L.cb.job[s2] = bp.cj;
+        L.cb.start[s2 + 1] = L.cb.start[s2] + bp.cj.wblocks * h->C;
+        L.pb.job[s2] = bp.pj;
+        L.pb.start[s2 + 1] = L.pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
+        L.kb.p[s2] = bp.pp;
+        L.kb.g_start[s2 + 1] = L.kb.g_start[s2] + bp.G;
+        L.rb.bar[s2] = h->d_bar; L.rb.iter_cnt[s2] = h->d_iter_cnt; L.rb.n_iter[s2] = h->max_iter;
+        L.rb.err[s2] = h->d_err; L.rb.prof[s2] = h->d_prof; L.rb.active[s2] = &L.d_xptr[s2];
     }
-    if (g->exec) { HIPCHK(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
-    if (g->graph) { HIPCHK(hipGraphDestroy(g->graph)); g->graph = nullptr; }
-    hipStream_t st = g->stream;
+    L.drop_graph();
+    hipStream_t st = L.stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    HIPCHK(hipMemcpyAsync(g->d_xptr, g->h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
-    launch_reset_batch(g->rb, st);
-    launch_whiten(g->wb, st);
-    for (int s2 = 0; s2 < n; ++s2)
-        if (g->hs[s2]->prop) run_gate(g->hs[s2].get(), g->W, st, &g->d_xptr[s2]);
-    launch_pcconv(g->cb, g->hs[0]->K, st);
-    launch_project(g->pb, g->hs[0]->bf16, st);
-    HIPCHK(launch_peel(g->kb, g->R, false, st));
-    HIPCHK(hipStreamEndCapture(st, &g->graph));
-    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
-    g->ld_captured.assign(ld, ld + n);
+    HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
+    launch_reset_batch(L.rb, st);
+    launch_whiten(L.wb, st);
+    {   // the proposal networks of all gated streams: one dispatch per layer
+        PnFwdJob fj[PN_MAX_JOBS];
+        PnGateJob gj[PN_MAX_JOBS];
+        int ng = 0;
+        for (int s2 = 0; s2 < n; ++s2) {
+            lss_handle* h = g->hs[s2].get();
+            if (!h->prop) continue;
+            fj[ng] = PnFwdJob{h->d_Xw, h->XS, XOFF, h->C, g->W, h->prop->wt, h->pnb, &L.d_xptr[s2]};
+            gj[ng] = PnGateJob{h->pnb, h->gate, g->W, &L.d_xptr[s2]};
+            ++ng;
+        }
+        pn_forward_batch(fj, ng, st);
+        pn_gate_batch(gj, ng, st);
+    }
+    launch_pcconv(L.cb, g->hs[0]->K, st);
+    launch_project(L.pb, g->hs[0]->bf16, st);
+    HIPCHK(launch_peel(L.kb, g->R, false, st));
+    HIPCHK(hipStreamEndCapture(st, &L.graph));
+    HIPCHK(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
+    L.ld_captured.assign(ld, ld + n);
 }
 
 }  // namespace
@@ -1383,9 +1409,11 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
             throw LssError(LSS_ERR_ARG, "the group's peel grid (" + std::to_string(gsum) +
                                             " workgroups) is not co-resident on this device");
         HIPCHK(hipSetDevice(g->dev));
-        HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-        HIPCHK(hipHostMalloc((void**)&g->h_xptr, sizeof(void*) * MAX_JOBS, hipHostMallocDefault));
-        HIPCHK(hipMalloc((void**)&g->d_xptr, sizeof(void*) * MAX_JOBS));
+        for (auto& L : g->lanes) {
+            HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+            HIPCHK(hipHostMalloc((void**)&L.h_xptr, sizeof(void*) * MAX_JOBS, hipHostMallocDefault));
+            HIPCHK(hipMalloc((void**)&L.d_xptr, sizeof(void*) * MAX_JOBS));
+        }
         *out = g.release();
     });
 }
@@ -1393,9 +1421,17 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
 int lss_group_process(lss_group* g, const int16_t* const* samples, const int32_t* ld, int32_t device_input,
                       const int64_t* first_sample, lss_event* const* ev, const int64_t* ev_cap, int64_t* n_ev,
                       float* const* feat) {
+    return lss_group_process_lane(g, 0, samples, ld, device_input, first_sample, ev, ev_cap, n_ev, feat);
+}
+
+int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* samples, const int32_t* ld,
+                           int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
+                           const int64_t* ev_cap, int64_t* n_ev, float* const* feat) {
     return guarded([&] {
         if (!g || !samples || !ld || !first_sample || !ev || !ev_cap || !n_ev)
             throw LssError(LSS_ERR_ARG, "null argument");
+        if (lane < 0 || lane >= GROUP_LANES) throw LssError(LSS_ERR_ARG, "no such lane");
+        GroupLane& L = g->lanes[lane];
         HIPCHK(hipSetDevice(g->dev));
         auto t0 = std::chrono::steady_clock::now();
         const int n = g->n;
@@ -1404,24 +1440,24 @@ int lss_group_process(lss_group* g, const int16_t* const* samples, const int32_t
             lss_handle* h = g->hs[s2].get();
             n_ev[s2] = 0;
             if (!samples[s2]) {             // idle this tick: every kernel of the stream returns at once
-                g->h_xptr[s2] = nullptr;
+                L.h_xptr[s2] = nullptr;
                 continue;
             }
             ++active;
             set_cmap(h, nullptr, ld[s2]);
             if (device_input) {
-                g->h_xptr[s2] = samples[s2];
+                L.h_xptr[s2] = samples[s2];
             } else {
                 int16_t* dx = ensure_dx(h, (size_t)g->W * ld[s2]);
                 HIPCHK(hipMemcpyAsync(dx, samples[s2], (size_t)g->W * ld[s2] * sizeof(int16_t),
-                                      hipMemcpyHostToDevice, g->stream));
-                g->h_xptr[s2] = dx;
+                                      hipMemcpyHostToDevice, L.stream));
+                L.h_xptr[s2] = dx;
             }
         }
         if (!active) return;
-        if (!g->exec || !std::equal(ld, ld + n, g->ld_captured.begin())) group_capture(g, ld);
-        HIPCHK(hipGraphLaunch(g->exec, g->stream));
-        HIPCHK(hipStreamSynchronize(g->stream));
+        if (!L.exec || !std::equal(ld, ld + n, L.ld_captured.begin())) group_capture(g, L, ld);
+        HIPCHK(hipGraphLaunch(L.exec, L.stream));
+        HIPCHK(hipStreamSynchronize(L.stream));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         for (int s2 = 0; s2 < n; ++s2) {
             if (!samples[s2]) continue;
@@ -1446,11 +1482,10 @@ int lss_group_set_proposal(lss_group* g, int32_t stream, lss_proposal* net, floa
     return guarded([&] {
         if (!g || stream < 0 || stream >= g->n) throw LssError(LSS_ERR_ARG, "bad argument");
         HIPCHK(hipSetDevice(g->dev));
-        HIPCHK(hipStreamSynchronize(g->stream));
+        for (auto& L : g->lanes) HIPCHK(hipStreamSynchronize(L.stream));
         const int rc = lss_set_proposal(g->hs[stream].get(), net, lo, hi, reach);
         if (rc != LSS_OK) throw LssError(rc, g_last_error);
-        if (g->exec) { HIPCHK(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
-        if (g->graph) { HIPCHK(hipGraphDestroy(g->graph)); g->graph = nullptr; }
+        for (auto& L : g->lanes) L.drop_graph();
     });
 }
 

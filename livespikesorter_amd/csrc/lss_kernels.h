@@ -178,6 +178,7 @@ struct ResetBatch {
     int n_iter[MAX_JOBS];
     int* err[MAX_JOBS];
     unsigned long long* prof[MAX_JOBS];
+    const int16_t* const* active[MAX_JOBS];   // nullable: a stream idle this launch keeps its counters
 };
 void launch_reset_batch(const ResetBatch& b, hipStream_t s);
 

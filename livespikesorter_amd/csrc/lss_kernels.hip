@@ -776,20 +776,25 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 // (a column under lo2 in BF16 cannot beat an accepted one in FP32).
                 // The correction of (t, w) is fixed (subtractions are applied to both
                 // alike), so it is computed once per (column, template) and cached.
-                __syncthreads();
+                // The columns' own threads (tid < CW, written just above) list the
+                // columns that need a dot product; one barrier publishes the list.
                 const int lane = tid & 63, wv = tid >> 6;
                 bool need = false;
                 if (tid < CW && c0 + tid < c1) need = s_cf[tid] >= p.lo2 && s_rt[tid] != s_im[tid];
                 const uint64_t bal = __ballot(need);
                 if (tid < CW && lane == 0) s_hcnt_w[wv] = __popcll(bal);
-                __syncthreads();
+                if (CW > 64) __syncthreads();            // wave counts of other waves
                 int nres = 0, base = 0;
-                for (int k = 0; k < (CW + 63) / 64; ++k) {
-                    if (k == wv) base = nres;
-                    nres += s_hcnt_w[k];
+                if (tid < CW) {
+                    for (int k = 0; k < (CW + 63) / 64; ++k) {
+                        if (k == wv) base = nres;
+                        nres += s_hcnt_w[k];
+                    }
+                    if (need) s_flag[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
+                    if (tid == 0) s_ncand = nres;
                 }
-                if (need) s_flag[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
                 __syncthreads();
+                nres = s_ncand;
                 for (int q = wv; q < nres; q += PEEL_THREADS / 64) {
                     const int cl = s_flag[q];
                     const int t = s_im[cl], ww = c0 + cl;
@@ -798,10 +803,20 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     const int mt = t / GEMM_BM;
                     const int e0 = p.kb_ptr[mt] * GEMM_BK, e1 = p.kb_ptr[mt + 1] * GEMM_BK;
                     const float* urow = p.U2 + (int64_t)t * p.ldu;
+                    // 8 elements per lane in flight at once: the block index, then the
+                    // template weight and the (scattered) Bpc column entry
                     float acc = 0.f;
-                    for (int e = e0 + lane; e < e1; e += 64) {
-                        const int kc = p.kb_idx[e / GEMM_BK] * GEMM_BK + e % GEMM_BK;
-                        acc += urow[kc] * p.Bpc[(int64_t)kc * p.BS + ww];
+                    for (int e = e0 + lane; e < e1; e += 64 * 8) {
+                        float uq[8], bq[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int ee = e + 64 * q;
+                            const int kc = ee < e1 ? p.kb_idx[ee / GEMM_BK] * GEMM_BK + ee % GEMM_BK : -1;
+                            uq[q] = kc >= 0 ? urow[kc] : 0.f;
+                            bq[q] = kc >= 0 ? p.Bpc[(int64_t)kc * p.BS + ww] : 0.f;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) acc += uq[q] * bq[q];
                     }
                     // butterfly: every lane ends with the same, order-fixed sum
                     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -810,7 +825,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         s_rt[cl] = t;
                     }
                 }
-                __syncthreads();
+                if (nres > 0) __syncthreads();
                 int flip = 0;
                 if (tid < CW && c0 + tid < c1) {
                     float cf = s_cf[tid];
@@ -1097,6 +1112,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
 __global__ void k_reset_batch(ResetBatch b) {
     const int j = blockIdx.x;
     if (j >= b.n) return;
+    if (b.active[j] && !*b.active[j]) return;     // idle stream (maybe running in another launch)
     for (int i = threadIdx.x; i < b.n_iter[j]; i += blockDim.x) b.iter_cnt[j][i] = 0;
     if ((int)threadIdx.x < PROF_WORDS) b.prof[j][threadIdx.x] = 0ull;
     if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) b.bar[j][threadIdx.x * BAR_LINE] = 0u;

@@ -61,6 +61,24 @@ struct PnGate {
     unsigned long long* stats;               // tiles scored, tiles, fallback blocks, blocks (running)
 };
 
+constexpr int PN_MAX_JOBS = 8;   // streams per batched launch
+
+// one stream's forward pass: input (whitened rows, data at column xoff), weights, activations
+struct PnFwdJob {
+    const float* xw; int64_t xs; int xoff; int C; int W;
+    PnWeights wt;
+    PnBuffers b;
+    const int16_t* const* active;    // nullable: the job is skipped when *active is null
+};
+struct PnGateJob {
+    PnBuffers b;
+    PnGate gt;
+    int W;
+    const int16_t* const* active;
+};
+void pn_forward_batch(const PnFwdJob* jobs, int n, hipStream_t s);
+void pn_gate_batch(const PnGateJob* jobs, int n, hipStream_t s);
+
 int pn_groups(int C);
 size_t pn_part_floats(int G, int Wp);
 

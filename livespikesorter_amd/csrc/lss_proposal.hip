@@ -26,13 +26,23 @@ constexpr int TW = PN_TILE + PN_KW - 1;      // input tile columns of a convolut
 // ---------------------------------------------------------------------------
 // input: group g, row ci = whitened channel 16 g - 4 + ci (zero off the probe)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pn_input(const float* __restrict__ xw, int64_t xs, int xoff, int C,
-                                                  int W, int Wp, __bf16* __restrict__ a0,
-                                                  const int16_t* const* active) {
-    if (active && !*active) return;
+struct PnInputArgs {
+    const float* xw; int64_t xs; int xoff; int C; int W; int Wp; int G;
+    __bf16* a0;
+    const int16_t* const* active;
+};
+struct PnInputTable { int n; PnInputArgs a[PN_MAX_JOBS]; };
+
+__global__ __launch_bounds__(256) void k_pn_input(PnInputTable tb) {
+    const PnInputArgs& A = tb.a[blockIdx.z];
+    if (A.active && !*A.active) return;
     const int w = blockIdx.x * 256 + threadIdx.x;
     const int row = blockIdx.y;                       // g * PN_CIN + ci
-    if (w >= Wp) return;
+    if (w >= A.Wp || row >= A.G * PN_CIN) return;
+    const float* xw = A.xw;
+    const int64_t xs = A.xs;
+    const int xoff = A.xoff, C = A.C, W = A.W, Wp = A.Wp;
+    __bf16* a0 = A.a0;
     const int g = row / PN_CIN, ci = row - g * PN_CIN;
     const int ch = g * PN_GROUP - PN_HALO + ci;
     float v = 0.f;
@@ -44,12 +54,24 @@ __global__ __launch_bounds__(256) void k_pn_input(const float* __restrict__ xw, 
 // convolution layer, 32 output channels.  MODE 0: + bias, ReLU (forward).
 // MODE 1: multiplied by (mask > 0) (data gradient through the previous ReLU).
 // ---------------------------------------------------------------------------
+struct PnConvArgs {
+    const __bf16* in; const __bf16* wt; const float* bias; const __bf16* mask; __bf16* out;
+    int W; int Wp; int G;
+    const int16_t* const* active;
+};
+struct PnConvTable { int n; PnConvArgs a[PN_MAX_JOBS]; };
+
 template <int CI, int MODE>
-__global__ __launch_bounds__(256) void k_pn_conv(const __bf16* __restrict__ in, const __bf16* __restrict__ wt,
-                                                 const float* __restrict__ bias, const __bf16* __restrict__ mask,
-                                                 __bf16* __restrict__ out, int W, int Wp,
-                                                 const int16_t* const* active) {
-    if (active && !*active) return;
+__global__ __launch_bounds__(256) void k_pn_conv(PnConvTable tb) {
+    const PnConvArgs& A = tb.a[blockIdx.z];
+    if (A.active && !*A.active) return;
+    if ((int)blockIdx.y >= A.G || (int)blockIdx.x * PN_TILE >= A.W) return;
+    const __bf16* __restrict__ in = A.in;
+    const __bf16* __restrict__ wt = A.wt;
+    const float* __restrict__ bias = A.bias;
+    const __bf16* __restrict__ mask = A.mask;
+    __bf16* __restrict__ out = A.out;
+    const int W = A.W, Wp = A.Wp;
     constexpr int KR = CI * PN_KW;
     constexpr int KP = (KR + 15) / 16 * 16;
     constexpr int XR = TW + 8;
@@ -109,12 +131,23 @@ __global__ __launch_bounds__(256) void k_pn_conv(const __bf16* __restrict__ in, 
 // head: logit z = b4 + sum_{ci,k} W4[ci][k] a3[ci][w + k - 4]; p = sigmoid(z)
 // (HIP's device expf)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pn_head(const __bf16* __restrict__ a3, const float* __restrict__ params,
-                                                 int W, int Wp, float* __restrict__ z, float* __restrict__ p,
-                                                 const int16_t* const* active) {
-    if (active && !*active) return;
+struct PnHeadArgs {
+    const __bf16* a3; const float* params; float* z; float* p;
+    int W; int Wp; int G;
+    const int16_t* const* active;
+};
+struct PnHeadTable { int n; PnHeadArgs a[PN_MAX_JOBS]; };
+
+__global__ __launch_bounds__(256) void k_pn_head(PnHeadTable tb) {
+    const PnHeadArgs& A = tb.a[blockIdx.z];
+    if (A.active && !*A.active) return;
+    if ((int)blockIdx.y >= A.G) return;
+    const __bf16* __restrict__ a3 = A.a3;
+    float* __restrict__ z = A.z;
+    float* __restrict__ p = A.p;
+    const int W = A.W, Wp = A.Wp;
     __shared__ float w4[PN_CH * PN_KW + 1];
-    for (int i = threadIdx.x; i < PN_CH * PN_KW + 1; i += 256) w4[i] = params[PN_W4 + i];
+    for (int i = threadIdx.x; i < PN_CH * PN_KW + 1; i += 256) w4[i] = A.params[PN_W4 + i];
     __syncthreads();
     const int w = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
     if (w >= Wp) return;
@@ -346,9 +379,20 @@ __global__ __launch_bounds__(256) void k_pn_pack(PnWeights wt) {
 // scoring, otherwise a template tile is scored when a group within one of its
 // group span has P >= hi.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pn_gate(const float* __restrict__ p, int G, int W, int Wp, PnGate gt,
-                                                 const int16_t* const* active) {
-    if (active && !*active) return;
+struct PnGateArgs {
+    const float* p; int G; int W; int Wp;
+    PnGate gt;
+    const int16_t* const* active;
+};
+struct PnGateTable { int n; PnGateArgs a[PN_MAX_JOBS]; };
+
+__global__ __launch_bounds__(256) void k_pn_gate(PnGateTable tb) {
+    const PnGateArgs& A = tb.a[blockIdx.z];
+    if (A.active && !*A.active) return;
+    const float* __restrict__ p = A.p;
+    const int G = A.G, W = A.W, Wp = A.Wp;
+    const PnGate& gt = A.gt;
+    if ((int)blockIdx.x * 64 >= W) return;
     __shared__ int s_state[64];                       // per group: 0 off, 1 uncertain, 2 on
     __shared__ int s_unc;
     const int n = blockIdx.x;
@@ -394,24 +438,51 @@ size_t pn_part_floats(int G, int Wp) {
     return (size_t)G * slabs * PN_CH * ((PN_CH * PN_KW + 31) / 32 * 32);
 }
 
+// All streams of a launch run each layer in one dispatch (grid z = stream).
+void pn_forward_batch(const PnFwdJob* jobs, int n, hipStream_t s) {
+    if (n <= 0) return;
+    PnInputTable ti{}; PnConvTable t1{}, t2{}, t3{}; PnHeadTable th{};
+    ti.n = t1.n = t2.n = t3.n = th.n = n;
+    int Wpm = 0, Gm = 0, Wm = 0;
+    for (int j = 0; j < n; ++j) {
+        const PnFwdJob& J = jobs[j];
+        const PnBuffers& b = J.b;
+        ti.a[j] = PnInputArgs{J.xw, J.xs, J.xoff, J.C, J.W, b.Wp, b.G, b.a0, J.active};
+        t1.a[j] = PnConvArgs{b.a0, J.wt.wf1, J.wt.params + PN_B1, nullptr, b.a1, J.W, b.Wp, b.G, J.active};
+        t2.a[j] = PnConvArgs{b.a1, J.wt.wf2, J.wt.params + PN_B2, nullptr, b.a2, J.W, b.Wp, b.G, J.active};
+        t3.a[j] = PnConvArgs{b.a2, J.wt.wf3, J.wt.params + PN_B3, nullptr, b.a3, J.W, b.Wp, b.G, J.active};
+        th.a[j] = PnHeadArgs{b.a3, J.wt.params, b.z, b.p, J.W, b.Wp, b.G, J.active};
+        Wpm = max(Wpm, b.Wp); Gm = max(Gm, b.G); Wm = max(Wm, J.W);
+    }
+    hipLaunchKernelGGL(k_pn_input, dim3((Wpm + 255) / 256, Gm * PN_CIN, n), dim3(256), 0, s, ti);
+    const dim3 grid((Wm + PN_TILE - 1) / PN_TILE, Gm, n);
+    hipLaunchKernelGGL((k_pn_conv<PN_CIN, 0>), grid, dim3(256), 0, s, t1);
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, t2);
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, t3);
+    hipLaunchKernelGGL(k_pn_head, dim3((Wpm + 255) / 256, Gm, n), dim3(256), 0, s, th);
+}
+
+void pn_gate_batch(const PnGateJob* jobs, int n, hipStream_t s) {
+    if (n <= 0) return;
+    PnGateTable tg{};
+    tg.n = n;
+    int Wm = 0;
+    for (int j = 0; j < n; ++j) {
+        tg.a[j] = PnGateArgs{jobs[j].b.p, jobs[j].b.G, jobs[j].W, jobs[j].b.Wp, jobs[j].gt, jobs[j].active};
+        Wm = max(Wm, jobs[j].W);
+    }
+    hipLaunchKernelGGL(k_pn_gate, dim3((Wm + 63) / 64, 1, n), dim3(256), 0, s, tg);
+}
+
 void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
                 hipStream_t s, const int16_t* const* active) {
-    const int G = b.G, Wp = b.Wp;
-    hipLaunchKernelGGL(k_pn_input, dim3((Wp + 255) / 256, G * PN_CIN), dim3(256), 0, s, xw, xs, xoff, C, W, Wp, b.a0,
-                       active);
-    const dim3 grid((W + PN_TILE - 1) / PN_TILE, G);
-    hipLaunchKernelGGL((k_pn_conv<PN_CIN, 0>), grid, dim3(256), 0, s, b.a0, wt.wf1, wt.params + PN_B1,
-                       (const __bf16*)nullptr, b.a1, W, Wp, active);
-    hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, b.a1, wt.wf2, wt.params + PN_B2,
-                       (const __bf16*)nullptr, b.a2, W, Wp, active);
-    hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, b.a2, wt.wf3, wt.params + PN_B3,
-                       (const __bf16*)nullptr, b.a3, W, Wp, active);
-    hipLaunchKernelGGL(k_pn_head, dim3((Wp + 255) / 256, G), dim3(256), 0, s, b.a3, wt.params, W, Wp, b.z, b.p,
-                       active);
+    const PnFwdJob j{xw, xs, xoff, C, W, wt, b, active};
+    pn_forward_batch(&j, 1, s);
 }
 
 void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s, const int16_t* const* active) {
-    hipLaunchKernelGGL(k_pn_gate, dim3((W + 63) / 64), dim3(256), 0, s, b.p, b.G, W, b.Wp, gt, active);
+    const PnGateJob j{b, gt, W, active};
+    pn_gate_batch(&j, 1, s);
 }
 
 void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight, float* grad,
@@ -424,10 +495,12 @@ void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int 
                        W, Wp, b.d3);
     const dim3 grid((W + PN_TILE - 1) / PN_TILE, G);
     // data gradients through layers 3 and 2 (masked by the ReLU of the layer below)
-    hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, b.d3, wt.wb3, (const float*)nullptr, b.a2,
-                       b.d2, W, Wp, (const int16_t* const*)nullptr);
-    hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, b.d2, wt.wb2, (const float*)nullptr, b.a1,
-                       b.d1, W, Wp, (const int16_t* const*)nullptr);
+    PnConvTable c3{}, c2{};
+    c3.n = c2.n = 1;
+    c3.a[0] = PnConvArgs{b.d3, wt.wb3, nullptr, b.a2, b.d2, W, Wp, G, nullptr};
+    c2.a[0] = PnConvArgs{b.d2, wt.wb2, nullptr, b.a1, b.d1, W, Wp, G, nullptr};
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, c3);
+    hipLaunchKernelGGL((k_pn_conv<PN_CH, 1>), grid, dim3(256), 0, s, c2);
     // weight gradients of layers 3, 2, 1
     const int slabs = (W + PN_SLAB - 1) / PN_SLAB;
     const dim3 sg(slabs, G);

@@ -196,7 +196,7 @@ class StreamGroup:
         self._props = getattr(self, "_props", {})
         self._props[stream] = net
 
-    def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536):
+    def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536, lane: int = 0):
         """``batches``: per stream an int16 array ``[batch][ld]`` (or, with
         ``device=True``, a device pointer and ``lds`` the row strides); ``None``
         for a stream with no batch this call (it returns no events)."""
@@ -216,10 +216,12 @@ class StreamGroup:
             ld = list(lds) if lds is not None else [b.shape[1] if b is not None else last[s]
                                                      for s, b in enumerate(keep)]
             self._ld = ld
-        if getattr(self, "_bufs", None) is None or self._bufs[0] != cap:     # reused across calls
-            self._bufs = (cap, [np.zeros(cap, EVENT_DTYPE) for _ in range(n)],
+        bufs = getattr(self, "_bufs", {})                 # reused across calls, one set per lane
+        if lane not in bufs or bufs[lane][0] != cap:
+            bufs[lane] = (cap, [np.zeros(cap, EVENT_DTYPE) for _ in range(n)],
                           [np.zeros((cap, self.info[s].nC, self.info[s].K), np.float32) for s in range(n)])
-        evs, feats = self._bufs[1], self._bufs[2]
+        self._bufs = bufs
+        evs, feats = bufs[lane][1], bufs[lane][2]
         c_ptrs = (ct.c_void_p * n)(*ptrs)
         c_ld = (ct.c_int32 * n)(*ld)
         c_first = (ct.c_int64 * n)(*[int(f) for f in first_samples])
@@ -227,7 +229,8 @@ class StreamGroup:
         c_cap = (ct.c_int64 * n)(*([cap] * n))
         c_n = (ct.c_int64 * n)()
         c_feat = (ct.c_void_p * n)(*[f.ctypes.data for f in feats])
-        check(lib().lss_group_process(self._g, c_ptrs, c_ld, int(device), c_first, c_ev, c_cap, c_n, c_feat))
+        check(lib().lss_group_process_lane(self._g, lane, c_ptrs, c_ld, int(device), c_first, c_ev, c_cap, c_n,
+                                           c_feat))
         del keep
         return [(evs[s][:c_n[s]].copy(), feats[s][:c_n[s]].copy()) for s in range(n)]
 

@@ -292,5 +292,15 @@ def test_stream_group_matches_single_handles(model_dirs):
         out = grp.process(bs, [4 * 1500] * 3)
         e1, f1 = singles[1].process_batch(bs[1], first_sample=4 * 1500)
         assert np.array_equal(out[1][0], e1)
+        # two lanes driven from two threads at once, on disjoint streams
+        from concurrent.futures import ThreadPoolExecutor
+        bs = [x[2 * 1500:2 * 1500 + W] for x in recs]
+        with ThreadPoolExecutor(2) as pool:
+            fa = pool.submit(grp.process, [bs[0], None, bs[2]], [3000] * 3, lane=0)
+            fb = pool.submit(grp.process, [None, bs[1], None], [3000] * 3, lane=1)
+            oa, ob = fa.result(), fb.result()
+        for k, o in ((0, oa[0]), (1, ob[1]), (2, oa[2])):
+            e1, f1 = singles[k].process_batch(bs[k], first_sample=3000)
+            assert np.array_equal(o[0], e1) and np.array_equal(o[1], f1), f"lane run, stream {k}"
     for sp in singles:
         sp.close()
