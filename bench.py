@@ -293,8 +293,9 @@ def bench_target(args) -> dict:
     net = ProposalNet(C, W, seed=1)
     net.load(weights)
     modes = {}
-    for name, prec, gated in (("group_fp32", PREC_FP32, False), ("group_bf16", PREC_BF16_RESCORE, False),
-                              ("group_bf16_gated", PREC_BF16_RESCORE, True)):
+    all_modes = (("group_fp32", PREC_FP32, False), ("group_bf16", PREC_BF16_RESCORE, False),
+                 ("group_bf16_gated", PREC_BF16_RESCORE, True))
+    for name, prec, gated in [m for m in all_modes if m[0] in args.modes.split(",")]:
         grp = StreamGroup([d] * S, batch=W, precision=prec)
         if gated:
             for s in range(S):
@@ -339,7 +340,7 @@ def bench_target(args) -> dict:
     net.close()
     for b in bufs:
         b.free()
-    g = modes["group_bf16_gated"]
+    g = modes.get("group_bf16_gated") or next(iter(modes.values()))
     return {
         "metric": METRIC + " [target: gated BF16 batched, 384 ch x 1024 templates]",
         "value": g["x_realtime"],
@@ -539,6 +540,8 @@ def main() -> int:
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
     ap.add_argument("--lanes", type=int, default=4, help="concurrent launch lanes of the paced group runs (1-4)")
+    ap.add_argument("--modes", default="group_fp32,group_bf16,group_bf16_gated",
+                    help="target workload: which group modes to run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized buffer three times (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -644,9 +647,10 @@ def main() -> int:
                 "frac": round(achieved / peak, 5), "traffic": traffic, "kernel": dom,
                 "ms_per_launch": round(avg[dom], 5),
                 "algorithmic_bytes" if bound == "hbm" else "algorithmic_tflop": round(amount * (1e9 if bound == "hbm" else 1.0), 3)}
-    stages = {k: {"ms": round(avg[k], 5), "bound": work[k][0] if k in work else "pcie",
+    # stage times (the proposal stage is empty without a network: it only measures the event gap)
+    stages = {k: {"ms": round(avg[k], 5), "bound": work[k][0] if k in work else None,
                   "achieved": round(work[k][1] / (avg[k] / 1e3), 3) if k in work and avg[k] > 0 else None,
-                  "unit": work[k][2] if k in work else None} for k in avg if k != "h2d_copy"}
+                  "unit": work[k][2] if k in work else None} for k in avg if k not in ("h2d_copy", "proposal")}
 
     cpu = None
     parity = None
