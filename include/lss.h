@@ -54,7 +54,11 @@ typedef struct lss_config {
     int32_t profile;       /* nonzero: time each device stage with HIP events (lss_stage_times) */
     int32_t peel_split;    /* template subsets per peel column block: 0 = automatic,
                               1 = none, n > 1 = at most n (Kilosort semantics only) */
-    int32_t reserved[4];
+    int32_t relabel;       /* nonzero: an event's template_id is the final cluster nearest to
+                              the spike's position (cluster_centroids.npy), as saveSpikes
+                              reports it (OnlineSpikesV2.cu:1186-1189); 0 (default): the
+                              matched template's file index */
+    int32_t reserved[3];
 } lss_config;
 
 /* One spike: (sample, template, amplitude), as OnlineSpikesPayload carries them
@@ -155,6 +159,14 @@ int lss_debug_stage(struct lss_handle* h, int32_t stage, void* out, int64_t* byt
  * the reference's per-stage Timer scopes print nothing (src/Helpers/Timer.h:12-16). */
 #define LSS_NSTAGES 6
 int lss_stage_times(struct lss_handle* h, double* ms, int64_t* batches, int32_t reset);
+
+/* Spike positions (x, y) of the events the last call returned, in the same
+ * order: the PC-feature mass of each of the nC nearest channels weights the
+ * channel coordinates xc/yc.  Replaces compute_spike_positions_kernel
+ * (OnlineSpikesV2.cu:190-245; Kilosort's compute_spike_positions,
+ * KS/postprocessing.py:33-46).  Needs xc.npy / yc.npy in the oss_input
+ * directory (otherwise LSS_ERR_STATE).  xy: [cap][2]; *n = events. */
+int lss_last_positions(struct lss_handle* h, float* xy, int64_t cap, int64_t* n);
 
 int lss_destroy(struct lss_handle* h);
 

@@ -33,7 +33,7 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_group_query", "lss_group_destroy", "lss_proposal_n_params", "lss_proposal_create",
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
-                 "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane")
+                 "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions")
 NSTAGES = 6
 STAGE_NAMES = ("h2d_copy", "whiten", "proposal", "pcconv", "project", "peel")
 
@@ -43,7 +43,7 @@ class Config(ct.Structure):
                 ("max_iter", ct.c_int32), ("semantics", ct.c_int32), ("precision", ct.c_int32),
                 ("dedup", ct.c_int32), ("max_events", ct.c_int32), ("keep_stages", ct.c_int32),
                 ("sparse", ct.c_int32), ("profile", ct.c_int32), ("peel_split", ct.c_int32),
-                ("reserved", ct.c_int32 * 4)]
+                ("relabel", ct.c_int32), ("reserved", ct.c_int32 * 3)]
 
 
 class Event(ct.Structure):
@@ -110,6 +110,7 @@ def lib() -> ct.CDLL:
                                         ct.POINTER(ct.c_int64), ct.POINTER(ct.c_void_p)]
         L.lss_group_process_lane.argtypes = [ct.c_void_p, ct.c_int32] + L.lss_group_process.argtypes[1:]
         L.lss_group_query.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(Info)]
+        L.lss_last_positions.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64)]
         L.lss_group_destroy.argtypes = [ct.c_void_p]
         L.lss_proposal_n_params.restype = ct.c_int32
         L.lss_proposal_n_params.argtypes = []

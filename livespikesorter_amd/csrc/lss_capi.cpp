@@ -205,6 +205,11 @@ struct lss_handle {
     OutHeader* d_hdr = nullptr;
     lss_event* d_ev = nullptr;
     float* d_feat = nullptr;
+    // spike positions (xc/yc present) and nearest-cluster relabelling (cfg.relabel)
+    float *pos = nullptr, *d_pos = nullptr;
+    float *d_xc = nullptr, *d_yc = nullptr, *d_cent = nullptr;
+    int n_cent = 0;
+    std::vector<float> last_pos;
     int ev_cap = 0;
 
     std::vector<int> orig, cmap_default, cmap_cur;
@@ -234,7 +239,7 @@ struct lss_handle {
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
                         d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_Pcf, d_Pbv, d_Pim,
-                        d_gate, d_mt_glo, d_mt_ghi, d_gate_stats};
+                        d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent};
         (void)hipSetDevice(dev);
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -623,10 +628,27 @@ void load(lss_handle* h, const std::string& dir_in) {
         h->d_dbgCf = dalloc<float>(Wmax);
         h->d_dbgIm = dalloc<int>(Wmax);
     }
+    // probe geometry for spike positions, final-cluster centroids for relabelling
+    if (file_exists(dir + "xc.npy") && file_exists(dir + "yc.npy")) {
+        auto xa = npy_load(dir + "xc.npy"), ya = npy_load(dir + "yc.npy");
+        if (xa.numel() != C || ya.numel() != C) throw LssError(LSS_ERR_IO, "xc/yc must have C entries");
+        if (nC > 16) throw LssError(LSS_ERR_IO, "spike positions support at most 16 nearest channels");
+        h->d_xc = dalloc<float>(C); upload(h->d_xc, xa.as<float>());
+        h->d_yc = dalloc<float>(C); upload(h->d_yc, ya.as<float>());
+    }
+    if (h->cfg.relabel) {
+        if (!h->d_xc) throw LssError(LSS_ERR_IO, "relabelling needs xc.npy and yc.npy");
+        // cluster centres as (x, y) pairs (loadKilosortClusteringData, OnlineSpikesV2.cu:648-659)
+        auto ca = npy_load(dir + "cluster_centroids.npy");
+        if (ca.numel() < 2 || ca.numel() % 2) throw LssError(LSS_ERR_IO, "cluster_centroids.npy must hold (x, y) pairs");
+        const auto cv = ca.as<float>();
+        h->n_cent = (int)(cv.size() / 2);
+        h->d_cent = dalloc<float>(cv.size()); upload(h->d_cent, cv);
+    }
     // mapped, coherent host memory: the peel kernel writes events straight into it
     h->ev_cap = h->cfg.max_events;
     const size_t out_bytes = sizeof(OutHeader) + sizeof(lss_event) * (size_t)h->ev_cap +
-                             sizeof(float) * (size_t)h->ev_cap * nC * K;
+                             sizeof(float) * (size_t)h->ev_cap * nC * K + sizeof(float) * 2 * (size_t)h->ev_cap;
     HIPCHK(hipHostMalloc(&h->h_out, out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(h->h_out, 0, out_bytes);
     void* dout = nullptr;
@@ -637,6 +659,10 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_hdr = static_cast<OutHeader*>(dout);
     h->d_ev = reinterpret_cast<lss_event*>(h->d_hdr + 1);
     h->d_feat = reinterpret_cast<float*>(h->d_ev + h->ev_cap);
+    if (h->d_xc) {
+        h->pos = h->feat + (size_t)h->ev_cap * nC * K;
+        h->d_pos = h->d_feat + (size_t)h->ev_cap * nC * K;
+    }
 
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     if (h->cfg.profile)
@@ -644,7 +670,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     launch_norms(h->d_U, K * C, T, h->d_nm, h->d_rnm, h->d_nm64, h->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->last_spike.assign(T, INT64_MIN / 2);
+    h->last_spike.assign(std::max(T, h->n_cent), INT64_MIN / 2);   // indexed by template, or by cluster (relabel)
 }
 
 void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
@@ -733,6 +759,8 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.iCC = h->d_iCC; p.iU = h->d_iU; p.Gt = h->d_Gt;
     p.do_feat = 1;
     p.hdr = h->d_hdr; p.ev = h->d_ev; p.feat = h->d_feat; p.ev_cap = h->ev_cap;
+    p.pos = h->d_pos; p.xc = h->d_xc; p.yc = h->d_yc;
+    p.cent = h->d_cent; p.n_cent = h->n_cent;
     p.half = M / 2; p.nt0min = h->nt0min;
     p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
     p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
@@ -893,6 +921,7 @@ int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_
     if (h->hdr->count > h->ev_cap) throw LssError(LSS_ERR_CAPACITY, "device event buffer overflow (max_events)");
     const int64_t fw = (int64_t)h->nC * h->K;
     int64_t out = 0, total = 0;
+    h->last_pos.clear();
     for (int64_t i = 0; i < n; ++i) {
         lss_event e = h->ev[i];
         e.sample += h->cur_first;           // the kernel writes samples relative to column 0
@@ -906,6 +935,10 @@ int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_
         if (out < cap) {
             if (ev) ev[out] = e;
             if (feat) std::memcpy(feat + out * fw, h->feat + i * fw, sizeof(float) * fw);
+            if (h->pos) {
+                h->last_pos.push_back(h->pos[2 * i]);
+                h->last_pos.push_back(h->pos[2 * i + 1]);
+            }
             ++out;
         }
     }
@@ -1241,6 +1274,18 @@ int lss_autotune(lss_handle* h, const int16_t* d_samples, int64_t n, int32_t ld,
         out->n = (int32_t)cand.size();
         out->chosen = best;
         h->tuned_split[W] = out->split[best];
+    });
+}
+
+int lss_last_positions(lss_handle* h, float* xy, int64_t cap, int64_t* n) {
+    return guarded([&] {
+        if (!h || !n) throw LssError(LSS_ERR_ARG, "null argument");
+        if (!h->pos) throw LssError(LSS_ERR_STATE, "no probe geometry (xc.npy / yc.npy) was loaded");
+        const int64_t m = (int64_t)h->last_pos.size() / 2;
+        *n = m;
+        if (m > cap) throw LssError(LSS_ERR_CAPACITY, "position buffer too small");
+        if (m && !xy) throw LssError(LSS_ERR_ARG, "null argument");
+        if (m) std::memcpy(xy, h->last_pos.data(), sizeof(float) * 2 * m);
     });
 }
 

@@ -112,6 +112,10 @@ struct PeelParams {
     const uint8_t* gate; int gate_mt;
     // optional: the stream is idle this launch when *active is null (every workgroup returns)
     const int16_t* const* active;
+    // spike positions from the features (pos [ev_cap][2], nullable; channel coordinates xc, yc)
+    // and, with cent (n_cent (x, y) pairs), the event's template replaced by the nearest cluster
+    float* pos; const float* xc; const float* yc;
+    const float* cent; int n_cent;
 };
 
 // ---- batched launch descriptors (kernel arguments) ----

@@ -1056,7 +1056,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         }
         s_rank[i] = r;
         const int d = off + r;
-        if (d < p.ev_cap) {
+        if (d < p.ev_cap && !p.cent) {              // with relabelling the event is written below
             lss_event e;
             e.sample = si.w - p.half + p.nt0min;     // the host adds the batch's first sample
             e.template_id = p.orig[si.t];
@@ -1069,27 +1069,85 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         const int lane = tid & 63, wv = tid >> 6;
         const int K = p.K, nC = p.nC, C = p.C, L2 = 2 * M - 1;
         const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.NB - 1, (c1 - 1 + M - 1) / CW);
+        float* stage = reinterpret_cast<float*>(s_idx) + wv * 64;    // s_idx is free after the offset sum
         for (int i = wv; i < nmine; i += PEEL_THREADS / 64) {
             const int d = off + s_rank[i];
-            if (d >= p.ev_cap || lane >= nC * K) continue;
+            if (d >= p.ev_cap) continue;                             // uniform per wave
             const Spike si = s_list[i];
-            const int j = lane / K, k = lane % K;
+            const bool mine = lane < nC * K;
+            const int j = mine ? lane / K : 0, k = mine ? lane % K : 0;
             const int ch = p.iCC[j * C + p.iU[si.t]];
-            float f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
-            for (int hh = flo; hh <= fhi; ++hh) {
-                const int cnt = hh == cb ? nmine : ld_c(&p.list_cnt[hh]);
-                const Spike* lst = p.list + (int64_t)hh * p.cap_g;
-                for (int q = 0; q < cnt; ++q) {
-                    const Spike sq = hh == cb ? s_list[q] : ld_spike(&lst[q]);
-                    const int dd = si.w - sq.w;
-                    if (dd < -(M - 1) || dd > M - 1) continue;
-                    float gs = 0.f;
-                    for (int k2 = 0; k2 < K; ++k2)
-                        gs += p.U[((int64_t)sq.t * K + k2) * C + ch] * p.Gt[(k2 * K + k) * L2 + dd + M - 1];
-                    f -= sq.amp * gs;
+            float f = 0.f;
+            if (mine) {
+                f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
+                for (int hh = flo; hh <= fhi; ++hh) {
+                    const int cnt = hh == cb ? nmine : ld_c(&p.list_cnt[hh]);
+                    const Spike* lst = p.list + (int64_t)hh * p.cap_g;
+                    for (int q = 0; q < cnt; ++q) {
+                        const Spike sq = hh == cb ? s_list[q] : ld_spike(&lst[q]);
+                        const int dd = si.w - sq.w;
+                        if (dd < -(M - 1) || dd > M - 1) continue;
+                        float gs = 0.f;
+                        for (int k2 = 0; k2 < K; ++k2)
+                            gs += p.U[((int64_t)sq.t * K + k2) * C + ch] * p.Gt[(k2 * K + k) * L2 + dd + M - 1];
+                        f -= sq.amp * gs;
+                    }
+                }
+                p.feat[(int64_t)d * nC * K + lane] = f;
+            }
+            if (p.pos) {
+                // position (compute_spike_positions_kernel, OnlineSpikesV2.cu:190-245): the
+                // mass of nearest channel jj is the sum of its squared features; the
+                // channel coordinates are averaged with weights mass / total, in order
+                stage[lane] = f;
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0xc07f);               // lgkmcnt(0): the features are in LDS
+                float x = 0.f, y = 0.f;
+                if (lane == 0) {
+                    float mass[16], total = 0.f;
+                    for (int jj = 0; jj < nC; ++jj) {
+                        float m = 0.f;
+                        for (int kk = 0; kk < K; ++kk) m += stage[jj * K + kk] * stage[jj * K + kk];
+                        mass[jj] = m;
+                        total += m;
+                    }
+                    const int iu = p.iU[si.t];
+                    for (int jj = 0; jj < nC; ++jj) {
+                        const float wgt = total > 0.f ? mass[jj] / total : 0.f;
+                        const int c2 = p.iCC[jj * C + iu];
+                        x += p.xc[c2] * wgt;
+                        y += p.yc[c2] * wgt;
+                    }
+                    p.pos[2 * (int64_t)d] = x;
+                    p.pos[2 * (int64_t)d + 1] = y;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (p.cent) {
+                    // nearest final cluster (closestCluster, OnlineSpikesV2.cu:1058-1072):
+                    // Euclidean distance, the first index among equals
+                    x = __shfl(x, 0);
+                    y = __shfl(y, 0);
+                    float bd = INFINITY;
+                    int bc = 0x7fffffff;
+                    for (int c2 = lane; c2 < p.n_cent; c2 += 64) {
+                        const float dx = x - p.cent[2 * c2], dy = y - p.cent[2 * c2 + 1];
+                        const float dist = sqrtf(dx * dx + dy * dy);
+                        if (dist < bd) { bd = dist; bc = c2; }
+                    }
+                    for (int m = 32; m > 0; m >>= 1) {
+                        const float od = __shfl_xor(bd, m);
+                        const int oc = __shfl_xor(bc, m);
+                        if (od < bd || (od == bd && oc < bc)) { bd = od; bc = oc; }
+                    }
+                    if (lane == 0) {
+                        lss_event e;
+                        e.sample = si.w - p.half + p.nt0min;
+                        e.template_id = bc != 0x7fffffff ? bc : p.orig[si.t];
+                        e.amplitude = si.amp;
+                        p.ev[d] = e;
+                    }
                 }
             }
-            p.feat[(int64_t)d * nC * K + lane] = f;
         }
     }
     mark(5);

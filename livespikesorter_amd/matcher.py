@@ -31,15 +31,23 @@ class OnlineSpikesV2:
                  max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
                  precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
                  keep_stages: bool = False, sparse: bool = True, profile: bool = False,
-                 peel_split: int = 0):
+                 peel_split: int = 0, relabel: bool = False):
         self._h = ct.c_void_p()
         cfg = default_config(device=device, chunk=chunk, max_batch=max_batch, max_iter=max_iter,
                              semantics=semantics, precision=precision, dedup=dedup,
                              max_events=max_events, keep_stages=int(keep_stages), sparse=int(sparse),
-                             profile=int(profile), peel_split=peel_split)
+                             profile=int(profile), peel_split=peel_split, relabel=int(relabel))
         check(lib().lss_create(os.fsencode(str(oss_input_dir)), ct.byref(cfg), ct.byref(self._h)))
         self.info = self.query()
         self.last_W = 0
+
+    def last_positions(self) -> np.ndarray:
+        """(x, y) per event of the last call, in event order (lss_last_positions)."""
+        n = ct.c_int64()
+        cap = 65536
+        out = np.empty((cap, 2), np.float32)
+        check(lib().lss_last_positions(self._h, out.ctypes.data, cap, ct.byref(n)))
+        return out[:n.value].copy()
 
     def autotune(self, dev_ptr: int, n_samples: int, ld: int, reps: int = 5) -> list:
         """Time each peel split on a device batch and keep the fastest for this

@@ -196,6 +196,9 @@ def write_oss_input(model: ProbeModel, path: str | os.PathLike, *, with_ctc: boo
     np.save(p / "Ucc.npy", Ucc)
     np.save(p / "xc.npy", xc)
     np.save(p / "yc.npy", yc)
+    # final clusters: one per template, centred on its footprint (x = 0, y = 20 um x centre)
+    np.save(p / "cluster_centroids.npy",
+            np.stack([np.zeros(T), 20.0 * model.mu], axis=1).astype(np.float32))
     with open(p / "misc.txt", "w") as f:
         f.write(f"nt0min:{model.nt0min}\n")
         f.write(f"numNearestChans:{model.nC}\n")

@@ -251,3 +251,31 @@ def stage_conv_project(x: np.ndarray, Wr, wPCA, U, *, bits: int = 64, threads: i
     getattr(lib, f"orc{bits}_project")(ct.byref(prm), _p(Bpc, creal), ct.c_long(W), _p(U, ct.c_float),
                                        _p(B, creal))
     return Xw, Bpc, B
+
+
+# ---------------------------------------------------------------------------
+# Spike positions and final-cluster relabelling (pure numpy, float64)
+# ---------------------------------------------------------------------------
+
+def spike_positions(feat: np.ndarray, templates: np.ndarray, iCC: np.ndarray, iU: np.ndarray,
+                    xc: np.ndarray, yc: np.ndarray):
+    """Kilosort's compute_spike_positions (KS/postprocessing.py:33-46), as LSS's
+    compute_spike_positions_kernel computes it (OnlineSpikesV2.cu:190-245):
+    feat [n][nC][K] PC features, templates [n] file indices; the mass of
+    nearest channel j is sum_k feat^2, the position the mass-weighted mean of
+    the nearest channels' coordinates."""
+    feat = np.asarray(feat, np.float64)
+    n, nC, K = feat.shape
+    mass = (feat ** 2).sum(-1)
+    tot = mass.sum(1, keepdims=True)
+    w = np.divide(mass, tot, out=np.zeros_like(mass), where=tot > 0)
+    chs = np.asarray(iCC)[:nC, np.asarray(iU)[np.asarray(templates)]].T      # [n][nC]
+    return (np.asarray(xc, np.float64)[chs] * w).sum(1), (np.asarray(yc, np.float64)[chs] * w).sum(1)
+
+
+def closest_cluster(x, y, centroids: np.ndarray) -> np.ndarray:
+    """OnlineSpikesV2::closestCluster (OnlineSpikesV2.cu:1058-1072): index of the
+    nearest (x, y) centroid, the first among equals."""
+    c = np.asarray(centroids, np.float64).reshape(-1, 2)
+    d = np.hypot(np.asarray(x, np.float64)[:, None] - c[None, :, 0], np.asarray(y, np.float64)[:, None] - c[None, :, 1])
+    return np.argmin(d, axis=1)

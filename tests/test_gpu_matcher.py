@@ -173,6 +173,47 @@ def test_autotune_keeps_results(cfg2):
     assert np.array_equal(ea, eb) and np.array_equal(fa, fb)
 
 
+def test_positions_and_relabel(cfg2):
+    """Spike positions from the GPU features against the position oracle, and
+    the nearest-cluster relabelling (saveSpikes' template index) against the
+    oracle's nearest centroid."""
+    m, d, tab, x = cfg2
+    batch = x[:3122]
+    xc = np.load(d / "xc.npy")
+    yc = np.load(d / "yc.npy")
+    cent = np.load(d / "cluster_centroids.npy")
+    with OnlineSpikesV2(d) as a, OnlineSpikesV2(d, relabel=True) as b:
+        ea, fa = a.process_batch(batch)
+        pa = a.last_positions()
+        eb, fb = b.process_batch(batch)
+        pb = b.last_positions()
+    assert len(pa) == len(ea) > 0
+    px, py = oracle.spike_positions(fa, ea["template"], tab["iCC"], tab["iU"], xc, yc)
+    err = np.abs(pa - np.stack([px, py], 1)).max()
+    scale = max(np.abs(xc).max(), np.abs(yc).max())
+    print(f"positions: {len(ea)} events, max |gpu - oracle(gpu features)| {err:.2e} um (scale {scale:.0f} um)")
+    assert err <= 1e-6 * scale                                  # FP32 rounding of the weighted sums
+    # against the FP64 oracle's own features
+    ref = _oracle(tab, batch)
+    rx, ry = oracle.spike_positions(ref.feat, ref_templates(ref), tab["iCC"], tab["iU"], xc, yc)
+    ref_s = ref.w - 61 // 2 + 20
+    _, _, pairs = agreement(ref_s, ref_templates(ref), ea["sample"], ea["template"], tol=0)
+    ri = np.array([p[0] for p in pairs]); gi = np.array([p[1] for p in pairs])
+    e64 = max(np.abs(pa[gi, 0] - rx[ri]).max(), np.abs(pa[gi, 1] - ry[ri]).max())
+    print(f"positions vs FP64 oracle features: max error {e64:.2e} um over {len(pairs)} events")
+    assert e64 <= 1e-5 * scale
+    # relabelling: same spikes (features identical), template = nearest centroid
+    assert np.array_equal(eb["sample"], ea["sample"]) and np.array_equal(fb, fa)
+    want = oracle.closest_cluster(pa[:, 0], pa[:, 1], cent)
+    same = np.mean(eb["template"] == want)
+    print(f"relabel: {same:.5f} of events on the oracle's nearest centroid")
+    assert same >= 0.999
+
+
+def ref_templates(ref):
+    return np.asarray(ref.t)
+
+
 def test_deterministic(cfg2):
     m, d, tab, x = cfg2
     batch = x[3000:6122]

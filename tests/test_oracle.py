@@ -283,3 +283,46 @@ def test_lss_semantics_switch():
     assert len(w) == 0                                   # Kilosort: Cmax > Th^2 is strict
     w, t, *_ = oracle.peel(B, nm, ctc, m.U, m.wPCA, Th=8.0, max_iter=1, semantics=oracle.SEM_LSS)
     assert sorted(t.tolist()) == [1, 3] and set(w.tolist()) == {w0}
+
+
+def test_spike_positions_closed_forms():
+    """Kilosort's spike position: mass on one channel gives that channel's
+    coordinates; equal masses give the midpoint; no mass gives the origin
+    (compute_spike_positions_kernel sets weights to 0 when the total is 0);
+    a random case equals a loop over the definition."""
+    iCC = np.array([[0, 1, 2], [1, 2, 1], [2, 0, 0]])        # [nC=3][C=3]
+    iU = np.array([0, 1, 2])
+    xc = np.array([0.0, 10.0, 30.0])
+    yc = np.array([5.0, 25.0, 45.0])
+    f = np.zeros((4, 3, 2))
+    f[0, 0, :] = [3.0, 4.0]                                     # template 0: all mass on channel 0
+    f[1, 0, 1] = 2.0; f[1, 1, 0] = 2.0                          # template 1: channels 1 and 2, equal
+    x, y = oracle.spike_positions(f, np.array([0, 1, 2, 0]), iCC, iU, xc, yc)
+    assert (x[0], y[0]) == (0.0, 5.0)
+    assert (x[1], y[1]) == (20.0, 35.0)
+    assert (x[2], y[2]) == (0.0, 0.0) and (x[3], y[3]) == (0.0, 0.0)
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=(5, 3, 2))
+    t = rng.integers(0, 3, 5)
+    x, y = oracle.spike_positions(f, t, iCC, iU, xc, yc)
+    for i in range(5):
+        m = [(f[i, j] ** 2).sum() for j in range(3)]
+        s = sum(m)
+        ch = [iCC[j, iU[t[i]]] for j in range(3)]
+        assert np.isclose(x[i], sum(xc[ch[j]] * m[j] / s for j in range(3)))
+        assert np.isclose(y[i], sum(yc[ch[j]] * m[j] / s for j in range(3)))
+
+
+def test_closest_cluster_brute_force_and_ties():
+    cent = np.array([[0.0, 0.0], [10.0, 0.0], [0.0, 10.0], [10.0, 0.0]])
+    x = np.array([1.0, 9.0, 5.0, 0.0])
+    y = np.array([1.0, 1.0, 0.0, 9.0])
+    got = oracle.closest_cluster(x, y, cent)
+    assert got.tolist() == [0, 1, 0, 2]                          # (5, 0): equidistant 0 and 1 -> first
+    rng = np.random.default_rng(4)
+    cent = rng.normal(size=(40, 2)) * 50
+    x, y = rng.normal(size=20) * 50, rng.normal(size=20) * 50
+    got = oracle.closest_cluster(x, y, cent)
+    for i in range(20):
+        d = [((x[i] - cx) ** 2 + (y[i] - cy) ** 2) ** 0.5 for cx, cy in cent]
+        assert got[i] == min(range(40), key=lambda k: (d[k], k))
