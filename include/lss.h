@@ -103,8 +103,23 @@ int lss_config_init(lss_config* cfg);
 
 /* Load an oss_input directory and allocate the device state.
  * Replaces OnlineSpikesV2::OnlineSpikesV2 / initializeSorter
- * (OnlineSpikesV2.cu:248-281,322-382) and its loaders (:454-660). */
+ * (OnlineSpikesV2.cu:248-281,322-382) and its loaders (:454-660).
+ * Required: misc.txt, Wall3.npy, wPCA.npy, whiteningMat.npy, and either
+ * iCC.npy + iU.npy or the geometry xc.npy + yc.npy.  The tables the launcher
+ * derives from these (run_online_spikes.py:609-667) are computed at load time
+ * when their files are absent: ctc on the GPU (FP64 accumulation rounded to
+ * f32), iCC and iU from the geometry and Wall3; templates.npy is optional
+ * (M and C then come from wPCA and Wall3). */
 int lss_create(const char* oss_input_dir, const lss_config* cfg, struct lss_handle** out);
+
+/* lss_create over a subset of the preclustered templates: keep[0..n_keep) are
+ * ascending Wall3 row indices; the kept rows of Wall3, ctc and iU are used
+ * and renumbered 0..n_keep-1, as in a directory written by
+ * subset_oss_input_inplace (src/Python/subset_templates.py:291-347), which
+ * subsets the same tensors on disk.  Final-cluster tensors (centroids) are
+ * untouched, as there. */
+int lss_create_subset(const char* oss_input_dir, const lss_config* cfg, const int32_t* keep, int32_t n_keep,
+                      struct lss_handle** out);
 
 /* Match one batch whose columns are already framed with margins: W = n_samples
  * columns of int16 samples, row stride `ld` (>= C) in the host buffer,
@@ -147,7 +162,9 @@ int lss_query(struct lss_handle* h, lss_info* info);
  * u64 ticks of the 100 MHz device clock: [0..6] summed over workgroups and
  * [7..13] the largest workgroup's (top-of-iteration exchange, scan, barrier,
  * acceptance, barrier, events and features, subtraction), [14] scan passes
- * that did work, [15] workgroups.
+ * that did work, [15] workgroups, 7 the ctc table the matcher uses, file layout
+ * [T][T][2M+1] f32 (as ctc.npy, or as computed at load time without it; needs
+ * neither keep_stages nor a batch).
  * *bytes: in = capacity, out = size written. */
 int lss_debug_stage(struct lss_handle* h, int32_t stage, void* out, int64_t* bytes);
 

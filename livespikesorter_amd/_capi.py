@@ -33,7 +33,8 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_group_query", "lss_group_destroy", "lss_proposal_n_params", "lss_proposal_create",
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
-                 "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions")
+                 "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions",
+                 "lss_create_subset")
 NSTAGES = 6
 STAGE_NAMES = ("h2d_copy", "whiten", "proposal", "pcconv", "project", "peel")
 
@@ -93,6 +94,8 @@ def lib() -> ct.CDLL:
         L.lss_last_error.restype = ct.c_char_p
         L.lss_config_init.argtypes = [ct.POINTER(Config)]
         L.lss_create.argtypes = [ct.c_char_p, ct.POINTER(Config), ct.POINTER(ct.c_void_p)]
+        L.lss_create_subset.argtypes = [ct.c_char_p, ct.POINTER(Config), ct.c_void_p, ct.c_int32,
+                                        ct.POINTER(ct.c_void_p)]
         batch_args = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_int64,
                       ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64), ct.c_void_p]
         L.lss_process_batch.argtypes = batch_args

@@ -214,6 +214,7 @@ struct lss_handle {
 
     std::vector<int> orig, cmap_default, cmap_cur;
     std::vector<int> iU_int;         // main channel of each template, internal order
+    std::vector<int> keep;           // template subset (file indices), empty = all
     int last_W = 0;
 
     // streaming state
@@ -317,26 +318,46 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->dt = std::stoi(kv["duplicate_spike_bins"]);
     if (h->sem == LSS_SEM_LSS) h->Th = (float)(int)h->Th;
 
-    // templates.npy gives M and C (loadTemplatesShape, OnlineSpikesV2.cu:454-462)
-    auto tpl = npy_load(dir + "templates.npy");
-    if (tpl.shape.size() != 3) throw LssError(LSS_ERR_IO, "templates.npy must be [T][M][C]");
-    h->M = (int)tpl.shape[1];
-    h->C = (int)tpl.shape[2];
-    // Wall3 gives T (unclu_T) and K (loadPreclusterShapes, :489-497)
+    // Wall3 gives T (unclu_T) and K (loadPreclusterShapes, :489-497); templates.npy gives
+    // M and C (loadTemplatesShape, OnlineSpikesV2.cu:454-462).  Without templates.npy, C
+    // comes from Wall3 and M from wPCA, which carry the same sizes.
     auto wall3 = npy_load(dir + "Wall3.npy");
-    if (wall3.shape.size() != 3 || wall3.shape[2] != h->C)
-        throw LssError(LSS_ERR_IO, "Wall3.npy must be [T][K][C]");
-    h->T = (int)wall3.shape[0];
+    if (wall3.shape.size() != 3) throw LssError(LSS_ERR_IO, "Wall3.npy must be [T][K][C]");
+    const int T_file = (int)wall3.shape[0];
+    h->T = T_file;
     h->K = (int)wall3.shape[1];
+    h->C = (int)wall3.shape[2];
+    // template subset (subset_templates.py:291-347 applied at load time): the kept rows of
+    // the preclustered tensors, renumbered 0..n-1 in the order given (ascending)
+    std::vector<int>& keep = h->keep;
+    if (!keep.empty()) {
+        for (size_t i = 0; i < keep.size(); ++i)
+            if (keep[i] < 0 || keep[i] >= T_file || (i && keep[i] <= keep[i - 1]))
+                throw LssError(LSS_ERR_ARG, "template subset must be ascending indices below Wall3's template count");
+        h->T = (int)keep.size();
+    }
+    auto wp_a = npy_load(dir + "wPCA.npy");
+    if (wp_a.shape.size() != 2) throw LssError(LSS_ERR_IO, "wPCA.npy must be [K][M]");
+    h->M = (int)wp_a.shape[1];
+    if (file_exists(dir + "templates.npy")) {
+        auto tpl = npy_load(dir + "templates.npy");
+        if (tpl.shape.size() != 3 || tpl.shape[2] != h->C || tpl.shape[1] != h->M)
+            throw LssError(LSS_ERR_IO, "templates.npy must be [T][M][C] with Wall3's C and wPCA's M");
+    }
     const int C = h->C, T = h->T, M = h->M, K = h->K, nC = h->nC;
     if (M % 2 == 0 || M > MAX_M) throw LssError(LSS_ERR_IO, "template length must be odd and <= 63");
     if (K < 1 || K > MAX_K) throw LssError(LSS_ERR_IO, "PC count must be 1..8");
     if (nC < 1 || nC * K > 64 || nC > C) throw LssError(LSS_ERR_IO, "numNearestChans out of range");
     auto U = wall3.as<float>();
+    if (!keep.empty()) {
+        const size_t row = (size_t)K * C;
+        std::vector<float> Us((size_t)T * row);
+        for (int t = 0; t < T; ++t) std::memcpy(&Us[t * row], &U[(size_t)keep[t] * row], row * sizeof(float));
+        U.swap(Us);
+    }
     auto wr_a = npy_load(dir + "whiteningMat.npy");
     expect_shape(wr_a, {C, C}, "whiteningMat.npy");
     auto Wr = wr_a.as<double>();
-    auto wp_a = npy_load(dir + "wPCA.npy");
     expect_shape(wp_a, {K, M}, "wPCA.npy");
     auto wpca = wp_a.as<double>();
 
@@ -380,8 +401,13 @@ void load(lss_handle* h, const std::string& dir_in) {
             throw LssError(LSS_ERR_IO, "iCC.npy must be [>=nC][C]");
         iCC = a.as<int64_t>();
         auto b = npy_load(dir + "iU.npy");
-        expect_shape(b, {T}, "iU.npy");
+        expect_shape(b, {T_file}, "iU.npy");
         iU = b.as<int64_t>();
+        if (!keep.empty()) {
+            std::vector<int64_t> s2(T);
+            for (int t = 0; t < T; ++t) s2[t] = iU[keep[t]];
+            iU.swap(s2);
+        }
     } else {
         auto xa = npy_load(dir + "xc.npy"), ya = npy_load(dir + "yc.npy");
         auto xc = xa.as<double>(), yc = ya.as<double>();
@@ -454,59 +480,30 @@ void load(lss_handle* h, const std::string& dir_in) {
         if (T <= REG_MAX_T && (T + REG_NRG - 1) / REG_NRG <= R) { h->ctc_rows = REG_NRG * R; break; }
     h->ctc_rows = std::max(h->ctc_rows, h->min_rows);
     const int TS = h->ctc_rows;
-    std::vector<float> ctc_i((size_t)T * TS * LP, 0.f);
+    // Without ctc.npy the table is computed on the GPU once the templates are resident
+    // (launch_ctc, FP64 accumulation); here only WtW and the channel spans are prepared.
+    const int L = 2 * M + 1;
+    const bool ctc_file = file_exists(dir + "ctc.npy");
+    std::vector<float> ctc_i;
     std::vector<int> tlo(T, 0), thi(T, 0);
-    {
-        const int L = 2 * M + 1;
+    std::vector<double> wtw;
+    if (ctc_file) {
+        ctc_i.assign((size_t)T * TS * LP, 0.f);
         std::vector<float> ctc;
-        if (file_exists(dir + "ctc.npy")) {
-            auto a = npy_load(dir + "ctc.npy");
-            expect_shape(a, {T, T, L}, "ctc.npy");
-            if (a.kind == 'f' && a.itemsize == 4) {
-                ctc.resize((size_t)a.numel());
-                std::memcpy(ctc.data(), a.bytes.data(), a.bytes.size());
-            } else {
-                ctc = a.as<float>();
-            }
+        auto a = npy_load(dir + "ctc.npy");
+        expect_shape(a, {T_file, T_file, L}, "ctc.npy");
+        if (a.kind == 'f' && a.itemsize == 4) {
+            ctc.resize((size_t)a.numel());
+            std::memcpy(ctc.data(), a.bytes.data(), a.bytes.size());
         } else {
-            // load-time precompute from Wall3 and wPCA (FP64)
-            std::vector<double> Wt((size_t)K * K * L, 0.0);
-            for (int k1 = 0; k1 < K; ++k1)
-                for (int k2 = 0; k2 < K; ++k2)
-                    for (int l = 0; l < L; ++l) {
-                        const int d = l - M;
-                        double s = 0;
-                        for (int m = 0; m < M; ++m) {
-                            const int mi = m - d;
-                            if (mi >= 0 && mi < M) s += wpca[(size_t)k1 * M + mi] * wpca[(size_t)k2 * M + m];
-                        }
-                        Wt[((size_t)k1 * K + k2) * L + l] = s;
-                    }
-            ctc.assign((size_t)T * T * L, 0.f);
-            std::vector<double> utu((size_t)K * K);
-            for (int i = 0; i < T; ++i)
-                for (int j = 0; j < T; ++j) {
-                    for (int k1 = 0; k1 < K; ++k1)
-                        for (int k2 = 0; k2 < K; ++k2) {
-                            double s = 0;
-                            const float* a = &U[((size_t)i * K + k1) * C];
-                            const float* b = &U[((size_t)j * K + k2) * C];
-                            for (int c = 0; c < C; ++c) s += (double)a[c] * b[c];
-                            utu[k1 * K + k2] = s;
-                        }
-                    for (int l = 0; l < L; ++l) {
-                        double s = 0;
-                        for (int q = 0; q < K * K; ++q) s += utu[q] * Wt[(size_t)q * L + l];
-                        ctc[((size_t)i * T + j) * L + l] = (float)s;
-                    }
-                }
+            ctc = a.as<float>();
         }
         for (int sp = 0; sp < T; ++sp) {
-            const int s = h->orig[sp];
+            const int s = keep.empty() ? h->orig[sp] : keep[h->orig[sp]];
             int lo = T, hi = 0;
             for (int tp = 0; tp < T; ++tp) {
-                const int t = h->orig[tp];
-                const float* src = &ctc[((size_t)t * T + s) * L];
+                const int t = keep.empty() ? h->orig[tp] : keep[h->orig[tp]];
+                const float* src = &ctc[((size_t)t * T_file + s) * L];
                 float* dst = &ctc_i[((size_t)sp * TS + tp) * LP];
                 bool nz = false;
                 for (int l = 0; l < L; ++l) {
@@ -515,11 +512,23 @@ void load(lss_handle* h, const std::string& dir_in) {
                 }
                 if (nz) { lo = std::min(lo, tp); hi = tp + 1; }
             }
-            if (!h->cfg.sparse) { lo = 0; hi = T; }
-            if (lo > hi) lo = hi = 0;
             tlo[sp] = lo;
             thi[sp] = hi;
         }
+    } else {
+        // WtW[k1][k2][l] = sum_m W[k1][m - (l - M)] W[k2][m] (KS/template_matching.py:113-114)
+        wtw.assign((size_t)K * K * L, 0.0);
+        for (int k1 = 0; k1 < K; ++k1)
+            for (int k2 = 0; k2 < K; ++k2)
+                for (int l = 0; l < L; ++l) {
+                    const int d = l - M;
+                    double s = 0;
+                    for (int m = 0; m < M; ++m) {
+                        const int mi = m - d;
+                        if (mi >= 0 && mi < M) s += wpca[(size_t)k1 * M + mi] * wpca[(size_t)k2 * M + m];
+                    }
+                    wtw[((size_t)k1 * K + k2) * L + l] = s;
+                }
     }
 
     // G[k'][k][d + M - 1] = sum_m W[k'][m + d] W[k][m]: PC projection of a shifted waveform,
@@ -557,9 +566,42 @@ void load(lss_handle* h, const std::string& dir_in) {
     }
     h->d_U = dalloc<float>(Ui.size()); upload(h->d_U, Ui);
     h->d_wpca = dalloc<float>(wpf.size()); upload(h->d_wpca, wpf);
-    h->d_ctc = dalloc<float>(ctc_i.size()); upload(h->d_ctc, ctc_i);
-    h->d_tlo = dalloc<int>(T); upload(h->d_tlo, tlo);
-    h->d_thi = dalloc<int>(T); upload(h->d_thi, thi);
+    h->d_ctc = dalloc<float>((size_t)T * TS * LP);
+    h->d_tlo = dalloc<int>(T);
+    h->d_thi = dalloc<int>(T);
+    if (ctc_file) {
+        upload(h->d_ctc, ctc_i);
+    } else {
+        // nonzero channel span of each template (internal order)
+        std::vector<int> span(2 * (size_t)T);
+        for (int tp = 0; tp < T; ++tp) {
+            int c0 = C, c1 = 0;
+            for (int k = 0; k < K; ++k)
+                for (int c = 0; c < C; ++c)
+                    if (Ui[((size_t)tp * K + k) * C + c] != 0.f) { c0 = std::min(c0, c); c1 = std::max(c1, c + 1); }
+            span[2 * tp] = c0;
+            span[2 * tp + 1] = std::max(c0, c1);
+        }
+        double* d_wtw = dalloc<double>(wtw.size());
+        int* d_span = dalloc<int>(span.size());
+        upload(d_wtw, wtw);
+        upload(d_span, span);
+        HIPCHK(hipMemset(h->d_ctc, 0, (size_t)T * TS * LP * sizeof(float)));
+        launch_ctc(h->d_U, d_wtw, reinterpret_cast<const int2*>(d_span), T, K, C, M, TS, h->d_ctc, h->d_tlo,
+                   h->d_thi, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(tlo.data(), h->d_tlo, T * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(thi.data(), h->d_thi, T * sizeof(int), hipMemcpyDeviceToHost));
+        (void)hipFree(d_wtw);
+        (void)hipFree(d_span);
+    }
+    for (int sp = 0; sp < T; ++sp) {
+        if (!h->cfg.sparse) { tlo[sp] = 0; thi[sp] = T; }
+        if (tlo[sp] > thi[sp]) tlo[sp] = thi[sp] = 0;
+    }
+    upload(h->d_tlo, tlo);
+    upload(h->d_thi, thi);
     h->d_Gt = dalloc<float>(Gt.size()); upload(h->d_Gt, Gt);
     {
         std::vector<int> orig_p(TS, INT32_MAX);
@@ -977,8 +1019,10 @@ bool dedup_on(const lss_handle* h) {
 }
 
 // Build a handle from an oss_input directory (lss_create; also each stream of a group).
-std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_config* cfg, int min_rows) {
+std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_config* cfg, int min_rows,
+                                          const int32_t* keep = nullptr, int n_keep = 0) {
     auto h = std::make_unique<lss_handle>();
+    if (keep) h->keep.assign(keep, keep + n_keep);
     if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
     if (h->cfg.precision != LSS_PREC_FP32 && h->cfg.precision != LSS_PREC_BF16_RESCORE)
         throw LssError(LSS_ERR_ARG, "unknown precision mode");
@@ -1031,6 +1075,15 @@ int lss_create(const char* oss_input_dir, const lss_config* cfg, lss_handle** ou
         if (!oss_input_dir || !out) throw LssError(LSS_ERR_ARG, "null argument");
         *out = nullptr;
         *out = create_handle(oss_input_dir, cfg, 0).release();
+    });
+}
+
+int lss_create_subset(const char* oss_input_dir, const lss_config* cfg, const int32_t* keep, int32_t n_keep,
+                      lss_handle** out) {
+    return guarded([&] {
+        if (!oss_input_dir || !out || !keep || n_keep <= 0) throw LssError(LSS_ERR_ARG, "null argument or empty subset");
+        *out = nullptr;
+        *out = create_handle(oss_input_dir, cfg, 0, keep, n_keep).release();
     });
 }
 
@@ -1181,6 +1234,21 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
             HIPCHK(hipSetDevice(h->dev));
             HIPCHK(hipMemcpy(out, h->d_prof, PROF_WORDS * 8, hipMemcpyDeviceToHost));
             *bytes = PROF_WORDS * 8;
+            return;
+        }
+        if (stage == 7) {
+            const int T = h->T, L = 2 * h->M + 1, TS = h->ctc_rows;
+            const int64_t need = (int64_t)T * T * L * 4;
+            if (*bytes < need) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+            HIPCHK(hipSetDevice(h->dev));
+            std::vector<float> ci((size_t)T * TS * LP);
+            HIPCHK(hipMemcpy(ci.data(), h->d_ctc, ci.size() * 4, hipMemcpyDeviceToHost));
+            float* of = static_cast<float*>(out);
+            for (int sp = 0; sp < T; ++sp)
+                for (int tp = 0; tp < T; ++tp)
+                    std::memcpy(of + ((size_t)h->orig[tp] * T + h->orig[sp]) * L, &ci[((size_t)sp * TS + tp) * LP],
+                                (size_t)L * 4);
+            *bytes = need;
             return;
         }
         if (h->last_W <= 0) throw LssError(LSS_ERR_STATE, "no batch processed yet");

@@ -161,6 +161,11 @@ void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s);
 void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s);
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
+// load-time ctc table on the GPU (FP64 accumulation): U [T][K][C] internal order,
+// wtw [K*K][2M+1] FP64, span [T] = [first, last+1) nonzero channel of each template;
+// ctc [T][TS][LP] zero on entry; tlo/thi = nonzero t' range of each s' (thi = 0 if none)
+void launch_ctc(const float* U, const double* wtw, const int2* span, int T, int K, int C, int M, int TS, float* ctc,
+                int* tlo, int* thi, hipStream_t s);
 
 // Rows of the score tile each peel thread keeps in registers (0: tile stays in global memory).
 int peel_rows(int T, int CW);

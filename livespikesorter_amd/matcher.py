@@ -21,7 +21,7 @@ from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_BF16_RESCORE, PREC
 __all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS", "PREC_FP32",
            "PREC_BF16_RESCORE"]
 
-STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES = range(7)
+STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES, STAGE_CTC = range(8)
 
 
 class OnlineSpikesV2:
@@ -31,13 +31,20 @@ class OnlineSpikesV2:
                  max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
                  precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
                  keep_stages: bool = False, sparse: bool = True, profile: bool = False,
-                 peel_split: int = 0, relabel: bool = False):
+                 peel_split: int = 0, relabel: bool = False, keep=None):
+        """``keep``: ascending Wall3 template indices to load (a template subset,
+        lss_create_subset); event template ids are then positions in ``keep``."""
         self._h = ct.c_void_p()
         cfg = default_config(device=device, chunk=chunk, max_batch=max_batch, max_iter=max_iter,
                              semantics=semantics, precision=precision, dedup=dedup,
                              max_events=max_events, keep_stages=int(keep_stages), sparse=int(sparse),
                              profile=int(profile), peel_split=peel_split, relabel=int(relabel))
-        check(lib().lss_create(os.fsencode(str(oss_input_dir)), ct.byref(cfg), ct.byref(self._h)))
+        if keep is None:
+            check(lib().lss_create(os.fsencode(str(oss_input_dir)), ct.byref(cfg), ct.byref(self._h)))
+        else:
+            k = np.ascontiguousarray(keep, dtype=np.int32)
+            check(lib().lss_create_subset(os.fsencode(str(oss_input_dir)), ct.byref(cfg), k.ctypes.data, len(k),
+                                          ct.byref(self._h)))
         self.info = self.query()
         self.last_W = 0
 
@@ -169,6 +176,14 @@ class OnlineSpikesV2:
         out = np.zeros(shape, dt)
         nb = ct.c_int64(out.nbytes)
         check(lib().lss_debug_stage(self._h, stage, out.ctypes.data, ct.byref(nb)))
+        return out
+
+    def ctc(self) -> np.ndarray:
+        """The ctc table the matcher uses, [T][T][2M+1] f32 (file layout of ctc.npy)."""
+        info = self.info
+        out = np.zeros((info.T, info.T, 2 * info.M + 1), np.float32)
+        nb = ct.c_int64(out.nbytes)
+        check(lib().lss_debug_stage(self._h, STAGE_CTC, out.ctypes.data, ct.byref(nb)))
         return out
 
 
