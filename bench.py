@@ -44,7 +44,7 @@ def make_data(cache: Path, seconds: float):
     cache.mkdir(parents=True, exist_ok=True)
     d = cache / "cfg2_oss_input"
     m = synth.make_model(**synth.CONFIGS["cfg2"])
-    if not (d / "ctc.npy").exists():
+    if not (d / "ctc.npy").exists() or not (d / "hp_filter.npy").exists():
         synth.write_oss_input(m, d)
     rec_path = cache / f"cfg2_rec_{int(seconds)}s.npy"
     if rec_path.exists():
@@ -532,6 +532,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-steps", type=int, default=100, help="chunks timed with host-resident input")
+    ap.add_argument("--pre-steps", type=int, default=200,
+                    help="chunks timed again with preprocessing (means, CAR, high-pass) switched on")
     ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
     ap.add_argument("--workload", choices=("cfg2", "group", "bf16", "target"), default="cfg2",
                     help="cfg2: the headline plain FP32 matcher; group: 8 streams per graph launch; "
@@ -625,6 +627,33 @@ def main() -> int:
         host_lat.append((time.perf_counter() - t1) * 1e3)
     sp.stage_times(reset=True)
 
+    # the same chunks with preprocessing (channel means, CAR, high-pass) in front of whitening
+    pre = None
+    if args.pre_steps > 0:
+        from livespikesorter_amd import PRE_ALL
+        with OnlineSpikesV2(d, chunk=N, profile=True, preprocess=PRE_ALL) as spp:
+            def pstep(j):
+                b = j % n_batches
+                return spp.process_batch_device(base + b * N * C * 2, W, C, first_sample=b * N)
+            for j in range(5):
+                pstep(j)
+            spp.stage_times(reset=True)
+            plat = []
+            synchronize()
+            ts = time.perf_counter()
+            for j in range(args.pre_steps):
+                t1 = time.perf_counter()
+                pstep(j)
+                plat.append((time.perf_counter() - t1) * 1e3)
+            synchronize()
+            pel = time.perf_counter() - ts
+            pst, pnb = spp.stage_times(reset=True)
+        pavg = {k: round(v / max(pnb, 1), 5) for k, v in pst.items() if k not in ("h2d_copy", "proposal")}
+        pre = {"steps": "mean removal + CAR lower median + high-pass (hp_filter.npy, Kilosort 300 Hz) per chunk",
+               "chunks": args.pre_steps, "value": round(args.pre_steps * N / FS / pel, 3), "unit": "x realtime",
+               "p50_ms": round(pct(plat, 50), 4), "p99_ms": round(pct(plat, 99), 4), "stage_ms": pavg}
+        log(f"with preprocessing: {pre['value']}x realtime, p99 {pre['p99_ms']} ms, stages {pavg}")
+
     # roofline of the dominant device stage (algorithmic work per launch / measured time)
     T = info.T
     I = float(np.mean(iters))
@@ -650,7 +679,7 @@ def main() -> int:
     # stage times (the proposal stage is empty without a network: it only measures the event gap)
     stages = {k: {"ms": round(avg[k], 5), "bound": work[k][0] if k in work else None,
                   "achieved": round(work[k][1] / (avg[k] / 1e3), 3) if k in work and avg[k] > 0 else None,
-                  "unit": work[k][2] if k in work else None} for k in avg if k not in ("h2d_copy", "proposal")}
+                  "unit": work[k][2] if k in work else None} for k in avg if k not in ("h2d_copy", "proposal", "preprocess")}
 
     cpu = None
     parity = None
@@ -725,6 +754,7 @@ def main() -> int:
         "mean_iterations": round(I, 2),
         "mean_spikes_per_chunk": round(S, 2),
         "stages": stages,
+        "with_preprocessing": pre,
         "peel_phases_us": peel_phases,
         "roofline": roofline,
         "cpu_baseline": cpu,

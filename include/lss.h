@@ -58,8 +58,32 @@ typedef struct lss_config {
                               the spike's position (cluster_centroids.npy), as saveSpikes
                               reports it (OnlineSpikesV2.cu:1186-1189); 0 (default): the
                               matched template's file index */
-    int32_t reserved[3];
+    int32_t preprocess;    /* LSS_PRE_* bits: preprocessing in front of whitening, per batch
+                              (OnlineSpikesV2.cu:771-794); 0 (default): the samples are
+                              whitened as given */
+    int32_t reserved[2];
 } lss_config;
+
+/* Preprocessing steps (lss_config.preprocess), applied in this order to each
+ * batch of W samples of the C mapped channels:
+ *   LSS_PRE_MEAN      subtract each channel's mean over the batch
+ *                     (meanRemove, OnlineSpikesV2.cu:774-779; myGPUhelpers.cu:99-119)
+ *   LSS_PRE_CAR       subtract, per sample, the lower median over channels
+ *                     (medianRemove, :782-786; myGPUhelpers.cu:480-543; torch.median)
+ *   LSS_PRE_HIGHPASS  the oss_input's hp_filter.npy impulse response cropped to W,
+ *                     applied circularly and shifted by W/2 (highpassFilter,
+ *                     :1006-1056; Kilosort fft_highpass, KS/preprocessing.py:139-154):
+ *                     the shift follows torch.fft.fftshift under LSS_SEM_KILOSORT
+ *                     and the reference's shift_kernel under LSS_SEM_LSS (they
+ *                     differ by one sample for odd W).  Under LSS_SEM_LSS a batch
+ *                     longer than the filter is an error, as there (:1010-1013);
+ *                     Kilosort semantics zero-pad the filter instead.
+ * With any step on, lss_info.p2p is the peak-to-peak of the raw batch
+ * (P2P_calc, :771,1137-1146). */
+#define LSS_PRE_MEAN 1
+#define LSS_PRE_CAR 2
+#define LSS_PRE_HIGHPASS 4
+#define LSS_PRE_ALL 7
 
 /* One spike: (sample, template, amplitude), as OnlineSpikesPayload carries them
  * (src/SpikeSorter/onlineSpikesPayload.h:8-48).  `sample` follows Kilosort's
@@ -85,7 +109,7 @@ typedef struct lss_info {
     int64_t proj_kblocks;   /* 64x16 projection operand blocks computed per 64-sample column tile */
     int64_t proj_kblocks_dense;  /* the same without exact-zero skipping */
     int32_t precision;      /* LSS_PREC_FP32 or LSS_PREC_BF16_RESCORE */
-    int32_t reserved0;
+    float p2p;                   /* peak-to-peak of the last batch's raw samples (preprocessing on) */
     int64_t rescored;       /* BF16 mode: local maxima rescored in FP32 (running total) */
     int64_t rescore_flips;  /* BF16 mode: of those, accept/reject decisions the rescore changed */
     int32_t peel_cw;        /* last batch's peel geometry: columns per workgroup, */
@@ -164,17 +188,20 @@ int lss_query(struct lss_handle* h, lss_info* info);
  * acceptance, barrier, events and features, subtraction), [14] scan passes
  * that did work, [15] workgroups, 7 the ctc table the matcher uses, file layout
  * [T][T][2M+1] f32 (as ctc.npy, or as computed at load time without it; needs
- * neither keep_stages nor a batch).
+ * neither keep_stages nor a batch); with preprocessing on: 8 the preprocessed
+ * batch [C][W] f32 (the whitening input), 9 channel means [C] f32, 10 per-sample
+ * medians [W] f32.
  * *bytes: in = capacity, out = size written. */
 int lss_debug_stage(struct lss_handle* h, int32_t stage, void* out, int64_t* bytes);
 
 /* Device time per stage accumulated over the batches run with cfg.profile set:
- * ms[0] input copy, ms[1] whitening, ms[2] proposal network and gate (0 without
- * one), ms[3] PC conv, ms[4] projection, ms[5] peel loop with events and
- * features (LSS_NSTAGES entries); *batches = batches timed.
+ * ms[0] input copy, ms[1] preprocessing (0 without), ms[2] whitening, ms[3]
+ * proposal network and gate (0 without one), ms[4] PC conv, ms[5] projection,
+ * ms[6] peel loop with events and features (LSS_NSTAGES entries);
+ * *batches = batches timed.
  * reset != 0 clears the accumulators after reading.  No reference counterpart:
  * the reference's per-stage Timer scopes print nothing (src/Helpers/Timer.h:12-16). */
-#define LSS_NSTAGES 6
+#define LSS_NSTAGES 7
 int lss_stage_times(struct lss_handle* h, double* ms, int64_t* batches, int32_t reset);
 
 /* Spike positions (x, y) of the events the last call returned, in the same

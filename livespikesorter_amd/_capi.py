@@ -26,6 +26,8 @@ SEM_KILOSORT = 0
 SEM_LSS = 1
 PREC_FP32 = 0
 PREC_BF16_RESCORE = 1
+PRE_MEAN, PRE_CAR, PRE_HIGHPASS = 1, 2, 4
+PRE_ALL = PRE_MEAN | PRE_CAR | PRE_HIGHPASS
 
 ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_process_batch_device",
                  "lss_process_chunk", "lss_flush", "lss_query", "lss_debug_stage", "lss_stage_times",
@@ -35,8 +37,8 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
                  "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions",
                  "lss_create_subset")
-NSTAGES = 6
-STAGE_NAMES = ("h2d_copy", "whiten", "proposal", "pcconv", "project", "peel")
+NSTAGES = 7
+STAGE_NAMES = ("h2d_copy", "preprocess", "whiten", "proposal", "pcconv", "project", "peel")
 
 
 class Config(ct.Structure):
@@ -44,7 +46,7 @@ class Config(ct.Structure):
                 ("max_iter", ct.c_int32), ("semantics", ct.c_int32), ("precision", ct.c_int32),
                 ("dedup", ct.c_int32), ("max_events", ct.c_int32), ("keep_stages", ct.c_int32),
                 ("sparse", ct.c_int32), ("profile", ct.c_int32), ("peel_split", ct.c_int32),
-                ("relabel", ct.c_int32), ("reserved", ct.c_int32 * 3)]
+                ("relabel", ct.c_int32), ("preprocess", ct.c_int32), ("reserved", ct.c_int32 * 2)]
 
 
 class Event(ct.Structure):
@@ -62,7 +64,7 @@ class Info(ct.Structure):
                 ("chunk", ct.c_int32), ("max_iter", ct.c_int32), ("batches", ct.c_int64),
                 ("events", ct.c_int64), ("last_iters", ct.c_int32), ("last_spikes", ct.c_int32),
                 ("last_ms", ct.c_double), ("white_kblocks", ct.c_int64), ("proj_kblocks", ct.c_int64),
-                ("proj_kblocks_dense", ct.c_int64), ("precision", ct.c_int32), ("reserved0", ct.c_int32),
+                ("proj_kblocks_dense", ct.c_int64), ("precision", ct.c_int32), ("p2p", ct.c_float),
                 ("rescored", ct.c_int64), ("rescore_flips", ct.c_int64), ("peel_cw", ct.c_int32),
                 ("peel_rows", ct.c_int32), ("peel_split", ct.c_int32), ("peel_groups", ct.c_int32),
                 ("gate_tiles_scored", ct.c_int64), ("gate_tiles", ct.c_int64),

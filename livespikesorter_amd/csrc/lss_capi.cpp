@@ -215,6 +215,13 @@ struct lss_handle {
     std::vector<int> orig, cmap_default, cmap_cur;
     std::vector<int> iU_int;         // main channel of each template, internal order
     std::vector<int> keep;           // template subset (file indices), empty = all
+    // preprocessing in front of whitening (cfg.preprocess)
+    int pre = 0;
+    float* d_hp = nullptr;           // hp_filter.npy, whole impulse response
+    int hp_F = 0, hp_lo = 0, hp_hi = 0;   // its length and the span of taps that matter
+    int *d_psum = nullptr, *d_pmin = nullptr, *d_pmax = nullptr;
+    float *d_pmean = nullptr, *d_pmed = nullptr, *d_X1 = nullptr, *d_Xp = nullptr;
+    int64_t PS = 0;
     int last_W = 0;
 
     // streaming state
@@ -240,7 +247,8 @@ struct lss_handle {
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
                         d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_Pcf, d_Pbv, d_Pim,
-                        d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent};
+                        d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
+                        d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -670,6 +678,42 @@ void load(lss_handle* h, const std::string& dir_in) {
         h->d_dbgCf = dalloc<float>(Wmax);
         h->d_dbgIm = dalloc<int>(Wmax);
     }
+    // preprocessing (cfg.preprocess): hp_filter.npy (loadKilosortTrainingData, OnlineSpikesV2.cu:597-616)
+    h->pre = h->cfg.preprocess;
+    if (h->pre & ~(PRE_MEAN | PRE_CAR | PRE_HP)) throw LssError(LSS_ERR_ARG, "unknown preprocessing bits");
+    if (h->pre) {
+        if (C > PRE_MAX_C) throw LssError(LSS_ERR_ARG, "preprocessing supports at most 1024 channels");
+        if (h->pre & PRE_HP) {
+            if (!file_exists(dir + "hp_filter.npy"))
+                throw LssError(LSS_ERR_IO, "high-pass filtering needs hp_filter.npy");
+            auto hp = npy_load(dir + "hp_filter.npy").as<float>();
+            if (hp.empty()) throw LssError(LSS_ERR_IO, "hp_filter.npy is empty");
+            h->hp_F = (int)hp.size();
+            // the span of taps that matter: the response's tails outside [hp_lo, hp_hi) hold
+            // at most 2^-30 of its summed magnitude (each side 2^-31)
+            double tot = 0;
+            for (float v : hp) tot += std::fabs((double)v);
+            const double eps = tot * std::ldexp(1.0, -31);
+            double acc = 0;
+            int lo = 0;
+            while (lo < h->hp_F && acc + std::fabs((double)hp[lo]) <= eps) acc += std::fabs((double)hp[lo++]);
+            acc = 0;
+            int hi = h->hp_F;
+            while (hi > lo && acc + std::fabs((double)hp[hi - 1]) <= eps) acc += std::fabs((double)hp[--hi]);
+            h->hp_lo = lo;
+            h->hp_hi = hi;
+            h->d_hp = dalloc<float>(hp.size()); upload(h->d_hp, hp);
+        }
+        h->PS = round_up(Wmax, 64);
+        const int nslab = (Wmax + PRE_SLAB - 1) / PRE_SLAB;
+        h->d_psum = dalloc<int>((size_t)nslab * C);
+        h->d_pmin = dalloc<int>((size_t)nslab * C);
+        h->d_pmax = dalloc<int>((size_t)nslab * C);
+        h->d_pmean = dalloc<float>(C);
+        h->d_pmed = dalloc<float>(Wmax);
+        h->d_X1 = dalloc<float>((size_t)C * h->PS);
+        if (h->pre & PRE_HP) h->d_Xp = dalloc<float>((size_t)C * h->PS);
+    }
     // probe geometry for spike positions, final-cluster centroids for relabelling
     if (file_exists(dir + "xc.npy") && file_exists(dir + "yc.npy")) {
         auto xa = npy_load(dir + "xc.npy"), ya = npy_load(dir + "yc.npy");
@@ -733,6 +777,8 @@ void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
 }  // namespace
 
 struct BatchPlan {
+    PreJob pre;
+    bool has_pre = false;
     WhitenJob wj;
     PcconvJob cj;
     ProjectJob pj;
@@ -754,6 +800,34 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     wj.ld = ld; wj.CP = h->CP; wj.W = W; wj.C = h->C; wj.XS = h->XS;
     wj.mtiles = h->CP / GEMM_BM;
     wj.ntiles = (W + XOFF + GEMM_BN - 1) / GEMM_BN;          // cover the right zero pad
+    bp.has_pre = h->pre != 0;
+    if (bp.has_pre) {
+        PreJob& q = bp.pre;
+        q = PreJob{};
+        q.x = d_x; q.xptr = xptr; q.ld = ld; q.cmap = h->d_cmap;
+        q.C = h->C; q.W = W; q.flags = h->pre; q.nslab = (W + PRE_SLAB - 1) / PRE_SLAB;
+        q.psum = h->d_psum; q.pmin = h->d_pmin; q.pmax = h->d_pmax;
+        q.mean = h->d_pmean; q.med = h->d_pmed; q.X1 = h->d_X1; q.Xp = h->d_Xp; q.PS = h->PS;
+        q.p2p = &h->d_hdr->p2p;
+        if (h->pre & PRE_HP) {
+            // taps h[m] = hp[m + crop], m in [0, W): the response cropped (F >= W) or zero-padded
+            // to W (fft_highpass, KS/preprocessing.py:143-153); only m with hp index in
+            // [hp_lo, hp_hi) are kept.  Kilosort's fftshift reads y[(k - W/2) mod W], the
+            // reference's shift_kernel y[(k + W/2) mod W] (myGPUhelpers.cu:72-83).
+            const int F = h->hp_F;
+            if (h->sem == LSS_SEM_LSS && W > F)
+                throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) + " samples is longer than the " +
+                                            std::to_string(F) + "-sample high-pass filter");
+            const int crop = F >= W ? (F - W) / 2 : -((W - F) / 2);
+            const int m_lo = std::max(0, h->hp_lo - crop), m_hi = std::min(W, h->hp_hi - crop);
+            const int s0 = h->sem == LSS_SEM_KILOSORT ? (W - W / 2) % W : (W / 2) % W;
+            q.ntap = std::max(0, m_hi - m_lo);
+            q.taps = h->d_hp + crop + m_lo;
+            q.shift = (int)(((int64_t)s0 + std::max(0, m_lo)) % W);
+        }
+        wj.xf = (h->pre & PRE_HP) ? h->d_Xp : h->d_X1;
+        wj.xfs = h->PS;
+    }
     PcconvJob& cj = bp.cj;
     cj = PcconvJob{};
     cj.Xw = h->d_Xw; cj.wpca = h->d_wpca; cj.Bpc = h->d_Bpc;
@@ -924,19 +998,26 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     kb.n = 1; kb.g_start[0] = 0; kb.g_start[1] = bp.G; kb.p[0] = bp.pp;
     if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
     launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, h->d_prof, s);
-    launch_whiten(wb, s);
+    if (bp.has_pre) {
+        PreBatch pre{};
+        pre.n = 1;
+        pre.job[0] = bp.pre;
+        launch_preprocess(pre, s);
+    }
     if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
-    if (h->prop) run_gate(h, W, s);
+    launch_whiten(wb, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
-    launch_pcconv(cb, h->K, s);
+    if (h->prop) run_gate(h, W, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
-    launch_project(pb, h->bf16, s);
+    launch_pcconv(cb, h->K, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
+    launch_project(pb, h->bf16, s);
+    if (prof) HIPCHK(hipEventRecord(h->evt[6], s));
     HIPCHK(hipGetLastError());
     if (h->cfg.keep_stages)
         HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
     HIPCHK(launch_peel(kb, R, h->coop, s));
-    if (prof) HIPCHK(hipEventRecord(h->evt[6], s));
+    if (prof) HIPCHK(hipEventRecord(h->evt[7], s));
     HIPCHK(hipStreamSynchronize(s));
     if (prof) {
         float ms = 0.f;
@@ -961,6 +1042,7 @@ int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_
              bool dedup) {
     const int64_t n = std::min<int64_t>(h->hdr->count, h->ev_cap);
     if (h->hdr->count > h->ev_cap) throw LssError(LSS_ERR_CAPACITY, "device event buffer overflow (max_events)");
+    if (h->pre) h->info.p2p = h->hdr->p2p;
     const int64_t fw = (int64_t)h->nC * h->K;
     int64_t out = 0, total = 0;
     h->last_pos.clear();
@@ -1256,6 +1338,24 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
         HIPCHK(hipStreamSynchronize(h->stream));
         const int W = h->last_W, C = h->C, T = h->T, K = h->K;
         int64_t need = 0;
+        if (stage >= 8 && stage <= 10) {
+            if (!h->pre) throw LssError(LSS_ERR_STATE, "preprocessing is off");
+            if (stage == 8) {
+                if (*bytes < (int64_t)C * W * 4) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+                HIPCHK(hipMemcpy2D(out, (size_t)W * 4, (h->pre & PRE_HP) ? h->d_Xp : h->d_X1, (size_t)h->PS * 4,
+                                   (size_t)W * 4, C, hipMemcpyDeviceToHost));
+                *bytes = (int64_t)C * W * 4;
+            } else if (stage == 9) {
+                if (*bytes < (int64_t)C * 4) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+                HIPCHK(hipMemcpy(out, h->d_pmean, (size_t)C * 4, hipMemcpyDeviceToHost));
+                *bytes = (int64_t)C * 4;
+            } else {
+                if (*bytes < (int64_t)W * 4) throw LssError(LSS_ERR_ARG, "stage buffer too small");
+                HIPCHK(hipMemcpy(out, h->d_pmed, (size_t)W * 4, hipMemcpyDeviceToHost));
+                *bytes = (int64_t)W * 4;
+            }
+            return;
+        }
         if (stage == 0) need = (int64_t)C * W * 4;
         else if (stage == 1) need = (int64_t)K * C * W * 4;
         else if (stage == 2) need = (int64_t)T * W * 4;
@@ -1443,10 +1543,12 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     const int n = g->n;
     L.wb = WhitenBatch{}; L.cb = PcconvBatch{}; L.pb = ProjectBatch{}; L.kb = PeelBatch{}; L.rb = ResetBatch{};
     L.wb.n = L.cb.n = L.pb.n = L.kb.n = L.rb.n = n;
+    PreBatch pre{};
     for (int s2 = 0; s2 < n; ++s2) {
         lss_handle* h = g->hs[s2].get();
         BatchPlan bp;
         plan_batch(h, nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, 1, bp);
+        if (bp.has_pre) pre.job[pre.n++] = bp.pre;
         L.wb.job[s2] = bp.wj;
         L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
         L.cb.job[s2] = bp.cj;
@@ -1463,6 +1565,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
     launch_reset_batch(L.rb, st);
+    launch_preprocess(pre, st);
     launch_whiten(L.wb, st);
     {   // the proposal networks of all gated streams: one dispatch per layer
         PnFwdJob fj[PN_MAX_JOBS];
@@ -1742,6 +1845,12 @@ int lss_proposal_train_step(lss_proposal* net, lss_handle* h, const int16_t* d_s
         set_cmap(h, nullptr, ld);
         BatchPlan bp;
         plan_batch(h, d_samples, nullptr, (int)n, ld, MIN_CW, 0, 1, bp);
+        if (bp.has_pre) {
+            PreBatch pre{};
+            pre.n = 1;
+            pre.job[0] = bp.pre;
+            launch_preprocess(pre, h->stream);
+        }
         WhitenBatch wb{};
         wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
         launch_whiten(wb, h->stream);

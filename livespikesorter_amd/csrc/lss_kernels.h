@@ -53,7 +53,8 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
     int64_t count;
     int32_t n_iter;
     int32_t err;
-    int32_t pad[4];
+    float p2p;         // written by the preprocessing (peak-to-peak of the raw batch)
+    int32_t pad[3];
     int32_t pad2[4];
 };
 // Peel-kernel time per phase in 100 MHz realtime ticks, summed and maxed over
@@ -125,8 +126,36 @@ struct WhitenJob {
     const float* Wr; const int16_t* x;
     const int16_t* const* xptr;        // when set, the input pointer is read from device memory
     const int* cmap; const int* kb_ptr; const int* kb_idx; float* Xw;
+    const float* xf; int64_t xfs;      // when set, the input is the preprocessed batch xf[c][w]
     int64_t ld; int CP, W, C, XS, mtiles, ntiles;
 };
+
+// ---- preprocessing in front of whitening (lss_preprocess.hip) ----
+constexpr int PRE_MEAN = 1, PRE_CAR = 2, PRE_HP = 4;
+constexpr int PRE_SLAB = 128;     // samples per partial-statistics workgroup
+constexpr int PRE_ROWS = 32;      // samples per CAR workgroup
+constexpr int PRE_OUT = 1024;     // high-pass outputs per workgroup (256 threads x 4)
+constexpr int PRE_TCH = 2048;     // taps staged in LDS at a time
+constexpr int PRE_MAX_C = 1024;
+
+struct PreJob {
+    const int16_t* x; const int16_t* const* xptr;   // input [W][ld] (as the whitening job)
+    int64_t ld; const int* cmap;
+    int C, W, flags, nslab;
+    int shift;             // high-pass: out[k] = sum_{m < ntap} X1[(k + shift + m) mod W] taps[m]
+    int ntap;
+    const float* taps;
+    int* psum; int* pmin; int* pmax;   // [nslab][C] partial statistics
+    float* mean; float* med;           // [C], [W]
+    float* X1; float* Xp; int64_t PS;  // [C][PS]: after mean and CAR, after the high-pass
+    float* p2p;                        // nullable (host-mapped): max - min of the raw batch
+};
+struct PreBatch {
+    int n;
+    int start_stats[MAX_JOBS + 1], start_car[MAX_JOBS + 1], start_fir[MAX_JOBS + 1];
+    PreJob job[MAX_JOBS];
+};
+void launch_preprocess(PreBatch& b, hipStream_t s);
 struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
     const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)

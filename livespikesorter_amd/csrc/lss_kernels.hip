@@ -113,7 +113,8 @@ __global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int c = k0 + bk + i;
-            rb[i] = (n < W && c < C) ? (float)x[(int64_t)n * ld + cmap[c]] : 0.f;
+            rb[i] = (n < W && c < C) ? (jb.xf ? jb.xf[(int64_t)c * jb.xfs + n] : (float)x[(int64_t)n * ld + cmap[c]])
+                                     : 0.f;
         }
     };
     if (p0 < p1) gload(jb.kb_idx[p0]);

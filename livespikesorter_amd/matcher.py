@@ -21,7 +21,8 @@ from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_BF16_RESCORE, PREC
 __all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS", "PREC_FP32",
            "PREC_BF16_RESCORE"]
 
-STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES, STAGE_CTC = range(8)
+(STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES, STAGE_CTC, STAGE_PRE,
+ STAGE_PRE_MEAN, STAGE_PRE_MED) = range(11)
 
 
 class OnlineSpikesV2:
@@ -31,14 +32,15 @@ class OnlineSpikesV2:
                  max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
                  precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
                  keep_stages: bool = False, sparse: bool = True, profile: bool = False,
-                 peel_split: int = 0, relabel: bool = False, keep=None):
+                 peel_split: int = 0, relabel: bool = False, preprocess: int = 0, keep=None):
         """``keep``: ascending Wall3 template indices to load (a template subset,
         lss_create_subset); event template ids are then positions in ``keep``."""
         self._h = ct.c_void_p()
         cfg = default_config(device=device, chunk=chunk, max_batch=max_batch, max_iter=max_iter,
                              semantics=semantics, precision=precision, dedup=dedup,
                              max_events=max_events, keep_stages=int(keep_stages), sparse=int(sparse),
-                             profile=int(profile), peel_split=peel_split, relabel=int(relabel))
+                             profile=int(profile), peel_split=peel_split, relabel=int(relabel),
+                             preprocess=preprocess)
         if keep is None:
             check(lib().lss_create(os.fsencode(str(oss_input_dir)), ct.byref(cfg), ct.byref(self._h)))
         else:
@@ -165,13 +167,15 @@ class OnlineSpikesV2:
         return dict(zip(STAGE_NAMES, list(ms))), nb.value
 
     def debug_stage(self, stage: int) -> np.ndarray:
-        """Per-stage device buffers of the last batch (needs ``keep_stages=True``)."""
+        """Per-stage device buffers of the last batch (stages B, CFMAX and IMAX
+        need ``keep_stages=True``; the PRE stages need preprocessing on)."""
         info = self.info
         W = self.last_W
         shapes = {STAGE_XW: ((info.C, W), np.float32), STAGE_BPC: ((info.K, info.C, W), np.float32),
                   STAGE_B: ((info.T, W), np.float32), STAGE_CFMAX: ((W,), np.float32),
                   STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32),
-                  STAGE_PHASES: ((16,), np.uint64)}
+                  STAGE_PHASES: ((16,), np.uint64), STAGE_PRE: ((info.C, W), np.float32),
+                  STAGE_PRE_MEAN: ((info.C,), np.float32), STAGE_PRE_MED: ((W,), np.float32)}
         shape, dt = shapes[stage]
         out = np.zeros(shape, dt)
         nb = ct.c_int64(out.nbytes)
@@ -197,12 +201,12 @@ class StreamGroup:
 
     def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
                  semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True,
-                 precision: int = PREC_FP32):
+                 precision: int = PREC_FP32, preprocess: int = 0):
         self._g = ct.c_void_p()
         self.n = len(oss_input_dirs)
         cfg = default_config(device=device, chunk=batch, max_batch=batch, max_iter=max_iter,
                              semantics=semantics, max_events=max_events, sparse=int(sparse),
-                             precision=precision)
+                             precision=precision, preprocess=preprocess)
         dirs = (ct.c_char_p * self.n)(*[os.fsencode(str(d)) for d in oss_input_dirs])
         check(lib().lss_group_create(self.n, dirs, ct.byref(cfg), ct.byref(self._g)))
         self.batch = batch

@@ -170,6 +170,17 @@ def extract_tables(U: np.ndarray, xc: np.ndarray, yc: np.ndarray, nC: int):
     return iCC, iU, np.ascontiguousarray(Ucc.astype(np.float32))
 
 
+def highpass_impulse(fs: float = 30000.0, cutoff: float = 300.0, NT: int = 30122) -> np.ndarray:
+    """hp_filter.npy: impulse response of filtfilt with a 3rd-order Butterworth
+    high-pass, from a unit impulse at NT//2 of NT samples, as FP32
+    (get_highpass_filter, KS/preprocessing.py:121-136)."""
+    from scipy.signal import butter, filtfilt
+    b, a = butter(3, cutoff, fs=fs, btype="high")
+    x = np.zeros(NT)
+    x[NT // 2] = 1.0
+    return filtfilt(b, a, x).astype(np.float32)
+
+
 def write_oss_input(model: ProbeModel, path: str | os.PathLike, *, with_ctc: bool = True) -> Path:
     """Write an oss_input directory (file names as read at OnlineSpikesV2.cu:500-510,580-598)."""
     p = Path(path)
@@ -199,6 +210,7 @@ def write_oss_input(model: ProbeModel, path: str | os.PathLike, *, with_ctc: boo
     # final clusters: one per template, centred on its footprint (x = 0, y = 20 um x centre)
     np.save(p / "cluster_centroids.npy",
             np.stack([np.zeros(T), 20.0 * model.mu], axis=1).astype(np.float32))
+    np.save(p / "hp_filter.npy", highpass_impulse(model.fs))
     with open(p / "misc.txt", "w") as f:
         f.write(f"nt0min:{model.nt0min}\n")
         f.write(f"numNearestChans:{model.nC}\n")

@@ -76,11 +76,16 @@ class OracleResult:
 def run_batch(x: np.ndarray, Wr, wPCA, U, ctc, iCC, iU, *, Th: float = 8.0, nC: int = 10,
               max_iter: int = 100, semantics: int = SEM_KILOSORT, bits: int = 64,
               threads: int = 0, chan_map=None, stages: bool = False, cap: int | None = None,
-              features: bool = True) -> OracleResult:
-    """Run one batch (``x`` int16 [W][ld]) through the oracle."""
+              features: bool = True, xf: np.ndarray | None = None) -> OracleResult:
+    """Run one batch (``x`` int16 [W][ld]) through the oracle.  With ``xf``
+    (preprocessed input [C][W], FP64, channel-gathered) ``x`` is ignored and
+    whitening starts from ``xf`` (see oracle.preprocess)."""
     lib = _lib(bits)
     real = np.float64 if bits == 64 else np.float32
     creal = ct.c_double if bits == 64 else ct.c_float
+    if xf is not None:
+        xf = np.ascontiguousarray(xf, dtype=np.float64)
+        x = np.zeros((xf.shape[1], 1), np.int16)
     x = np.ascontiguousarray(x, dtype=np.int16)
     W, ld = x.shape
     T, K, C = U.shape
@@ -106,14 +111,20 @@ def run_batch(x: np.ndarray, Wr, wPCA, U, ctc, iCC, iU, *, Th: float = 8.0, nC: 
     sB = np.zeros((T, W), real) if stages else None
     sCf = np.zeros(W, real) if stages else None
     sIm = np.zeros(W, np.int32) if stages else None
-    fn = getattr(lib, f"orc{bits}_run_batch")
-    fn.restype = ct.c_long
-    n = fn(ct.byref(prm), _p(x, ct.c_int16), ct.c_long(W), ct.c_long(ld), _p(cm, ct.c_int),
-           _p(Wr, ct.c_float), _p(wPCA, ct.c_float), _p(U, ct.c_float), _p(ctc, ct.c_float),
-           _p(iCC, ct.c_int64), _p(iU, ct.c_int64), ct.c_long(cap),
-           _p(ow, ct.c_long), _p(ot, ct.c_int), _p(oa, creal), _p(oi, ct.c_int), _p(of, creal),
-           ct.byref(nit), _p(sXw, creal), _p(sBpc, creal), _p(sB, creal), _p(sCf, creal),
-           _p(sIm, ct.c_int))
+    outs = (ct.c_long(cap), _p(ow, ct.c_long), _p(ot, ct.c_int), _p(oa, creal), _p(oi, ct.c_int), _p(of, creal),
+            ct.byref(nit), _p(sXw, creal), _p(sBpc, creal), _p(sB, creal), _p(sCf, creal), _p(sIm, ct.c_int))
+    tabs = (_p(Wr, ct.c_float), _p(wPCA, ct.c_float), _p(U, ct.c_float), _p(ctc, ct.c_float),
+            _p(iCC, ct.c_int64), _p(iU, ct.c_int64))
+    if xf is not None:
+        if xf.shape != (C, W):
+            raise ValueError(f"xf must be [C][W] = [{C}][{W}]")
+        fn = getattr(lib, f"orc{bits}_run_batch_f")
+        fn.restype = ct.c_long
+        n = fn(ct.byref(prm), _p(xf, ct.c_double), ct.c_long(W), *tabs, *outs)
+    else:
+        fn = getattr(lib, f"orc{bits}_run_batch")
+        fn.restype = ct.c_long
+        n = fn(ct.byref(prm), _p(x, ct.c_int16), ct.c_long(W), ct.c_long(ld), _p(cm, ct.c_int), *tabs, *outs)
     if n < 0:
         raise RuntimeError("oracle spike capacity exceeded")
     return OracleResult(ow[:n].copy(), ot[:n].copy(), oa[:n].copy(), oi[:n].copy(),

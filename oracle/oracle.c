@@ -314,6 +314,12 @@ void ORC(features)(const orc_params *p, const REAL *Xres, long W, const float *w
  * returned sorted by (column, detection order), the order extract() returns
  * after its final argsort (KS/template_matching.py:80-83) with ties kept in
  * detection order.  Stage buffers are optional (NULL to skip). */
+static long ORC(match_whitened)(const orc_params *p, REAL *Xw, long W, const float *wPCA, const float *U,
+                                const float *ctc, const int64_t *iCC, const int64_t *iU,
+                                long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
+                                int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
+                                REAL *stage_Cfmax, int *stage_imax);
+
 long ORC(run_batch)(const orc_params *p, const int16_t *x, long W, long ld, const int *chan_map,
                     const float *Wr, const float *wPCA, const float *U, const float *ctc,
                     const int64_t *iCC, const int64_t *iU,
@@ -321,12 +327,49 @@ long ORC(run_batch)(const orc_params *p, const int16_t *x, long W, long ld, cons
                     int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
                     REAL *stage_Cfmax, int *stage_imax)
 {
-    const int C = p->C, K = p->K, T = p->T, nC = p->nC;
+    REAL *Xw = (REAL *)malloc(sizeof(REAL) * (size_t)p->C * W);
+    ORC(whiten)(p, x, W, ld, chan_map, Wr, Xw);
+    return ORC(match_whitened)(p, Xw, W, wPCA, U, ctc, iCC, iU, cap, out_w, out_t, out_amp, out_iter, out_feat,
+                               n_iter, stage_Xw, stage_Bpc, stage_B, stage_Cfmax, stage_imax);
+}
+
+/* The same batch from a preprocessed input xf [C][W] (FP64, channel-gathered):
+ * the mean, CAR and high-pass stages in front of whitening are applied by the
+ * caller (oracle/preprocess.py). */
+long ORC(run_batch_f)(const orc_params *p, const double *xf, long W, const float *Wr, const float *wPCA,
+                      const float *U, const float *ctc, const int64_t *iCC, const int64_t *iU,
+                      long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
+                      int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
+                      REAL *stage_Cfmax, int *stage_imax)
+{
+    const int C = p->C;
     REAL *Xw = (REAL *)malloc(sizeof(REAL) * (size_t)C * W);
+    #pragma omp parallel for schedule(static) num_threads(NTHR(p))
+    for (int c = 0; c < C; ++c) {
+        REAL *o = Xw + (size_t)c * W;
+        for (long w = 0; w < W; ++w) o[w] = 0;
+        for (int c2 = 0; c2 < C; ++c2) {
+            const REAL a = (REAL)Wr[(size_t)c * C + c2];
+            if (a == 0) continue;
+            const double *xi = xf + (size_t)c2 * W;
+            for (long w = 0; w < W; ++w) o[w] += a * (REAL)xi[w];
+        }
+    }
+    return ORC(match_whitened)(p, Xw, W, wPCA, U, ctc, iCC, iU, cap, out_w, out_t, out_amp, out_iter, out_feat,
+                               n_iter, stage_Xw, stage_Bpc, stage_B, stage_Cfmax, stage_imax);
+}
+
+/* Xw is taken over (freed here). */
+static long ORC(match_whitened)(const orc_params *p, REAL *Xw, long W, const float *wPCA, const float *U,
+                                const float *ctc, const int64_t *iCC, const int64_t *iU,
+                                long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
+                                int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
+                                REAL *stage_Cfmax, int *stage_imax)
+{
+    const int C = p->C, K = p->K, T = p->T, nC = p->nC;
     REAL *Bpc = (REAL *)malloc(sizeof(REAL) * (size_t)C * K * W);
     REAL *B = (REAL *)malloc(sizeof(REAL) * (size_t)T * W);
     REAL *nm = (REAL *)malloc(sizeof(REAL) * T);
-    ORC(whiten)(p, x, W, ld, chan_map, Wr, Xw);
     ORC(pcconv)(p, Xw, W, wPCA, Bpc);
     ORC(project)(p, Bpc, W, U, B);
     ORC(norms)(p, U, nm);
