@@ -299,6 +299,121 @@ int lss_set_proposal(struct lss_handle* h, struct lss_proposal* net, float lo, f
 /* Message of the last error on this thread ("" if none). */
 const char* lss_last_error(void);
 
+/* ==== Streaming around the matcher: the live loop's window policy, event
+ * bookkeeping, output formats and a recording replay source (host code) ==== */
+
+/* The window and skip policy of runSpikeSorting (OnlineSpikesV2.cu:698-745):
+ * after waiting for min_window new samples past the last fetch, one of five
+ * fetches is chosen from how far the stream count is ahead of processedCt. */
+typedef struct lss_window_cfg {
+    int64_t min_window;      /* iMinScanWindow: samples waited for per batch */
+    int64_t max_window;      /* iMaxScanWindow: samples fetched at most (the socket's maximum) */
+    int32_t time_behind_ms;  /* iTimeBehind: 0 = skip ahead whenever behind */
+    float samp_rate;         /* fImecSamplingRate / downsampling */
+    int32_t small_skip;      /* bSmallskip */
+    int32_t lookback;        /* overlap kept between batches, 2M (OnlineSpikesV2.cu:461) */
+} lss_window_cfg;
+
+#define LSS_WIN_LATEST 0       /* within max_window: fetch [processedCt, latest) to row 0 (:709-717) */
+#define LSS_WIN_SKIP_LATEST 1  /* behind, time_behind 0: latest max_window samples (:720-724) */
+#define LSS_WIN_CATCH_UP 2     /* behind within time_behind: max_window from processedCt (:727-731) */
+#define LSS_WIN_SMALL_SKIP 3   /* too far behind: max_window from the allowed count (:734-738) */
+#define LSS_WIN_BIG_SKIP 4     /* too far behind: latest max_window samples (:741-745) */
+
+typedef struct lss_window_step {
+    int32_t kind;               /* LSS_WIN_* */
+    int32_t skip;               /* the batch starts at row min_window of the fetch buffer (:750-757) */
+    int64_t fetch_start;        /* stream samples [fetch_start, fetch_start + fetch_n) are fetched */
+    int64_t fetch_n;
+    int64_t buf_row;            /* ... into the fetch buffer from this row (0 or min_window) */
+    int64_t batch_row;          /* the batch is buffer rows [batch_row, batch_row + batch_n) */
+    int64_t batch_n;            /* currBatchNumSamples */
+    int64_t latest_ct;          /* latestCt after the fetch */
+    int64_t allowed_ct;         /* allowedCt = latestCt - time_behind * rate / 1000, in FP32 as there */
+    int64_t save_start;         /* saveSpikes start offset: latest_ct - batch_n + 1 (:822) */
+    int64_t save_end;           /* saveSpikes end offset: batch_n - min_window */
+    int64_t next_processed_ct;  /* processedCt after the batch: latest_ct - lookback (:844) */
+} lss_window_step;
+
+/* Decide the next batch from processedCt and the stream's sample count (the
+ * count is taken as unchanged while fetching).  Pure function. */
+int lss_window_plan(const lss_window_cfg* cfg, int64_t processed_ct, int64_t stream_ct, lss_window_step* step);
+
+/* saveSpikes (OnlineSpikesV2.cu:1173-1206) over the events of one batch matched
+ * with first_sample 0 (sample = column - M/2 + nt0min): index = sample - M -
+ * lookback, time = save_start + index; a template's spike within dt of its
+ * last kept time is dropped, the last time is updated before the end-valid
+ * test (index >= save_end drops it), and kept spikes are inserted in time
+ * order after equal times.  last_spike [n_templates] carries across batches
+ * (zero-initialised, as the reference's lastSpikeTime).  Events are taken in
+ * the order given (the reference's detection order is not kept by the
+ * device; this matcher gives them in sample order).  src (nullable) receives
+ * each kept spike's index in ev. */
+int lss_save_spikes(const lss_event* ev, int64_t n, int32_t M, int32_t lookback, int32_t dt, int64_t save_start,
+                    int64_t save_end, int64_t* last_spike, int32_t n_templates, lss_event* out, int64_t* src,
+                    int64_t cap, int64_t* n_out);
+
+/* spikeOutput.txt lines "time,template,amplitude,y\n" as writeSpikesToFile
+ * writes them (OnlineSpikesV2.cu:1161-1164; std::ostream defaults, so floats
+ * print as %g): time = recording_offset + sample.  y [n] is the spike's
+ * position (nullable: 0).  The reference indexes y by the sorted position
+ * against detection-order positions; here each line carries its own spike's
+ * y.  out NULL: *size = bytes needed. */
+int lss_format_spikes(int64_t recording_offset, const lss_event* ev, const float* y, int64_t n, char* out,
+                      int64_t cap, int64_t* size);
+
+/* OnlineSpikesPayload (src/Networking/onlineSpikesPayload.h:8-48) in cereal's
+ * binary archive layout: fields in declaration order, raw little-endian,
+ * vectors as a uint64 length then the elements.  `long` fields take
+ * long_bytes bytes: 4 as the reference's Windows (LLP64) build sends them,
+ * 8 for an LP64 build. */
+typedef struct lss_payload_header {
+    int64_t recording_offset;
+    int64_t stream_sample_ct;
+    double vrms;
+    float p2p;
+    int64_t process_time;
+    int64_t event_stream_sample_ct;
+    int16_t predict_label, label, n_trials, n_correct;
+    double confidence;
+} lss_payload_header;
+
+/* out NULL: *size = bytes needed. */
+int lss_payload_encode(const lss_payload_header* h, const int64_t* times, const int64_t* templates,
+                       const float* amplitudes, int64_t n, int32_t long_bytes, uint8_t* out, int64_t cap,
+                       int64_t* size);
+/* times/templates/amplitudes NULL: only the header and *n. */
+int lss_payload_decode(const uint8_t* buf, int64_t size, int32_t long_bytes, lss_payload_header* h, int64_t* times,
+                       int64_t* templates, float* amplitudes, int64_t cap, int64_t* n);
+
+/* SorterParameters (src/Networking/sorterParameters.h:6-29), same layout rules;
+ * getSorterParams fills it with T, C, M, W, the sampling rate, the active
+ * channels' positions and the template map (OnlineSpikesV2.cu:1076-1086). */
+typedef struct lss_sorter_params {
+    int64_t T, C, M, N;
+    float samp_rate;
+} lss_sorter_params;
+
+int lss_sorter_params_encode(const lss_sorter_params* p, const double* chanpos, int64_t n_chanpos,
+                             const int32_t* neuron_indices, int64_t n_indices, int32_t long_bytes, uint8_t* out,
+                             int64_t cap, int64_t* size);
+int lss_sorter_params_decode(const uint8_t* buf, int64_t size, int32_t long_bytes, lss_sorter_params* p,
+                             double* chanpos, int64_t chanpos_cap, int64_t* n_chanpos, int32_t* neuron_indices,
+                             int64_t indices_cap, int64_t* n_indices);
+
+/* Replay source over a recorded .ap.bin (interleaved int16, n_chan_file
+ * channels per sample, channels fastest; an imec file has 385, the last being
+ * sync), memory-mapped.  channels NULL: all but the last.  Fetches return
+ * [n][n_channels] int16 rows as the reference's socket fetch does for its
+ * channel subset (dataSocket.cpp:103-168) and verify_batch.py:33-39 reads. */
+struct lss_replay;
+int lss_replay_open(const char* path, int32_t n_chan_file, const int32_t* channels, int32_t n_channels,
+                    struct lss_replay** out);
+int lss_replay_samples(struct lss_replay* r, int64_t* n_samples, int32_t* n_channels);
+/* *n_out = samples copied (fewer at the end of the file) */
+int lss_replay_fetch(struct lss_replay* r, int64_t start, int64_t n, int16_t* out, int64_t* n_out);
+int lss_replay_close(struct lss_replay* r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,10 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
                  "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions",
-                 "lss_create_subset")
+                 "lss_create_subset", "lss_window_plan", "lss_save_spikes", "lss_format_spikes",
+                 "lss_payload_encode", "lss_payload_decode", "lss_sorter_params_encode",
+                 "lss_sorter_params_decode", "lss_replay_open", "lss_replay_samples", "lss_replay_fetch",
+                 "lss_replay_close")
 NSTAGES = 7
 STAGE_NAMES = ("h2d_copy", "preprocess", "whiten", "proposal", "pcconv", "project", "peel")
 
@@ -81,6 +84,32 @@ class LssError(RuntimeError):
         super().__init__(f"[{code}] {msg}")
         self.code = code
 
+
+class WindowCfg(ct.Structure):
+    _fields_ = [("min_window", ct.c_int64), ("max_window", ct.c_int64), ("time_behind_ms", ct.c_int32),
+                ("samp_rate", ct.c_float), ("small_skip", ct.c_int32), ("lookback", ct.c_int32)]
+
+
+class WindowStep(ct.Structure):
+    _fields_ = [("kind", ct.c_int32), ("skip", ct.c_int32), ("fetch_start", ct.c_int64), ("fetch_n", ct.c_int64),
+                ("buf_row", ct.c_int64), ("batch_row", ct.c_int64), ("batch_n", ct.c_int64),
+                ("latest_ct", ct.c_int64), ("allowed_ct", ct.c_int64), ("save_start", ct.c_int64),
+                ("save_end", ct.c_int64), ("next_processed_ct", ct.c_int64)]
+
+
+class PayloadHeader(ct.Structure):
+    _fields_ = [("recording_offset", ct.c_int64), ("stream_sample_ct", ct.c_int64), ("vrms", ct.c_double),
+                ("p2p", ct.c_float), ("process_time", ct.c_int64), ("event_stream_sample_ct", ct.c_int64),
+                ("predict_label", ct.c_int16), ("label", ct.c_int16), ("n_trials", ct.c_int16),
+                ("n_correct", ct.c_int16), ("confidence", ct.c_double)]
+
+
+class SorterParams(ct.Structure):
+    _fields_ = [("T", ct.c_int64), ("C", ct.c_int64), ("M", ct.c_int64), ("N", ct.c_int64),
+                ("samp_rate", ct.c_float)]
+
+
+WIN_LATEST, WIN_SKIP_LATEST, WIN_CATCH_UP, WIN_SMALL_SKIP, WIN_BIG_SKIP = range(5)
 
 _LIB = None
 
@@ -133,6 +162,24 @@ def lib() -> ct.CDLL:
         L.lss_group_set_proposal.argtypes = [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_float, ct.c_float,
                                              ct.c_int32]
         L.lss_set_proposal.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_float, ct.c_int32]
+        P64 = ct.POINTER(ct.c_int64)
+        L.lss_window_plan.argtypes = [ct.POINTER(WindowCfg), ct.c_int64, ct.c_int64, ct.POINTER(WindowStep)]
+        L.lss_save_spikes.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int64,
+                                      ct.c_int64, ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_void_p, ct.c_int64, P64]
+        L.lss_format_spikes.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64,
+                                        P64]
+        L.lss_payload_encode.argtypes = [ct.POINTER(PayloadHeader), ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                         ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_int64, P64]
+        L.lss_payload_decode.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int32, ct.POINTER(PayloadHeader),
+                                         ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64, P64]
+        L.lss_sorter_params_encode.argtypes = [ct.POINTER(SorterParams), ct.c_void_p, ct.c_int64, ct.c_void_p,
+                                               ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_int64, P64]
+        L.lss_sorter_params_decode.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int32, ct.POINTER(SorterParams),
+                                               ct.c_void_p, ct.c_int64, P64, ct.c_void_p, ct.c_int64, P64]
+        L.lss_replay_open.argtypes = [ct.c_char_p, ct.c_int32, ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_void_p)]
+        L.lss_replay_samples.argtypes = [ct.c_void_p, P64, ct.POINTER(ct.c_int32)]
+        L.lss_replay_fetch.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p, P64]
+        L.lss_replay_close.argtypes = [ct.c_void_p]
         _LIB = L
     return _LIB
 

@@ -1913,3 +1913,7 @@ int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi, int32
 }
 
 }  // extern "C"
+
+namespace lss {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace lss
