@@ -414,6 +414,37 @@ int lss_replay_samples(struct lss_replay* r, int64_t* n_samples, int32_t* n_chan
 int lss_replay_fetch(struct lss_replay* r, int64_t start, int64_t n, int16_t* out, int64_t* n_out);
 int lss_replay_close(struct lss_replay* r);
 
+/* ==== Long-run z-score baseline of binned spike counts (FP64 on the GPU) ====
+ * The Decoder's ZScoreSdmProcessor (src/Decoder/ZScoreSdmProcessor.cpp): spikes
+ * are counted per bin of bin_samples samples (bin = sample / bin_samples,
+ * C++ division; onSpikes :85-94); a completed bin's count updates a running
+ * mean and M2 by Welford's method while the baseline is open (:96-114) and,
+ * once frozen, is scored z = (count - mean) / sd with direction +1 / -1 beyond
+ * +-trigger_z (:116-140).  Series 0 is the total count (the reference's one
+ * series), series 1 + t counts template t.  Counts wait in a ring of
+ * ring_bins bins on the device; an event for a bin already closed is counted
+ * as late and dropped, one past the ring makes lss_zscore_complete fail. */
+struct lss_zscore;
+/* max(1, llround(bin_ms / 1000 * rate)) (:27-28) */
+int lss_zscore_bin_samples(double bin_ms, float samp_rate, int64_t* bin_samples);
+int lss_zscore_create(int32_t device, int32_t n_templates, int64_t bin_samples, int32_t ring_bins,
+                      struct lss_zscore** out);
+/* count events (host array, any order; e.g. what a process call returned) */
+int lss_zscore_add(struct lss_zscore* z, const lss_event* ev, int64_t n);
+/* close every open bin that ends at or before end_sample, in order; with z_out
+ * (and dir_out) each closed bin's z [n_closed][1 + n_templates] (0 while the
+ * baseline is open or its sd is not positive). */
+int lss_zscore_complete(struct lss_zscore* z, int64_t end_sample, float* z_out, int8_t* dir_out, int64_t cap_bins,
+                        int64_t* n_closed);
+/* freeze the baseline at mean/sd [1 + n_templates] (the reference takes them
+ * from the operator, :36-70); NULL: at the running mean and the sample
+ * standard deviation sqrt(M2 / (n - 1)). */
+int lss_zscore_freeze(struct lss_zscore* z, const double* mean, const double* sd, double trigger_z);
+/* running statistics [1 + n_templates] (each nullable), late events, first open bin */
+int lss_zscore_stats(struct lss_zscore* z, int64_t* bins_seen, double* mean, double* m2, int64_t* late,
+                     int64_t* first_open);
+int lss_zscore_destroy(struct lss_zscore* z);
+
 #ifdef __cplusplus
 }
 #endif

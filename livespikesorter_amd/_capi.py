@@ -39,7 +39,8 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_create_subset", "lss_window_plan", "lss_save_spikes", "lss_format_spikes",
                  "lss_payload_encode", "lss_payload_decode", "lss_sorter_params_encode",
                  "lss_sorter_params_decode", "lss_replay_open", "lss_replay_samples", "lss_replay_fetch",
-                 "lss_replay_close")
+                 "lss_replay_close", "lss_zscore_bin_samples", "lss_zscore_create", "lss_zscore_add",
+                 "lss_zscore_complete", "lss_zscore_freeze", "lss_zscore_stats", "lss_zscore_destroy")
 NSTAGES = 7
 STAGE_NAMES = ("h2d_copy", "preprocess", "whiten", "proposal", "pcconv", "project", "peel")
 
@@ -180,6 +181,13 @@ def lib() -> ct.CDLL:
         L.lss_replay_samples.argtypes = [ct.c_void_p, P64, ct.POINTER(ct.c_int32)]
         L.lss_replay_fetch.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p, P64]
         L.lss_replay_close.argtypes = [ct.c_void_p]
+        L.lss_zscore_bin_samples.argtypes = [ct.c_double, ct.c_float, P64]
+        L.lss_zscore_create.argtypes = [ct.c_int32, ct.c_int32, ct.c_int64, ct.c_int32, ct.POINTER(ct.c_void_p)]
+        L.lss_zscore_add.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
+        L.lss_zscore_complete.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int64, P64]
+        L.lss_zscore_freeze.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_double]
+        L.lss_zscore_stats.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, P64, P64]
+        L.lss_zscore_destroy.argtypes = [ct.c_void_p]
         _LIB = L
     return _LIB
 

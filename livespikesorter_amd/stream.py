@@ -292,3 +292,79 @@ def _has_positions(sorter) -> bool:
         return True
     except LssError:
         return False
+
+
+class ZScoreBaseline:
+    """Long-run z-score baseline of binned spike counts on the GPU, FP64
+    (lss_zscore_*; the Decoder's ZScoreSdmProcessor, src/Decoder/
+    ZScoreSdmProcessor.cpp).  Series 0 is the total count, 1 + t template t."""
+
+    def __init__(self, n_templates: int, *, bin_ms: float = 50.0, samp_rate: float = 30000.0,
+                 bin_samples: int | None = None, ring_bins: int = 4096, device: int = 0):
+        if bin_samples is None:
+            b = ct.c_int64()
+            check(lib().lss_zscore_bin_samples(bin_ms, samp_rate, ct.byref(b)))
+            bin_samples = b.value
+        self.bin_samples = int(bin_samples)
+        self.S = 1 + n_templates
+        self._z = ct.c_void_p()
+        check(lib().lss_zscore_create(device, n_templates, self.bin_samples, ring_bins, ct.byref(self._z)))
+
+    def add(self, events: np.ndarray) -> None:
+        ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        check(lib().lss_zscore_add(self._z, ev.ctypes.data, len(ev)))
+
+    def complete(self, end_sample: int, cap_bins: int = 4096):
+        """Close the bins ending at or before ``end_sample``: (z [bins][S] f32, direction int8)."""
+        z = np.zeros((cap_bins, self.S), np.float32)
+        d = np.zeros((cap_bins, self.S), np.int8)
+        n = ct.c_int64()
+        check(lib().lss_zscore_complete(self._z, int(end_sample), z.ctypes.data, d.ctypes.data, cap_bins,
+                                        ct.byref(n)))
+        return z[:n.value].copy(), d[:n.value].copy()
+
+    def freeze(self, mean=None, sd=None, trigger_z: float = 1.0) -> None:
+        if (mean is None) != (sd is None):
+            raise ValueError("give both mean and sd, or neither")
+        m = None if mean is None else np.ascontiguousarray(mean, dtype=np.float64)
+        s = None if sd is None else np.ascontiguousarray(sd, dtype=np.float64)
+        check(lib().lss_zscore_freeze(self._z, None if m is None else m.ctypes.data,
+                                      None if s is None else s.ctypes.data, trigger_z))
+
+    def stats(self) -> dict:
+        seen = np.zeros(self.S, np.int64)
+        mean = np.zeros(self.S, np.float64)
+        m2 = np.zeros(self.S, np.float64)
+        late, first = ct.c_int64(), ct.c_int64()
+        check(lib().lss_zscore_stats(self._z, seen.ctypes.data, mean.ctypes.data, m2.ctypes.data, ct.byref(late),
+                                     ct.byref(first)))
+        return {"bins_seen": seen, "mean": mean, "m2": m2, "late": late.value, "first_open": first.value}
+
+    def close(self) -> None:
+        if self._z:
+            lib().lss_zscore_destroy(self._z)
+            self._z = ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def sorter_parameters(oss_input_dir, batch_samples: int, samp_rate: float = 30000.0) -> SorterParameters:
+    """getSorterParams (OnlineSpikesV2.cu:1076-1086) from an oss_input directory:
+    T final clusters, C channels and M samples from templates.npy, N the batch
+    size, the active channels 0..C-1 as positions and templateMap.npy
+    (loadTemplateMap, :568-577)."""
+    d = Path(oss_input_dir)
+    tpl = np.load(d / "templates.npy", mmap_mode="r")
+    T, M, C = tpl.shape
+    tmap = np.load(d / "templateMap.npy").ravel().astype(np.int32)
+    if len(tmap) != T:
+        raise ValueError("templateMap.npy: incorrect template map size")
+    return SorterParameters(int(T), int(C), int(M), int(batch_samples), float(samp_rate),
+                            np.arange(C, dtype=np.float64), tmap)
