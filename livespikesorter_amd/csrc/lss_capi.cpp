@@ -185,6 +185,7 @@ struct lss_handle {
     int* d_mt_ghi = nullptr;
     unsigned long long* d_gate_stats = nullptr;
     int slot_cap = 0, cap_g = 0, max_groups = 0;
+    int g_cap = 0, sync_words = 0;       // peel workgroups at most; words of the iteration/flag block
     // template split of the peel: per-subset column maxima [max_split][Wmax]
     int max_split = 1;
     int force_split = 0;                 // autotuning: the split being timed (0: none forced)
@@ -669,7 +670,11 @@ void load(lss_handle* h, const std::string& dir_in) {
         h->d_Pbv = dalloc<float>((size_t)h->max_split * Wmax);
     }
     h->d_list_cnt = dalloc<int>(h->max_groups);
-    h->d_iter_cnt = dalloc<int>(std::max(1, h->max_iter));
+    // per-iteration spike totals and done counters, then the peel's neighbour flags
+    // (per workgroup, per column block); all cleared by the reset kernel each batch
+    h->g_cap = h->max_groups * std::max(1, h->max_split);
+    h->sync_words = 2 * std::max(1, h->max_iter) + h->g_cap + h->max_groups;
+    h->d_iter_cnt = dalloc<int>(h->sync_words);
     h->d_err = dalloc<int>(1);
     h->d_bar = dalloc<unsigned>(BAR_WORDS);
     h->d_prof = dalloc<unsigned long long>(PROF_WORDS);
@@ -852,6 +857,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p = PeelParams{};
     const int NB = (W + CW - 1) / CW;
     const int G = S * NB;
+    if (NB > h->max_groups || G > h->g_cap) throw LssError(LSS_ERR_ARG, "peel grid larger than the handle's flag space");
     p.B = h->d_B; p.BS = h->BS;
     p.B_bytes = (int)std::min<int64_t>((int64_t)h->TP * h->BS * 4, INT32_MAX);
     p.ctc_bytes = (int)std::min<int64_t>((int64_t)h->T * h->ctc_rows * LP * 4, INT32_MAX);
@@ -997,7 +1003,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     PeelBatch kb{};
     kb.n = 1; kb.g_start[0] = 0; kb.g_start[1] = bp.G; kb.p[0] = bp.pp;
     if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
-    launch_reset(h->d_bar, h->d_iter_cnt, h->max_iter, h->d_err, h->d_prof, s);
+    launch_reset(h->d_bar, h->d_iter_cnt, h->sync_words, h->d_err, h->d_prof, s);
     if (bp.has_pre) {
         PreBatch pre{};
         pre.n = 1;
@@ -1557,7 +1563,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         L.pb.start[s2 + 1] = L.pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
         L.kb.p[s2] = bp.pp;
         L.kb.g_start[s2 + 1] = L.kb.g_start[s2] + bp.G;
-        L.rb.bar[s2] = h->d_bar; L.rb.iter_cnt[s2] = h->d_iter_cnt; L.rb.n_iter[s2] = h->max_iter;
+        L.rb.bar[s2] = h->d_bar; L.rb.iter_cnt[s2] = h->d_iter_cnt; L.rb.n_iter[s2] = h->sync_words;
         L.rb.err[s2] = h->d_err; L.rb.prof[s2] = h->d_prof; L.rb.active[s2] = &L.d_xptr[s2];
     }
     L.drop_graph();

@@ -86,6 +86,10 @@ struct PeelParams {
     float* Cfmax; int* imax;
     Spike* slots; int* slot_cnt; int slot_cap;
     Spike* list; int* list_cnt; int cap_g;
+    // iter_cnt [max_iter] spike totals, followed by the peel loop's neighbour
+    // synchronisation words (all zero at batch start): owners done with phase B per
+    // iteration [max_iter], then per column block the iteration its owner finished
+    // phase B [NB], then per workgroup the iteration it finished phase A [G]
     unsigned* bar; int* iter_cnt; int* err;
     // features
     const float* Bpc; int CP;

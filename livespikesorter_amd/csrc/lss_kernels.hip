@@ -392,6 +392,33 @@ __device__ __forceinline__ void grid_sync(unsigned* bar, int G, int g, unsigned&
     __syncthreads();
 }
 
+// Local synchronisation of the peel loop.  A workgroup publishes that it is done
+// with a phase by raising its flag to the iteration number + 1 after its coherent
+// stores have completed; a workgroup that depends on a contiguous range of flags
+// waits (one thread per flag) until all of them reach the target.  A workgroup
+// leaving the loop raises its flags to FLAG_DONE so no neighbour waits on it.
+constexpr int FLAG_DONE = 0x7fffffff;
+
+__device__ __forceinline__ void raise_flag(int* f, int v) {
+    __builtin_amdgcn_s_waitcnt(0);       // this wave's coherent stores have completed
+    __syncthreads();                     // ... and every wave's
+    if (threadIdx.x == 0) st_c(f, v);
+}
+
+__device__ __forceinline__ void wait_flags(const int* f, int n, int target, int* err) {
+    if ((int)threadIdx.x < n) {
+        unsigned spins = 0;
+        while (ld_c(&f[threadIdx.x]) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 26)) {
+                __hip_atomic_fetch_or(err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
     return v > bv || (v == bv && o < bo);
 }
@@ -450,7 +477,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ int s_hcnt_w[MAX_CW / 64];
     __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
-    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand;
+    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk, s_cntadd;
+    __shared__ unsigned long long s_wkey[R > 0 ? PEEL_THREADS / REG_NRG : 1];   // split: winner key per column
+    __shared__ int s_wt[R > 0 ? PEEL_THREADS / REG_NRG : 1];                   //   its template
+    __shared__ float s_wb[R > 0 ? PEEL_THREADS / REG_NRG : 1];                 //   and its B value
     int* const s_rank = reinterpret_cast<int*>(s_val);
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
 
@@ -528,7 +558,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         }
     };
 
-    int it_found = 0;
+    if (tid == 0) s_kchk = 0;
+    // per iteration: owners done with phase B (high word) and spikes accepted (low word)
+    unsigned long long* const itc = reinterpret_cast<unsigned long long*>(p.iter_cnt);
+    int* const flagB = p.iter_cnt + 2 * p.max_iter;
+    int* const flagA = flagB + p.NB;
     // phase clock: thread 0 accumulates in LDS (no registers held across the loop)
     const bool timing = (tid == 0);
     __shared__ uint64_t s_ph[NPHASE + 2];      // phases, then the previous mark and the pass count
@@ -547,7 +581,21 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         if (it > 0) {
             // one coherent round trip: spike total of the last iteration, the
             // neighbours' counts, and their first PRE spikes
-            if (tid == 0) { s_tot = ld_c(&p.iter_cnt[it - 1]); s_err = ld_c(p.err); }
+            if (tid == 0) {
+                // Stop once some iteration is known to have accepted no spike anywhere:
+                // all owners are done with it (done_cnt) and its total is zero.  Every
+                // later iteration then changes nothing, so when a workgroup notices
+                // does not change any result.
+                int stop = 0;
+                while (s_kchk < it) {
+                    const unsigned long long v = ld_c(&itc[s_kchk]);
+                    if ((int)(v >> 32) < p.NB) break;
+                    if ((unsigned)v == 0u) { stop = 1; break; }
+                    ++s_kchk;
+                }
+                s_tot = !stop;
+                s_err = ld_c(p.err);
+            }
             if (tid >= 64 && tid < 64 + nh) s_hcnt[tid - 64] = min(ld_c(&p.slot_cnt[hlo + tid - 64]), p.slot_cap);
             if (tid >= 128 && tid < 128 + nh * PRE) {
                 const int hh = (tid - 128) / PRE, k = (tid - 128) % PRE;
@@ -555,7 +603,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
             __syncthreads();
             if (s_tot == 0 || (s_err & ERR_BARRIER)) break;
-            it_found = it;
             if (it == p.max_iter) break;
             if (tid == 0) {
                 int n = 0;
@@ -849,7 +896,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
         }
         mark(1);
-        grid_sync(p.bar, p.G, g, bar_gen, p.err);
+        // phase A done: the owners of this and the neighbouring blocks may read this
+        // workgroup's column maxima (or partials); an owner waits for every workgroup
+        // of blocks hlo..hhi (contiguous ids) before pooling over +-M columns
+        raise_flag(&flagA[g], it + 1);
+        if (owner) wait_flags(flagA + hlo * S, (hhi - hlo + 1) * S, it + 1, p.err);
         mark(2);
 
         // ---------------- phase B (the column block's owner) ----------------
@@ -868,9 +919,34 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             int* const s_wi = reinterpret_cast<int*>(s_win);
             for (int i = tid; i < nwin; i += PEEL_THREADS) s_wi[i] = 0;
             __syncthreads();
+            if (R > 0 && tid < CW) s_wkey[tid] = 0ull;
+            __syncthreads();
             for (int i = tid; i < nwin * S; i += PEEL_THREADS) {
                 const int s2 = i / nwin, iw = i - s2 * nwin, ww = c0 - M + iw;
                 if (ww >= 0 && ww < W) atomicMax(&s_wi[iw], __float_as_int(ld_c(&p.Pcf[(int64_t)s2 * W + ww])));
+            }
+            if (R > 0) {
+                // the winning partial of every own column, from the same round trip:
+                // key = (score bits, ~file index), so the largest score wins and the
+                // smaller file index breaks ties (scores are >= 0)
+                const int s2 = tid / CW, cl = tid - s2 * CW;
+                float v = 0.f, b = 0.f;
+                int t = 0;
+                const bool in = s2 < S && c0 + cl < c1;
+                if (in) {
+                    const int64_t k = (int64_t)s2 * W + c0 + cl;
+                    v = ld_c(&p.Pcf[k]);
+                    t = ld_c(&p.Pim[k]);
+                    b = ld_c(&p.Pbv[k]);
+                }
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(v) << 32) | (unsigned)(0x7fffffff - s_or[in ? t : 0]);
+                if (in) atomicMax(&s_wkey[cl], key);
+                __syncthreads();
+                if (in && s_wkey[cl] == key) {
+                    s_wt[cl] = t;
+                    s_wb[cl] = b;
+                }
             }
         }
         if (!owner) {
@@ -878,11 +954,24 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         } else {
         if (tid == 0) s_ncand = 0;
         __syncthreads();
+        float* const s_part = reinterpret_cast<float*>(s_whit);     // free in phase B (R > 0)
+        if (R > 0) {
+            // pooled maximum over each own column's 2M+1 window: the nrg row groups take
+            // interleaved slices, the column's thread then combines the nrg slices
+            float m = -INFINITY;
+            for (int i = rg; i <= 2 * M; i += nrg) m = fmaxf(m, s_win[col + i]);
+            s_part[tid] = m;
+            __syncthreads();
+        }
         if (tid < CW) {
             int f = 0;
             if (c0 + tid < c1) {
                 float cm = -INFINITY;
-                for (int i = 0; i <= 2 * M; ++i) cm = fmaxf(cm, s_win[tid + i]);
+                if (R > 0) {
+                    for (int q = 0; q < nrg; ++q) cm = fmaxf(cm, s_part[q * CW + tid]);
+                } else {
+                    for (int i = 0; i <= 2 * M; ++i) cm = fmaxf(cm, s_win[tid + i]);
+                }
                 const float cfw = s_win[tid + M];
                 f = p.lss ? (cm >= p.Th2 && (cm - cfw) < 1e-4f) : (cm > p.Th2 && fabsf(cm - cfw) < 1e-9f);
                 s_cm[tid] = cm;
@@ -960,26 +1049,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 // butterfly argmax (largest score, smaller file index on ties)
                 int n = 0;
                 for (int k = 0; k < nwv; ++k) n += s_hcnt_w[k];
-                for (int q = wv; q < n; q += PEEL_THREADS / 64) {
-                    const int ww = s_sp[q].w;
-                    float v = -1.f, b = 0.f;
-                    int t = 0, o = 0x7fffffff;
-                    if (lane < S) {
-                        const int64_t k = (int64_t)lane * W + ww;
-                        v = ld_c(&p.Pcf[k]);
-                        t = ld_c(&p.Pim[k]);
-                        b = ld_c(&p.Pbv[k]);
-                        o = p.orig[t];
-                    }
-                    for (int m = 32; m > 0; m >>= 1) {
-                        const float v2 = __shfl_xor(v, m), b2 = __shfl_xor(b, m);
-                        const int t2 = __shfl_xor(t, m), o2 = __shfl_xor(o, m);
-                        if (v2 > v || (v2 == v && o2 < o)) { v = v2; b = b2; t = t2; o = o2; }
-                    }
-                    if (lane == 0) {
-                        s_sp[q].t = t;
-                        s_sp[q].amp = b / p.nm[t];
-                    }
+                for (int q = tid; q < n; q += PEEL_THREADS) {
+                    const int cl = s_sp[q].w - c0;
+                    const int t = s_wt[cl];
+                    s_sp[q].t = t;
+                    s_sp[q].amp = s_wb[cl] / p.nm[t];
                 }
                 __syncthreads();
                 for (int i = tid; i < min(n, p.slot_cap); i += PEEL_THREADS)
@@ -993,7 +1067,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 if (s_listn + n > MAX_LIST) atomicOr(p.err, ERR_LIST);
                 s_n = min(n, p.slot_cap);
                 s_listn = min(s_listn + n, MAX_LIST);
-                if (n > 0) __hip_atomic_fetch_add(&p.iter_cnt[it], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_cntadd = n;
             }
         } else if (tid == 0) {
             int n = min(s_ncand, MAX_STAGE), ln = s_listn;
@@ -1015,7 +1089,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
             s_n = n;
             s_listn = min(ln, MAX_LIST);
-            if (n > 0) __hip_atomic_fetch_add(&p.iter_cnt[it], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_cntadd = n;
         }
         __syncthreads();
         {
@@ -1025,17 +1099,41 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         }
         }   // owner
         mark(3);
-        grid_sync(p.bar, p.G, g, bar_gen, p.err);
+        if (owner) {
+            // one atomic counts the owner done and adds its spikes, so a reader that
+            // sees every owner done also sees the complete total
+            if (tid == 0)
+                __hip_atomic_fetch_add(&itc[it], (1ull << 32) | (unsigned)s_cntadd, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            raise_flag(&flagB[cb], it + 1);
+        }
+        // the next exchange reads the slots of blocks hlo..hhi; waiting for their owners
+        // also keeps this workgroup's next phase A from overwriting column maxima that
+        // a neighbouring owner is still pooling
+        wait_flags(flagB + hlo, hhi - hlo + 1, it + 1, p.err);
         mark(4);
     }
     mark(0);
+    raise_flag(&flagA[g], FLAG_DONE);
+    if (owner && tid == 0) st_c(&flagB[cb], FLAG_DONE);
 
     // ---------------- events and features ----------------
-    const int nmine = owner ? min(s_listn, MAX_LIST) : 0;
-    for (int i = tid; i < nmine; i += PEEL_THREADS) st_spike(&p.list[(int64_t)cb * p.cap_g + i], s_list[i]);
-    if (tid == 0 && owner) st_c(&p.list_cnt[cb], nmine);
-    grid_sync(p.bar, p.G, g, bar_gen, p.err);
+    // The owners publish their lists.  After the barrier every workgroup of a
+    // column block holds the block's list and takes a share of its spikes'
+    // features (spike i goes to subset i % S), so a block dense in spikes is
+    // spread over its S workgroups; the owner writes the events.
     if (owner) {
+        const int nl = min(s_listn, MAX_LIST);
+        for (int i = tid; i < nl; i += PEEL_THREADS) st_spike(&p.list[(int64_t)cb * p.cap_g + i], s_list[i]);
+        if (tid == 0) st_c(&p.list_cnt[cb], nl);
+    }
+    grid_sync(p.bar, p.G, g, bar_gen, p.err);
+    if (!owner) {
+        if (tid == 0) s_listn = min(ld_c(&p.list_cnt[cb]), MAX_LIST);
+        __syncthreads();
+        for (int i = tid; i < s_listn; i += PEEL_THREADS) s_list[i] = ld_spike(&p.list[(int64_t)cb * p.cap_g + i]);
+    }
+    {   // spikes of the blocks before this one: the block's offset in the event list
         int part = 0;
         for (int hh = tid; hh < cb; hh += PEEL_THREADS) part += ld_c(&p.list_cnt[hh]);
         s_idx[tid] = part;
@@ -1047,6 +1145,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         if (tid == 0) s_off = s_idx[0];
         __syncthreads();
     }
+    const int nmine = min(s_listn, MAX_LIST);
     const int off = s_off;
     for (int i = tid; i < nmine; i += PEEL_THREADS) {
         const Spike si = s_list[i];
@@ -1057,7 +1156,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         }
         s_rank[i] = r;
         const int d = off + r;
-        if (d < p.ev_cap && !p.cent) {              // with relabelling the event is written below
+        if (owner && d < p.ev_cap && !p.cent) {     // with relabelling the event is written below
             lss_event e;
             e.sample = si.w - p.half + p.nt0min;     // the host adds the batch's first sample
             e.template_id = p.orig[si.t];
@@ -1071,7 +1170,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         const int K = p.K, nC = p.nC, C = p.C, L2 = 2 * M - 1;
         const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.NB - 1, (c1 - 1 + M - 1) / CW);
         float* stage = reinterpret_cast<float*>(s_idx) + wv * 64;    // s_idx is free after the offset sum
-        for (int i = wv; i < nmine; i += PEEL_THREADS / 64) {
+        for (int i = sb + S * wv; i < nmine; i += S * (PEEL_THREADS / 64)) {
             const int d = off + s_rank[i];
             if (d >= p.ev_cap) continue;                             // uniform per wave
             const Spike si = s_list[i];
@@ -1079,23 +1178,39 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             const int j = mine ? lane / K : 0, k = mine ? lane % K : 0;
             const int ch = p.iCC[j * C + p.iU[si.t]];
             float f = 0.f;
-            if (mine) {
+            if (mine)
                 f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
-                for (int hh = flo; hh <= fhi; ++hh) {
-                    const int cnt = hh == cb ? nmine : ld_c(&p.list_cnt[hh]);
-                    const Spike* lst = p.list + (int64_t)hh * p.cap_g;
-                    for (int q = 0; q < cnt; ++q) {
-                        const Spike sq = hh == cb ? s_list[q] : ld_spike(&lst[q]);
+            // the spikes within +-(M-1) columns, in list order: a wave reads 64 at a time
+            // and takes the near ones from the ballot, lowest index first
+            for (int hh = flo; hh <= fhi; ++hh) {
+                const int cnt = hh == cb ? nmine : ld_c(&p.list_cnt[hh]);
+                const Spike* lst = p.list + (int64_t)hh * p.cap_g;
+                for (int q0 = 0; q0 < cnt; q0 += 64) {
+                    const int q = q0 + lane;
+                    Spike sq{};
+                    bool near = false;
+                    if (q < cnt) {
+                        sq = hh == cb ? s_list[q] : ld_spike(&lst[q]);
                         const int dd = si.w - sq.w;
-                        if (dd < -(M - 1) || dd > M - 1) continue;
-                        float gs = 0.f;
-                        for (int k2 = 0; k2 < K; ++k2)
-                            gs += p.U[((int64_t)sq.t * K + k2) * C + ch] * p.Gt[(k2 * K + k) * L2 + dd + M - 1];
-                        f -= sq.amp * gs;
+                        near = dd >= -(M - 1) && dd <= M - 1;
+                    }
+                    unsigned long long mask = __ballot(near);
+                    while (mask) {
+                        const int b = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const int wq = __shfl(sq.w, b), tq = __shfl(sq.t, b);
+                        const float aq = __shfl(sq.amp, b);
+                        if (mine) {
+                            const int dd = si.w - wq;
+                            float gs = 0.f;
+                            for (int k2 = 0; k2 < K; ++k2)
+                                gs += p.U[((int64_t)tq * K + k2) * C + ch] * p.Gt[(k2 * K + k) * L2 + dd + M - 1];
+                            f -= aq * gs;
+                        }
                     }
                 }
-                p.feat[(int64_t)d * nC * K + lane] = f;
             }
+            if (mine) p.feat[(int64_t)d * nC * K + lane] = f;
             if (p.pos) {
                 // position (compute_spike_positions_kernel, OnlineSpikesV2.cu:190-245): the
                 // mass of nearest channel jj is the sum of its squared features; the
@@ -1103,22 +1218,26 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 stage[lane] = f;
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_s_waitcnt(0xc07f);               // lgkmcnt(0): the features are in LDS
+                // lane jj < nC loads nearest channel jj's mass and coordinates; the sums
+                // then run over jj in order on every lane (broadcasts), as a serial loop would
+                const int iu = p.iU[si.t];
+                float mj = 0.f, cxj = 0.f, cyj = 0.f;
+                if (lane < nC) {
+                    for (int kk = 0; kk < K; ++kk) mj += stage[lane * K + kk] * stage[lane * K + kk];
+                    const int c2 = p.iCC[lane * C + iu];
+                    cxj = p.xc[c2];
+                    cyj = p.yc[c2];
+                }
+                float total = 0.f;
+                for (int jj = 0; jj < nC; ++jj) total += __shfl(mj, jj);
                 float x = 0.f, y = 0.f;
+                for (int jj = 0; jj < nC; ++jj) {
+                    const float m = __shfl(mj, jj);
+                    const float wgt = total > 0.f ? m / total : 0.f;
+                    x += __shfl(cxj, jj) * wgt;
+                    y += __shfl(cyj, jj) * wgt;
+                }
                 if (lane == 0) {
-                    float mass[16], total = 0.f;
-                    for (int jj = 0; jj < nC; ++jj) {
-                        float m = 0.f;
-                        for (int kk = 0; kk < K; ++kk) m += stage[jj * K + kk] * stage[jj * K + kk];
-                        mass[jj] = m;
-                        total += m;
-                    }
-                    const int iu = p.iU[si.t];
-                    for (int jj = 0; jj < nC; ++jj) {
-                        const float wgt = total > 0.f ? mass[jj] / total : 0.f;
-                        const int c2 = p.iCC[jj * C + iu];
-                        x += p.xc[c2] * wgt;
-                        y += p.yc[c2] * wgt;
-                    }
                     p.pos[2 * (int64_t)d] = x;
                     p.pos[2 * (int64_t)d + 1] = y;
                 }
@@ -1126,8 +1245,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 if (p.cent) {
                     // nearest final cluster (closestCluster, OnlineSpikesV2.cu:1058-1072):
                     // Euclidean distance, the first index among equals
-                    x = __shfl(x, 0);
-                    y = __shfl(y, 0);
                     float bd = INFINITY;
                     int bc = 0x7fffffff;
                     for (int c2 = lane; c2 < p.n_cent; c2 += 64) {
@@ -1161,8 +1278,13 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         atomicAdd(&p.prof[2 * NPHASE + 1], 1ull);
     }
     if (cb == p.NB - 1 && owner && tid == 0) {
+        // iterations run: the first that accepted nothing, or the cap (the grid
+        // barrier of the epilogue has made every count final)
+        int n_iter = p.max_iter;
+        for (int k = 0; k < p.max_iter; ++k)
+            if ((unsigned)ld_c(&itc[k]) == 0u) { n_iter = k; break; }
         p.hdr->count = off + nmine;
-        p.hdr->n_iter = it_found;
+        p.hdr->n_iter = n_iter;
         p.hdr->err = ld_c(p.err);
     }
 }
