@@ -476,6 +476,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ float s_rd[MAX_CW];             //   and the correction (FP32 score - BF16 score)
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ int s_hcnt_w[MAX_CW / 64];
+    __shared__ int s_qlo[R == 0 ? MAX_STAGE : 1], s_qhi[R == 0 ? MAX_STAGE : 1];   // staged spikes' row bands
     __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
     __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk, s_cntadd;
     __shared__ unsigned long long s_wkey[R > 0 ? PEEL_THREADS / REG_NRG : 1];   // split: winner key per column
@@ -621,6 +622,16 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
             __syncthreads();
             nsp = s_nsp;
+            if (R == 0) {
+                // the global tile tests every (row, spike) pair: the spikes' nonzero
+                // ctc row bands go to LDS once
+                for (int q = tid; q < nsp; q += PEEL_THREADS) {
+                    const int st = s_sp[q].t;
+                    s_qlo[q] = p.tlo[st];
+                    s_qhi[q] = p.thi[st];
+                }
+                __syncthreads();
+            }
         }
         mark(0);
 
@@ -631,7 +642,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             int bi = 0, bo = 0x7fffffff;
             // A wave whose rows no spike touched keeps last pass's per-thread
             // maxima in LDS (its registers did not change): it skips the scan.
-            bool wdirty = it == 0 || R == 0;
+            bool wdirty = it == 0;
             // a wave whose 64-template tile the gate did not propose for this column
             // block scores zero everywhere (its stale rows are neither updated nor read)
             const bool woff = R > 0 && p.gate && !p.gate[cb * p.gate_mt + (rbase + rg * R) / GEMM_BM];
@@ -694,6 +705,26 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     __builtin_amdgcn_wave_barrier();
                 }
                 mark(6);
+            }
+            // Global tile: the staged spikes are in sample order, so the ones whose
+            // window covers this column form one range [qa, qb).  A column no new
+            // spike reaches keeps last pass's maxima (its rows did not change); the
+            // others apply only that range, in the same order.
+            int qa = 0, qb = 0;
+            if (R == 0) {
+                int lo = 0, hi = nsp;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_sp[mid].w < w - M) lo = mid + 1; else hi = mid;
+                }
+                qa = lo;
+                hi = nsp;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_sp[mid].w <= w + M) lo = mid + 1; else hi = mid;
+                }
+                qb = lo;
+                if (qb > qa) wdirty = true;
             }
             if (!wdirty) {
                 // nothing to rescan
@@ -763,23 +794,26 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     }
                     bi = rbase + vb + bslot;
                 } else {
+                    // each covering spike, in order, updates this thread's rows of its band
+                    // (every element sees its updates in spike order, as a row-outer loop
+                    // would apply them), then one pass takes the column maximum
+                    for (int q = qa; q < qb; ++q) {
+                        const Spike sp = s_sp[q];
+                        const int l = w - sp.w + M;                 // in [0, 2M] for q in [qa, qb)
+                        const int t0 = s_qlo[q], t1 = s_qhi[q];
+                        const float* cp = p.ctc + (int64_t)sp.t * p.ctc_rows * LP + l;
+                        for (int t = t0 + (rg - t0 % nrg + nrg) % nrg; t < t1; t += nrg) {
+                            if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) continue;
+                            float* bp = p.B + (int64_t)t * p.BS + w;
+                            *bp -= sp.amp * cp[(int64_t)t * LP];
+                        }
+                    }
                     for (int t = rg; t < T; t += nrg) {
                         if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) {
                             if (better(0.f, p.orig[t], best, bo)) { best = 0.f; bi = t; bo = p.orig[t]; bval = 0.f; }
                             continue;
                         }
-                        float* bp = p.B + (int64_t)t * p.BS + w;
-                        float b = *bp;
-                        bool changed = false;
-                        for (int q = 0; q < nsp; ++q) {
-                            const Spike sp = s_sp[q];
-                            const int l = w - sp.w + M;
-                            if (l >= 0 && l <= 2 * M && t >= p.tlo[sp.t] && t < p.thi[sp.t]) {
-                                b -= sp.amp * p.ctc[((int64_t)sp.t * p.ctc_rows + t) * LP + l];
-                                changed = true;
-                            }
-                        }
-                        if (changed) *bp = b;
+                        const float b = p.B[(int64_t)t * p.BS + w];
                         float cf = 0.f;
                         if (!edge) {
                             const float r = fmaxf(b, 0.f);
