@@ -186,12 +186,16 @@ struct lss_handle {
     unsigned long long* d_gate_stats = nullptr;
     int slot_cap = 0, cap_g = 0, max_groups = 0;
     int g_cap = 0, sync_words = 0;       // peel workgroups at most; words of the iteration/flag block
-    // template split of the peel: per-subset column maxima [max_split][Wmax]
+    // template split of the peel: per-subset column maxima [2][max_split][Wmax]
     int max_split = 1;
     int force_split = 0;                 // autotuning: the split being timed (0: none forced)
     std::map<int, int> tuned_split;      // batch width -> split chosen by lss_autotune
-    float *d_Pcf = nullptr, *d_Pbv = nullptr;
-    int* d_Pim = nullptr;
+    // Kilosort rules: the peel's tagged column maxima (score, B value and template
+    // arrays of [2][S][W] each, back to back, for the batch's S and W) and the
+    // epoch of their tags
+    unsigned long long* d_ptag = nullptr;
+    int64_t ptag_words = 0;
+    unsigned* d_epoch = nullptr;
     // debug copies
     float *d_dbgB = nullptr, *d_dbgCf = nullptr;
     int* d_dbgIm = nullptr;
@@ -247,7 +251,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_Pcf, d_Pbv, d_Pim,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
@@ -664,11 +668,13 @@ void load(lss_handle* h, const std::string& dir_in) {
         h->max_split = std::min<int>(16, (h->T + 127) / 128);
         if (h->cfg.peel_split > 0) h->max_split = std::min(h->max_split, (int)h->cfg.peel_split);
     }
-    if (h->max_split > 1) {
-        h->d_Pcf = dalloc<float>((size_t)h->max_split * Wmax);
-        h->d_Pim = dalloc<int>((size_t)h->max_split * Wmax);
-        h->d_Pbv = dalloc<float>((size_t)h->max_split * Wmax);
-    }
+    // per-subset column maxima (score, B value, template), one copy per iteration
+    // parity, as tagged words; all zero (a tag no epoch uses) to start with
+    h->ptag_words = 3 * 2 * (int64_t)std::max(1, h->max_split) * Wmax;
+    h->d_ptag = dalloc<unsigned long long>(h->ptag_words);
+    HIPCHK(hipMemset(h->d_ptag, 0, h->ptag_words * sizeof(unsigned long long)));
+    h->d_epoch = dalloc<unsigned>(1);
+    HIPCHK(hipMemset(h->d_epoch, 0, sizeof(unsigned)));
     h->d_list_cnt = dalloc<int>(h->max_groups);
     // per-iteration spike totals and done counters, then the peel's neighbour flags
     // (per workgroup, per column block); all cleared by the reset kernel each batch
@@ -870,7 +876,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.max_iter = h->max_iter;
     p.CW = CW; p.G = G;
     p.S = S; p.NB = NB;
-    p.Pcf = h->d_Pcf; p.Pim = h->d_Pim; p.Pbv = h->d_Pbv;
+    p.Pa = h->d_ptag; p.epoch = h->d_epoch;
     p.lss = h->sem == LSS_SEM_LSS;
     p.Cfmax = h->d_Cfmax; p.imax = h->d_imax;
     p.slots = h->d_slots; p.slot_cnt = h->d_slot_cnt; p.slot_cap = h->slot_cap;
@@ -917,7 +923,11 @@ int split_rows(int T, int S) {
 // Whether S subsets with R rows each form a valid co-resident split geometry.
 bool split_fits(const lss_handle* h, int W, int S, int R) {
     const int NB = (W + MIN_CW - 1) / MIN_CW;
+    // local acceptance reads (column, subset) pairs within 2M of a block: at most
+    // one per thread with 32 rows, three with fewer
+    const int pairs = S * (MIN_CW + 4 * h->M);
     return S >= 2 && S <= h->max_split && R > 0 && R <= 32 && S * REG_NRG * R >= h->T &&
+           pairs <= (R >= 32 ? 1 : 3) * PEEL_THREADS &&
            S * REG_NRG * R <= h->ctc_rows && NB <= h->max_groups && S * NB <= h->peel_cap[peel_cap_index(R)];
 }
 
@@ -1003,7 +1013,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     PeelBatch kb{};
     kb.n = 1; kb.g_start[0] = 0; kb.g_start[1] = bp.G; kb.p[0] = bp.pp;
     if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
-    launch_reset(h->d_bar, h->d_iter_cnt, h->sync_words, h->d_err, h->d_prof, s);
+    launch_reset(h->d_bar, h->d_iter_cnt, h->sync_words, h->d_err, h->d_prof, h->d_epoch, h->d_ptag, h->ptag_words, s);
     if (bp.has_pre) {
         PreBatch pre{};
         pre.n = 1;
@@ -1124,7 +1134,7 @@ std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_c
     h->dev = h->cfg.device;
     h->sem = h->cfg.semantics;
     h->max_iter = h->cfg.max_iter > 0 ? h->cfg.max_iter : (h->sem == LSS_SEM_LSS ? 50 : 100);
-    if (h->max_iter > 4096) throw LssError(LSS_ERR_ARG, "max_iter too large");
+    if (h->max_iter > MAX_ITER) throw LssError(LSS_ERR_ARG, "max_iter too large");
     h->min_rows = min_rows;
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
@@ -1565,6 +1575,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         L.kb.g_start[s2 + 1] = L.kb.g_start[s2] + bp.G;
         L.rb.bar[s2] = h->d_bar; L.rb.iter_cnt[s2] = h->d_iter_cnt; L.rb.n_iter[s2] = h->sync_words;
         L.rb.err[s2] = h->d_err; L.rb.prof[s2] = h->d_prof; L.rb.active[s2] = &L.d_xptr[s2];
+        L.rb.epoch[s2] = h->d_epoch; L.rb.ptag[s2] = h->d_ptag; L.rb.ptag_words[s2] = h->ptag_words;
     }
     L.drop_graph();
     hipStream_t st = L.stream;

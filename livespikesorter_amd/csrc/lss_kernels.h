@@ -35,6 +35,18 @@ constexpr int BAR_MAX_GROUPS = 32;
 constexpr int BAR_HIER_MIN = 96;
 constexpr int BAR_WORDS = BAR_LINE * (1 + 2 * BAR_MAX_GROUPS);
 
+// Tags of the column maxima the peel publishes under Kilosort rules: the
+// low 32 bits of each 64-bit word are (epoch << TAG_ITER_BITS) | (iteration +
+// 1), or | TAG_DONE for the final maxima of a workgroup that has left the loop
+// (valid for every later iteration).  The epoch advances once per batch; when it
+// wraps the reset kernel clears every word, so an old word never carries a
+// current tag.
+constexpr int TAG_ITER_BITS = 13;
+constexpr unsigned TAG_DONE = (1u << TAG_ITER_BITS) - 1;
+constexpr unsigned EPOCH_MASK = (1u << (32 - TAG_ITER_BITS)) - 1;
+constexpr int MAX_ITER = 4096;
+static_assert(MAX_ITER + 1 < (int)TAG_DONE, "iteration tags stay below the final tag");
+
 // peel error flags
 constexpr int ERR_BARRIER = 1;
 constexpr int ERR_LIST = 2;
@@ -58,10 +70,12 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
     int32_t pad2[4];
 };
 // Peel-kernel time per phase in 100 MHz realtime ticks, summed and maxed over
-// workgroups: top-of-iteration exchange, phase A scan, barrier 1, phase B,
-// barrier 2, epilogue, phase A subtraction; then the number of phase-A passes
-// that did work and the number of workgroups (summed).
-constexpr int NPHASE = 7;
+// workgroups: top-of-iteration exchange, phase A scan, publishing the column
+// maxima and raising the flag (LSS rules: and the wait for the neighbours'
+// maxima), acceptance, the wait for the neighbours' flags, epilogue, phase A
+// subtraction, the gather of the neighbours' maxima; then the number of
+// phase-A passes that did work and the number of workgroups (summed).
+constexpr int NPHASE = 8;
 constexpr int PROF_WORDS = 2 * NPHASE + 2;
 
 // word 3 of a raw buffer descriptor on gfx9-family parts (gfx950): 32-bit data format
@@ -81,9 +95,12 @@ struct PeelParams {
     int max_iter;
     int CW, G;
     int S, NB;                         // template subsets per column block, column blocks (G = S * NB)
-    float* Pcf; int* Pim; float* Pbv;  // per-subset column maxima [S][W] (S > 1)
+    // Kilosort rules: per-subset column maxima by iteration parity, 64-bit words
+    // (value << 32 | tag): scores [2][S][W], then B values, then templates (same shape)
+    unsigned long long* Pa;
+    const unsigned* epoch;             // this batch's epoch (advanced by the reset kernel)
     int lss;
-    float* Cfmax; int* imax;
+    float* Cfmax; int* imax;           // column maxima (LSS rules)
     Spike* slots; int* slot_cnt; int slot_cap;
     Spike* list; int* list_cnt; int cap_g;
     // iter_cnt [max_iter] spike totals, followed by the peel loop's neighbour
@@ -186,8 +203,11 @@ using PcconvBatch = JobBatch<PcconvJob>;
 // Blocks a GEMM job occupies: M-tiles x N-tiles padded to a multiple of 8 (XCD order).
 inline int gemm_blocks(int mtiles, int ntiles) { return mtiles * ((ntiles + 7) / 8 * 8); }
 
-// Clears the per-batch counters the peel kernel relies on.
-void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, hipStream_t s);
+// Clears the per-batch counters the peel kernel relies on and advances the
+// epoch of the published column maxima (ptag: all of their words, cleared when
+// the epoch wraps).
+void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, unsigned* epoch,
+                  unsigned long long* ptag, int64_t ptag_words, hipStream_t s);
 
 void launch_whiten(const WhitenBatch& b, hipStream_t s);
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s);
@@ -220,6 +240,9 @@ struct ResetBatch {
     int n_iter[MAX_JOBS];
     int* err[MAX_JOBS];
     unsigned long long* prof[MAX_JOBS];
+    unsigned* epoch[MAX_JOBS];
+    unsigned long long* ptag[MAX_JOBS];
+    int64_t ptag_words[MAX_JOBS];
     const int16_t* const* active[MAX_JOBS];   // nullable: a stream idle this launch keeps its counters
 };
 void launch_reset_batch(const ResetBatch& b, hipStream_t s);

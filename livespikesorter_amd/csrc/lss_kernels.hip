@@ -310,11 +310,27 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ U, int 
     }
 }
 
-__global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof) {
+// The next epoch of a stream's published column maxima; on wrap-around every
+// word is cleared (tag 0 matches no epoch's tags, which are never 0).
+__device__ void advance_epoch(unsigned* epoch, unsigned long long* ptag, int64_t words) {
+    __shared__ int wrap;
+    if (threadIdx.x == 0) {
+        const unsigned e = (*epoch + 1u) & EPOCH_MASK;
+        *epoch = e;
+        wrap = e == 0u;
+    }
+    __syncthreads();
+    if (wrap)
+        for (int64_t i = threadIdx.x; i < words; i += blockDim.x) ptag[i] = 0ull;
+}
+
+__global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof,
+                        unsigned* epoch, unsigned long long* ptag, int64_t ptag_words) {
     for (int i = threadIdx.x; i < n_iter; i += blockDim.x) iter_cnt[i] = 0;
     if ((int)threadIdx.x < PROF_WORDS) prof[threadIdx.x] = 0ull;
     if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) bar[threadIdx.x * BAR_LINE] = 0u;
     if (threadIdx.x == 0) *err = 0;
+    if (epoch) advance_epoch(epoch, ptag, ptag_words);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,17 +421,19 @@ __device__ __forceinline__ void raise_flag(int* f, int v) {
     if (threadIdx.x == 0) st_c(f, v);
 }
 
-__device__ __forceinline__ void wait_flags(const int* f, int n, int target, int* err) {
-    if ((int)threadIdx.x < n) {
-        unsigned spins = 0;
-        while (ld_c(&f[threadIdx.x]) < target) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1u << 26)) {
-                __hip_atomic_fetch_or(err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
+__device__ __forceinline__ void spin_flag(const int* f, int target, int* err) {
+    unsigned spins = 0;
+    while (ld_c(f) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 26)) {
+            __hip_atomic_fetch_or(err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
         }
     }
+}
+
+__device__ __forceinline__ void wait_flags(const int* f, int n, int target, int* err) {
+    if ((int)threadIdx.x < n) spin_flag(&f[threadIdx.x], target, err);
     __syncthreads();
 }
 
@@ -424,21 +442,28 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 }
 
 // ---------------------------------------------------------------------------
-// Persistent peel kernel: one workgroup owns CW consecutive columns of the
-// score matrix B (all templates); only the owner reads or writes them.  With
-// R > 0 the owned tile lives in registers for the whole batch (thread (col,
-// rg) holds rows rg, rg + nrg, ...); R == 0 keeps it in global memory (large
-// template counts).  Per iteration:
+// Persistent peel kernel: workgroup g owns CW consecutive columns of the score
+// matrix B and one subset of its templates (rows); only it reads or writes
+// them.  With R > 0 the owned tile lives in registers for the whole batch
+// (thread (col, rg) holds R consecutive rows); R == 0 keeps it in global
+// memory (large template counts).  Per iteration:
 //   A  apply the previous iteration's spikes that reach the owned columns
 //      (B[t][w] -= a * ctc[s][t][w - w_s + M], spikes in sample order), then
 //      Cf = relu(B)^2 / nm (zero in the first and last M columns) and the
-//      column max / first-index argmax over templates;        -- barrier --
-//   B  Cmax = max of Cfmax over [w-M, w+M]; accept w when Cmax > Th^2 and
-//      Cmax == Cfmax[w] (|diff| < 1e-9); amplitude B[t][w] / nm[t];
-//                                                              -- barrier --
-//   the next iteration starts with one coherent round trip that reads the
-//   spike total (stop when zero) and the neighbours' new spikes.
-// One owner per column and a fixed application order make the result
+//      column max / first-index argmax over the owned templates; publish them
+//      and raise the workgroup's flag.
+//   B  (Kilosort rules) after the flags of the workgroups within 2M columns:
+//      the column maxima over all templates there; Cmax = their max over
+//      [w-M, w+M]; accept w when Cmax > Th^2 and Cmax == Cf[w] (|diff| <
+//      1e-9), amplitude B[t][w] / nm[t].  Every workgroup does this for the
+//      columns within M of its own, so it has the next iteration's
+//      subtractions without a further exchange.
+//      (LSS rules, one subset) the block accepts its own columns from its
+//      neighbours' maxima and publishes the spikes; the neighbours read them
+//      after its second flag.
+// No grid-wide barrier inside the loop: the waits are on neighbours' flags,
+// and the stop test reads a per-iteration count that lags behind.  One owner
+// per (row, column) and a fixed application order make the result
 // deterministic; no data atomics are used.  Afterwards each workgroup sorts
 // its spikes by (column, iteration), writes the events, and computes the PC
 // features of the final residual.
@@ -457,8 +482,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     if (p.active && !*p.active) return;                         // stream idle: every workgroup leaves
     // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
     // immediate offset (no per-row address registers); the norm table comes first.
-    __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // per-template reciprocal norm (0 past T)
-    __shared__ int s_or[R > 0 ? REG_MAX_T : 1];      // per-template file index (INT_MAX past T)
+    constexpr int CWX = R > 0 ? PEEL_THREADS / REG_NRG : MAX_CW;   // columns a workgroup owns at most
+    constexpr int NWIN = CWX + 4 * MAX_M + 2;    // columns within 2M of the block
+    constexpr int NACC = CWX + 2 * MAX_M + 2;    // columns within M of the block
+    __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // own rows' reciprocal norms (0 past T)
+    __shared__ float s_nmf[R > 0 ? REG_MAX_T : 1];   // every row's norm (+inf past T)
+    __shared__ int s_or[R > 0 ? REG_MAX_T : 1];      // every row's file index (INT_MAX past T)
     __shared__ Spike s_sp[MAX_STAGE];          // staged / newly accepted spikes
     __shared__ Spike s_list[MAX_LIST];         // this workgroup's spikes, whole batch
     __shared__ Spike s_pre[MAX_NBR * PRE];
@@ -466,23 +495,26 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ int s_idx[PEEL_THREADS];
     __shared__ int s_org[PEEL_THREADS];
     __shared__ float s_bvv[PEEL_THREADS];
-    __shared__ float s_win[MAX_CW + 2 * MAX_M + 2];
-    __shared__ float s_cf[MAX_CW];             // own column maxima, their argmax and its score
-    __shared__ int s_im[MAX_CW];
-    __shared__ float s_bv[MAX_CW];
-    __shared__ float s_cm[MAX_CW];
-    __shared__ int s_flag[MAX_CW];
-    __shared__ int s_rt[MAX_CW];               // FP32 rescoring: template whose correction is cached
-    __shared__ float s_rd[MAX_CW];             //   and the correction (FP32 score - BF16 score)
+    // local acceptance: per column within 2M of the block, (score bits, ~file index)
+    // of its best template; the LSS-rule exchange keeps its pooling window here
+    __shared__ unsigned long long s_key[NWIN];
+    float* const s_win = reinterpret_cast<float*>(s_key);
+    __shared__ float s_cf[CWX];                // own column maxima, their argmax and its score
+    __shared__ int s_im[CWX];
+    __shared__ float s_bv[CWX];
+    __shared__ float s_cm[CWX];                // LSS pooled maxima; block maxima of local acceptance
+    __shared__ int s_flag[CWX];
+    __shared__ int s_rt[CWX];                  // FP32 rescoring: template whose correction is cached
+    __shared__ float s_rd[CWX];                //   and the correction (FP32 score - BF16 score)
     __shared__ int s_hcnt[MAX_NBR + 1];
-    __shared__ int s_hcnt_w[MAX_CW / 64];
+    __shared__ int s_hcnt_w[PEEL_THREADS / 64], s_hcnt_o[PEEL_THREADS / 64];
     __shared__ int s_qlo[R == 0 ? MAX_STAGE : 1], s_qhi[R == 0 ? MAX_STAGE : 1];   // staged spikes' row bands
     __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
-    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk, s_cntadd;
-    __shared__ unsigned long long s_wkey[R > 0 ? PEEL_THREADS / REG_NRG : 1];   // split: winner key per column
-    __shared__ int s_wt[R > 0 ? PEEL_THREADS / REG_NRG : 1];                   //   its template
-    __shared__ float s_wb[R > 0 ? PEEL_THREADS / REG_NRG : 1];                 //   and its B value
+    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk;
+    __shared__ int s_wt[NACC];                 // local acceptance: winning template per column
+    __shared__ float s_wb[NACC];               //   and its B value
     int* const s_rank = reinterpret_cast<int*>(s_val);
+    static_assert(2 * NWIN >= CWX + 2 * MAX_M + 2, "the LSS pooling window fits the key array");
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
 
     const int g = blockIdx.x - bt.g_start[sj], tid = threadIdx.x;
@@ -506,9 +538,16 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     const int w = c0 + col;
     const bool own = w < c1;
     const bool edge = (w < M) || (w >= W - M);
+    // Kilosort rules: every workgroup decides acceptance itself for the columns
+    // within M of its block, from the published maxima of the columns within 2M
+    // (blocks hlo2..hhi2).  LSS rules (one subset) exchange accepted spikes instead.
+    const bool local = !p.lss;
+    unsigned long long* const Pa = p.Pa;                   // scores
+    unsigned long long* const Pb = p.Pa + (int64_t)2 * S * W;   // B values
+    unsigned long long* const Pt = p.Pa + (int64_t)4 * S * W;   // templates
     unsigned bar_gen = 0;
     if (tid == 0) s_listn = 0;
-    for (int i = tid; i < MAX_CW; i += PEEL_THREADS) s_rt[i] = -1;
+    for (int i = tid; i < CWX; i += PEEL_THREADS) s_rt[i] = -1;
 
     // Buffer descriptors: one scalar base per table plus a per-thread offset, so
     // the R rows of a tile need no address registers (row strides go in SGPRs).
@@ -516,9 +555,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p.ctc), 0, p.ctc_bytes, BUF_DWORD3);
     float breg[R > 0 ? R : 1];
     if (R > 0) {
-        for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) {
-            s_nm[t] = p.rnm[rbase + t];
-            s_or[t] = p.orig[rbase + t];
+        for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) s_nm[t] = p.rnm[rbase + t];
+        for (int t = tid; t < min(p.ctc_rows, REG_MAX_T); t += PEEL_THREADS) {
+            s_nmf[t] = p.nm[t];
+            s_or[t] = p.orig[t];
         }
         // rows past the score matrix read as zero (buffer loads beyond the range)
         const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
@@ -531,35 +571,32 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         __syncthreads();
     }
 
-    // Publish own column cl's maximum: the column maximum itself (one subset) or
-    // this subset's partial (score, template, B value).
-    auto publish = [&](int cl) {
+    // Publish own column cl's maximum.  LSS rules: the column maximum itself.
+    // Kilosort rules: this subset's (score, B value, template) of the column, each
+    // a 64-bit word carrying the tag, into the buffer of the iteration's parity.
+    // A reader of iteration i polls the words until they carry i's tag (or the
+    // final tag).  The writer overwrites them only in iteration i + 2, after it
+    // has read the iteration i + 1 words of every workgroup within 2M columns
+    // (its readers), which they publish after reading iteration i's.
+    auto publish = [&](int cl, int buf, unsigned tag) {
         const int ww = c0 + cl;
-        if (S == 1) {
+        if (!local) {
             st_c(&p.Cfmax[ww], s_cf[cl]);
         } else {
-            const int64_t k = (int64_t)sb * W + ww;
-            st_c(&p.Pcf[k], s_cf[cl]);
-            st_c(&p.Pim[k], s_im[cl]);
-            st_c(&p.Pbv[k], s_bv[cl]);
+            const int64_t k = ((int64_t)buf * S + sb) * W + ww;
+            st_c(&Pa[k], ((unsigned long long)__float_as_uint(s_cf[cl]) << 32) | tag);
+            st_c(&Pb[k], ((unsigned long long)__float_as_uint(s_bv[cl]) << 32) | tag);
+            st_c(&Pt[k], ((unsigned long long)(unsigned)s_im[cl] << 32) | tag);
         }
     };
 
-    // The winning partial of column ww over the subsets: largest score, smaller
-    // file index on ties (the argmax over all templates).
-    auto subset_winner = [&](int ww, float& v, int& t, float& b) {
-        v = -1.f;
-        int bo2 = 0x7fffffff;
-        for (int s2 = 0; s2 < S; ++s2) {
-            const int64_t k = (int64_t)s2 * W + ww;
-            const float c = ld_c(&p.Pcf[k]);
-            const int tt = ld_c(&p.Pim[k]);
-            const int o = p.orig[tt];
-            if (c > v || (c == v && o < bo2)) { v = c; bo2 = o; t = tt; b = ld_c(&p.Pbv[k]); }
-        }
-    };
-
-    if (tid == 0) s_kchk = 0;
+    __shared__ unsigned s_ep;      // this batch's epoch, shifted into tag position
+    __shared__ int s_abort;        // a gather timed out
+    if (tid == 0) {
+        s_kchk = 0;
+        s_abort = 0;
+        s_ep = local ? ld_c(p.epoch) << TAG_ITER_BITS : 0u;
+    }
     // per iteration: owners done with phase B (high word) and spikes accepted (low word)
     unsigned long long* const itc = reinterpret_cast<unsigned long long*>(p.iter_cnt);
     int* const flagB = p.iter_cnt + 2 * p.max_iter;
@@ -569,6 +606,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ uint64_t s_ph[NPHASE + 2];      // phases, then the previous mark and the pass count
     if (tid < NPHASE + 2) s_ph[tid] = 0;
     __syncthreads();
+    const unsigned ep = s_ep;
     if (timing) s_ph[NPHASE] = rt_now();
     auto mark = [&](int k) {
         if (timing) {
@@ -577,50 +615,55 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             s_ph[NPHASE] = t;
         }
     };
+    // Stop once some iteration is known to have accepted no spike anywhere: every
+    // owner is done with it (high word of its packed count) and its total (low
+    // word) is zero.  Every later iteration then changes nothing, so when a
+    // workgroup notices does not change any result.  Run by one thread.
+    auto stop_test = [&](int it) {
+        int stop = 0;
+        while (s_kchk < it) {
+            const unsigned long long v = ld_c(&itc[s_kchk]);
+            if ((int)(v >> 32) < p.NB) break;
+            if ((unsigned)v == 0u) { stop = 1; break; }
+            ++s_kchk;
+        }
+        s_tot = !stop;
+        s_err = ld_c(p.err);
+    };
     for (int it = 0; it <= p.max_iter; ++it) {
         int nsp = 0;
+        if (local && it == p.max_iter) break;
         if (it > 0) {
-            // one coherent round trip: spike total of the last iteration, the
-            // neighbours' counts, and their first PRE spikes
-            if (tid == 0) {
-                // Stop once some iteration is known to have accepted no spike anywhere:
-                // all owners are done with it (done_cnt) and its total is zero.  Every
-                // later iteration then changes nothing, so when a workgroup notices
-                // does not change any result.
-                int stop = 0;
-                while (s_kchk < it) {
-                    const unsigned long long v = ld_c(&itc[s_kchk]);
-                    if ((int)(v >> 32) < p.NB) break;
-                    if ((unsigned)v == 0u) { stop = 1; break; }
-                    ++s_kchk;
+            if (!local) {
+                // one coherent round trip: the stop test, the neighbours' counts, and
+                // their first PRE spikes
+                if (tid == 0) stop_test(it);
+                if (tid >= 64 && tid < 64 + nh) s_hcnt[tid - 64] = min(ld_c(&p.slot_cnt[hlo + tid - 64]), p.slot_cap);
+                if (tid >= 128 && tid < 128 + nh * PRE) {
+                    const int hh = (tid - 128) / PRE, k = (tid - 128) % PRE;
+                    s_pre[hh * PRE + k] = ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + k]);
                 }
-                s_tot = !stop;
-                s_err = ld_c(p.err);
-            }
-            if (tid >= 64 && tid < 64 + nh) s_hcnt[tid - 64] = min(ld_c(&p.slot_cnt[hlo + tid - 64]), p.slot_cap);
-            if (tid >= 128 && tid < 128 + nh * PRE) {
-                const int hh = (tid - 128) / PRE, k = (tid - 128) % PRE;
-                s_pre[hh * PRE + k] = ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + k]);
-            }
-            __syncthreads();
-            if (s_tot == 0 || (s_err & ERR_BARRIER)) break;
-            if (it == p.max_iter) break;
-            if (tid == 0) {
-                int n = 0;
-                for (int hh = 0; hh < nh; ++hh) {
-                    const int cnt = s_hcnt[hh];
-                    for (int k = 0; k < cnt; ++k) {
-                        const Spike sp = k < PRE ? s_pre[hh * PRE + k]
-                                                 : ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + k]);
-                        if (sp.w + M >= c0 && sp.w - M < c1) {
-                            if (n < MAX_STAGE) s_sp[n++] = sp;
-                            else atomicOr(p.err, ERR_STAGE);
+                __syncthreads();
+                if (s_tot == 0 || (s_err & ERR_BARRIER)) break;
+                if (it == p.max_iter) break;
+                if (tid == 0) {
+                    int n = 0;
+                    for (int hh = 0; hh < nh; ++hh) {
+                        const int cnt = s_hcnt[hh];
+                        for (int k = 0; k < cnt; ++k) {
+                            const Spike sp = k < PRE ? s_pre[hh * PRE + k]
+                                                     : ld_spike(&p.slots[(int64_t)(hlo + hh) * p.slot_cap + k]);
+                            if (sp.w + M >= c0 && sp.w - M < c1) {
+                                if (n < MAX_STAGE) s_sp[n++] = sp;
+                                else atomicOr(p.err, ERR_STAGE);
+                            }
                         }
                     }
+                    s_nsp = n;
                 }
-                s_nsp = n;
+                __syncthreads();
             }
-            __syncthreads();
+            // local acceptance has left the spikes within M of this block in s_sp
             nsp = s_nsp;
             if (R == 0) {
                 // the global tile tests every (row, spike) pair: the spikes' nonzero
@@ -738,7 +781,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     int vb = rg * R;
                     asm volatile("" : "+v"(vb));
                     const float* rnp = s_nm + vb;
-                    const int* orp = s_or + vb;
+                    const int* orp = s_or + rbase + vb;
+                    const float* nmp = s_nmf + rbase + vb;
                     // Argmax in two steps.  a = r^2 * (1/nm) is within 3 * 2^-24 (relative)
                     // of Cf = r^2 / nm (Kilosort; LSS multiplies, so a is Cf exactly).  One
                     // branch-free pass keeps the largest a, its row, and the second largest;
@@ -766,12 +810,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     const float thr = p.lss ? amax : amax * (1.f - 0x1p-20f);
                     if (a2 < thr) {
                         const float r = fmaxf(bval, 0.f);
-                        best = p.lss ? amax : (r * r) / p.nm[rbase + vb + bslot];
+                        best = p.lss ? amax : (r * r) / nmp[bslot];
                         bo = orp[bslot];
                     } else {
                         best = -1.f;
                         bo = 0x7fffffff;
-                        const float* nmp = p.nm + rbase + vb;
                         // recompute the squares here instead of keeping 64 of them live
 #pragma unroll
                         for (int i = 0; i < R; ++i) asm volatile("" : "+v"(breg[i]));
@@ -847,7 +890,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     s_cf[tid] = bv;
                     s_im[tid] = bI;
                     s_bv[tid] = bb;
-                    if (!p.rescore) publish(tid);
+                    if (!local) publish(tid, 0, 0u);
                     if (it == 0 && p.dbg_cfmax && S == 1) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
                 }
             }
@@ -920,7 +963,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         s_bv[tid] = b;
                         s_cf[tid] = c32;
                     }
-                    publish(tid);
                 }
                 const uint64_t fb = __ballot(flip);
                 if (lane == 0 && wv == 0) {
@@ -930,16 +972,203 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             }
         }
         mark(1);
-        // phase A done: the owners of this and the neighbouring blocks may read this
-        // workgroup's column maxima (or partials); an owner waits for every workgroup
-        // of blocks hlo..hhi (contiguous ids) before pooling over +-M columns
-        raise_flag(&flagA[g], it + 1);
-        if (owner) wait_flags(flagA + hlo * S, (hhi - hlo + 1) * S, it + 1, p.err);
-        mark(2);
-
-        // ---------------- phase B (the column block's owner) ----------------
-        const int nwin = (c1 - c0) + 2 * M;
-        if (S == 1) {
+        if (local) {
+            // ---------------- local acceptance (Kilosort rules) ----------------
+            if (tid < CW && c0 + tid < c1) publish(tid, it & 1, ep | (unsigned)(it + 1));
+            mark(2);
+            const int x0 = c0 - 2 * M;                 // window column 0
+            const int nw = (c1 - c0) + 4 * M;          // window: columns [c0 - 2M, c1 + 2M)
+            const int na = (c1 - c0) + 2 * M;          // columns [c0 - M, c1 + M): window M .. M + na - 1
+            const int64_t pbuf = (int64_t)(it & 1) * S * W;
+            // (window column, subset) pairs, at most MAXP per thread (the host keeps
+            // S * (CW + 4M) within MAXP * PEEL_THREADS)
+            constexpr int MAXP = (R == 0 || R >= 32) ? 1 : 3;
+            auto score = [&](int j) { return __uint_as_float((unsigned)(s_key[j] >> 32)); };
+            if (S > 1)
+                for (int i = tid; i < nw; i += PEEL_THREADS) s_key[i] = 0ull;
+            // the stop test, on one thread, while the neighbours' words are on their way
+            if (tid == PEEL_THREADS - 64) stop_test(it);
+            float pv[MAXP], pbv[MAXP];
+            int pt[MAXP];
+            bool pin[MAXP], pacc[MAXP];
+            int64_t pkx[MAXP];
+            unsigned pend = 0;
+            // (the opaque thread index keeps per-thread addresses out of the peel loop)
+            int pid = tid;
+            asm volatile("" : "+v"(pid));
+#pragma unroll
+            for (int q = 0; q < MAXP; ++q) {
+                const int i = pid + q * PEEL_THREADS;
+                const int s2 = i / nw, j = i - s2 * nw, ww = x0 + j;
+                pin[q] = s2 < S && ww >= 0 && ww < W;
+                pacc[q] = pin[q] && j >= M && j < M + na;
+                pkx[q] = pbuf + (int64_t)s2 * W + ww;
+                pv[q] = 0.f; pt[q] = 0; pbv[q] = 0.f;
+                if (pin[q]) pend |= 1u << q;
+            }
+            {
+                // poll this thread's words until each carries this iteration's tag (or
+                // the final tag); every load of a round is in flight at once
+                const unsigned tg = ep | (unsigned)(it + 1), tdone = ep | TAG_DONE;
+                auto ok = [&](unsigned long long x) { return (unsigned)x == tg || (unsigned)x == tdone; };
+                unsigned spins = 0;
+                while (pend) {
+                    unsigned long long wa[MAXP], wb[MAXP], wt[MAXP];
+#pragma unroll
+                    for (int q = 0; q < MAXP; ++q) {
+                        wa[q] = wb[q] = wt[q] = 0ull;
+                        if ((pend >> q) & 1u) {
+                            wa[q] = ld_c(&Pa[pkx[q]]);
+                            if (pacc[q]) {
+                                wb[q] = ld_c(&Pb[pkx[q]]);
+                                wt[q] = ld_c(&Pt[pkx[q]]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < MAXP; ++q) {
+                        if (((pend >> q) & 1u) && ok(wa[q]) && (!pacc[q] || (ok(wb[q]) && ok(wt[q])))) {
+                            pv[q] = __uint_as_float((unsigned)(wa[q] >> 32));
+                            pbv[q] = __uint_as_float((unsigned)(wb[q] >> 32));
+                            pt[q] = (int)(wt[q] >> 32);
+                            pend &= ~(1u << q);
+                        }
+                    }
+                    if (pend) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > (1u << 26)) {
+                            __hip_atomic_fetch_or(p.err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            s_abort = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            mark(4);
+            if (s_tot == 0 || (s_err & ERR_BARRIER) || s_abort) break;
+            // each column's best template over the subsets: the largest score, then the
+            // smaller file index (scores are >= 0, so the key orders as an integer)
+            unsigned long long pk[MAXP];
+#pragma unroll
+            for (int q = 0; q < MAXP; ++q) {
+                const int i = pid + q * PEEL_THREADS;
+                const int s2 = i / nw, j = i - s2 * nw;
+                const unsigned lo = (R > 0 && S > 1 && pin[q]) ? 0x7fffffffu - (unsigned)s_or[pt[q]] : 0u;
+                pk[q] = ((unsigned long long)__float_as_uint(pv[q]) << 32) | lo;
+                if (S == 1) {
+                    if (s2 == 0) {
+                        s_key[j] = pk[q];                  // every window column, 0 outside the batch
+                        if (j >= M && j < M + na) { s_wt[j - M] = pt[q]; s_wb[j - M] = pbv[q]; }
+                    }
+                } else if (pin[q]) {
+                    atomicMax(&s_key[j], pk[q]);
+                }
+            }
+            if (tid == 0 && nw * S > MAXP * PEEL_THREADS) atomicOr(p.err, ERR_STAGE);
+            __syncthreads();
+            mark(7);
+            float* const s_bm = s_cm;                  // maxima of 8-column groups of the window
+            if (S > 1) {
+#pragma unroll
+                for (int q = 0; q < MAXP; ++q) {
+                    const int i = pid + q * PEEL_THREADS;
+                    const int s2 = i / nw, j = i - s2 * nw;
+                    if (pin[q] && j >= M && j < M + na && s_key[j] == pk[q]) {
+                        s_wt[j - M] = pt[q];
+                        s_wb[j - M] = pbv[q];
+                    }
+                }
+            }
+            if (tid < (nw + 7) / 8) {
+                float m = 0.f;
+                for (int e = 8 * tid; e < min(8 * tid + 8, nw); ++e) {
+                    const float v = score(e);
+                    m = v > m ? v : m;
+                }
+                s_bm[tid] = m;
+            }
+            const int ln0 = s_listn;
+            __syncthreads();
+            // accept column ww when Cmax = max of the scores over [ww - M, ww + M]
+            // exceeds Th^2 and equals the column's own score (scores are finite and
+            // >= 0: selects rather than fmaxf).  The opaque thread index keeps the
+            // per-thread LDS addresses from being hoisted out of the peel loop.
+            int ct = tid;
+            asm volatile("" : "+v"(ct));
+            const int lane = tid & 63, wv = tid >> 6;
+            bool f = false, mine = false;
+            if (ct < na) {
+                const int j = M + ct, ww = c0 - M + ct;
+                const float cfw = score(j);
+                if (ww >= 0 && ww < W && cfw > p.Th2) {
+                    // window [lo, hi] = the columns before the first whole 8-column
+                    // group (< 8), the whole groups (< 16, since 2M + 1 < 128), the
+                    // columns after the last (< 8).  Fixed trip counts keep every
+                    // read in flight at once; a slot outside the window reads the
+                    // column itself, which is in it.
+                    const int lo = j - M, hi = j + M;
+                    const int blo = (lo + 7) >> 3, bhi = (hi + 1) >> 3;
+                    const int hend = min(8 * blo, hi + 1), t0 = 8 * max(blo, bhi);
+                    float v[7 + 16 + 7];
+#pragma unroll
+                    for (int e = 0; e < 7; ++e) v[e] = score(lo + e < hend ? lo + e : j);
+#pragma unroll
+                    for (int b = 0; b < 16; ++b) v[7 + b] = blo + b < bhi ? s_bm[blo + b] : cfw;
+#pragma unroll
+                    for (int e = 0; e < 7; ++e) v[23 + e] = score(t0 + e <= hi ? t0 + e : j);
+                    float cm = cfw;
+#pragma unroll
+                    for (int e = 0; e < 30; ++e) cm = v[e] > cm ? v[e] : cm;
+                    f = cm > p.Th2 && fabsf(cm - cfw) < 1e-9f;
+                }
+                mine = f && ww >= c0 && ww < c1;
+            }
+            // the accepted columns in sample order: all of them are the next
+            // iteration's subtractions, those of this block go to its list
+            const uint64_t bf = __ballot(f), bm = __ballot(mine);
+            if (lane == 0) { s_hcnt_w[wv] = __popcll(bf); s_hcnt_o[wv] = __popcll(bm); }
+            __syncthreads();
+            if (f) {
+                int base = 0, baso = 0;
+                for (int k = 0; k < wv; ++k) { base += s_hcnt_w[k]; baso += s_hcnt_o[k]; }
+                const uint64_t below = (1ull << lane) - 1ull;
+                const int idx = base + __popcll(bf & below);
+                const int t = s_wt[tid];
+                const float nmt = R > 0 ? s_nmf[t] : p.nm[t];
+                const Spike sp{c0 - M + tid, t, s_wb[tid] / nmt, it};
+                if (idx < MAX_STAGE) s_sp[idx] = sp;
+                if (mine) {
+                    const int ln = ln0 + baso + __popcll(bm & below);
+                    if (ln < MAX_LIST) s_list[ln] = sp;
+                }
+            }
+            if (S > 1 && it == 0 && p.dbg_cfmax && owner && tid >= M && tid < M + (c1 - c0)) {
+                p.dbg_cfmax[c0 - M + tid] = score(M + tid);
+                p.dbg_imax[c0 - M + tid] = s_wt[tid];
+            }
+            if (tid == 0) {
+                int n = 0, no = 0;
+                for (int k = 0; k < (na + 63) / 64; ++k) { n += s_hcnt_w[k]; no += s_hcnt_o[k]; }
+                if (n > MAX_STAGE) atomicOr(p.err, ERR_STAGE);
+                if (ln0 + no > MAX_LIST) atomicOr(p.err, ERR_LIST);
+                s_nsp = min(n, MAX_STAGE);
+                s_listn = min(ln0 + no, MAX_LIST);
+                // one atomic counts the owner done and adds its block's spikes, so a
+                // reader that sees every owner done also sees the complete total
+                if (owner)
+                    __hip_atomic_fetch_add(&itc[it], (1ull << 32) | (unsigned)no, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            mark(3);
+        } else {
+            // ---------------- LSS rules (one subset): spike exchange ----------------
+            // phase A done: the neighbouring blocks may read this block's column maxima
+            raise_flag(&flagA[g], it + 1);
+            wait_flags(flagA + hlo, nh, it + 1, p.err);
+            mark(2);
+            const int nwin = (c1 - c0) + 2 * M;
             for (int i = tid; i < nwin; i += PEEL_THREADS) {
                 const int ww = c0 - M + i;
                 float v = -INFINITY;
@@ -947,79 +1176,33 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 else if (ww >= 0 && ww < W) v = ld_c(&p.Cfmax[ww]);
                 s_win[i] = v;
             }
-        } else if (owner) {
-            // column maxima = max over the subsets' partials; scores are >= 0, so their
-            // bit patterns order as integers (columns outside the batch stay at 0)
-            int* const s_wi = reinterpret_cast<int*>(s_win);
-            for (int i = tid; i < nwin; i += PEEL_THREADS) s_wi[i] = 0;
+            if (tid == 0) s_ncand = 0;
             __syncthreads();
-            if (R > 0 && tid < CW) s_wkey[tid] = 0ull;
-            __syncthreads();
-            for (int i = tid; i < nwin * S; i += PEEL_THREADS) {
-                const int s2 = i / nwin, iw = i - s2 * nwin, ww = c0 - M + iw;
-                if (ww >= 0 && ww < W) atomicMax(&s_wi[iw], __float_as_int(ld_c(&p.Pcf[(int64_t)s2 * W + ww])));
-            }
+            float* const s_part = reinterpret_cast<float*>(s_whit);     // free here (R > 0)
             if (R > 0) {
-                // the winning partial of every own column, from the same round trip:
-                // key = (score bits, ~file index), so the largest score wins and the
-                // smaller file index breaks ties (scores are >= 0)
-                const int s2 = tid / CW, cl = tid - s2 * CW;
-                float v = 0.f, b = 0.f;
-                int t = 0;
-                const bool in = s2 < S && c0 + cl < c1;
-                if (in) {
-                    const int64_t k = (int64_t)s2 * W + c0 + cl;
-                    v = ld_c(&p.Pcf[k]);
-                    t = ld_c(&p.Pim[k]);
-                    b = ld_c(&p.Pbv[k]);
-                }
-                const unsigned long long key =
-                    ((unsigned long long)__float_as_uint(v) << 32) | (unsigned)(0x7fffffff - s_or[in ? t : 0]);
-                if (in) atomicMax(&s_wkey[cl], key);
+                // pooled maximum over each own column's 2M+1 window: the nrg row groups take
+                // interleaved slices, the column's thread then combines the nrg slices
+                float m = -INFINITY;
+                for (int i = rg; i <= 2 * M; i += nrg) m = fmaxf(m, s_win[col + i]);
+                s_part[tid] = m;
                 __syncthreads();
-                if (in && s_wkey[cl] == key) {
-                    s_wt[cl] = t;
-                    s_wb[cl] = b;
-                }
             }
-        }
-        if (!owner) {
-            // the owner accepts and publishes; nothing to do until the barrier
-        } else {
-        if (tid == 0) s_ncand = 0;
-        __syncthreads();
-        float* const s_part = reinterpret_cast<float*>(s_whit);     // free in phase B (R > 0)
-        if (R > 0) {
-            // pooled maximum over each own column's 2M+1 window: the nrg row groups take
-            // interleaved slices, the column's thread then combines the nrg slices
-            float m = -INFINITY;
-            for (int i = rg; i <= 2 * M; i += nrg) m = fmaxf(m, s_win[col + i]);
-            s_part[tid] = m;
+            if (tid < CW) {
+                int f = 0;
+                if (c0 + tid < c1) {
+                    float cm = -INFINITY;
+                    if (R > 0) {
+                        for (int q = 0; q < nrg; ++q) cm = fmaxf(cm, s_part[q * CW + tid]);
+                    } else {
+                        for (int i = 0; i <= 2 * M; ++i) cm = fmaxf(cm, s_win[tid + i]);
+                    }
+                    const float cfw = s_win[tid + M];
+                    f = cm >= p.Th2 && (cm - cfw) < 1e-4f;
+                    s_cm[tid] = cm;
+                }
+                s_flag[tid] = f;
+            }
             __syncthreads();
-        }
-        if (tid < CW) {
-            int f = 0;
-            if (c0 + tid < c1) {
-                float cm = -INFINITY;
-                if (R > 0) {
-                    for (int q = 0; q < nrg; ++q) cm = fmaxf(cm, s_part[q * CW + tid]);
-                } else {
-                    for (int i = 0; i <= 2 * M; ++i) cm = fmaxf(cm, s_win[tid + i]);
-                }
-                const float cfw = s_win[tid + M];
-                f = p.lss ? (cm >= p.Th2 && (cm - cfw) < 1e-4f) : (cm > p.Th2 && fabsf(cm - cfw) < 1e-9f);
-                s_cm[tid] = cm;
-                if (S > 1 && it == 0 && p.dbg_cfmax) {
-                    float v; int t; float b;
-                    subset_winner(c0 + tid, v, t, b);
-                    p.dbg_cfmax[c0 + tid] = v;
-                    p.dbg_imax[c0 + tid] = t;
-                }
-            }
-            s_flag[tid] = f;
-        }
-        __syncthreads();
-        if (p.lss) {
             // every template within 1e-4 of the pooled max (myGPUhelpers.cu:465); the
             // candidates are collected, then ordered by (column, template)
             if (own && s_flag[col]) {
@@ -1051,122 +1234,67 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 }
             }
             __syncthreads();
-        }
-        if (!p.lss) {
-            // one spike per accepted column: compact the flags with per-wave ballots
-            const int lane = tid & 63, wv = tid >> 6;
-            const int nwv = (CW + 63) / 64;
-            const bool f = tid < CW && s_flag[tid];
-            const uint64_t bal = __ballot(f);
-            if (tid < CW && lane == 0) s_hcnt_w[wv] = __popcll(bal);
-            __syncthreads();
-            if (tid < CW) {
-                int base = 0;
-                for (int k = 0; k < wv; ++k) base += s_hcnt_w[k];
-                if (f) {
-                    const int idx = base + __popcll(bal & ((1ull << lane) - 1ull));
-                    if (S == 1) {
-                        const int t = s_im[tid];
-                        const Spike sp{c0 + tid, t, s_bv[tid] / p.nm[t], it};
-                        if (idx < p.slot_cap) s_sp[idx] = sp;
-                        const int ln = s_listn + idx;
-                        if (ln < MAX_LIST) s_list[ln] = sp;
-                    } else {
-                        s_sp[idx] = Spike{c0 + tid, 0, 0.f, it};   // idx < CW <= MAX_STAGE
-                    }
-                }
-            }
-            __syncthreads();
-            if (S > 1) {
-                // template and amplitude of each accepted column from the subsets'
-                // partials: one wave per column, one lane per subset, then a
-                // butterfly argmax (largest score, smaller file index on ties)
-                int n = 0;
-                for (int k = 0; k < nwv; ++k) n += s_hcnt_w[k];
-                for (int q = tid; q < n; q += PEEL_THREADS) {
-                    const int cl = s_sp[q].w - c0;
-                    const int t = s_wt[cl];
-                    s_sp[q].t = t;
-                    s_sp[q].amp = s_wb[cl] / p.nm[t];
-                }
-                __syncthreads();
-                for (int i = tid; i < min(n, p.slot_cap); i += PEEL_THREADS)
-                    if (s_listn + i < MAX_LIST) s_list[s_listn + i] = s_sp[i];
-                __syncthreads();        // s_listn is advanced below
-            }
             if (tid == 0) {
-                int n = 0;
-                for (int k = 0; k < nwv; ++k) n += s_hcnt_w[k];
-                if (n > p.slot_cap) atomicOr(p.err, ERR_SLOT);
-                if (s_listn + n > MAX_LIST) atomicOr(p.err, ERR_LIST);
-                s_n = min(n, p.slot_cap);
-                s_listn = min(s_listn + n, MAX_LIST);
-                s_cntadd = n;
-            }
-        } else if (tid == 0) {
-            int n = min(s_ncand, MAX_STAGE), ln = s_listn;
-            if (s_ncand > MAX_STAGE) atomicOr(p.err, ERR_SLOT);
-            for (int i = 1; i < n; ++i) {          // insertion sort by (column, template)
-                const Spike x = s_sp[i];
-                int j = i - 1;
-                while (j >= 0 && (s_sp[j].w > x.w || (s_sp[j].w == x.w && s_sp[j].t > x.t))) {
-                    s_sp[j + 1] = s_sp[j];
-                    --j;
+                int n = min(s_ncand, MAX_STAGE), ln = s_listn;
+                if (s_ncand > MAX_STAGE) atomicOr(p.err, ERR_SLOT);
+                for (int i = 1; i < n; ++i) {          // insertion sort by (column, template)
+                    const Spike x = s_sp[i];
+                    int j = i - 1;
+                    while (j >= 0 && (s_sp[j].w > x.w || (s_sp[j].w == x.w && s_sp[j].t > x.t))) {
+                        s_sp[j + 1] = s_sp[j];
+                        --j;
+                    }
+                    s_sp[j + 1] = x;
                 }
-                s_sp[j + 1] = x;
+                if (n > p.slot_cap) { atomicOr(p.err, ERR_SLOT); n = p.slot_cap; }
+                for (int i = 0; i < n; ++i) {
+                    if (ln < MAX_LIST) s_list[ln] = s_sp[i];
+                    else atomicOr(p.err, ERR_LIST);
+                    ++ln;
+                }
+                s_n = n;
+                s_listn = min(ln, MAX_LIST);
             }
-            if (n > p.slot_cap) { atomicOr(p.err, ERR_SLOT); n = p.slot_cap; }
-            for (int i = 0; i < n; ++i) {
-                if (ln < MAX_LIST) s_list[ln] = s_sp[i];
-                else atomicOr(p.err, ERR_LIST);
-                ++ln;
-            }
-            s_n = n;
-            s_listn = min(ln, MAX_LIST);
-            s_cntadd = n;
-        }
-        __syncthreads();
-        {
+            __syncthreads();
             const int n = s_n;
             for (int i = tid; i < n; i += PEEL_THREADS) st_spike(&p.slots[(int64_t)cb * p.slot_cap + i], s_sp[i]);
             if (tid == 0) st_c(&p.slot_cnt[cb], n);
-        }
-        }   // owner
-        mark(3);
-        if (owner) {
-            // one atomic counts the owner done and adds its spikes, so a reader that
-            // sees every owner done also sees the complete total
+            mark(3);
+            // one atomic counts the block done and adds its spikes, so a reader that
+            // sees every block done also sees the complete total
             if (tid == 0)
-                __hip_atomic_fetch_add(&itc[it], (1ull << 32) | (unsigned)s_cntadd, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(&itc[it], (1ull << 32) | (unsigned)n, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             raise_flag(&flagB[cb], it + 1);
+            // the next exchange reads the slots of blocks hlo..hhi; waiting for them also
+            // keeps this workgroup's next phase A from overwriting column maxima that a
+            // neighbour is still pooling
+            wait_flags(flagB + hlo, nh, it + 1, p.err);
+            mark(4);
         }
-        // the next exchange reads the slots of blocks hlo..hhi; waiting for their owners
-        // also keeps this workgroup's next phase A from overwriting column maxima that
-        // a neighbouring owner is still pooling
-        wait_flags(flagB + hlo, hhi - hlo + 1, it + 1, p.err);
-        mark(4);
     }
     mark(0);
+    if (local && tid < CW && c0 + tid < c1) {
+        // final maxima for the neighbours still iterating: every later iteration's
+        // (no score changes after an iteration without spikes, or none follows)
+        publish(tid, 0, ep | TAG_DONE);
+        publish(tid, 1, ep | TAG_DONE);
+    }
     raise_flag(&flagA[g], FLAG_DONE);
     if (owner && tid == 0) st_c(&flagB[cb], FLAG_DONE);
 
     // ---------------- events and features ----------------
-    // The owners publish their lists.  After the barrier every workgroup of a
-    // column block holds the block's list and takes a share of its spikes'
-    // features (spike i goes to subset i % S), so a block dense in spikes is
-    // spread over its S workgroups; the owner writes the events.
+    // The owners publish their lists for the neighbours' features.  Every
+    // workgroup of a column block holds the block's list (all of them accepted
+    // the block's spikes alike) and takes a share of its spikes' features (spike
+    // i goes to subset i % S), so a block dense in spikes is spread over its S
+    // workgroups; the owner writes the events.
     if (owner) {
         const int nl = min(s_listn, MAX_LIST);
         for (int i = tid; i < nl; i += PEEL_THREADS) st_spike(&p.list[(int64_t)cb * p.cap_g + i], s_list[i]);
         if (tid == 0) st_c(&p.list_cnt[cb], nl);
     }
     grid_sync(p.bar, p.G, g, bar_gen, p.err);
-    if (!owner) {
-        if (tid == 0) s_listn = min(ld_c(&p.list_cnt[cb]), MAX_LIST);
-        __syncthreads();
-        for (int i = tid; i < s_listn; i += PEEL_THREADS) s_list[i] = ld_spike(&p.list[(int64_t)cb * p.cap_g + i]);
-    }
     {   // spikes of the blocks before this one: the block's offset in the event list
         int part = 0;
         for (int hh = tid; hh < cb; hh += PEEL_THREADS) part += ld_c(&p.list_cnt[hh]);
@@ -1332,13 +1460,15 @@ __global__ void k_reset_batch(ResetBatch b) {
     if ((int)threadIdx.x < PROF_WORDS) b.prof[j][threadIdx.x] = 0ull;
     if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) b.bar[j][threadIdx.x * BAR_LINE] = 0u;
     if (threadIdx.x == 0) *b.err[j] = 0;
+    if (b.epoch[j]) advance_epoch(b.epoch[j], b.ptag[j], b.ptag_words[j]);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err, prof);
+void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, unsigned* epoch,
+                  unsigned long long* ptag, int64_t ptag_words, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err, prof, epoch, ptag, ptag_words);
 }
 
 void launch_reset_batch(const ResetBatch& b, hipStream_t s) {
