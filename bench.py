@@ -590,7 +590,7 @@ def main() -> int:
         step(j)
     sp.stage_times(reset=True)
     lat, iters, spikes = [], [], []
-    phases = np.zeros(18)
+    phases = np.zeros(16)
     synchronize()
     t_start = time.perf_counter()
     for j in range(args.warmup, args.warmup + args.steps):
@@ -610,12 +610,12 @@ def main() -> int:
     avg = {k: v / max(nb, 1) for k, v in stage_ms.items()}
     log("device ms per chunk by stage: " + ", ".join(f"{k} {v:.4f}" for k, v in avg.items()))
     # the peel kernel's own clock (100 MHz ticks), workgroup 0, per chunk
-    phase_names = ("exchange", "scan", "publish", "accept", "wait", "epilogue", "subtract", "gather")
-    nwg = max(phases[17], 1.0)
+    phase_names = ("exchange", "scan", "publish", "accept", "wait", "epilogue", "subtract")
+    nwg = max(phases[15], 1.0)
     peel_phases = {k: {"mean_wg": round(phases[i] / nwg / 100.0, 3),
-                       "max_wg": round(phases[8 + i] / args.steps / 100.0, 3)}
+                       "max_wg": round(phases[7 + i] / args.steps / 100.0, 3)}
                    for i, k in enumerate(phase_names)}
-    peel_phases["scan_passes_per_wg"] = round(phases[16] / nwg, 2)
+    peel_phases["scan_passes_per_wg"] = round(phases[14] / nwg, 2)
     log(f"peel phases, us per chunk (mean over workgroups / largest): {peel_phases}")
 
     # host-resident input: chunk latency from int16 on the host to events on the host

@@ -923,11 +923,7 @@ int split_rows(int T, int S) {
 // Whether S subsets with R rows each form a valid co-resident split geometry.
 bool split_fits(const lss_handle* h, int W, int S, int R) {
     const int NB = (W + MIN_CW - 1) / MIN_CW;
-    // local acceptance reads (column, subset) pairs within 2M of a block: at most
-    // one per thread with 32 rows, three with fewer
-    const int pairs = S * (MIN_CW + 4 * h->M);
     return S >= 2 && S <= h->max_split && R > 0 && R <= 32 && S * REG_NRG * R >= h->T &&
-           pairs <= (R >= 32 ? 1 : 3) * PEEL_THREADS &&
            S * REG_NRG * R <= h->ctc_rows && NB <= h->max_groups && S * NB <= h->peel_cap[peel_cap_index(R)];
 }
 

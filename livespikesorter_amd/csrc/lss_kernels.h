@@ -71,11 +71,11 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
 };
 // Peel-kernel time per phase in 100 MHz realtime ticks, summed and maxed over
 // workgroups: top-of-iteration exchange, phase A scan, publishing the column
-// maxima and raising the flag (LSS rules: and the wait for the neighbours'
-// maxima), acceptance, the wait for the neighbours' flags, epilogue, phase A
-// subtraction, the gather of the neighbours' maxima; then the number of
-// phase-A passes that did work and the number of workgroups (summed).
-constexpr int NPHASE = 8;
+// maxima (LSS rules: and the wait for the neighbours'), acceptance, the wait
+// for and the reading of the neighbours' maxima (LSS rules: for their spikes),
+// epilogue, phase A subtraction; then the number of phase-A passes that did
+// work and the number of workgroups (summed).
+constexpr int NPHASE = 7;
 constexpr int PROF_WORDS = 2 * NPHASE + 2;
 
 // word 3 of a raw buffer descriptor on gfx9-family parts (gfx950): 32-bit data format

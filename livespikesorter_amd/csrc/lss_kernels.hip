@@ -495,26 +495,29 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ int s_idx[PEEL_THREADS];
     __shared__ int s_org[PEEL_THREADS];
     __shared__ float s_bvv[PEEL_THREADS];
-    // local acceptance: per column within 2M of the block, (score bits, ~file index)
-    // of its best template; the LSS-rule exchange keeps its pooling window here
-    __shared__ unsigned long long s_key[NWIN];
-    float* const s_win = reinterpret_cast<float*>(s_key);
+    // local acceptance: the best score of each column within 2M of the block and
+    // the maxima of its aligned 8-column groups; the LSS-rule exchange keeps its
+    // pooling window in the score array
+    __shared__ float s_sc[NWIN];
+    __shared__ float s_bm[NWIN / 8 + 1];
+    float* const s_win = s_sc;
     __shared__ float s_cf[CWX];                // own column maxima, their argmax and its score
     __shared__ int s_im[CWX];
     __shared__ float s_bv[CWX];
-    __shared__ float s_cm[CWX];                // LSS pooled maxima; block maxima of local acceptance
+    __shared__ float s_cm[CWX];                // LSS pooled maxima
     __shared__ int s_flag[CWX];
     __shared__ int s_rt[CWX];                  // FP32 rescoring: template whose correction is cached
     __shared__ float s_rd[CWX];                //   and the correction (FP32 score - BF16 score)
     __shared__ int s_hcnt[MAX_NBR + 1];
-    __shared__ int s_hcnt_w[PEEL_THREADS / 64], s_hcnt_o[PEEL_THREADS / 64];
+    __shared__ __align__(16) int s_hcnt_w[PEEL_THREADS / 64];     // per-wave counts (read as int4)
+    __shared__ __align__(16) int s_hcnt_o[PEEL_THREADS / 64];
     __shared__ int s_qlo[R == 0 ? MAX_STAGE : 1], s_qhi[R == 0 ? MAX_STAGE : 1];   // staged spikes' row bands
     __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
     __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk;
     __shared__ int s_wt[NACC];                 // local acceptance: winning template per column
     __shared__ float s_wb[NACC];               //   and its B value
     int* const s_rank = reinterpret_cast<int*>(s_val);
-    static_assert(2 * NWIN >= CWX + 2 * MAX_M + 2, "the LSS pooling window fits the key array");
+    static_assert(NWIN >= CWX + 2 * MAX_M + 2, "the LSS pooling window fits the score array");
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
 
     const int g = blockIdx.x - bt.g_start[sj], tid = threadIdx.x;
@@ -979,129 +982,98 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             const int x0 = c0 - 2 * M;                 // window column 0
             const int nw = (c1 - c0) + 4 * M;          // window: columns [c0 - 2M, c1 + 2M)
             const int na = (c1 - c0) + 2 * M;          // columns [c0 - M, c1 + M): window M .. M + na - 1
-            const int64_t pbuf = (int64_t)(it & 1) * S * W;
-            // (window column, subset) pairs, at most MAXP per thread (the host keeps
-            // S * (CW + 4M) within MAXP * PEEL_THREADS)
-            constexpr int MAXP = (R == 0 || R >= 32) ? 1 : 3;
-            auto score = [&](int j) { return __uint_as_float((unsigned)(s_key[j] >> 32)); };
-            if (S > 1)
-                for (int i = tid; i < nw; i += PEEL_THREADS) s_key[i] = 0ull;
-            // the stop test, on one thread, while the neighbours' words are on their way
+            // the stop test, on a thread of the last wave (no window column there)
             if (tid == PEEL_THREADS - 64) stop_test(it);
-            float pv[MAXP], pbv[MAXP];
-            int pt[MAXP];
-            bool pin[MAXP], pacc[MAXP];
-            int64_t pkx[MAXP];
-            unsigned pend = 0;
-            // (the opaque thread index keeps per-thread addresses out of the peel loop)
+            // Thread j < nw reads window column j of every subset: the words of this
+            // iteration (score; for columns within M also B value and template), and
+            // keeps the best (largest score, then the smaller file index: the argmax
+            // over all templates).  (The opaque index keeps per-thread addresses out
+            // of the peel loop.)
             int pid = tid;
             asm volatile("" : "+v"(pid));
-#pragma unroll
-            for (int q = 0; q < MAXP; ++q) {
-                const int i = pid + q * PEEL_THREADS;
-                const int s2 = i / nw, j = i - s2 * nw, ww = x0 + j;
-                pin[q] = s2 < S && ww >= 0 && ww < W;
-                pacc[q] = pin[q] && j >= M && j < M + na;
-                pkx[q] = pbuf + (int64_t)s2 * W + ww;
-                pv[q] = 0.f; pt[q] = 0; pbv[q] = 0.f;
-                if (pin[q]) pend |= 1u << q;
-            }
+            const int ww = x0 + pid;
+            const bool jin = pid < nw && ww >= 0 && ww < W;
+            const bool jacc = jin && pid >= M && pid < M + na;
+            float bv = -1.f, bb = 0.f;              // scores are >= 0: the first read one wins
+            int bt = 0, bo = 0x7fffffff;
             {
-                // poll this thread's words until each carries this iteration's tag (or
-                // the final tag); every load of a round is in flight at once
                 const unsigned tg = ep | (unsigned)(it + 1), tdone = ep | TAG_DONE;
                 auto ok = [&](unsigned long long x) { return (unsigned)x == tg || (unsigned)x == tdone; };
+                const int64_t base = (int64_t)(it & 1) * S * W + ww;
+                // subsets per round, every load of a round in flight at once (tiles of
+                // 64 rows and the global tile are never split)
+                constexpr int QS = (R == 0 || R == 64) ? 1 : 4;
                 unsigned spins = 0;
-                while (pend) {
-                    unsigned long long wa[MAXP], wb[MAXP], wt[MAXP];
+                for (int s0 = 0; s0 < S; s0 += QS) {
+                    unsigned pend = jin ? ((1u << min(QS, S - s0)) - 1u) : 0u;
+                    while (pend) {
+                        unsigned long long wa[QS], wb[QS], wt[QS];
 #pragma unroll
-                    for (int q = 0; q < MAXP; ++q) {
-                        wa[q] = wb[q] = wt[q] = 0ull;
-                        if ((pend >> q) & 1u) {
-                            wa[q] = ld_c(&Pa[pkx[q]]);
-                            if (pacc[q]) {
-                                wb[q] = ld_c(&Pb[pkx[q]]);
-                                wt[q] = ld_c(&Pt[pkx[q]]);
+                        for (int q = 0; q < QS; ++q) {
+                            wa[q] = wb[q] = wt[q] = 0ull;
+                            if ((pend >> q) & 1u) {
+                                const int64_t k = base + (int64_t)(s0 + q) * W;
+                                wa[q] = ld_c(&Pa[k]);
+                                if (jacc) {
+                                    wb[q] = ld_c(&Pb[k]);
+                                    wt[q] = ld_c(&Pt[k]);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < QS; ++q) {
+                            if (((pend >> q) & 1u) && ok(wa[q]) && (!jacc || (ok(wb[q]) && ok(wt[q])))) {
+                                pend &= ~(1u << q);
+                                const float v = __uint_as_float((unsigned)(wa[q] >> 32));
+                                const int t = (int)(wt[q] >> 32);
+                                const int o = (R > 0 && S > 1 && jacc) ? s_or[t] : 0;
+                                // subsets come in the order their words arrive; equal
+                                // (score, file index) keys only occur on the edge
+                                // columns, where every subset reports the same winner
+                                if (v > bv || (v == bv && o < bo)) {
+                                    bv = v; bo = o; bt = t;
+                                    bb = __uint_as_float((unsigned)(wb[q] >> 32));
+                                }
+                            }
+                        }
+                        if (pend) {
+                            __builtin_amdgcn_s_sleep(1);
+                            if (++spins > (1u << 26)) {
+                                __hip_atomic_fetch_or(p.err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                s_abort = 1;
+                                break;
                             }
                         }
                     }
-#pragma unroll
-                    for (int q = 0; q < MAXP; ++q) {
-                        if (((pend >> q) & 1u) && ok(wa[q]) && (!pacc[q] || (ok(wb[q]) && ok(wt[q])))) {
-                            pv[q] = __uint_as_float((unsigned)(wa[q] >> 32));
-                            pbv[q] = __uint_as_float((unsigned)(wb[q] >> 32));
-                            pt[q] = (int)(wt[q] >> 32);
-                            pend &= ~(1u << q);
-                        }
-                    }
-                    if (pend) {
-                        __builtin_amdgcn_s_sleep(1);
-                        if (++spins > (1u << 26)) {
-                            __hip_atomic_fetch_or(p.err, ERR_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            s_abort = 1;
-                            break;
-                        }
-                    }
                 }
             }
-            __syncthreads();
-            mark(4);
-            if (s_tot == 0 || (s_err & ERR_BARRIER) || s_abort) break;
-            // each column's best template over the subsets: the largest score, then the
-            // smaller file index (scores are >= 0, so the key orders as an integer)
-            unsigned long long pk[MAXP];
+            if (!jin) bv = 0.f;                     // outside the batch
+            const int lane = tid & 63, wv = tid >> 6;
+            if (wv * 64 < nw) {
+                // scores of the window (0 outside the batch), their maxima over aligned
+                // 8-column groups (lane butterflies), the winners within M
+                if (pid < nw) s_sc[pid] = bv;
+                float m = bv;
 #pragma unroll
-            for (int q = 0; q < MAXP; ++q) {
-                const int i = pid + q * PEEL_THREADS;
-                const int s2 = i / nw, j = i - s2 * nw;
-                const unsigned lo = (R > 0 && S > 1 && pin[q]) ? 0x7fffffffu - (unsigned)s_or[pt[q]] : 0u;
-                pk[q] = ((unsigned long long)__float_as_uint(pv[q]) << 32) | lo;
-                if (S == 1) {
-                    if (s2 == 0) {
-                        s_key[j] = pk[q];                  // every window column, 0 outside the batch
-                        if (j >= M && j < M + na) { s_wt[j - M] = pt[q]; s_wb[j - M] = pbv[q]; }
-                    }
-                } else if (pin[q]) {
-                    atomicMax(&s_key[j], pk[q]);
+                for (int o = 1; o < 8; o <<= 1) {
+                    const float x = __shfl_xor(m, o);
+                    m = x > m ? x : m;
                 }
-            }
-            if (tid == 0 && nw * S > MAXP * PEEL_THREADS) atomicOr(p.err, ERR_STAGE);
-            __syncthreads();
-            mark(7);
-            float* const s_bm = s_cm;                  // maxima of 8-column groups of the window
-            if (S > 1) {
-#pragma unroll
-                for (int q = 0; q < MAXP; ++q) {
-                    const int i = pid + q * PEEL_THREADS;
-                    const int s2 = i / nw, j = i - s2 * nw;
-                    if (pin[q] && j >= M && j < M + na && s_key[j] == pk[q]) {
-                        s_wt[j - M] = pt[q];
-                        s_wb[j - M] = pbv[q];
-                    }
-                }
-            }
-            if (tid < (nw + 7) / 8) {
-                float m = 0.f;
-                for (int e = 8 * tid; e < min(8 * tid + 8, nw); ++e) {
-                    const float v = score(e);
-                    m = v > m ? v : m;
-                }
-                s_bm[tid] = m;
+                if ((lane & 7) == 0) s_bm[pid >> 3] = m;
+                if (jacc) { s_wt[pid - M] = bt; s_wb[pid - M] = bb; }
             }
             const int ln0 = s_listn;
             __syncthreads();
+            mark(4);
+            if (s_tot == 0 || (s_err & ERR_BARRIER) || s_abort) break;
             // accept column ww when Cmax = max of the scores over [ww - M, ww + M]
             // exceeds Th^2 and equals the column's own score (scores are finite and
-            // >= 0: selects rather than fmaxf).  The opaque thread index keeps the
-            // per-thread LDS addresses from being hoisted out of the peel loop.
-            int ct = tid;
-            asm volatile("" : "+v"(ct));
-            const int lane = tid & 63, wv = tid >> 6;
+            // >= 0: selects rather than fmaxf)
             bool f = false, mine = false;
-            if (ct < na) {
-                const int j = M + ct, ww = c0 - M + ct;
-                const float cfw = score(j);
-                if (ww >= 0 && ww < W && cfw > p.Th2) {
+            if (pid < na) {
+                const int j = M + pid, wc = c0 - M + pid;
+                const float cfw = s_sc[j];
+                if (wc >= 0 && wc < W && cfw > p.Th2) {
                     // window [lo, hi] = the columns before the first whole 8-column
                     // group (< 8), the whole groups (< 16, since 2M + 1 < 128), the
                     // columns after the last (< 8).  Fixed trip counts keep every
@@ -1112,17 +1084,17 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     const int hend = min(8 * blo, hi + 1), t0 = 8 * max(blo, bhi);
                     float v[7 + 16 + 7];
 #pragma unroll
-                    for (int e = 0; e < 7; ++e) v[e] = score(lo + e < hend ? lo + e : j);
+                    for (int e = 0; e < 7; ++e) v[e] = s_sc[lo + e < hend ? lo + e : j];
 #pragma unroll
                     for (int b = 0; b < 16; ++b) v[7 + b] = blo + b < bhi ? s_bm[blo + b] : cfw;
 #pragma unroll
-                    for (int e = 0; e < 7; ++e) v[23 + e] = score(t0 + e <= hi ? t0 + e : j);
+                    for (int e = 0; e < 7; ++e) v[23 + e] = s_sc[t0 + e <= hi ? t0 + e : j];
                     float cm = cfw;
 #pragma unroll
                     for (int e = 0; e < 30; ++e) cm = v[e] > cm ? v[e] : cm;
                     f = cm > p.Th2 && fabsf(cm - cfw) < 1e-9f;
                 }
-                mine = f && ww >= c0 && ww < c1;
+                mine = f && wc >= c0 && wc < c1;
             }
             // the accepted columns in sample order: all of them are the next
             // iteration's subtractions, those of this block go to its list
@@ -1130,26 +1102,34 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             if (lane == 0) { s_hcnt_w[wv] = __popcll(bf); s_hcnt_o[wv] = __popcll(bm); }
             __syncthreads();
             if (f) {
-                int base = 0, baso = 0;
-                for (int k = 0; k < wv; ++k) { base += s_hcnt_w[k]; baso += s_hcnt_o[k]; }
+                int nb = 0, nbo = 0;
+#pragma unroll
+                for (int k4 = 0; k4 < PEEL_THREADS / 256; ++k4) {
+                    const int4 a = reinterpret_cast<const int4*>(s_hcnt_w)[k4];
+                    const int4 o = reinterpret_cast<const int4*>(s_hcnt_o)[k4];
+                    nb += (4 * k4 < wv ? a.x : 0) + (4 * k4 + 1 < wv ? a.y : 0) +
+                          (4 * k4 + 2 < wv ? a.z : 0) + (4 * k4 + 3 < wv ? a.w : 0);
+                    nbo += (4 * k4 < wv ? o.x : 0) + (4 * k4 + 1 < wv ? o.y : 0) +
+                           (4 * k4 + 2 < wv ? o.z : 0) + (4 * k4 + 3 < wv ? o.w : 0);
+                }
                 const uint64_t below = (1ull << lane) - 1ull;
-                const int idx = base + __popcll(bf & below);
-                const int t = s_wt[tid];
+                const int idx = nb + __popcll(bf & below);
+                const int t = s_wt[pid];
                 const float nmt = R > 0 ? s_nmf[t] : p.nm[t];
-                const Spike sp{c0 - M + tid, t, s_wb[tid] / nmt, it};
+                const Spike sp{c0 - M + pid, t, s_wb[pid] / nmt, it};
                 if (idx < MAX_STAGE) s_sp[idx] = sp;
                 if (mine) {
-                    const int ln = ln0 + baso + __popcll(bm & below);
+                    const int ln = ln0 + nbo + __popcll(bm & below);
                     if (ln < MAX_LIST) s_list[ln] = sp;
                 }
             }
-            if (S > 1 && it == 0 && p.dbg_cfmax && owner && tid >= M && tid < M + (c1 - c0)) {
-                p.dbg_cfmax[c0 - M + tid] = score(M + tid);
-                p.dbg_imax[c0 - M + tid] = s_wt[tid];
+            if (S > 1 && it == 0 && p.dbg_cfmax && owner && pid >= M && pid < M + (c1 - c0)) {
+                p.dbg_cfmax[c0 - M + pid] = s_sc[M + pid];
+                p.dbg_imax[c0 - M + pid] = s_wt[pid];
             }
             if (tid == 0) {
                 int n = 0, no = 0;
-                for (int k = 0; k < (na + 63) / 64; ++k) { n += s_hcnt_w[k]; no += s_hcnt_o[k]; }
+                for (int k = 0; k < PEEL_THREADS / 64; ++k) { n += s_hcnt_w[k]; no += s_hcnt_o[k]; }   // 0 past na
                 if (n > MAX_STAGE) atomicOr(p.err, ERR_STAGE);
                 if (ln0 + no > MAX_LIST) atomicOr(p.err, ERR_LIST);
                 s_nsp = min(n, MAX_STAGE);

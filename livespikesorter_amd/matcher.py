@@ -174,7 +174,7 @@ class OnlineSpikesV2:
         shapes = {STAGE_XW: ((info.C, W), np.float32), STAGE_BPC: ((info.K, info.C, W), np.float32),
                   STAGE_B: ((info.T, W), np.float32), STAGE_CFMAX: ((W,), np.float32),
                   STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32),
-                  STAGE_PHASES: ((18,), np.uint64), STAGE_PRE: ((info.C, W), np.float32),
+                  STAGE_PHASES: ((16,), np.uint64), STAGE_PRE: ((info.C, W), np.float32),
                   STAGE_PRE_MEAN: ((info.C,), np.float32), STAGE_PRE_MED: ((W,), np.float32)}
         shape, dt = shapes[stage]
         out = np.zeros(shape, dt)
