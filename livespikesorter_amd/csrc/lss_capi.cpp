@@ -218,6 +218,7 @@ struct lss_handle {
     int ev_cap = 0;
 
     std::vector<int> orig, cmap_default, cmap_cur;
+    bool cmap_id = false;                // cmap_cur maps every channel to itself
     std::vector<int> iU_int;         // main channel of each template, internal order
     std::vector<int> keep;           // template subset (file indices), empty = all
     // preprocessing in front of whitening (cfg.preprocess)
@@ -781,6 +782,8 @@ void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipMemcpy(h->d_cmap, want.data(), want.size() * sizeof(int), hipMemcpyHostToDevice));
         h->cmap_cur = want;
+        h->cmap_id = true;
+        for (int c = 0; c < h->C; ++c) h->cmap_id = h->cmap_id && want[c] == c;
     }
 }
 
@@ -810,6 +813,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     wj.kb_ptr = h->d_kbw_ptr; wj.kb_idx = h->d_kbw_idx; wj.Xw = h->d_Xw;
     wj.ld = ld; wj.CP = h->CP; wj.W = W; wj.C = h->C; wj.XS = h->XS;
     wj.mtiles = h->CP / GEMM_BM;
+    wj.cmap_id = h->cmap_id ? 1 : 0;
     wj.ntiles = (W + XOFF + GEMM_BN - 1) / GEMM_BN;          // cover the right zero pad
     bp.has_pre = h->pre != 0;
     if (bp.has_pre) {

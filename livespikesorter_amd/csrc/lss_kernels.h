@@ -149,6 +149,7 @@ struct WhitenJob {
     const int* cmap; const int* kb_ptr; const int* kb_idx; float* Xw;
     const float* xf; int64_t xfs;      // when set, the input is the preprocessed batch xf[c][w]
     int64_t ld; int CP, W, C, XS, mtiles, ntiles;
+    int cmap_id;                       // the channel map is the identity (c -> c)
 };
 
 // ---- preprocessing in front of whitening (lss_preprocess.hip) ----

@@ -83,57 +83,134 @@ __device__ __forceinline__ bool xcd_tile(int k, int mtiles, int ntiles, int& m, 
 // Xw[c][XOFF + w] = sum_c' Wr[c][c'] * x[w][cmap[c']]   (rows c < CP)
 // Columns w >= W read zero input and so store zero: they are the right-hand
 // zero padding the PC convolution reads.
+//
+// Each step stages four listed 16-deep K-blocks (64 of the reduction) in LDS,
+// and the next step's operands are loaded into registers while the MFMAs run,
+// so a tile waits for memory once per 64 of the reduction instead of once per
+// 16.  Blocks are accumulated in list order, 8 MFMAs each, as a one-block step
+// would.  The channel map is staged in LDS once per workgroup, so an int16
+// sample load never waits for a map entry; with the identity map and aligned
+// rows a thread reads its 16 channels as two 16-byte loads.
+constexpr int WBK = 4 * GEMM_BK;     // reduction depth staged per step
+constexpr int WCM = 2048;            // channel maps up to this size are staged in LDS
+
+struct WhitenTile {
+    float As[WBK][GEMM_BM + LDS_PAD];
+    float Bs[WBK][GEMM_BN + LDS_PAD];
+};
+
+// One 16-deep block of the 64x64 tile: rows kb .. kb + 15 of the staged operands.
+__device__ __forceinline__ f32x16 mfma_block(const WhitenTile& s, int kb, f32x16 acc, int lane, int wm, int wn) {
+#pragma unroll
+    for (int kk = 0; kk < GEMM_BK / 2; ++kk) {
+        const float a = s.As[kb + kk * 2 + (lane >> 5)][wm * 32 + (lane & 31)];
+        const float b = s.Bs[kb + kk * 2 + (lane >> 5)][wn * 32 + (lane & 31)];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
-    __shared__ GemmTile s;
+    __shared__ WhitenTile s;
+    __shared__ int s_cm[WCM];
+    __shared__ int s_kb[WCM / GEMM_BK];
     const int j = job_of(bt.start, bt.n, blockIdx.x);
     const WhitenJob& jb = bt.job[j];
     int mt, nt;
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     const float* __restrict__ Wr = jb.Wr;
     const int16_t* __restrict__ x = jb.xptr ? *jb.xptr : jb.x;
+    const float* __restrict__ xf = jb.xf;
     if (!x) return;                                             // stream idle this launch
-    const int* __restrict__ cmap = jb.cmap;
     const int CP = jb.CP, W = jb.W, C = jb.C;
     const int64_t ld = jb.ld;
     const int m0 = mt * GEMM_BM, n0 = nt * GEMM_BN;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
-    const int ar = tid >> 2, ak = (tid & 3) * 4;   // A: row, 4 consecutive k
-    const int bn = tid >> 2, bk = (tid & 3) * 4;   // B: column (sample), 4 consecutive channels
+    const int q = tid & 3, ar = tid >> 2;        // listed block of the step, A row / B sample (int16 input)
+    const bool cm_lds = !xf && CP <= WCM;
+    if (cm_lds)
+        for (int i = tid; i < CP; i += 256) s_cm[i] = jb.cmap[i];
+    const int* __restrict__ cm = cm_lds ? s_cm : jb.cmap;
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
     const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
-    float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
-    float rb[4] = {0.f, 0.f, 0.f, 0.f};
-    auto gload = [&](int kb) {
-        const int k0 = kb * GEMM_BK;
-        ra = *reinterpret_cast<const float4*>(Wr + (int64_t)(m0 + ar) * CP + k0 + ak);
-        const int n = n0 + bn;
+    // the tile's K-block list, staged too (indexed from p0)
+    const bool kb_lds = p1 - p0 <= WCM / GEMM_BK;
+    if (kb_lds)
+        for (int i = tid; i < p1 - p0; i += 256) s_kb[i] = jb.kb_idx[p0 + i];
+    auto kbi = [&](int p) { return kb_lds ? s_kb[p - p0] : jb.kb_idx[p]; };
+    float4 ra[4];
+    float rb[16];
+    // registers of step pb (list entries pb .. pb + 3): this thread's A row segment
+    // of block q, and 16 B-operand values
+    auto gload = [&](int pb) {
+        const bool have = pb + q < p1;
+        const int k0 = have ? kbi(pb + q) * GEMM_BK : 0;
+        const float4* ap = reinterpret_cast<const float4*>(Wr + (int64_t)(m0 + ar) * CP + k0);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int c = k0 + bk + i;
-            rb[i] = (n < W && c < C) ? (jb.xf ? jb.xf[(int64_t)c * jb.xfs + n] : (float)x[(int64_t)n * ld + cmap[c]])
-                                     : 0.f;
+        for (int i = 0; i < 4; ++i) ra[i] = have ? ap[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (xf) {
+            // preprocessed input, channel-major: lane = sample, value i = staged row
+            // 4 i + wave (block i / 4, channel 4 (i % 4) + wave)
+            const int n = n0 + lane;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int r = 4 * i + wv, pp = pb + r / GEMM_BK;
+                const int c = pp < p1 ? kbi(pp) * GEMM_BK + r % GEMM_BK : C;
+                rb[i] = (n < W && c < C) ? xf[(int64_t)c * jb.xfs + n] : 0.f;
+            }
+        } else {
+            // int16 input, sample-major: sample n0 + ar, channels k0 .. k0 + 15 of block q
+            const int n = n0 + ar;
+            const int16_t* row = x + (int64_t)n * ld;
+            const bool vec = have && n < W && jb.cmap_id && k0 + GEMM_BK <= C &&
+                             ((reinterpret_cast<uintptr_t>(row + k0) & 15) == 0);
+            if (vec) {
+                const int4 v0 = reinterpret_cast<const int4*>(row + k0)[0];
+                const int4 v1 = reinterpret_cast<const int4*>(row + k0)[1];
+                const int w8[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    rb[2 * i] = (float)(int16_t)(w8[i] & 0xffff);
+                    rb[2 * i + 1] = (float)(int16_t)((unsigned)w8[i] >> 16);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int c = k0 + i;
+                    rb[i] = (have && n < W && c < C) ? (float)row[cm[c]] : 0.f;
+                }
+            }
         }
     };
-    if (p0 < p1) gload(jb.kb_idx[p0]);
-    for (int p = p0; p < p1; ++p) {
+    __syncthreads();                             // the staged channel map and block list
+    if (p0 < p1) gload(p0);
+    for (int pb = p0; pb < p1; pb += 4) {
         __syncthreads();
-        s.As[ak + 0][ar] = ra.x;
-        s.As[ak + 1][ar] = ra.y;
-        s.As[ak + 2][ar] = ra.z;
-        s.As[ak + 3][ar] = ra.w;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) s.Bs[bk + i][bn] = rb[i];
+        for (int i = 0; i < 4; ++i) {
+            s.As[q * GEMM_BK + 4 * i + 0][ar] = ra[i].x;
+            s.As[q * GEMM_BK + 4 * i + 1][ar] = ra[i].y;
+            s.As[q * GEMM_BK + 4 * i + 2][ar] = ra[i].z;
+            s.As[q * GEMM_BK + 4 * i + 3][ar] = ra[i].w;
+        }
+        if (xf) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s.Bs[4 * i + wv][lane] = rb[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s.Bs[q * GEMM_BK + i][ar] = rb[i];
+        }
         __syncthreads();
-        if (p + 1 < p1) gload(jb.kb_idx[p + 1]);
-        acc = mfma_tile(s, acc, lane, wm, wn);
+        if (pb + 4 < p1) gload(pb + 4);
+        const int nq = min(4, p1 - pb);
+        for (int qq = 0; qq < nq; ++qq) acc = mfma_block(s, qq * GEMM_BK, acc, lane, wm, wn);
     }
     store_tile(jb.Xw + XOFF, jb.XS, m0, n0, acc, lane, wm, wn);
 }
 
-// B[t][w] = sum_kk A[t][kk] * Bm[kk][w]   (A = spatial templates [TP][K*CP], Bm = PC conv [K*CP][BS])
 __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     __shared__ GemmTile s;
     const int j = job_of(bt.start, bt.n, blockIdx.x);
