@@ -675,7 +675,11 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_ptag = dalloc<unsigned long long>(h->ptag_words);
     HIPCHK(hipMemset(h->d_ptag, 0, h->ptag_words * sizeof(unsigned long long)));
     h->d_epoch = dalloc<unsigned>(1);
-    HIPCHK(hipMemset(h->d_epoch, 0, sizeof(unsigned)));
+    if (const char* es = std::getenv("LSS_EPOCH_START")) {
+        // test support: start the tag epoch elsewhere (near its wrap-around)
+        const unsigned e0 = (unsigned)std::strtoul(es, nullptr, 10) & EPOCH_MASK;
+        HIPCHK(hipMemcpy(h->d_epoch, &e0, sizeof(unsigned), hipMemcpyHostToDevice));
+    }
     h->d_list_cnt = dalloc<int>(h->max_groups);
     // per-iteration spike totals and done counters, then the peel's neighbour flags
     // (per workgroup, per column block); all cleared by the reset kernel each batch

@@ -214,6 +214,25 @@ def ref_templates(ref):
     return np.asarray(ref.t)
 
 
+def test_epoch_wrap_keeps_results(cfg2, monkeypatch):
+    """The peel's published column maxima carry a per-batch epoch in their tags;
+    when it wraps the reset kernel clears every word.  A handle started three
+    batches before the wrap gives the same events over batches of changing
+    width (words left from other layouts) as one started at zero."""
+    m, d, tab, x = cfg2
+    widths = [3122, 2000, 3122, 1500, 3122, 2500]
+    monkeypatch.setenv("LSS_EPOCH_START", str((1 << 19) - 3))
+    a = OnlineSpikesV2(d)
+    monkeypatch.delenv("LSS_EPOCH_START")
+    with a, OnlineSpikesV2(d) as b:
+        for j, wd in enumerate(widths):
+            batch = x[j * 1000:j * 1000 + wd]
+            ea, fa = a.process_batch(batch)
+            eb, fb = b.process_batch(batch)
+            assert len(ea) > 0
+            assert np.array_equal(ea, eb) and np.array_equal(fa, fb), f"batch {j} (width {wd})"
+
+
 def test_deterministic(cfg2):
     m, d, tab, x = cfg2
     batch = x[3000:6122]
