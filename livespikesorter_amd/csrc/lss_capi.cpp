@@ -804,6 +804,9 @@ struct BatchPlan {
     int R = 0, CW = 0, G = 0;
 };
 
+// PC conv workgroups per 256-column block: channel rows PC_ROWS_PER_BLOCK at a time.
+inline int pc_groups(int C) { return (C + PC_ROWS_PER_BLOCK - 1) / PC_ROWS_PER_BLOCK; }
+
 inline int peel_cap_index(int R) { return R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 : R == 32 ? 3 : 4; }
 
 // Job descriptors of one handle for one batch of W columns.  The input is read
@@ -852,7 +855,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     cj.Xw = h->d_Xw; cj.wpca = h->d_wpca; cj.Bpc = h->d_Bpc;
     cj.Bpcb = reinterpret_cast<__bf16*>(h->d_Bpcb);
     cj.active = xptr;
-    cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP;
+    cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP; cj.C = h->C;
     cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
     ProjectJob& pj = bp.pj;
     pj = ProjectJob{};
@@ -1011,7 +1014,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     WhitenBatch wb{};
     wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
     PcconvBatch cb{};
-    cb.n = 1; cb.start[0] = 0; cb.start[1] = bp.cj.wblocks * h->C; cb.job[0] = bp.cj;
+    cb.n = 1; cb.start[0] = 0; cb.start[1] = bp.cj.wblocks * pc_groups(h->C); cb.job[0] = bp.cj;
     ProjectBatch pb{};
     pb.n = 1; pb.start[0] = 0; pb.start[1] = gemm_blocks(bp.pj.mtiles, bp.pj.ntiles); pb.job[0] = bp.pj;
     PeelBatch kb{};
@@ -1572,7 +1575,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         L.wb.job[s2] = bp.wj;
         L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
         L.cb.job[s2] = bp.cj;
-        L.cb.start[s2 + 1] = L.cb.start[s2] + bp.cj.wblocks * h->C;
+        L.cb.start[s2 + 1] = L.cb.start[s2] + bp.cj.wblocks * pc_groups(h->C);
         L.pb.job[s2] = bp.pj;
         L.pb.start[s2 + 1] = L.pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
         L.kb.p[s2] = bp.pp;

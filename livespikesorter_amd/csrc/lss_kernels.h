@@ -191,7 +191,9 @@ struct PcconvJob {
     __bf16* Bpcb;                         // optional BF16 copy of the PC projection
     const int16_t* const* active;         // optional: the job is skipped when *active is null
     int XS, M, BS, CP, wblocks;
+    int C;                                // channels (rows PC_ROWS to a workgroup)
 };
+constexpr int PC_ROWS_PER_BLOCK = 4;      // channel rows of one PC conv workgroup
 template <class J> struct JobBatch {
     int n;
     int start[MAX_JOBS + 1];       // first block of each job; start[n] = blocks in the launch

@@ -258,34 +258,63 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
 // One workgroup stages 256 + 64 samples of one whitened channel row in LDS
 // (coalesced), each thread produces the K outputs of one column.
 // ---------------------------------------------------------------------------
+// Bpc[k][c][w] = sum_m wpca[k][m] * Xw[c][w + m - M/2].  A workgroup takes four
+// channel rows of one 256-column block; a thread computes 4 consecutive columns
+// of its row for every PC, reading the staged row 4 taps at a time as two
+// aligned 16-byte words (the row is staged from column w0 - M/2).  Each output
+// accumulates its taps in order with one fused multiply-add per tap.
+constexpr int PC_ROWS = PC_ROWS_PER_BLOCK;
+
 template <int KK>
 __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
-    __shared__ float xs[256 + 2 * XOFF];
+    constexpr int XW = 256 + 2 * XOFF;           // staged columns per row (>= 256 + M - 1 + 4)
+    __shared__ __align__(16) float xs[PC_ROWS][XW];
     const int j = job_of(bt.start, bt.n, blockIdx.x);
     const PcconvJob& jb = bt.job[j];
     if (jb.active && !*jb.active) return;                      // stream idle this launch
     const int k = blockIdx.x - bt.start[j];
-    const int c = k / jb.wblocks, w0 = (k % jb.wblocks) * 256, tid = threadIdx.x;
-    const int M = jb.M;
-    const float* __restrict__ wpca = jb.wpca;
-    const float* row = jb.Xw + (int64_t)c * jb.XS + XOFF;      // row[w] valid for w >= -XOFF
-    for (int i = tid; i < 256 + 2 * XOFF; i += 256) xs[i] = row[w0 - XOFF + i];
-    __syncthreads();
-    const int h = M / 2;
-    float acc[KK];
-#pragma unroll
-    for (int q = 0; q < KK; ++q) acc[q] = 0.f;
-    const float* xp = xs + tid + XOFF - h;
-    for (int m = 0; m < M; ++m) {
-        const float v = xp[m];
-#pragma unroll
-        for (int q = 0; q < KK; ++q) acc[q] = fmaf(wpca[q * M + m], v, acc[q]);
+    const int cg = k / jb.wblocks, w0 = (k % jb.wblocks) * 256, tid = threadIdx.x;
+    const int M = jb.M, h = M / 2, C = jb.C;
+    for (int i = tid; i < PC_ROWS * XW; i += 256) {
+        const int r = i / XW, o = i - r * XW, cc = cg * PC_ROWS + r;
+        // Xw rows are valid from column -XOFF and past BS + XOFF
+        xs[r][o] = (cc < C && o < 256 + M - 1) ? jb.Xw[(int64_t)cc * jb.XS + XOFF + w0 - h + o] : 0.f;
     }
+    __syncthreads();
+    const int rr = tid >> 6, l = tid & 63, c = cg * PC_ROWS + rr;
+    if (c >= C) return;
+    const float* __restrict__ wpca = jb.wpca;
+    float acc[4][KK];
 #pragma unroll
-    for (int q = 0; q < KK; ++q) jb.Bpc[(int64_t)(q * jb.CP + c) * jb.BS + w0 + tid] = acc[q];
-    if (jb.Bpcb) {
+    for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
-        for (int q = 0; q < KK; ++q) jb.Bpcb[(int64_t)(q * jb.CP + c) * jb.BS + w0 + tid] = (__bf16)acc[q];
+        for (int q = 0; q < KK; ++q) acc[jj][q] = 0.f;
+    const float* xp = &xs[rr][4 * l];
+    for (int m0 = 0; m0 < M; m0 += 4) {
+        const float4 a = *reinterpret_cast<const float4*>(xp + m0);
+        const float4 b = *reinterpret_cast<const float4*>(xp + m0 + 4);
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) {
+            if (m0 + mm < M) {
+#pragma unroll
+                for (int q = 0; q < KK; ++q) {
+                    const float wq = wpca[q * M + m0 + mm];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) acc[jj][q] = fmaf(wq, v[jj + mm], acc[jj][q]);
+                }
+            }
+        }
+    }
+    const int64_t o0 = (int64_t)c * jb.BS + w0 + 4 * l;
+#pragma unroll
+    for (int q = 0; q < KK; ++q) {
+        const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
+        *reinterpret_cast<float4*>(jb.Bpc + o) = make_float4(acc[0][q], acc[1][q], acc[2][q], acc[3][q]);
+        if (jb.Bpcb) {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) jb.Bpcb[o + jj] = (__bf16)acc[jj][q];
+        }
     }
 }
 
