@@ -594,6 +594,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ float s_nm[R > 0 ? REG_MAX_T : 1];    // own rows' reciprocal norms (0 past T)
     __shared__ float s_nmf[R > 0 ? REG_MAX_T : 1];   // every row's norm (+inf past T)
     __shared__ int s_or[R > 0 ? REG_MAX_T : 1];      // every row's file index (INT_MAX past T)
+    __shared__ int s_band[R > 0 ? REG_MAX_T : 1];    // every template's nonzero ctc rows: tlo | thi << 16
     __shared__ Spike s_sp[MAX_STAGE];          // staged / newly accepted spikes
     __shared__ Spike s_list[MAX_LIST];         // this workgroup's spikes, whole batch
     __shared__ Spike s_pre[MAX_NBR * PRE];
@@ -668,6 +669,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         for (int t = tid; t < min(p.ctc_rows, REG_MAX_T); t += PEEL_THREADS) {
             s_nmf[t] = p.nm[t];
             s_or[t] = p.orig[t];
+            s_band[t] = t < T ? (p.tlo[t] | (p.thi[t] << 16)) : 0;
         }
         // rows past the score matrix read as zero (buffer loads beyond the range)
         const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(p.B, 0, p.B_bytes, BUF_DWORD3);
@@ -817,8 +819,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     const int qq = q0 + lane;
                     bool bh = false;
                     if (qq < nsp && !woff) {
-                        const int st = s_sp[qq].t;
-                        bh = r0 < p.thi[st] && r0 + R > p.tlo[st];
+                        const int bd = s_band[s_sp[qq].t];
+                        bh = r0 < (bd >> 16) && r0 + R > (bd & 0xffff);
                     }
                     const uint64_t bal = __ballot(bh);
                     if (bh) whit[__popcll(bal & ((1ull << lane) - 1ull))] = qq;
