@@ -1368,8 +1368,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         publish(tid, 0, ep | TAG_DONE);
         publish(tid, 1, ep | TAG_DONE);
     }
-    raise_flag(&flagA[g], FLAG_DONE);
-    if (owner && tid == 0) st_c(&flagB[cb], FLAG_DONE);
+    if (!local) {
+        raise_flag(&flagA[g], FLAG_DONE);
+        if (tid == 0) st_c(&flagB[cb], FLAG_DONE);
+    }
 
     // ---------------- events and features ----------------
     // The owners publish their lists for the neighbours' features.  Every
@@ -1383,18 +1385,35 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         if (tid == 0) st_c(&p.list_cnt[cb], nl);
     }
     grid_sync(p.bar, p.G, g, bar_gen, p.err);
-    {   // spikes of the blocks before this one: the block's offset in the event list
-        int part = 0;
-        for (int hh = tid; hh < cb; hh += PEEL_THREADS) part += ld_c(&p.list_cnt[hh]);
-        s_idx[tid] = part;
-        __syncthreads();
-        for (int o = PEEL_THREADS / 2; o > 0; o >>= 1) {
-            if (tid < o) s_idx[tid] += s_idx[tid + o];
-            __syncthreads();
+    // Blocks whose spikes lie within M - 1 columns (the features' overlap terms).
+    const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.NB - 1, (c1 - 1 + M - 1) / CW);
+    const int nfb = fhi - flo + 1;               // <= MAX_NBR because CW >= M
+    __shared__ int s_nbn[MAX_NBR], s_nbo[MAX_NBR], s_nbt;
+    {   // one round trip: the spike counts of the blocks before this one (its offset in
+        // the event list: wave 0 sums them) and of the neighbour blocks
+        if (tid < 64) {
+            int part = 0;
+            for (int hh = tid; hh < cb; hh += 64) part += ld_c(&p.list_cnt[hh]);
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+            if (tid == 0) s_off = part;
+        } else if (tid < 64 + nfb) {
+            const int hh = flo + tid - 64;
+            s_nbn[tid - 64] = hh == cb ? 0 : min(ld_c(&p.list_cnt[hh]), MAX_LIST);
         }
-        if (tid == 0) s_off = s_idx[0];
         __syncthreads();
     }
+    // the neighbour blocks' lists, staged once in the (now free) spike array
+    if (tid == 0) {
+        int o = 0;
+        for (int i = 0; i < nfb; ++i) { s_nbo[i] = o; o += s_nbn[i]; }
+        s_nbt = o;
+    }
+    __syncthreads();
+    const bool nb_lds = p.do_feat && s_nbt <= MAX_STAGE;
+    if (nb_lds)
+        for (int i = 0; i < nfb; ++i)
+            for (int q = tid; q < s_nbn[i]; q += PEEL_THREADS)
+                s_sp[s_nbo[i] + q] = ld_spike(&p.list[(int64_t)(flo + i) * p.cap_g + q]);
     const int nmine = min(s_listn, MAX_LIST);
     const int off = s_off;
     for (int i = tid; i < nmine; i += PEEL_THREADS) {
@@ -1418,7 +1437,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     if (p.do_feat) {
         const int lane = tid & 63, wv = tid >> 6;
         const int K = p.K, nC = p.nC, C = p.C, L2 = 2 * M - 1;
-        const int flo = max(0, (c0 - (M - 1)) / CW), fhi = min(p.NB - 1, (c1 - 1 + M - 1) / CW);
         float* stage = reinterpret_cast<float*>(s_idx) + wv * 64;    // s_idx is free after the offset sum
         for (int i = sb + S * wv; i < nmine; i += S * (PEEL_THREADS / 64)) {
             const int d = off + s_rank[i];
@@ -1433,14 +1451,15 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             // the spikes within +-(M-1) columns, in list order: a wave reads 64 at a time
             // and takes the near ones from the ballot, lowest index first
             for (int hh = flo; hh <= fhi; ++hh) {
-                const int cnt = hh == cb ? nmine : ld_c(&p.list_cnt[hh]);
+                const int cnt = hh == cb ? nmine : s_nbn[hh - flo];
                 const Spike* lst = p.list + (int64_t)hh * p.cap_g;
+                const Spike* lsl = hh == cb ? s_list : s_sp + s_nbo[hh - flo];
                 for (int q0 = 0; q0 < cnt; q0 += 64) {
                     const int q = q0 + lane;
                     Spike sq{};
                     bool near = false;
                     if (q < cnt) {
-                        sq = hh == cb ? s_list[q] : ld_spike(&lst[q]);
+                        sq = (hh == cb || nb_lds) ? lsl[q] : ld_spike(&lst[q]);
                         const int dd = si.w - sq.w;
                         near = dd >= -(M - 1) && dd <= M - 1;
                     }
