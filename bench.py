@@ -535,6 +535,8 @@ def main() -> int:
     ap.add_argument("--pre-steps", type=int, default=200,
                     help="chunks timed again with preprocessing (means, CAR, high-pass) switched on")
     ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="cfg2: keep the default peel geometry instead of lss_autotune's choice")
     ap.add_argument("--workload", choices=("cfg2", "group", "bf16", "target"), default="cfg2",
                     help="cfg2: the headline plain FP32 matcher; group: 8 streams per graph launch; "
                          "bf16: BF16 projection with FP32 rescoring, 1024 templates, 60000-sample batches")
@@ -581,6 +583,11 @@ def main() -> int:
 
     sp = OnlineSpikesV2(d, chunk=N, profile=True)
     info = sp.info
+    # the peel geometry for this batch width, timed on a chunk (lss_autotune), as a
+    # caller would pick it before streaming
+    tuned = [] if args.no_autotune else sp.autotune(base + 10 * N * C * 2, W, C, reps=20)
+    if tuned:
+        log("peel autotune (split, rows, ms, chosen): " + str(tuned))
 
     def step(j):
         b = j % n_batches
@@ -603,6 +610,7 @@ def main() -> int:
         phases += sp.debug_stage(STAGE_PHASES)
     synchronize()
     elapsed = time.perf_counter() - t_start
+    q_last = sp.query()
     stage_ms, nb = sp.stage_times(reset=True)
     rtf = args.steps * N / FS / elapsed
     log(f"timed {args.steps} chunks in {elapsed:.3f}s: {rtf:.1f}x realtime, p50 {pct(lat, 50):.3f} ms, "
@@ -756,6 +764,8 @@ def main() -> int:
         "stages": stages,
         "with_preprocessing": pre,
         "peel_phases_us": peel_phases,
+        "peel_geometry": {"split": int(q_last.peel_split), "rows_per_thread": int(q_last.peel_rows),
+                          "workgroups": int(q_last.peel_groups), "autotuned": bool(tuned)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,

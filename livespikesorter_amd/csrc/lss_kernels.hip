@@ -152,14 +152,13 @@ __global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) ra[i] = have ? ap[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (xf) {
-            // preprocessed input, channel-major: lane = sample, value i = staged row
-            // 4 i + wave (block i / 4, channel 4 (i % 4) + wave)
-            const int n = n0 + lane;
+            // preprocessed input, channel-major: sample n0 + ar, channels k0 .. k0 + 15
+            // of block q (a load instruction reads 16 consecutive samples of 4 rows)
+            const int n = n0 + ar;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const int r = 4 * i + wv, pp = pb + r / GEMM_BK;
-                const int c = pp < p1 ? kbi(pp) * GEMM_BK + r % GEMM_BK : C;
-                rb[i] = (n < W && c < C) ? xf[(int64_t)c * jb.xfs + n] : 0.f;
+                const int c = k0 + i;
+                rb[i] = (have && n < W && c < C) ? xf[(int64_t)c * jb.xfs + n] : 0.f;
             }
         } else {
             // int16 input, sample-major: sample n0 + ar, channels k0 .. k0 + 15 of block q
@@ -196,13 +195,8 @@ __global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
             s.As[q * GEMM_BK + 4 * i + 2][ar] = ra[i].z;
             s.As[q * GEMM_BK + 4 * i + 3][ar] = ra[i].w;
         }
-        if (xf) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) s.Bs[4 * i + wv][lane] = rb[i];
-        } else {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) s.Bs[q * GEMM_BK + i][ar] = rb[i];
-        }
+        for (int i = 0; i < 16; ++i) s.Bs[q * GEMM_BK + i][ar] = rb[i];
         __syncthreads();
         if (pb + 4 < p1) gload(pb + 4);
         const int nq = min(4, p1 - pb);
