@@ -822,11 +822,16 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     wdirty = wdirty || nhit > 0;
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_s_waitcnt(0xc07f);     // lgkmcnt(0): the list is in LDS
-                    for (int h0 = 0; h0 < nhit; h0 += 4) {
-                        float am[4];
-                        int vo[4];
+                    // A load batch holds SPB spikes x RC rows (32 values): a wave usually
+                    // has one or two spikes in its band, so taller tiles take more rows of
+                    // fewer spikes per batch (each batch costs one memory latency).
+                    constexpr int SPB = R <= 8 ? 4 : (R == 16 ? 2 : 1);    // spikes per load batch
+                    constexpr int RC = R < 32 ? R : 32;                     // rows per load batch
+                    for (int h0 = 0; h0 < nhit; h0 += SPB) {
+                        float am[SPB];
+                        int vo[SPB];
     #pragma unroll
-                        for (int j = 0; j < 4; ++j) {
+                        for (int j = 0; j < SPB; ++j) {
                             const bool in = h0 + j < nhit;
                             const Spike sp = s_sp[whit[in ? h0 + j : h0]];
                             const int l = w - sp.w + M;
@@ -834,12 +839,11 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             am[j] = hit ? sp.amp : 0.f;
                             vo[j] = ((sp.t * p.ctc_rows + r0) * LP + (hit ? l : 0)) * 4;
                         }
-                        constexpr int RC = R < 8 ? R : (R >= 64 ? 4 : 8);    // rows per load batch
     #pragma unroll
                         for (int i0 = 0; i0 < R; i0 += RC) {
-                            float c[4][RC];
+                            float c[SPB][RC];
     #pragma unroll
-                            for (int j = 0; j < 4; ++j)
+                            for (int j = 0; j < SPB; ++j)
     #pragma unroll
                                 for (int i = 0; i < RC; ++i)
                                     c[j][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
@@ -847,7 +851,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     #pragma unroll
                             for (int i = 0; i < RC; ++i)
     #pragma unroll
-                                for (int j = 0; j < 4; ++j) breg[i0 + i] -= am[j] * c[j][i];
+                                for (int j = 0; j < SPB; ++j) breg[i0 + i] -= am[j] * c[j][i];
                         }
                     }
                     __builtin_amdgcn_wave_barrier();
