@@ -53,7 +53,8 @@ typedef struct lss_config {
     int32_t sparse;        /* nonzero (default): skip exact-zero template/channel blocks */
     int32_t profile;       /* nonzero: time each device stage with HIP events (lss_stage_times) */
     int32_t peel_split;    /* template subsets per peel column block: 0 = automatic,
-                              1 = none, n > 1 = at most n (Kilosort semantics only) */
+                              1 = none, n > 1 = n when that geometry fits, else at
+                              most n (Kilosort semantics only) */
     int32_t relabel;       /* nonzero: an event's template_id is the final cluster nearest to
                               the spike's position (cluster_centroids.npy), as saveSpikes
                               reports it (OnlineSpikesV2.cu:1186-1189); 0 (default): the

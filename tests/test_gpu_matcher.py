@@ -142,15 +142,20 @@ def test_template_split_is_exact(cfg2, model_dirs):
     ref = _oracle(tab, batch, stages=True, features=False)
     assert np.abs(cf_c - ref.Cfmax).max() <= 1e-4 * np.abs(ref.Cfmax).max()
     assert np.mean(im_c == ref.imax) >= 0.999
-    # 1024 templates, 1500-sample chunks: eight subsets of 128 rows
+    # 1024 templates, 1500-sample chunks: eight subsets of 128 rows, and the
+    # default (four subsets of 256 rows)
     m5, d5 = model_dirs("cfg2", T=1024)
     x5, _ = synth.make_recording(m5, 1622 * 2, seed=synth.TEST_SEED)
-    with OnlineSpikesV2(d5, max_batch=1622, peel_split=1) as a, OnlineSpikesV2(d5, max_batch=1622) as b:
+    with OnlineSpikesV2(d5, max_batch=1622, peel_split=1) as a, OnlineSpikesV2(d5, max_batch=1622) as b, \
+            OnlineSpikesV2(d5, max_batch=1622, peel_split=8) as c8:
         for j in range(2):
             ea, fa = a.process_batch(x5[j * 1500:j * 1500 + 1622])
             eb, fb = b.process_batch(x5[j * 1500:j * 1500 + 1622])
+            ec, fc = c8.process_batch(x5[j * 1500:j * 1500 + 1622])
             assert np.array_equal(ea, eb) and np.array_equal(fa, fb)
-        assert b.query().peel_split == 8
+            assert np.array_equal(ea, ec) and np.array_equal(fa, fc)
+        assert b.query().peel_split == 4 and b.query().peel_rows == 16
+        assert c8.query().peel_split == 8 and c8.query().peel_rows == 8
 
 
 def test_autotune_keeps_results(cfg2):
