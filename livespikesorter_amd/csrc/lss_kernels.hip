@@ -949,30 +949,58 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     // each covering spike, in order, updates this thread's rows of its band
                     // (every element sees its updates in spike order, as a row-outer loop
                     // would apply them), then one pass takes the column maximum
+                    // Eight rows at a time, every load of a batch in flight before its
+                    // stores (the rows of a batch are distinct; the B and ctc pointers
+                    // would otherwise keep each row's loads behind the previous store).
+                    auto gated = [&](int t) { return p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]; };
                     for (int q = qa; q < qb; ++q) {
                         const Spike sp = s_sp[q];
                         const int l = w - sp.w + M;                 // in [0, 2M] for q in [qa, qb)
                         const int t0 = s_qlo[q], t1 = s_qhi[q];
                         const float* cp = p.ctc + (int64_t)sp.t * p.ctc_rows * LP + l;
-                        for (int t = t0 + (rg - t0 % nrg + nrg) % nrg; t < t1; t += nrg) {
-                            if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) continue;
-                            float* bp = p.B + (int64_t)t * p.BS + w;
-                            *bp -= sp.amp * cp[(int64_t)t * LP];
+                        for (int ta = t0 + (rg - t0 % nrg + nrg) % nrg; ta < t1; ta += 8 * nrg) {
+                            float bv[8], cv[8];
+                            bool in[8];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                const int t = ta + k * nrg;
+                                in[k] = t < t1 && !gated(t);
+                                bv[k] = in[k] ? p.B[(int64_t)t * p.BS + w] : 0.f;
+                                cv[k] = in[k] ? cp[(int64_t)t * LP] : 0.f;
+                            }
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                if (in[k]) p.B[(int64_t)(ta + k * nrg) * p.BS + w] = bv[k] - sp.amp * cv[k];
                         }
                     }
-                    for (int t = rg; t < T; t += nrg) {
-                        if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) {
-                            if (better(0.f, p.orig[t], best, bo)) { best = 0.f; bi = t; bo = p.orig[t]; bval = 0.f; }
-                            continue;
+                    for (int ta = rg; ta < T; ta += 8 * nrg) {
+                        float bv[8], nv[8];
+                        int ov[8];
+                        bool in[8], on[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int t = ta + k * nrg;
+                            in[k] = t < T;
+                            on[k] = in[k] && !gated(t);
+                            bv[k] = on[k] ? p.B[(int64_t)t * p.BS + w] : 0.f;
+                            nv[k] = on[k] ? (p.lss ? p.rnm[t] : p.nm[t]) : 1.f;
+                            ov[k] = in[k] ? p.orig[t] : 0;
                         }
-                        const float b = p.B[(int64_t)t * p.BS + w];
-                        float cf = 0.f;
-                        if (!edge) {
-                            const float r = fmaxf(b, 0.f);
-                            cf = p.lss ? (r * r) * p.rnm[t] : (r * r) / p.nm[t];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            if (!in[k]) continue;
+                            const int t = ta + k * nrg;
+                            if (!on[k]) {
+                                if (better(0.f, ov[k], best, bo)) { best = 0.f; bi = t; bo = ov[k]; bval = 0.f; }
+                                continue;
+                            }
+                            float cf = 0.f;
+                            if (!edge) {
+                                const float r = fmaxf(bv[k], 0.f);
+                                cf = p.lss ? (r * r) * nv[k] : (r * r) / nv[k];
+                            }
+                            if (better(cf, ov[k], best, bo)) { best = cf; bi = t; bo = ov[k]; bval = bv[k]; }
                         }
-                        const int o = p.orig[t];
-                        if (better(cf, o, best, bo)) { best = cf; bi = t; bo = o; bval = b; }
                     }
                 }
             }
