@@ -1070,12 +1070,17 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     float acc = 0.f;
                     for (int e = e0 + lane; e < e1; e += 64 * 8) {
                         float uq[8], bq[8];
+                        int kc[8];
+                        // the 8 block indices, then the 16 loads they address, each group in flight at once
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int ee = e + 64 * q;
-                            const int kc = ee < e1 ? p.kb_idx[ee / GEMM_BK] * GEMM_BK + ee % GEMM_BK : -1;
-                            uq[q] = kc >= 0 ? urow[kc] : 0.f;
-                            bq[q] = kc >= 0 ? p.Bpc[(int64_t)kc * p.BS + ww] : 0.f;
+                            kc[q] = ee < e1 ? p.kb_idx[ee / GEMM_BK] * GEMM_BK + ee % GEMM_BK : -1;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            uq[q] = kc[q] >= 0 ? urow[kc[q]] : 0.f;
+                            bq[q] = kc[q] >= 0 ? p.Bpc[(int64_t)kc[q] * p.BS + ww] : 0.f;
                         }
 #pragma unroll
                         for (int q = 0; q < 8; ++q) acc += uq[q] * bq[q];
@@ -1497,9 +1502,17 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         const float aq = __shfl(sq.amp, b);
                         if (mine) {
                             const int dd = si.w - wq;
+                            // all 2K loads in flight before the sum (in k2 order)
+                            float uu[MAX_K], gg[MAX_K];
+#pragma unroll
+                            for (int k2 = 0; k2 < MAX_K; ++k2) {
+                                uu[k2] = k2 < K ? p.U[((int64_t)tq * K + k2) * C + ch] : 0.f;
+                                gg[k2] = k2 < K ? p.Gt[(k2 * K + k) * L2 + dd + M - 1] : 0.f;
+                            }
                             float gs = 0.f;
-                            for (int k2 = 0; k2 < K; ++k2)
-                                gs += p.U[((int64_t)tq * K + k2) * C + ch] * p.Gt[(k2 * K + k) * L2 + dd + M - 1];
+#pragma unroll
+                            for (int k2 = 0; k2 < MAX_K; ++k2)
+                                if (k2 < K) gs += uu[k2] * gg[k2];
                             f -= aq * gs;
                         }
                     }
