@@ -675,8 +675,9 @@ def main() -> int:
     traffic = None
     tf = ROOT / "profiles" / "cfg2_plain_fp32_traffic.json"
     if tf.exists():
+        # the peel's entry is the kernel of the timed chunks (its tile height)
         kname = {"whiten": "lss::k_whiten", "pcconv": "lss::k_pcconv", "project": "lss::k_project",
-                 "peel": "lss::k_peel"}[dom]
+                 "peel": f"lss::k_peel<{int(q_last.peel_rows)}>"}[dom]
         for k, v in json.loads(tf.read_text())["bytes_per_launch"].items():
             if k.startswith(kname):
                 traffic = round(v)
