@@ -263,6 +263,7 @@ template <int KK>
 __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     constexpr int XW = 256 + 2 * XOFF;           // staged columns per row (>= 256 + M - 1 + 4)
     __shared__ __align__(16) float xs[PC_ROWS][XW];
+    __shared__ __align__(16) float wt[MAX_M + 1][8];   // wt[m][k] = wpca[k][m]: a tap's weights, two 16-byte reads
     const int j = job_of(bt.start, bt.n, blockIdx.x);
     const PcconvJob& jb = bt.job[j];
     if (jb.active && !*jb.active) return;                      // stream idle this launch
@@ -274,10 +275,13 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
         // Xw rows are valid from column -XOFF and past BS + XOFF
         xs[r][o] = (cc < C && o < 256 + M - 1) ? jb.Xw[(int64_t)cc * jb.XS + XOFF + w0 - h + o] : 0.f;
     }
+    for (int i = tid; i < M * 8; i += 256) {
+        const int m = i >> 3, q = i & 7;
+        wt[m][q] = q < KK ? jb.wpca[q * M + m] : 0.f;
+    }
     __syncthreads();
     const int rr = tid >> 6, l = tid & 63, c = cg * PC_ROWS + rr;
     if (c >= C) return;
-    const float* __restrict__ wpca = jb.wpca;
     float acc[4][KK];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj)
@@ -291,11 +295,13 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
 #pragma unroll
         for (int mm = 0; mm < 4; ++mm) {
             if (m0 + mm < M) {
+                const float4 wa = *reinterpret_cast<const float4*>(&wt[m0 + mm][0]);
+                const float4 wb = *reinterpret_cast<const float4*>(&wt[m0 + mm][4]);
+                const float wv8[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
 #pragma unroll
                 for (int q = 0; q < KK; ++q) {
-                    const float wq = wpca[q * M + m0 + mm];
 #pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) acc[jj][q] = fmaf(wq, v[jj + mm], acc[jj][q]);
+                    for (int jj = 0; jj < 4; ++jj) acc[jj][q] = fmaf(wv8[q], v[jj + mm], acc[jj][q]);
                 }
             }
         }
