@@ -198,6 +198,7 @@ __global__ __launch_bounds__(1024) void k_pre_car(PreBatch bt) {
 // the four taps themselves through the scalar path (uniform loads), for 16 FMAs.
 __global__ __launch_bounds__(256) void k_pre_fir(PreBatch bt) {
     __shared__ __attribute__((aligned(16))) float s_w[PRE_OUT + PRE_TCH + 8];
+    __shared__ __attribute__((aligned(16))) float s_h[PRE_TCH];     // this chunk's taps (broadcast reads)
     const int j = [&] {
         int q = 0;
         while (q + 1 < bt.n && (int)blockIdx.x >= bt.start_fir[q + 1]) ++q;
@@ -220,9 +221,10 @@ __global__ __launch_bounds__(256) void k_pre_fir(PreBatch bt) {
         // window element e = x[(k0 + shift + base + e) mod W]
         const int i0 = (k0 + jb.shift + base) % W;
         for (int e = tid; e < PRE_OUT + nt4 + 4; e += 256) s_w[e] = xr[(i0 + e) % W];
+        for (int e = tid; e < nt; e += 256) s_h[e] = taps[base + e];
         __syncthreads();
         const float* wp = s_w + 4 * tid;
-        const float* hp = taps + base;
+        const float* hp = s_h;
         float4 A = *reinterpret_cast<const float4*>(wp);
         auto step4 = [&](const float4& B, float h0, float h1, float h2, float h3) {
             y0 = fmaf(A.x, h0, y0); y0 = fmaf(A.y, h1, y0); y0 = fmaf(A.z, h2, y0); y0 = fmaf(A.w, h3, y0);
@@ -231,13 +233,16 @@ __global__ __launch_bounds__(256) void k_pre_fir(PreBatch bt) {
             y3 = fmaf(A.w, h0, y3); y3 = fmaf(B.x, h1, y3); y3 = fmaf(B.y, h2, y3); y3 = fmaf(B.z, h3, y3);
             A = B;
         };
-        // sixteen taps per iteration: one wait covers 16 scalar taps and 4 window reads
+        // sixteen taps per iteration (four broadcast 16-byte reads) and four window reads
         const int nt16 = nt & ~15;
         int q = 0;
         for (; q < nt16; q += 16) {
             float h[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) h[i] = hp[q + i];
+            for (int i = 0; i < 4; ++i) {
+                const float4 t4 = *reinterpret_cast<const float4*>(hp + q + 4 * i);
+                h[4 * i] = t4.x; h[4 * i + 1] = t4.y; h[4 * i + 2] = t4.z; h[4 * i + 3] = t4.w;
+            }
             const float4 B0 = *reinterpret_cast<const float4*>(wp + q + 4);
             const float4 B1 = *reinterpret_cast<const float4*>(wp + q + 8);
             const float4 B2 = *reinterpret_cast<const float4*>(wp + q + 12);
