@@ -51,7 +51,8 @@ typedef struct lss_config {
     int32_t max_events;    /* event capacity per batch on the device (default 65536) */
     int32_t keep_stages;   /* nonzero: keep per-stage device copies for lss_debug_stage */
     int32_t sparse;        /* nonzero (default): skip exact-zero template/channel blocks */
-    int32_t profile;       /* nonzero: time each device stage with HIP events (lss_stage_times) */
+    int32_t profile;       /* 1: time each device stage with HIP events (lss_stage_times);
+                              2: the peel stage only (two events per batch) */
     int32_t peel_split;    /* template subsets per peel column block: 0 = automatic,
                               1 = none, n > 1 = n when that geometry fits, else at
                               most n (Kilosort semantics only) */

@@ -1013,7 +1013,8 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     if (W <= 0 || W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) +
                                               " samples outside 1..max_batch " + std::to_string(h->Wmax));
     hipStream_t s = h->stream;
-    const bool prof = h->cfg.profile != 0;
+    const bool prof = h->cfg.profile == 1;         // every stage
+    const bool prof_peel = h->cfg.profile != 0;    // the peel (level 2: the peel only)
     int CW = 0, R = 0, S = 1;
     peel_geometry(h, W, CW, R, S);
     BatchPlan bp;
@@ -1043,20 +1044,20 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     launch_pcconv(cb, h->K, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
     launch_project(pb, h->bf16, s);
-    if (prof) HIPCHK(hipEventRecord(h->evt[6], s));
-    HIPCHK(hipGetLastError());
     if (h->cfg.keep_stages)
         HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
+    if (prof_peel) HIPCHK(hipEventRecord(h->evt[6], s));
+    HIPCHK(hipGetLastError());
     HIPCHK(launch_peel(kb, R, h->coop, s));
-    if (prof) HIPCHK(hipEventRecord(h->evt[7], s));
+    if (prof_peel) HIPCHK(hipEventRecord(h->evt[7], s));
     HIPCHK(hipStreamSynchronize(s));
-    if (prof) {
+    if (prof_peel) {
         float ms = 0.f;
         if (h->copy_timed) {
             HIPCHK(hipEventElapsedTime(&ms, h->evt[0], h->evt[1]));
             h->stage_ms[0] += ms;
         }
-        for (int i = 1; i < LSS_NSTAGES; ++i) {
+        for (int i = prof ? 1 : LSS_NSTAGES - 1; i < LSS_NSTAGES; ++i) {
             HIPCHK(hipEventElapsedTime(&ms, h->evt[i], h->evt[i + 1]));
             h->stage_ms[i] += ms;
         }
@@ -1121,9 +1122,9 @@ void process_host_batch(lss_handle* h, const int16_t* samples, int64_t W, int32_
     if (W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch longer than max_batch");
     set_cmap(h, chan_map, ld);
     int16_t* dx = ensure_dx(h, (size_t)W * ld);
-    if (h->cfg.profile) HIPCHK(hipEventRecord(h->evt[0], h->stream));
+    if (h->cfg.profile == 1) HIPCHK(hipEventRecord(h->evt[0], h->stream));
     HIPCHK(hipMemcpyAsync(dx, samples, (size_t)W * ld * sizeof(int16_t), hipMemcpyHostToDevice, h->stream));
-    h->copy_timed = h->cfg.profile != 0;
+    h->copy_timed = h->cfg.profile == 1;
     run_batch(h, dx, (int)W, ld, first_sample);
 }
 
@@ -1437,7 +1438,7 @@ int lss_autotune(lss_handle* h, const int16_t* d_samples, int64_t n, int32_t ld,
         set_cmap(h, nullptr, ld);
         const int W = (int)n;
         const lss_info saved = h->info;
-        const bool prof = h->cfg.profile != 0;
+        const int32_t prof = h->cfg.profile;   // restored afterwards (the timings are host-side)
         h->cfg.profile = 0;
         *out = lss_tune_result{};
         std::vector<int> cand = split_candidates(h, W);

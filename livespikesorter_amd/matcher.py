@@ -31,7 +31,7 @@ class OnlineSpikesV2:
     def __init__(self, oss_input_dir: str | os.PathLike, *, device: int = 0, chunk: int = 3000,
                  max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
                  precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
-                 keep_stages: bool = False, sparse: bool = True, profile: bool = False,
+                 keep_stages: bool = False, sparse: bool = True, profile: int = 0,
                  peel_split: int = 0, relabel: bool = False, preprocess: int = 0, keep=None):
         """``keep``: ascending Wall3 template indices to load (a template subset,
         lss_create_subset); event template ids are then positions in ``keep``."""
@@ -159,7 +159,7 @@ class OnlineSpikesV2:
         return ev[:n.value].copy(), feat[:n.value].copy()
 
     def stage_times(self, reset: bool = False):
-        """Accumulated device ms per stage (needs ``profile=True``): (dict, batches)."""
+        """Accumulated device ms per stage (needs ``profile=1``; ``profile=2`` times the peel only): (dict, batches)."""
         from ._capi import NSTAGES, STAGE_NAMES
         ms = (ct.c_double * NSTAGES)()
         nb = ct.c_int64(0)
