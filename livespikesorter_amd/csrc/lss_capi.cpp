@@ -682,7 +682,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     }
     h->d_list_cnt = dalloc<int>(h->max_groups);
     // per-iteration spike totals and done counters, then the peel's neighbour flags
-    // (per workgroup, per column block); all cleared by the reset kernel each batch
+    // (per workgroup, per column block); all cleared by the whitening launch each batch
     h->g_cap = h->max_groups * std::max(1, h->max_split);
     h->sync_words = 2 * std::max(1, h->max_iter) + h->g_cap + h->max_groups;
     h->d_iter_cnt = dalloc<int>(h->sync_words);
@@ -1028,8 +1028,12 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     pb.n = 1; pb.start[0] = 0; pb.start[1] = gemm_blocks(bp.pj.mtiles, bp.pj.ntiles); pb.job[0] = bp.pj;
     PeelBatch kb{};
     kb.n = 1; kb.g_start[0] = 0; kb.g_start[1] = bp.G; kb.p[0] = bp.pp;
+    // the peel words of this batch are reset by the whitening launch
+    ResetBatch rb{};
+    rb.n = 1;
+    rb.bar[0] = h->d_bar; rb.iter_cnt[0] = h->d_iter_cnt; rb.n_iter[0] = h->sync_words; rb.err[0] = h->d_err;
+    rb.prof[0] = h->d_prof; rb.epoch[0] = h->d_epoch; rb.ptag[0] = h->d_ptag; rb.ptag_words[0] = h->ptag_words;
     if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
-    launch_reset(h->d_bar, h->d_iter_cnt, h->sync_words, h->d_err, h->d_prof, h->d_epoch, h->d_ptag, h->ptag_words, s);
     if (bp.has_pre) {
         PreBatch pre{};
         pre.n = 1;
@@ -1037,7 +1041,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
         launch_preprocess(pre, s);
     }
     if (prof) HIPCHK(hipEventRecord(h->evt[2], s));
-    launch_whiten(wb, s);
+    launch_whiten(wb, &rb, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[3], s));
     if (h->prop) run_gate(h, W, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
@@ -1597,9 +1601,8 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     hipStream_t st = L.stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
-    launch_reset_batch(L.rb, st);
     launch_preprocess(pre, st);
-    launch_whiten(L.wb, st);
+    launch_whiten(L.wb, &L.rb, st);
     {   // the proposal networks of all gated streams: one dispatch per layer
         PnFwdJob fj[PN_MAX_JOBS];
         PnGateJob gj[PN_MAX_JOBS];
@@ -1886,7 +1889,7 @@ int lss_proposal_train_step(lss_proposal* net, lss_handle* h, const int16_t* d_s
         }
         WhitenBatch wb{};
         wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
-        launch_whiten(wb, h->stream);
+        launch_whiten(wb, nullptr, h->stream);
         HIPCHK(hipStreamSynchronize(h->stream));
         pn_upload_labels(net, labels, (int32_t)n);
         pn_forward(net->wt, net->buf, h->d_Xw, h->XS, XOFF, h->C, (int)n, net->stream);

@@ -39,7 +39,7 @@ constexpr int BAR_WORDS = BAR_LINE * (1 + 2 * BAR_MAX_GROUPS);
 // low 32 bits of each 64-bit word are (epoch << TAG_ITER_BITS) | (iteration +
 // 1), or | TAG_DONE for the final maxima of a workgroup that has left the loop
 // (valid for every later iteration).  The epoch advances once per batch; when it
-// wraps the reset kernel clears every word, so an old word never carries a
+// wraps the reset (done by the whitening launch) clears every word, so an old word never carries a
 // current tag.
 constexpr int TAG_ITER_BITS = 13;
 constexpr unsigned TAG_DONE = (1u << TAG_ITER_BITS) - 1;
@@ -98,7 +98,7 @@ struct PeelParams {
     // Kilosort rules: per-subset column maxima by iteration parity, 64-bit words
     // (value << 32 | tag): scores [2][S][W], then B values, then templates (same shape)
     unsigned long long* Pa;
-    const unsigned* epoch;             // this batch's epoch (advanced by the reset kernel)
+    const unsigned* epoch;             // this batch's epoch (advanced by the whitening launch)
     int lss;
     float* Cfmax; int* imax;           // column maxima (LSS rules)
     Spike* slots; int* slot_cnt; int slot_cap;
@@ -206,13 +206,28 @@ using PcconvBatch = JobBatch<PcconvJob>;
 // Blocks a GEMM job occupies: M-tiles x N-tiles padded to a multiple of 8 (XCD order).
 inline int gemm_blocks(int mtiles, int ntiles) { return mtiles * ((ntiles + 7) / 8 * 8); }
 
-// Clears the per-batch counters the peel kernel relies on and advances the
-// epoch of the published column maxima (ptag: all of their words, cleared when
-// the epoch wraps).
-void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, unsigned* epoch,
-                  unsigned long long* ptag, int64_t ptag_words, hipStream_t s);
 
-void launch_whiten(const WhitenBatch& b, hipStream_t s);
+struct ResetBatch {
+    int n;
+    unsigned* bar[MAX_JOBS];
+    int* iter_cnt[MAX_JOBS];
+    int n_iter[MAX_JOBS];
+    int* err[MAX_JOBS];
+    unsigned long long* prof[MAX_JOBS];
+    unsigned* epoch[MAX_JOBS];
+    unsigned long long* ptag[MAX_JOBS];
+    int64_t ptag_words[MAX_JOBS];
+    const int16_t* const* active[MAX_JOBS];   // nullable: a stream idle this launch keeps its counters
+};
+
+// The whitening launch also resets the peel words of its streams (the peel of
+// the same batch comes after it on the stream): rb may be null.
+struct WhitenLaunch {
+    WhitenBatch w;
+    ResetBatch r;
+};
+static_assert(sizeof(WhitenLaunch) <= 4096, "whitening launch must fit the kernel-argument segment");
+void launch_whiten(const WhitenBatch& b, const ResetBatch* rb, hipStream_t s);
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s);
 void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s);
 
@@ -236,19 +251,7 @@ static_assert(sizeof(PeelBatch) <= 4096, "peel batch must fit the kernel-argumen
 
 hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t s);
 
-struct ResetBatch {
-    int n;
-    unsigned* bar[MAX_JOBS];
-    int* iter_cnt[MAX_JOBS];
-    int n_iter[MAX_JOBS];
-    int* err[MAX_JOBS];
-    unsigned long long* prof[MAX_JOBS];
-    unsigned* epoch[MAX_JOBS];
-    unsigned long long* ptag[MAX_JOBS];
-    int64_t ptag_words[MAX_JOBS];
-    const int16_t* const* active[MAX_JOBS];   // nullable: a stream idle this launch keeps its counters
-};
-void launch_reset_batch(const ResetBatch& b, hipStream_t s);
+
 
 int peel_max_blocks(int device, int R);
 

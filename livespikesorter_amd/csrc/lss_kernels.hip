@@ -110,10 +110,15 @@ __device__ __forceinline__ f32x16 mfma_block(const WhitenTile& s, int kb, f32x16
     return acc;
 }
 
-__global__ __launch_bounds__(256) void k_whiten(WhitenBatch bt) {
+__device__ void reset_stream(const ResetBatch& b, int j);
+
+__global__ __launch_bounds__(256) void k_whiten(WhitenLaunch wl) {
     __shared__ WhitenTile s;
     __shared__ int s_cm[WCM];
     __shared__ int s_kb[WCM / GEMM_BK];
+    // the first workgroups also reset the batch's peel words, one stream each
+    if ((int)blockIdx.x < wl.r.n) reset_stream(wl.r, blockIdx.x);
+    const WhitenBatch& bt = wl.w;
     const int j = job_of(bt.start, bt.n, blockIdx.x);
     const WhitenJob& jb = bt.job[j];
     int mt, nt;
@@ -430,13 +435,18 @@ __device__ void advance_epoch(unsigned* epoch, unsigned long long* ptag, int64_t
         for (int64_t i = threadIdx.x; i < words; i += blockDim.x) ptag[i] = 0ull;
 }
 
-__global__ void k_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof,
-                        unsigned* epoch, unsigned long long* ptag, int64_t ptag_words) {
-    for (int i = threadIdx.x; i < n_iter; i += blockDim.x) iter_cnt[i] = 0;
-    if ((int)threadIdx.x < PROF_WORDS) prof[threadIdx.x] = 0ull;
-    if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) bar[threadIdx.x * BAR_LINE] = 0u;
-    if (threadIdx.x == 0) *err = 0;
-    if (epoch) advance_epoch(epoch, ptag, ptag_words);
+// The per-batch words the peel relies on, for stream j of a launch: iteration
+// counts and neighbour flags, phase clock, barrier words, error flags cleared,
+// the tag epoch advanced.  Run by one whole workgroup of the whitening launch
+// (every peel of the batch comes after it on the stream); a stream idle in this
+// launch keeps its words.
+__device__ void reset_stream(const ResetBatch& b, int j) {
+    if (b.active[j] && !*b.active[j]) return;
+    for (int i = threadIdx.x; i < b.n_iter[j]; i += blockDim.x) b.iter_cnt[j][i] = 0;
+    if ((int)threadIdx.x < PROF_WORDS) b.prof[j][threadIdx.x] = 0ull;
+    if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) b.bar[j][threadIdx.x * BAR_LINE] = 0u;
+    if (threadIdx.x == 0) *b.err[j] = 0;
+    if (b.epoch[j]) advance_epoch(b.epoch[j], b.ptag[j], b.ptag_words[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1603,32 +1613,15 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     }
 }
 
-// One block per stream: the per-batch counters of every stream of a group.
-__global__ void k_reset_batch(ResetBatch b) {
-    const int j = blockIdx.x;
-    if (j >= b.n) return;
-    if (b.active[j] && !*b.active[j]) return;     // idle stream (maybe running in another launch)
-    for (int i = threadIdx.x; i < b.n_iter[j]; i += blockDim.x) b.iter_cnt[j][i] = 0;
-    if ((int)threadIdx.x < PROF_WORDS) b.prof[j][threadIdx.x] = 0ull;
-    if ((int)threadIdx.x < 1 + 2 * BAR_MAX_GROUPS) b.bar[j][threadIdx.x * BAR_LINE] = 0u;
-    if (threadIdx.x == 0) *b.err[j] = 0;
-    if (b.epoch[j]) advance_epoch(b.epoch[j], b.ptag[j], b.ptag_words[j]);
-}
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_reset(unsigned* bar, int* iter_cnt, int n_iter, int* err, unsigned long long* prof, unsigned* epoch,
-                  unsigned long long* ptag, int64_t ptag_words, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, s, bar, iter_cnt, n_iter, err, prof, epoch, ptag, ptag_words);
-}
-
-void launch_reset_batch(const ResetBatch& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_batch, dim3(b.n), dim3(256), 0, s, b);
-}
-
-void launch_whiten(const WhitenBatch& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_whiten, dim3(b.start[b.n]), dim3(256), 0, s, b);
+void launch_whiten(const WhitenBatch& b, const ResetBatch* rb, hipStream_t s) {
+    WhitenLaunch wl{};
+    wl.w = b;
+    if (rb) wl.r = *rb;                          // else n = 0: nothing to reset
+    hipLaunchKernelGGL(k_whiten, dim3(b.start[b.n]), dim3(256), 0, s, wl);
 }
 
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {

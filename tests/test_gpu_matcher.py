@@ -221,7 +221,7 @@ def ref_templates(ref):
 
 def test_epoch_wrap_keeps_results(cfg2, monkeypatch):
     """The peel's published column maxima carry a per-batch epoch in their tags;
-    when it wraps the reset kernel clears every word.  A handle started three
+    when it wraps the per-batch reset clears every word.  A handle started three
     batches before the wrap gives the same events over batches of changing
     width (words left from other layouts) as one started at zero."""
     m, d, tab, x = cfg2
