@@ -167,7 +167,6 @@ struct lss_handle {
     // per-batch device buffers
     float *d_Xw = nullptr, *d_Bpc = nullptr, *d_B = nullptr;
     float* d_Cfmax = nullptr;
-    int* d_imax = nullptr;
     Spike *d_slots = nullptr, *d_list = nullptr;
     int *d_slot_cnt = nullptr, *d_list_cnt = nullptr, *d_iter_cnt = nullptr, *d_err = nullptr;
     unsigned* d_bar = nullptr;
@@ -251,7 +250,7 @@ struct lss_handle {
     void release() {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
-                        d_Cfmax, d_imax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
+                        d_Cfmax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
                         d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
@@ -646,7 +645,6 @@ void load(lss_handle* h, const std::string& dir_in) {
         HIPCHK(hipMemset(h->d_rescnt, 0, 2 * sizeof(unsigned long long)));
     }
     h->d_Cfmax = dalloc<float>(Wmax);
-    h->d_imax = dalloc<int>(Wmax);
     {
         const int Rs[5] = {0, 8, 16, 32, 64};
         for (int i = 0; i < 5; ++i) {
@@ -889,7 +887,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.S = S; p.NB = NB;
     p.Pa = h->d_ptag; p.epoch = h->d_epoch;
     p.lss = h->sem == LSS_SEM_LSS;
-    p.Cfmax = h->d_Cfmax; p.imax = h->d_imax;
+    p.Cfmax = h->d_Cfmax;
     p.slots = h->d_slots; p.slot_cnt = h->d_slot_cnt; p.slot_cap = h->slot_cap;
     p.list = h->d_list; p.list_cnt = h->d_list_cnt; p.cap_g = h->cap_g;
     p.bar = h->d_bar; p.iter_cnt = h->d_iter_cnt; p.err = h->d_err;
@@ -910,7 +908,6 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
         // candidates within 10% of threshold on the score scale are decided in FP32
         p.rescore = 1;
         p.lo2 = 0.9f * p.Th2;
-        p.hi2 = 1.1f * p.Th2;
         p.U2 = h->d_U2; p.ldu = h->KCP;
         p.kb_ptr = h->d_kbp_ptr; p.kb_idx = h->d_kbp_idx;
         p.Binit = h->d_Binit;

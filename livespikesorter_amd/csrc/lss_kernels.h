@@ -100,7 +100,7 @@ struct PeelParams {
     unsigned long long* Pa;
     const unsigned* epoch;             // this batch's epoch (advanced by the whitening launch)
     int lss;
-    float* Cfmax; int* imax;           // column maxima (LSS rules)
+    float* Cfmax;                      // column maxima (LSS rules)
     Spike* slots; int* slot_cnt; int slot_cap;
     Spike* list; int* list_cnt; int cap_g;
     // iter_cnt [max_iter] spike totals, followed by the peel loop's neighbour
@@ -123,9 +123,9 @@ struct PeelParams {
     // score reaches lo2 gets that score recomputed as the FP32 dot product of U2
     // row t with Bpc column w (over the listed K-blocks of t's 64-row tile) minus
     // the BF16 value Binit[t][w], added to its current score before it is
-    // published; hi2 is the upper end of the band reported as near-threshold.
+    // published.
     int rescore;
-    float lo2, hi2;
+    float lo2;
     const float* U2; int ldu;          // [TP][K*CP], FP32
     const int* kb_ptr; const int* kb_idx;
     const float* Binit;                // the BF16 projection before any subtraction
