@@ -238,6 +238,23 @@ def test_epoch_wrap_keeps_results(cfg2, monkeypatch):
             assert np.array_equal(ea, eb) and np.array_equal(fa, fb), f"batch {j} (width {wd})"
 
 
+def test_profile_levels(cfg2):
+    """Stage timing changes no result; level 2 times the peel stage only."""
+    m, d, tab, x = cfg2
+    batch = x[:3122]
+    with OnlineSpikesV2(d) as a, OnlineSpikesV2(d, profile=1) as b, OnlineSpikesV2(d, profile=2) as c:
+        ea, fa = a.process_batch(batch)
+        eb, fb = b.process_batch(batch)
+        ec, fc = c.process_batch(batch)
+        sb, nb = b.stage_times()
+        sc, nc = c.stage_times()
+    assert np.array_equal(ea, eb) and np.array_equal(fa, fb)
+    assert np.array_equal(ea, ec) and np.array_equal(fa, fc)
+    assert nb == 1 and nc == 1
+    assert sb["whiten"] > 0 and sb["peel"] > 0
+    assert sc["peel"] > 0 and all(v == 0 for k, v in sc.items() if k != "peel")
+
+
 def test_deterministic(cfg2):
     m, d, tab, x = cfg2
     batch = x[3000:6122]
