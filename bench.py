@@ -94,8 +94,9 @@ def bench_group(args) -> dict:
     C = recs[0].shape[1]
     synchronize()
     t0 = time.time()
-    grp = StreamGroup(dirs, batch=W)
-    log(f"group created in {time.time() - t0:.1f}s")
+    grp = StreamGroup(dirs, batch=W, peel_split=args.group_split)
+    geom = group_geometry(grp.info)
+    log(f"group created in {time.time() - t0:.1f}s, peel geometry {geom}")
     rng = np.random.default_rng(20261203)
     jitter = rng.uniform(-5.0, 5.0, size=(args.steps + args.warmup, len(dirs)))
 
@@ -137,7 +138,7 @@ def bench_group(args) -> dict:
     seq = time.perf_counter() - ts2
     for sp in singles:
         sp.close()
-    grp = StreamGroup(dirs, batch=W)
+    grp = StreamGroup(dirs, batch=W, peel_split=args.group_split)
     plat, pbusy, pticks = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205,
                                         lanes=args.lanes)
     grp.close()
@@ -169,7 +170,14 @@ def bench_group(args) -> dict:
         "spikes_per_tick": round(spikes / args.steps, 1),
         "sequential_single_handles_x_realtime": round(seq_rtf, 3),
         "speedup_over_sequential": round(rtf / seq_rtf, 3),
+        "peel_geometry": geom,
     }
+
+
+def group_geometry(infos):
+    """The group's peel grid: tile rows per thread, subsets per stream, workgroups."""
+    return {"rows_per_thread": int(infos[0].peel_rows), "split": [int(x.peel_split) for x in infos],
+            "workgroups": int(sum(int(x.peel_groups) for x in infos))}
 
 
 def pct(a, q):
@@ -296,7 +304,7 @@ def bench_target(args) -> dict:
     all_modes = (("group_fp32", PREC_FP32, False), ("group_bf16", PREC_BF16_RESCORE, False),
                  ("group_bf16_gated", PREC_BF16_RESCORE, True))
     for name, prec, gated in [m for m in all_modes if m[0] in args.modes.split(",")]:
-        grp = StreamGroup([d] * S, batch=W, precision=prec)
+        grp = StreamGroup([d] * S, batch=W, precision=prec, peel_split=args.group_split)
         if gated:
             for s in range(S):
                 grp.set_proposal(s, net, 0.25, 0.75, 4)
@@ -330,7 +338,8 @@ def bench_target(args) -> dict:
                "paced_gpu_busy_fraction": round(busy, 4), "paced_launches": nt,
                "agreement_vs_plain_fp32": {"identical": round(ident_n / max(tot, 1), 6),
                                            "within_1_sample": round(within_n / max(tot, 1), 6), "events": tot},
-               "speedup_over_plain_fp32_sequential": round(rtf / seq_rtf, 3)}
+               "speedup_over_plain_fp32_sequential": round(rtf / seq_rtf, 3),
+               "peel_geometry": group_geometry(q)}
         if gated:
             sc = sum(x.gate_tiles_scored for x in q)
             tt = sum(x.gate_tiles for x in q)
@@ -543,6 +552,8 @@ def main() -> int:
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
+    ap.add_argument("--group-split", type=int, default=0,
+                    help="group and target workloads: peel template split (0 automatic, 1 none, n ways)")
     ap.add_argument("--lanes", type=int, default=4, help="concurrent launch lanes of the paced group runs (1-4)")
     ap.add_argument("--modes", default="group_fp32,group_bf16,group_bf16_gated",
                     help="target workload: which group modes to run")

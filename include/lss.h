@@ -240,7 +240,13 @@ int lss_autotune(struct lss_handle* h, const int16_t* d_samples, int64_t n_sampl
  * stream per call with a single hipGraph replay (one batched launch per
  * stage; the peel kernel is shared by all streams).  Streams may have
  * different template counts; they share K, M and cfg.  cfg->max_batch is the
- * framed batch width W used for every stream. */
+ * framed batch width W used for every stream.  All streams share one peel
+ * kernel instance: the register tile of the largest T, or with Kilosort rules
+ * a shorter one (16, then 32 rows per thread) with each stream's templates
+ * split over ceil(T / tile rows) workgroups per column block, when that grid
+ * is co-resident.  cfg->peel_split: 0 = automatic, 1 = one subset per stream,
+ * n > 1 = the tile that splits the largest stream n ways when it fits.
+ * lss_group_query reports each stream's peel_rows, peel_split, peel_groups. */
 struct lss_group;
 int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const lss_config* cfg,
                      struct lss_group** out);

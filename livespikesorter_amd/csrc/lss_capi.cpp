@@ -153,6 +153,7 @@ struct lss_handle {
     int peel_cap[5] = {};    // co-resident peel workgroups for R = 0, 8, 16, 32, 64
     int ctc_rows = 0, t_first = 0;
     int min_rows = 0;        // force the padded ctc row count up to this (streams sharing one peel launch)
+    int group_split = 0;     // a group's stream: its template subsets in the group's peel (0: one)
     int64_t cur_first = 0;   // absolute sample of column 0 of the last batch
     bool coop = true;        // cooperative launch of the peel kernel
     bool bf16 = false;       // BF16 projection with FP32 rescoring (LSS_PREC_BF16_RESCORE)
@@ -666,6 +667,8 @@ void load(lss_handle* h, const std::string& dir_in) {
     if (h->T <= REG_MAX_T && h->sem == LSS_SEM_KILOSORT && h->min_rows == 0) {
         h->max_split = std::min<int>(16, (h->T + 127) / 128);
         if (h->cfg.peel_split > 0) h->max_split = std::min(h->max_split, (int)h->cfg.peel_split);
+    } else if (h->min_rows > 0 && h->group_split > 1) {
+        h->max_split = h->group_split;   // the group chose this stream's split
     }
     // per-subset column maxima (score, B value, template), one copy per iteration
     // parity, as tagged words; all zero (a tag no epoch uses) to start with
@@ -1135,7 +1138,7 @@ bool dedup_on(const lss_handle* h) {
 
 // Build a handle from an oss_input directory (lss_create; also each stream of a group).
 std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_config* cfg, int min_rows,
-                                          const int32_t* keep = nullptr, int n_keep = 0) {
+                                          const int32_t* keep = nullptr, int n_keep = 0, int group_split = 0) {
     auto h = std::make_unique<lss_handle>();
     if (keep) h->keep.assign(keep, keep + n_keep);
     if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
@@ -1153,6 +1156,7 @@ std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_c
     h->max_iter = h->cfg.max_iter > 0 ? h->cfg.max_iter : (h->sem == LSS_SEM_LSS ? 50 : 100);
     if (h->max_iter > MAX_ITER) throw LssError(LSS_ERR_ARG, "max_iter too large");
     h->min_rows = min_rows;
+    h->group_split = group_split;
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (h->dev < 0 || h->dev >= ndev) throw LssError(LSS_ERR_ARG, "no such HIP device");
@@ -1549,6 +1553,7 @@ constexpr int GROUP_LANES = 4;
 struct lss_group {
     std::vector<std::unique_ptr<lss_handle>> hs;
     int n = 0, W = 0, R = 0, CW = 0, dev = 0;
+    std::vector<int> S;   // per stream: template subsets per column block in the peel grid
     GroupLane lanes[GROUP_LANES];
     ~lss_group() {
         (void)hipSetDevice(dev);
@@ -1580,7 +1585,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     for (int s2 = 0; s2 < n; ++s2) {
         lss_handle* h = g->hs[s2].get();
         BatchPlan bp;
-        plan_batch(h, nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, 1, bp);
+        plan_batch(h, nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, g->S[s2], bp);
         if (bp.has_pre) pre.job[pre.n++] = bp.pre;
         L.wb.job[s2] = bp.wj;
         L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
@@ -1638,22 +1643,54 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
         g->W = cfg->max_batch;
         g->dev = cfg->device;
         int64_t tmax = 0;
+        std::vector<int64_t> ts(n_streams);
         for (int s2 = 0; s2 < n_streams; ++s2) {
             if (!oss_input_dirs[s2]) throw LssError(LSS_ERR_ARG, "null directory");
-            tmax = std::max(tmax, peek_templates(oss_input_dirs[s2]));
+            ts[s2] = peek_templates(oss_input_dirs[s2]);
+            tmax = std::max(tmax, ts[s2]);
         }
-        // one peel kernel instance for all streams: the register tile of the largest T
+        HIPCHK(hipSetDevice(g->dev));
+        // One peel kernel instance for all streams: the register tile of the largest T.
+        // With Kilosort rules a shorter tile may serve instead, each stream's templates
+        // split over ceil(T / tile rows) workgroups per column block, when that grid is
+        // co-resident: 16 rows per thread first, then 32.  peel_split 1 keeps one subset;
+        // n > 1 asks for the tile that splits the largest stream n ways, when it fits.
         g->CW = MIN_CW;
         g->R = peel_rows((int)tmax, g->CW);
+        const int NB = (g->W + MIN_CW - 1) / MIN_CW;
+        auto subsets = [](int R, int64_t T) { return (int)((T + REG_NRG * R - 1) / (REG_NRG * R)); };
+        auto split_ok = [&](int R) {
+            if (R <= 0 || R >= g->R || subsets(R, tmax) > 16) return false;
+            int64_t sum = 0;
+            for (int64_t T : ts) sum += (int64_t)subsets(R, T) * NB;
+            return sum <= peel_max_blocks(g->dev, R);
+        };
+        int Rs = 0;
+        const int want = cfg->peel_split;
+        if (g->R > 0 && tmax <= REG_MAX_T && cfg->semantics == LSS_SEM_KILOSORT && want != 1) {
+            if (want > 1 && split_ok(split_rows((int)tmax, want))) Rs = split_rows((int)tmax, want);
+            for (int r : {16, 32})
+                if (Rs == 0 && split_ok(r) && (want <= 1 || subsets(r, tmax) <= want)) Rs = r;
+        }
+        if (Rs > 0) g->R = Rs;
+        g->S.assign(n_streams, 1);
+        for (int s2 = 0; s2 < n_streams; ++s2)
+            if (Rs > 0) g->S[s2] = subsets(Rs, ts[s2]);
         const int rows = g->R > 0 ? REG_NRG * g->R : 0;
         for (int s2 = 0; s2 < n_streams; ++s2) {
-            g->hs.push_back(create_handle(oss_input_dirs[s2], cfg, rows));
+            g->hs.push_back(create_handle(oss_input_dirs[s2], cfg, rows, nullptr, 0, g->S[s2]));
             if (g->hs.back()->K != g->hs[0]->K || g->hs.back()->M != g->hs[0]->M)
                 throw LssError(LSS_ERR_ARG, "streams of a group must share K and M");
         }
         if (g->R == 0) g->CW = 2 * MIN_CW;
         int gsum = 0;
-        for (size_t s2 = 0; s2 < g->hs.size(); ++s2) gsum += (g->W + g->CW - 1) / g->CW;
+        for (size_t s2 = 0; s2 < g->hs.size(); ++s2) {
+            lss_handle* h = g->hs[s2].get();
+            const int G = g->S[s2] * ((g->W + g->CW - 1) / g->CW);
+            gsum += G;
+            h->info.peel_cw = g->CW; h->info.peel_rows = g->R; h->info.peel_split = g->S[s2];
+            h->info.peel_groups = G;
+        }
         if (gsum > g->hs[0]->peel_cap[peel_cap_index(g->R)])
             throw LssError(LSS_ERR_ARG, "the group's peel grid (" + std::to_string(gsum) +
                                             " workgroups) is not co-resident on this device");

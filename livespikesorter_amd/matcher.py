@@ -196,17 +196,20 @@ class StreamGroup:
 
     No counterpart in the reference, which runs one sorter thread per probe
     (main.cpp:77-85).  Each call to :meth:`process` matches one framed batch
-    of ``batch`` columns of every stream.
+    of ``batch`` columns of every stream.  ``peel_split``: 0 lets the group
+    split each stream's templates over several peel workgroups per column
+    block when that grid is co-resident, 1 keeps one, n > 1 asks for n ways
+    for the largest stream (``info[s].peel_split`` reports the choice).
     """
 
     def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
                  semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True,
-                 precision: int = PREC_FP32, preprocess: int = 0):
+                 precision: int = PREC_FP32, preprocess: int = 0, peel_split: int = 0):
         self._g = ct.c_void_p()
         self.n = len(oss_input_dirs)
         cfg = default_config(device=device, chunk=batch, max_batch=batch, max_iter=max_iter,
                              semantics=semantics, max_events=max_events, sparse=int(sparse),
-                             precision=precision, preprocess=preprocess)
+                             precision=precision, preprocess=preprocess, peel_split=peel_split)
         dirs = (ct.c_char_p * self.n)(*[os.fsencode(str(d)) for d in oss_input_dirs])
         check(lib().lss_group_create(self.n, dirs, ct.byref(cfg), ct.byref(self._g)))
         self.batch = batch

@@ -386,3 +386,36 @@ def test_stream_group_matches_single_handles(model_dirs):
             assert np.array_equal(o[0], e1) and np.array_equal(o[1], f1), f"lane run, stream {k}"
     for sp in singles:
         sp.close()
+
+
+def test_stream_group_template_split(model_dirs):
+    """A group whose largest stream has its templates split over several peel
+    workgroups per column block (a shorter register tile for every stream)
+    gives, bit for bit, the events and features of the unsplit group and of
+    single handles; peel_split 1 keeps one subset, 2 asks for two ways."""
+    from livespikesorter_amd import StreamGroup
+    specs = [dict(C=64, T=96, seed=41), dict(C=64, T=256, seed=42), dict(C=64, T=520, seed=43)]
+    dirs, recs = [], []
+    for k, sp in enumerate(specs):
+        m, d = model_dirs("grps%d" % k, C=sp["C"], T=sp["T"], seed=sp["seed"], whitening="banded",
+                          rate=(2.0, 20.0), amp=(11.0, 3.0))
+        x, _ = synth.make_recording(m, 2 * 1500 + 1622, seed=sp["seed"] + 100)
+        dirs.append(d)
+        recs.append(x)
+    W = 1622
+    ticks = [[x[t * 1500:t * 1500 + W] for x in recs] for t in range(2)]
+    singles = [OnlineSpikesV2(d, max_batch=W) for d in dirs]
+    ref = [[singles[k].process_batch(ticks[t][k], first_sample=t * 1500) for k in range(3)] for t in range(2)]
+    for sp in singles:
+        sp.close()
+    want = {0: [(16, 1), (16, 1), (16, 3)], 1: [(64, 1)] * 3, 2: [(32, 1), (32, 1), (32, 2)]}
+    for split, geo in want.items():
+        with StreamGroup(dirs, batch=W, peel_split=split) as grp:
+            assert [(int(i.peel_rows), int(i.peel_split)) for i in grp.info] == geo, f"peel_split={split}"
+            for t in range(2):
+                out = grp.process(ticks[t], [t * 1500] * 3)
+                for k, (ev, feat) in enumerate(out):
+                    e1, f1 = ref[t][k]
+                    assert len(e1) > 0
+                    assert np.array_equal(ev, e1), f"peel_split={split} stream {k} tick {t}"
+                    assert np.array_equal(feat, f1)
