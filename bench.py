@@ -260,7 +260,7 @@ def make_target_data(cache: Path, seconds: float, n_streams: int = 8):
 
 
 def bench_target(args) -> dict:
-    """8 probe streams of 384 channels x 1024 templates, 1500-sample chunks: the
+    """``--streams`` (default 8) probe streams of 384 channels x 1024 templates, 1500-sample chunks: the
     plain FP32 matcher one stream at a time, and one graph launch per tick for
     the plain FP32, BF16 and gated BF16 (trained proposal network) matchers;
     throughput, paced latency under +-5 ms arrival jitter, and spike agreement
@@ -273,7 +273,7 @@ def bench_target(args) -> dict:
     N, M = 1500, 61
     W = N + 2 * M
     t0 = time.time()
-    m, d, recs = make_target_data(Path(args.cache), args.group_seconds)
+    m, d, recs = make_target_data(Path(args.cache), args.group_seconds, n_streams=args.streams)
     S = len(recs)
     C = recs[0].shape[1]
     n_ticks = min((len(x) - W) // N + 1 for x in recs)
@@ -353,12 +353,12 @@ def bench_target(args) -> dict:
     return {
         "metric": METRIC + " [target: gated BF16 batched, 384 ch x 1024 templates]",
         "value": g["x_realtime"],
-        "unit": "x realtime (aggregate over 8 streams)",
+        "unit": f"x realtime (aggregate over {S} streams)",
         "n_gpus": 1, "steps": n_ticks, "warmup": 3,
         "ms_per_step": g["tick_p50_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (projection, proposal), f32",
-        "data": "synthetic (seeded generator: 8 recordings of one 384-ch, 1024-template probe)",
-        "config": {"workload": "8 streams x 384 ch x 1024 templates, 1500-sample chunks + 61-sample margins "
+        "data": f"synthetic (seeded generator: {S} recordings of one 384-ch, 1024-template probe)",
+        "config": {"workload": f"{S} streams x 384 ch x 1024 templates, 1500-sample chunks + 61-sample margins "
                                "(W=1622), one graph launch per tick, BF16 projection with FP32 rescoring, "
                                "proposal-network gate; paced run with +-5 ms arrival jitter",
                    "streams": S, "templates": 1024, "chunk": N, "batch_columns": W},
@@ -552,6 +552,7 @@ def main() -> int:
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
+    ap.add_argument("--streams", type=int, default=8, help="target workload: probe streams (1-8)")
     ap.add_argument("--group-split", type=int, default=0,
                     help="group and target workloads: peel template split (0 automatic, 1 none, n ways)")
     ap.add_argument("--lanes", type=int, default=4, help="concurrent launch lanes of the paced group runs (1-4)")
