@@ -552,6 +552,9 @@ def main() -> int:
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="cfg2: also time this many handles on alternating chunks from as many threads "
+                         "(reported as `inflight`, beside `value`; 1 = skip)")
     ap.add_argument("--streams", type=int, default=8, help="target workload: probe streams (1-8)")
     ap.add_argument("--group-split", type=int, default=0,
                     help="group and target workloads: peel template split (0 automatic, 1 none, n ways)")
@@ -689,6 +692,42 @@ def main() -> int:
                "p50_ms": round(pct(plat, 50), 4), "p99_ms": round(pct(plat, 99), 4), "stage_ms": pavg}
         log(f"with preprocessing: {pre['value']}x realtime, p99 {pre['p99_ms']} ms, stages {pavg}")
 
+    # Optional: N handles of the same probe on alternating chunks from N host
+    # threads, as a caller with a backlog could run them: one chunk's whitening,
+    # PC conv and projection can run on the CUs that another chunk's peel leaves
+    # idle.  Reported beside `value` (one chunk in flight), never as it.
+    inflight = None
+    if args.inflight > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        n_h = args.inflight
+        hs = [OnlineSpikesV2(d, chunk=N, peel_split=int(q_last.peel_split)) for _ in range(n_h)]
+        n_cmp = min(50, args.steps)
+        outs = {}
+
+        def worker(k, record):
+            for j in range(k, args.steps, n_h):
+                b = j % n_batches
+                r = hs[k].process_batch_device(base + b * N * C * 2, W, C, first_sample=b * N)
+                if record and j < n_cmp:
+                    outs[j] = r
+
+        with ThreadPoolExecutor(n_h) as pool:
+            list(pool.map(lambda k: worker(k, True), range(n_h)))      # warm-up pass, outputs kept
+            synchronize()
+            ts = time.perf_counter()
+            list(pool.map(lambda k: worker(k, False), range(n_h)))
+            synchronize()
+            iel = time.perf_counter() - ts
+        for h in hs:
+            h.close()
+        same = all(np.array_equal(outs[j][0], ref[0]) and np.array_equal(outs[j][1], ref[1])
+                   for j in range(n_cmp)
+                   for ref in [sp.process_batch_device(base + (j % n_batches) * N * C * 2, W, C,
+                                                       first_sample=(j % n_batches) * N)])
+        inflight = {"handles": n_h, "chunks": args.steps, "x_realtime": round(args.steps * N / FS / iel, 3),
+                    "identical_to_one_handle": bool(same), "chunks_compared": n_cmp}
+        log(f"{n_h} handles on alternating chunks: {inflight}")
+
     # roofline of the dominant device stage (algorithmic work per launch / measured time)
     T = info.T
     I = float(np.mean(iters))
@@ -791,6 +830,7 @@ def main() -> int:
         "mean_spikes_per_chunk": round(S, 2),
         "stages": stages,
         "with_preprocessing": pre,
+        **({"inflight": inflight} if inflight else {}),
         "peel_phases_us": peel_phases,
         "peel_geometry": {"split": int(q_last.peel_split), "rows_per_thread": int(q_last.peel_rows),
                           "workgroups": int(q_last.peel_groups), "autotuned": bool(tuned)},
