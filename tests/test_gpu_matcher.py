@@ -419,3 +419,32 @@ def test_stream_group_template_split(model_dirs):
                     assert len(e1) > 0
                     assert np.array_equal(ev, e1), f"peel_split={split} stream {k} tick {t}"
                     assert np.array_equal(feat, f1)
+
+
+def test_two_handles_on_alternating_chunks(cfg2):
+    """Two handles of one probe driven from two threads on alternating chunks
+    (a caller with a backlog) give, bit for bit, what one handle gives."""
+    from concurrent.futures import ThreadPoolExecutor
+    m, d, tab, x = cfg2
+    W = 3122
+    starts = list(range(0, len(x) - W + 1, 750))
+    with OnlineSpikesV2(d, max_batch=W) as one:
+        ref = [one.process_batch(x[s:s + W], first_sample=s) for s in starts]
+    hs = [OnlineSpikesV2(d, max_batch=W) for _ in range(2)]
+    out = [None] * len(starts)
+
+    def worker(k):
+        for i in range(k, len(starts), 2):
+            out[i] = hs[k].process_batch(x[starts[i]:starts[i] + W], first_sample=starts[i])
+
+    try:
+        for _ in range(2):
+            with ThreadPoolExecutor(2) as pool:
+                list(pool.map(worker, range(2)))
+            for i, (e1, f1) in enumerate(ref):
+                assert len(e1) > 0
+                assert np.array_equal(out[i][0], e1), f"chunk {i}"
+                assert np.array_equal(out[i][1], f1), f"chunk {i}"
+    finally:
+        for h in hs:
+            h.close()
