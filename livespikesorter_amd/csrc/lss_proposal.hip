@@ -138,6 +138,12 @@ struct PnHeadArgs {
 };
 struct PnHeadTable { int n; PnHeadArgs a[PN_MAX_JOBS]; };
 
+// A workgroup takes 256 columns of one group: the 32 hidden rows of its window
+// (256 + 8 columns, zero outside 0..W-1) are staged in LDS with coalesced loads,
+// then each thread sums its column's taps in the same order as a direct loop.
+constexpr int PN_HEAD_COLS = 256;
+constexpr int PN_HEAD_WIN = PN_HEAD_COLS + 2 * PN_PAD;
+
 __global__ __launch_bounds__(256) void k_pn_head(PnHeadTable tb) {
     const PnHeadArgs& A = tb.a[blockIdx.z];
     if (A.active && !*A.active) return;
@@ -146,19 +152,26 @@ __global__ __launch_bounds__(256) void k_pn_head(PnHeadTable tb) {
     float* __restrict__ z = A.z;
     float* __restrict__ p = A.p;
     const int W = A.W, Wp = A.Wp;
+    const int w0 = blockIdx.x * PN_HEAD_COLS, g = blockIdx.y;
+    if (w0 >= Wp) return;                       // whole workgroup past this job's columns
     __shared__ float w4[PN_CH * PN_KW + 1];
+    __shared__ float s_a[PN_CH][PN_HEAD_WIN];
     for (int i = threadIdx.x; i < PN_CH * PN_KW + 1; i += 256) w4[i] = A.params[PN_W4 + i];
+    for (int i = threadIdx.x; i < PN_CH * PN_HEAD_WIN; i += 256) {
+        const int ci = i / PN_HEAD_WIN, c = i - ci * PN_HEAD_WIN;
+        const int j = w0 - PN_PAD + c;
+        s_a[ci][c] = (j >= 0 && j < W) ? (float)a3[((int64_t)g * PN_CH + ci) * Wp + j] : 0.f;
+    }
     __syncthreads();
-    const int w = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
+    const int w = w0 + threadIdx.x;
     if (w >= Wp) return;
     float acc = w4[PN_CH * PN_KW];
     if (w < W) {
         for (int ci = 0; ci < PN_CH; ++ci) {
-            const __bf16* row = a3 + ((int64_t)g * PN_CH + ci) * Wp;
 #pragma unroll
             for (int k = 0; k < PN_KW; ++k) {
                 const int j = w + k - PN_PAD;
-                if (j >= 0 && j < W) acc += w4[ci * PN_KW + k] * (float)row[j];
+                if (j >= 0 && j < W) acc += w4[ci * PN_KW + k] * s_a[ci][threadIdx.x + k];
             }
         }
     }
@@ -459,7 +472,7 @@ void pn_forward_batch(const PnFwdJob* jobs, int n, hipStream_t s) {
     hipLaunchKernelGGL((k_pn_conv<PN_CIN, 0>), grid, dim3(256), 0, s, t1);
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, t2);
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, t3);
-    hipLaunchKernelGGL(k_pn_head, dim3((Wpm + 255) / 256, Gm, n), dim3(256), 0, s, th);
+    hipLaunchKernelGGL(k_pn_head, dim3((Wpm + PN_HEAD_COLS - 1) / PN_HEAD_COLS, Gm, n), dim3(256), 0, s, th);
 }
 
 void pn_gate_batch(const PnGateJob* jobs, int n, hipStream_t s) {
