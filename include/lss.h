@@ -25,8 +25,14 @@ extern "C" {
 #define LSS_ERR_ARG -1       /* bad argument or size beyond what the handle was created for */
 #define LSS_ERR_IO -2        /* oss_input file missing or malformed */
 #define LSS_ERR_HIP -3       /* HIP runtime failure */
-#define LSS_ERR_CAPACITY -4  /* more events than the caller's buffer holds (buffer filled, *n_ev = total) */
+#define LSS_ERR_CAPACITY -4  /* more events than the caller's buffer holds: the buffer holds the first
+                                ev_cap of them, *n_ev = total, and the handle's state is as before
+                                the call (repeat it with a larger buffer) */
 #define LSS_ERR_STATE -5     /* call out of order (e.g. lss_flush before any chunk) */
+#define LSS_ERR_TIMEOUT -6   /* a wait inside the peel kernel timed out: its workgroups were not all
+                                resident at once (e.g. a group graph on a device other work holds) */
+#define LSS_ERR_LIMIT -7     /* a per-batch limit of the handle was exceeded (max_events, or the peel's
+                                per-column-block spike lists); not about the caller's buffer */
 
 /* Peak-acceptance rules.  KILOSORT follows run_matching
  * (KS/template_matching.py:159-198: Cmax > Th^2, exact peak, one template per

@@ -21,6 +21,8 @@ LSS_ERR_IO = -2
 LSS_ERR_HIP = -3
 LSS_ERR_CAPACITY = -4
 LSS_ERR_STATE = -5
+LSS_ERR_TIMEOUT = -6
+LSS_ERR_LIMIT = -7
 
 SEM_KILOSORT = 0
 SEM_LSS = 1

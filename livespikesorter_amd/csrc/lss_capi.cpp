@@ -983,16 +983,18 @@ void peel_geometry(const lss_handle* h, int W, int& CW, int& R, int& S) {
     }
 }
 
-// Check the peel kernel's error flags of one handle after a batch.
+// Check the peel kernel's error flags of one handle after a batch.  A wait that
+// timed out (workgroups not co-resident) is LSS_ERR_TIMEOUT; a full internal
+// spike list is LSS_ERR_LIMIT.  Neither is about the caller's event buffer.
 void check_peel(lss_handle* h, int W) {
     const OutHeader hd = *h->hdr;
     if (hd.err) {
         std::string why;
-        if (hd.err & ERR_BARRIER) why += " barrier-timeout";
-        if (hd.err & ERR_LIST) why += " per-column-group spike list full";
-        if (hd.err & ERR_SLOT) why += " per-iteration slots full";
-        if (hd.err & ERR_STAGE) why += " neighbour staging full";
-        throw LssError(LSS_ERR_CAPACITY, "peel kernel reported:" + why);
+        if (hd.err & ERR_BARRIER) why += " a cross-workgroup wait timed out (peel grid not co-resident);";
+        if (hd.err & ERR_LIST) why += " per-column-block spike list full;";
+        if (hd.err & ERR_SLOT) why += " per-iteration spike slots full;";
+        if (hd.err & ERR_STAGE) why += " neighbour spike staging full;";
+        throw LssError((hd.err & ERR_BARRIER) ? LSS_ERR_TIMEOUT : LSS_ERR_LIMIT, "peel kernel reported:" + why);
     }
     h->last_W = W;
     h->info.batches += 1;
@@ -1073,41 +1075,63 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
 }
 
 // Copy the kernel's events (already sorted by sample) to the caller, applying
-// per-template duplicate suppression when enabled.
-int64_t emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample,
-             bool dedup) {
-    const int64_t n = std::min<int64_t>(h->hdr->count, h->ev_cap);
-    if (h->hdr->count > h->ev_cap) throw LssError(LSS_ERR_CAPACITY, "device event buffer overflow (max_events)");
+// per-template duplicate suppression when enabled.  *n_out = the number of
+// events of the batch.  The handle's state (the dedup table, the event count,
+// the positions of the last call) changes only when they all fit; otherwise
+// the buffer holds the first `cap` of them, the call fails with
+// LSS_ERR_CAPACITY and can be repeated with a larger buffer.
+void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample, bool dedup,
+          int64_t* n_out) {
+    *n_out = 0;
+    if (h->hdr->count > h->ev_cap)
+        throw LssError(LSS_ERR_LIMIT, "the batch has " + std::to_string(h->hdr->count) +
+                                          " events, more than the handle's max_events " + std::to_string(h->ev_cap));
+    const int64_t n = h->hdr->count;
     if (h->pre) h->info.p2p = h->hdr->p2p;
     const int64_t fw = (int64_t)h->nC * h->K;
-    int64_t out = 0, total = 0;
-    h->last_pos.clear();
-    for (int64_t i = 0; i < n; ++i) {
-        lss_event e = h->ev[i];
-        e.sample += h->cur_first;           // the kernel writes samples relative to column 0
-        if (e.sample < lo_sample || e.sample >= hi_sample) continue;
-        if (dedup) {
-            int64_t& last = h->last_spike[e.template_id];
-            if (e.sample - last < h->dt) continue;
-            last = e.sample;
-        }
-        ++total;
-        if (out < cap) {
-            if (ev) ev[out] = e;
-            if (feat) std::memcpy(feat + out * fw, h->feat + i * fw, sizeof(float) * fw);
-            if (h->pos) {
-                h->last_pos.push_back(h->pos[2 * i]);
-                h->last_pos.push_back(h->pos[2 * i + 1]);
+    // pass 1: which events are reported (dedup against a scratch copy of the last times)
+    std::vector<int64_t> pick;
+    pick.reserve((size_t)n);
+    std::vector<std::pair<int, int64_t>> touched;    // template, its last time after this batch
+    {
+        std::map<int, int64_t> last;
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t s = h->ev[i].sample + h->cur_first;   // the kernel writes samples relative to column 0
+            if (s < lo_sample || s >= hi_sample) continue;
+            if (dedup) {
+                const int t = h->ev[i].template_id;
+                auto it = last.find(t);
+                const int64_t prev = it != last.end() ? it->second : h->last_spike[t];
+                if (s - prev < h->dt) continue;
+                last[t] = s;
             }
-            ++out;
+            pick.push_back(i);
         }
+        touched.assign(last.begin(), last.end());
     }
-    h->info.events += out;
-    if (total > cap) {
+    const int64_t total = (int64_t)pick.size();
+    *n_out = total;
+    // pass 2: copy out (the first cap of them when they do not all fit)
+    const int64_t out = std::min(total, cap);
+    for (int64_t k = 0; k < out; ++k) {
+        const int64_t i = pick[k];
+        if (ev) {
+            ev[k] = h->ev[i];
+            ev[k].sample += h->cur_first;
+        }
+        if (feat) std::memcpy(feat + k * fw, h->feat + i * fw, sizeof(float) * fw);
+    }
+    if (total > cap)
         throw LssError(LSS_ERR_CAPACITY, "event buffer holds " + std::to_string(cap) + " of " +
                                              std::to_string(total) + " events");
-    }
-    return out;
+    for (const auto& tl : touched) h->last_spike[tl.first] = tl.second;
+    h->last_pos.clear();
+    if (h->pos)
+        for (int64_t k = 0; k < total; ++k) {
+            h->last_pos.push_back(h->pos[2 * pick[k]]);
+            h->last_pos.push_back(h->pos[2 * pick[k] + 1]);
+        }
+    h->info.events += total;
 }
 
 int16_t* ensure_dx(lss_handle* h, size_t elems) {
@@ -1215,7 +1239,7 @@ int lss_process_batch(lss_handle* h, const int16_t* samples, int64_t n_samples, 
         auto t0 = std::chrono::steady_clock::now();
         *n_ev = 0;
         process_host_batch(h, samples, n_samples, ld, chan_map, first_sample);
-        *n_ev = emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false);
+        emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false, n_ev);
         h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     });
 }
@@ -1225,12 +1249,15 @@ int lss_process_batch_device(lss_handle* h, const int16_t* d_samples, int64_t n_
                              int64_t* n_ev, float* feat) {
     return guarded([&] {
         if (!h || !n_ev || !d_samples || ev_cap < 0) throw LssError(LSS_ERR_ARG, "null argument");
+        if (n_samples <= 0 || n_samples > h->Wmax)
+            throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(n_samples) + " samples outside 1..max_batch " +
+                                            std::to_string(h->Wmax));
         HIPCHK(hipSetDevice(h->dev));
         auto t0 = std::chrono::steady_clock::now();
         *n_ev = 0;
         set_cmap(h, chan_map, ld);
         run_batch(h, d_samples, (int)n_samples, ld, first_sample);
-        *n_ev = emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false);
+        emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false, n_ev);
         h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     });
 }
@@ -1279,7 +1306,7 @@ int lss_process_chunk(lss_handle* h, const int16_t* samples, int64_t n_samples, 
         process_host_batch(h, h->frame.data(), W, C, ident.data(), col0);
         // accepted columns [M, W-M) -> samples [col0 + M, col0 + W - M); report events there
         const int64_t shift = -(M / 2) + h->nt0min;
-        *n_ev = emit(h, ev, ev_cap, feat, col0 + M + shift, col0 + W - M + shift, dedup_on(h));
+        emit(h, ev, ev_cap, feat, col0 + M + shift, col0 + W - M + shift, dedup_on(h), n_ev);
         // keep the last 2M samples as the next frame's left context
         const int64_t keep = std::min<int64_t>(2 * M, W);
         h->hist.assign(h->frame.end() - keep * C, h->frame.end());
@@ -1309,7 +1336,7 @@ int lss_flush(lss_handle* h, lss_event* ev, int64_t ev_cap, int64_t* n_ev, float
         std::iota(ident.begin(), ident.end(), 0);
         process_host_batch(h, h->frame.data(), W, C, ident.data(), col0);
         const int64_t shift = -(M / 2) + h->nt0min;
-        *n_ev = emit(h, ev, ev_cap, feat, col0 + M + shift, col0 + W - M + shift, dedup_on(h));
+        emit(h, ev, ev_cap, feat, col0 + M + shift, col0 + W - M + shift, dedup_on(h), n_ev);
         h->flushed = true;
     });
 }
@@ -1745,14 +1772,25 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         HIPCHK(hipGraphLaunch(L.exec, L.stream));
         HIPCHK(hipStreamSynchronize(L.stream));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        // every stream's results are checked and returned; a failure of one stream
+        // does not hide the others' events.  The call then reports the failed
+        // streams (their n_ev as emit left them), a timeout taking precedence.
+        std::string failed;
+        int code = LSS_OK;
         for (int s2 = 0; s2 < n; ++s2) {
             if (!samples[s2]) continue;
             lss_handle* h = g->hs[s2].get();
             h->cur_first = first_sample[s2];
-            check_peel(h, g->W);
-            n_ev[s2] = emit(h, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr, INT64_MIN, INT64_MAX, false);
             h->info.last_ms = ms;
+            try {
+                check_peel(h, g->W);
+                emit(h, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr, INT64_MIN, INT64_MAX, false, &n_ev[s2]);
+            } catch (const LssError& e) {
+                failed += " stream " + std::to_string(s2) + ": " + e.what() + ";";
+                if (code != LSS_ERR_TIMEOUT) code = e.code;
+            }
         }
+        if (code != LSS_OK) throw LssError(code, "group launch:" + failed);
     });
 }
 

@@ -1,184 +1,220 @@
-"""Template subsetting: pick which preclustered templates a matcher loads.
+"""Choosing which preclustered templates a matcher instance loads.
 
-Mirrors src/Python/subset_templates.py: a channel filter narrows the
-candidates first (:224-256, :183-221), then an activity cap keeps the ``n``
-most active survivors (:87-137), ranked by spike count from
-``spike_templates.npy`` or, without that file, by the L2 norm of each
-template's Wall3 rows.  The kept indices go to
-``OnlineSpikesV2(..., keep=...)`` (``lss_create_subset``), which subsets
-Wall3, ctc and iU at load time; :func:`write_subset` instead writes a reduced
-copy of the directory, subsetting the same tensors on disk as
-``subset_oss_input_inplace`` (:291-347) does.  Errors that the reference
-reports by ``sys.exit`` or ``RuntimeError`` raise :class:`ValueError` here.
+The matcher's cost grows with the number of preclustered templates (the rows
+of ``Wall3.npy``), so a probe can be run on fewer of them.  The selection
+rule is the reference's template-subsetting tool
+(src/Python/subset_templates.py:87-137, 163-256), and the files it reduces
+are the ones its directory rewrite touches (:291-347):
+
+1. *Channel filter* (optional).  A probe-channel interval ``[lo, hi]`` is
+   turned into positions on the templates' channel axis through the
+   directory's channel map; a template is a candidate when its dominant
+   channel lies there (``"peak"``) or when any channel carrying at least a
+   tenth of its largest support does (``"overlap"``).  A channel's support is
+   the peak-to-peak amplitude of the preclustered waveform, or the L2 norm
+   over PCs of ``Wall3`` when the waveforms are not in the directory.
+2. *Activity cap* (optional).  Of the candidates, the ``n`` most active are
+   kept: activity is the number of spikes Kilosort detected with each
+   preclustered template (``spike_detection_templates.npy``, written by
+   KS/io.py:251 and copied into ``oss_input`` by run_online_spikes.py:647),
+   or, when that file is absent, the L2 norm of the template's ``Wall3`` row.
+   Equal activity is broken toward the lower template index (the reference
+   leaves ties to numpy's unstable ``argsort``).
+
+The kept indices either go to ``OnlineSpikesV2(..., keep=...)``
+(``lss_create_subset``: the tensors are reduced at load time) or to
+:func:`write_subset`, which writes a reduced copy of the directory.  Every
+problem is reported as :class:`ValueError`.
 """
 from __future__ import annotations
 
+import dataclasses
 import json
 import shutil
 from pathlib import Path
 
 import numpy as np
 
-RANK_SOURCE = "spike_templates.npy"
+#: preclustered spike ids used for the activity ranking
+DETECTION_IDS = "spike_detection_templates.npy"
 
-# preclustered-template axes of each file (subset_templates.py:47-53)
-UNCLU_T_AXES = {
-    "Wall3.npy": (0,),
-    "ctc.npy": (0, 1),
-    "ctc_permuted.npy": (0, 1),
-    "iU.npy": (0,),
-    "Ucc.npy": (1,),
-    "preclustered_template_waveforms.npy": (0,),
-}
+#: (file, axes whose length is the preclustered template count); every other
+#: file of the directory describes final clusters, channels or PCs and is
+#: copied unchanged
+PRECLUSTER_TENSORS = (
+    ("Wall3.npy", (0,)),                                   # [unclu_T][K][C]
+    ("ctc.npy", (0, 1)),                                   # [unclu_T][unclu_T][2M+1]
+    ("ctc_permuted.npy", (0, 1)),
+    ("iU.npy", (0,)),                                      # [unclu_T]
+    ("Ucc.npy", (1,)),                                     # [nC][unclu_T][K]
+    ("preclustered_template_waveforms.npy", (0,)),         # [unclu_T][M][C]
+)
 
 
-def shapes(oss_dir) -> dict:
-    """T (final clusters), M, C from templates.npy and unclu_T, K from Wall3.npy (:69-84)."""
+@dataclasses.dataclass(frozen=True)
+class ProbeDims:
+    """Sizes of an ``oss_input`` directory.  ``n_pre`` preclustered templates
+    (Wall3 rows), ``n_final`` final clusters (templates.npy rows; equal to
+    ``n_pre`` when templates.npy is absent)."""
+    n_pre: int
+    n_final: int
+    K: int
+    C: int
+    M: int
+
+
+@dataclasses.dataclass
+class Selection:
+    keep: list                 # ascending preclustered indices
+    ranked_by: str             # "spike count", "template L2 norm", or why nothing was ranked
+    channel_filter: dict | None = None
+
+
+def probe_dims(oss_dir) -> ProbeDims:
     d = Path(oss_dir)
-    w = np.load(d / "Wall3.npy", mmap_mode="r")
-    if w.ndim != 3:
-        raise ValueError(f"Wall3.npy expected [unclu_T, K, C], got {w.shape}")
-    unclu_T, K, C = w.shape
-    if (d / "templates.npy").exists():
-        t = np.load(d / "templates.npy", mmap_mode="r")
-        if t.ndim != 3 or t.shape[2] != C:
-            raise ValueError(f"templates.npy {t.shape} does not match Wall3 {w.shape}")
-        T, M = t.shape[0], t.shape[1]
+    wall3 = np.load(d / "Wall3.npy", mmap_mode="r")
+    if wall3.ndim != 3:
+        raise ValueError(f"Wall3.npy has shape {wall3.shape}; a [templates][PCs][channels] array is needed")
+    n_pre, K, C = (int(v) for v in wall3.shape)
+    tpath = d / "templates.npy"
+    if tpath.exists():
+        tmpl = np.load(tpath, mmap_mode="r")
+        if tmpl.ndim != 3 or tmpl.shape[2] != C:
+            raise ValueError(f"templates.npy {tuple(tmpl.shape)} and Wall3.npy {tuple(wall3.shape)} "
+                             f"disagree on the channel count")
+        n_final, M = int(tmpl.shape[0]), int(tmpl.shape[1])
     else:
-        T, M = unclu_T, np.load(d / "wPCA.npy", mmap_mode="r").shape[1]
-    return dict(T=int(T), M=int(M), C=int(C), unclu_T=int(unclu_T), K=int(K))
+        n_final, M = n_pre, int(np.load(d / "wPCA.npy", mmap_mode="r").shape[1])
+    return ProbeDims(n_pre=n_pre, n_final=n_final, K=K, C=C, M=M)
 
 
-def parse_channel_range(s):
-    """``'lo-hi'`` (inclusive probe channel numbers) -> (lo, hi), None when blank (:140-160)."""
-    if s is None or not str(s).strip():
+def parse_channel_range(text):
+    """``"a-b"`` (inclusive probe channel numbers, either order) -> ``(lo, hi)``;
+    ``None`` or blank text -> ``None``."""
+    if text is None:
         return None
-    s = str(s).strip()
-    if "-" not in s:
-        raise ValueError(f"channel range '{s}' must look like 'lo-hi', e.g. '100-150'")
-    a, b = s.split("-", 1)
-    lo, hi = int(a.strip()), int(b.strip())
-    return (lo, hi) if lo <= hi else (hi, lo)
+    text = str(text).strip()
+    if not text:
+        return None
+    lo_txt, sep, hi_txt = text.partition("-")
+    if not sep:
+        raise ValueError(f"cannot read {text!r} as a probe-channel interval; write it as two numbers "
+                         f"joined by '-', such as 96-191")
+    a, b = int(lo_txt), int(hi_txt)
+    return min(a, b), max(a, b)
 
 
-def channels_in_range(oss_dir, c_lo: int, c_hi: int) -> np.ndarray:
-    """C-axis positions j whose probe channel channelMap[j] lies in [c_lo, c_hi] (:163-180)."""
+def _channel_map(d: Path) -> np.ndarray:
+    for name in ("channelMap.npy", "channel_map.npy"):
+        if (d / name).exists():
+            return np.load(d / name, allow_pickle=False).reshape(-1).astype(np.int64)
+    raise ValueError(f"{d} holds no channelMap.npy or channel_map.npy, so probe channel numbers cannot "
+                     f"be placed on the templates' channel axis")
+
+
+def channel_positions(oss_dir, lo: int, hi: int) -> np.ndarray:
+    """Positions j on the templates' channel axis whose probe channel
+    (channel map entry j) lies in ``[lo, hi]``."""
+    cmap = _channel_map(Path(oss_dir))
+    return np.flatnonzero((lo <= cmap) & (cmap <= hi))
+
+
+def channel_support(oss_dir) -> np.ndarray:
+    """Support of each preclustered template on each channel, ``[n_pre][C]``."""
     d = Path(oss_dir)
-    p = d / "channelMap.npy"
-    if not p.exists():
-        p = d / "channel_map.npy"
-    if not p.exists():
-        raise ValueError("need channelMap.npy (or channel_map.npy) to map probe channels onto the template axis")
-    cmap = np.load(p).ravel().astype(np.int64)
-    return np.where((cmap >= c_lo) & (cmap <= c_hi))[0]
+    wpath = d / "preclustered_template_waveforms.npy"
+    if wpath.exists():
+        wave = np.load(wpath, allow_pickle=False)                  # [n_pre][M][C]
+        return np.ptp(wave, axis=1)
+    U = np.load(d / "Wall3.npy", allow_pickle=False).astype(np.float64)   # [n_pre][K][C]
+    return np.sqrt((U * U).sum(axis=1))
 
 
-def templates_on_channels(oss_dir, allowed_chan_idx, criterion: str = "peak", overlap_frac: float = 0.1) -> list:
-    """Preclustered templates with support on the allowed C-axis positions (:183-221).
-
-    Support per channel: peak-to-peak of preclustered_template_waveforms.npy
-    [unclu_T, M, C], else the L2 norm over PCs of Wall3 [unclu_T, K, C].
-    ``peak``: the largest-support channel is allowed; ``overlap``: some
-    channel with support >= overlap_frac x the template's peak is allowed.
-    """
-    d = Path(oss_dir)
-    pre = d / "preclustered_template_waveforms.npy"
-    if pre.exists():
-        w = np.load(pre)
-        amp = w.max(axis=1) - w.min(axis=1)
+def templates_on_channels(oss_dir, positions, rule: str = "peak", frac: float = 0.1) -> list:
+    """Preclustered templates whose support reaches the channel positions given."""
+    sup = channel_support(oss_dir)
+    inside = np.isin(np.arange(sup.shape[1]), np.asarray(positions, np.int64))
+    if rule == "peak":
+        hit = inside[np.argmax(sup, axis=1)]
+    elif rule == "overlap":
+        strong = sup >= frac * sup.max(axis=1)[:, None]
+        hit = (strong & inside).any(axis=1)
     else:
-        amp = np.linalg.norm(np.load(d / "Wall3.npy"), axis=1)
-    allowed = np.zeros(amp.shape[1], dtype=bool)
-    allowed[np.asarray(allowed_chan_idx, dtype=np.int64)] = True
-    if criterion == "peak":
-        keep = np.where(allowed[amp.argmax(axis=1)])[0]
-    elif criterion == "overlap":
-        on = amp >= overlap_frac * amp.max(axis=1, keepdims=True)
-        keep = np.where((on & allowed[None, :]).any(axis=1))[0]
-    else:
-        raise ValueError(f"unknown channel criterion '{criterion}' (peak|overlap)")
-    return [int(t) for t in keep]
+        raise ValueError(f"channel rule {rule!r} is neither 'peak' nor 'overlap'")
+    return np.flatnonzero(hit).tolist()
 
 
-def choose_keep(oss_dir, dims: dict, n: int, metric: str = "spikes", candidates=None):
-    """Sorted kept indices and the ranking used (:87-137).
-
-    Without a cap (``n <= 0``) or with ``n`` at least the number of
-    candidates, every candidate is kept.  Otherwise the candidates are walked
-    in decreasing activity and the first ``n`` kept.
-    """
+def activity(oss_dir, dims: ProbeDims, metric: str = "spikes"):
+    """(score per preclustered template, label): detection counts, or the
+    ``Wall3`` row norm when asked for or when the detection ids are absent."""
     d = Path(oss_dir)
-    unclu_T = dims["unclu_T"]
-    cand = sorted({int(c) for c in (range(unclu_T) if candidates is None else candidates)})
-    if not n or n <= 0 or n >= len(cand):
-        return cand, ("all" if len(cand) == unclu_T else "channel range")
-    order = None
-    if metric == "spikes":
-        p = d / RANK_SOURCE
-        if p.exists():
-            sp = np.load(p).astype(np.int64).ravel()
-            if sp.size and int(sp.max()) < dims["T"]:
-                raise ValueError(f"{RANK_SOURCE} max index {int(sp.max())} < T ({dims['T']}): it looks like "
-                                 f"final-cluster space, not preclustered")
-            sp = sp[(sp >= 0) & (sp < unclu_T)]
-            order = np.argsort(np.bincount(sp, minlength=unclu_T))[::-1]
-            ranked = "spike count"
-        else:
-            metric = "norm"
-    if metric == "norm":
-        energy = np.linalg.norm(np.load(d / "Wall3.npy").reshape(unclu_T, -1), axis=1)
-        order = np.argsort(energy)[::-1]
-        ranked = "template L2 norm"
-    if order is None:
-        raise ValueError(f"unknown metric '{metric}' (spikes|norm)")
-    cs = set(cand)
-    return sorted([int(i) for i in order if int(i) in cs][:n]), ranked
+    if metric not in ("spikes", "norm"):
+        raise ValueError(f"activity metric {metric!r} is neither 'spikes' nor 'norm'")
+    ids_path = d / DETECTION_IDS
+    if metric == "spikes" and ids_path.exists():
+        ids = np.load(ids_path, allow_pickle=False).reshape(-1).astype(np.int64)
+        # preclustered ids run up to n_pre - 1 >= n_final; a largest id below the final
+        # cluster count means the file holds final-cluster ids
+        if ids.size and ids.max() < dims.n_final:
+            raise ValueError(f"{DETECTION_IDS}: largest id {int(ids.max())} is below the {dims.n_final} final "
+                             f"clusters, so these are final-cluster ids; preclustered ids are needed")
+        ids = ids[(ids >= 0) & (ids < dims.n_pre)]
+        return np.bincount(ids, minlength=dims.n_pre).astype(np.float64), "spike count"
+    rows = np.load(d / "Wall3.npy", allow_pickle=False).astype(np.float64).reshape(dims.n_pre, -1)
+    return np.sqrt((rows * rows).sum(axis=1)), "template L2 norm"
 
 
-def compute_keep(oss_dir, n: int = 0, metric: str = "spikes", channel_range=None,
-                 channel_criterion: str = "peak"):
-    """(keep, ranked, channel metadata): channel filter first, then the cap (:224-256)."""
-    dims = shapes(oss_dir)
-    candidates, meta = None, None
+def select_templates(oss_dir, n: int = 0, metric: str = "spikes", channel_range=None,
+                     channel_rule: str = "peak") -> Selection:
+    """Channel filter first, then the activity cap of ``n`` (``n <= 0``: no cap)."""
+    dims = probe_dims(oss_dir)
+    pool = np.arange(dims.n_pre)
+    info = None
     if channel_range:
-        c_lo, c_hi = channel_range
-        idx = channels_in_range(oss_dir, c_lo, c_hi)
-        if idx.size == 0:
-            raise ValueError(f"no probe channels fall in range [{c_lo}, {c_hi}]")
-        candidates = templates_on_channels(oss_dir, idx, channel_criterion)
-        if not candidates:
-            raise ValueError(f"no preclustered templates have support on channels [{c_lo}, {c_hi}]")
-        meta = {"channel_range": [int(c_lo), int(c_hi)], "channel_criterion": channel_criterion,
-                "probe_channels_in_range": int(idx.size), "templates_on_channels": len(candidates)}
-    keep, ranked = choose_keep(oss_dir, dims, n, metric, candidates)
-    return keep, ranked, meta
-
-
-def subset_axes(arr: np.ndarray, axes, keep, unclu_T: int, name: str) -> np.ndarray:
-    """Take ``keep`` along each listed axis, checking its length is unclu_T (:278-288)."""
-    for ax in axes:
-        ra = ax if ax >= 0 else arr.ndim + ax
-        if arr.shape[ra] != unclu_T:
-            raise ValueError(f"{name}: axis {ax} length {arr.shape[ra]} != unclu_T {unclu_T}")
-        arr = np.take(arr, np.asarray(keep, np.int64), axis=ra)
-    return arr
+        lo, hi = channel_range
+        pos = channel_positions(oss_dir, lo, hi)
+        if pos.size == 0:
+            raise ValueError(f"the channel map has no probe channel in [{lo}, {hi}]")
+        pool = np.asarray(templates_on_channels(oss_dir, pos, channel_rule), np.int64)
+        if pool.size == 0:
+            raise ValueError(f"no preclustered template has its support on probe channels [{lo}, {hi}]")
+        info = {"channel_range": [int(lo), int(hi)], "channel_criterion": channel_rule,
+                "probe_channels_in_range": int(pos.size), "templates_on_channels": int(pool.size)}
+    if n is None or n <= 0 or n >= pool.size:
+        why = "channel range" if pool.size < dims.n_pre else "all"
+        return Selection(pool.tolist(), why, info)
+    score, label = activity(oss_dir, dims, metric)
+    # most active first; equal scores: lower index first
+    order = np.lexsort((pool, -score[pool]))
+    return Selection(np.sort(pool[order[:n]]).tolist(), label, info)
 
 
 def write_subset(src, dst, keep, *, allow_below_t: bool = False) -> dict:
-    """Copy ``src`` to ``dst`` with the preclustered tensors subset to ``keep``
-    (subset_oss_input_inplace, :291-347, on a copy); writes subset_manifest.json."""
+    """Copy directory ``src`` to ``dst`` and cut every preclustered tensor down
+    to the templates ``keep``; the final-cluster files stay as they are.  A
+    ``subset_manifest.json`` records the selection."""
     src, dst = Path(src), Path(dst)
-    dims = shapes(src)
-    keep = sorted(int(k) for k in keep)
-    if len(keep) < dims["T"] and not allow_below_t:
-        raise ValueError(f"subset of {len(keep)} templates is below the final-cluster count {dims['T']}; "
-                         f"pass allow_below_t=True (this matcher's relabelling scans every centroid)")
+    dims = probe_dims(src)
+    idx = np.unique(np.asarray(list(keep), np.int64))
+    if idx.size and (idx[0] < 0 or idx[-1] >= dims.n_pre):
+        raise ValueError(f"template indices must lie in [0, {dims.n_pre})")
+    if idx.size < dims.n_final and not allow_below_t:
+        raise ValueError(f"keeping {idx.size} preclustered templates leaves fewer than the {dims.n_final} "
+                         f"final clusters; pass allow_below_t=True to accept that (relabelling here scans "
+                         f"every centroid, so it stays correct)")
     shutil.copytree(src, dst)
-    for name, axes in UNCLU_T_AXES.items():
-        p = dst / name
-        if p.exists():
-            np.save(p, subset_axes(np.load(p, allow_pickle=False), axes, keep, dims["unclu_T"], name))
-    manifest = {"original_unclu_T": dims["unclu_T"], "new_unclu_T": len(keep), "T_unchanged": dims["T"],
-                "kept_indices": keep}
+    for name, axes in PRECLUSTER_TENSORS:
+        path = dst / name
+        if not path.exists():
+            continue
+        arr = np.load(path, allow_pickle=False)
+        for ax in axes:
+            if arr.shape[ax] != dims.n_pre:
+                raise ValueError(f"{name}: axis {ax} has length {arr.shape[ax]}, not the {dims.n_pre} "
+                                 f"preclustered templates")
+            arr = arr.take(idx, axis=ax)
+        np.save(path, arr)
+    manifest = {"original_unclu_T": dims.n_pre, "new_unclu_T": int(idx.size), "T_unchanged": dims.n_final,
+                "kept_indices": idx.tolist()}
     (dst / "subset_manifest.json").write_text(json.dumps(manifest, indent=2))
     return manifest

@@ -309,11 +309,19 @@ def make_recording(model: ProbeModel, n_samples: int, seed: int | None = None,
     if workers > 1 and len(jobs) > 1:
         import multiprocessing as mp
         _POOL_STATE.update(model=model, truth=truth, noise=noise)
+        # the workers are let finish and joined (close + join), never terminated:
+        # terminate() signals them, and a signal handler they inherited (a
+        # profiler's, say) would report that as an abort
+        pool = mp.get_context("fork").Pool(workers)
         try:
-            with mp.get_context("fork").Pool(workers) as pool:
-                for b0, q in pool.imap_unordered(_render_job, jobs, chunksize=2):
-                    out[b0:b0 + len(q)] = q
+            for b0, q in pool.imap_unordered(_render_job, jobs, chunksize=2):
+                out[b0:b0 + len(q)] = q
+            pool.close()
+        except BaseException:
+            pool.terminate()
+            raise
         finally:
+            pool.join()
             _POOL_STATE.clear()
     else:
         for b0, b1, sq in jobs:

@@ -88,8 +88,9 @@ def test_load_time_subset_matches_subset_directory(model_dirs, tmp_path):
     from livespikesorter_amd import subset
     from livespikesorter_amd.matcher import LssError
     m, full = model_dirs("cfg2")
-    keep, ranked, _ = subset.compute_keep(full, 200, "norm")
-    assert len(keep) == 200 and ranked == "template L2 norm"
+    sel = subset.select_templates(full, 200, "norm")
+    keep = sel.keep
+    assert len(keep) == 200 and sel.ranked_by == "template L2 norm"
     sub = tmp_path / "sub"
     subset.write_subset(full, sub, keep, allow_below_t=True)
     x, _ = synth.make_recording(m, 3000 * 4 + 122, seed=synth.TEST_SEED)
