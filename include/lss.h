@@ -118,7 +118,7 @@ typedef struct lss_info {
     int64_t proj_kblocks_dense;  /* the same without exact-zero skipping */
     int32_t precision;      /* LSS_PREC_FP32 or LSS_PREC_BF16_RESCORE */
     float p2p;                   /* peak-to-peak of the last batch's raw samples (preprocessing on) */
-    int64_t rescored;       /* BF16 mode: local maxima rescored in FP32 (running total) */
+    int64_t rescored;       /* BF16 mode: (column, template) scores rescored in FP32 (running total) */
     int64_t rescore_flips;  /* BF16 mode: of those, accept/reject decisions the rescore changed */
     int32_t peel_cw;        /* last batch's peel geometry: columns per workgroup, */
     int32_t peel_rows;      /*   score rows per thread held in registers (0: in memory), */
@@ -272,6 +272,13 @@ int lss_group_process_lane(struct lss_group* g, int32_t lane, const int16_t* con
                            int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
                            const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
 int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
+/* Device time of the launches on one lane of a group created with cfg->profile
+ * set, measured by HIP events recorded inside the launch graph: ms[0] the
+ * stages in front of the peel (input pointers, preprocessing, whitening,
+ * proposal network and gate, PC conv, projection), ms[1] the peel kernel
+ * (all streams' peels in one grid); *launches = launches timed.  reset != 0
+ * clears the accumulators after reading.  No reference counterpart. */
+int lss_group_stage_times(struct lss_group* g, int32_t lane, double* ms, int64_t* launches, int32_t reset);
 /* lss_set_proposal for one stream of a group (the launch graph is rebuilt). */
 int lss_group_set_proposal(struct lss_group* g, int32_t stream, struct lss_proposal* net, float lo, float hi,
                            int32_t reach);

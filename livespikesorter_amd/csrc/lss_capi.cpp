@@ -175,6 +175,7 @@ struct lss_handle {
     // BF16 projection operands, its unsubtracted result, and the rescore counters
     uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
     float* d_Binit = nullptr;
+    float* d_Nsq = nullptr;          // per 64-template tile and column: squared norm of the BF16 operand
     unsigned long long* d_rescnt = nullptr;
     // proposal gate (lss_set_proposal): the network, this handle's activations and the tile mask
     lss_proposal* prop = nullptr;
@@ -252,7 +253,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_rescnt, d_ptag, d_epoch,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_Nsq, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
@@ -642,6 +643,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     if (h->bf16) {
         h->d_Bpcb = dalloc<uint16_t>((size_t)KCP * h->BS);
         h->d_Binit = dalloc<float>((size_t)TP * h->BS);
+        h->d_Nsq = dalloc<float>((size_t)(TP / GEMM_BM) * h->BS);
         h->d_rescnt = dalloc<unsigned long long>(2);
         HIPCHK(hipMemset(h->d_rescnt, 0, 2 * sizeof(unsigned long long)));
     }
@@ -868,6 +870,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
         pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
         pj.Bmb = reinterpret_cast<const __bf16*>(h->d_Bpcb);
         pj.C2 = h->d_Binit;
+        pj.Nsq = h->d_Nsq;
     }
     pj.mtiles = h->TP / GEMM_BM;
     pj.ntiles = (W + GEMM_BN - 1) / GEMM_BN;
@@ -914,6 +917,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
         p.U2 = h->d_U2; p.ldu = h->KCP;
         p.kb_ptr = h->d_kbp_ptr; p.kb_idx = h->d_kbp_idx;
         p.Binit = h->d_Binit;
+        p.Nsq = h->d_Nsq;
         p.n_rescored = h->d_rescnt;
     }
     bp.R = R; bp.CW = CW; bp.G = G;
@@ -994,6 +998,7 @@ void check_peel(lss_handle* h, int W) {
         if (hd.err & ERR_LIST) why += " per-column-block spike list full;";
         if (hd.err & ERR_SLOT) why += " per-iteration spike slots full;";
         if (hd.err & ERR_STAGE) why += " neighbour spike staging full;";
+        if (hd.err & ERR_BAND) why += " FP32 rescoring pair list full;";
         throw LssError((hd.err & ERR_BARRIER) ? LSS_ERR_TIMEOUT : LSS_ERR_LIMIT, "peel kernel reported:" + why);
     }
     h->last_W = W;
@@ -1568,6 +1573,11 @@ struct GroupLane {
     ProjectBatch pb{};
     PeelBatch kb{};
     ResetBatch rb{};
+    // cfg.profile: events recorded inside the graph at the tick's start and around the
+    // peel, and the device time they measured on this lane (front stages, peel)
+    hipEvent_t ev[3] = {};
+    double ms[2] = {};
+    int64_t timed = 0;
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1580,12 +1590,15 @@ constexpr int GROUP_LANES = 4;
 struct lss_group {
     std::vector<std::unique_ptr<lss_handle>> hs;
     int n = 0, W = 0, R = 0, CW = 0, dev = 0;
+    bool profile = false;
     std::vector<int> S;   // per stream: template subsets per column block in the peel grid
     GroupLane lanes[GROUP_LANES];
     ~lss_group() {
         (void)hipSetDevice(dev);
         for (auto& L : lanes) {
             L.drop_graph();
+            for (auto& e : L.ev)
+                if (e) (void)hipEventDestroy(e);
             if (L.h_xptr) (void)hipHostFree(L.h_xptr);
             if (L.d_xptr) (void)hipFree(L.d_xptr);
             if (L.stream) (void)hipStreamDestroy(L.stream);
@@ -1629,6 +1642,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     L.drop_graph();
     hipStream_t st = L.stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    if (g->profile) HIPCHK(hipEventRecord(L.ev[0], st));
     HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
     launch_preprocess(pre, st);
     launch_whiten(L.wb, &L.rb, st);
@@ -1648,7 +1662,9 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     }
     launch_pcconv(L.cb, g->hs[0]->K, st);
     launch_project(L.pb, g->hs[0]->bf16, st);
+    if (g->profile) HIPCHK(hipEventRecord(L.ev[1], st));
     HIPCHK(launch_peel(L.kb, g->R, false, st));
+    if (g->profile) HIPCHK(hipEventRecord(L.ev[2], st));
     HIPCHK(hipStreamEndCapture(st, &L.graph));
     HIPCHK(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
     L.ld_captured.assign(ld, ld + n);
@@ -1722,7 +1738,10 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
             throw LssError(LSS_ERR_ARG, "the group's peel grid (" + std::to_string(gsum) +
                                             " workgroups) is not co-resident on this device");
         HIPCHK(hipSetDevice(g->dev));
+        g->profile = cfg->profile != 0;
         for (auto& L : g->lanes) {
+            if (g->profile)
+                for (auto& e : L.ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
             HIPCHK(hipHostMalloc((void**)&L.h_xptr, sizeof(void*) * MAX_JOBS, hipHostMallocDefault));
             HIPCHK(hipMalloc((void**)&L.d_xptr, sizeof(void*) * MAX_JOBS));
@@ -1772,6 +1791,14 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         HIPCHK(hipGraphLaunch(L.exec, L.stream));
         HIPCHK(hipStreamSynchronize(L.stream));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (g->profile) {
+            float a = 0.f, b = 0.f;
+            HIPCHK(hipEventElapsedTime(&a, L.ev[0], L.ev[1]));
+            HIPCHK(hipEventElapsedTime(&b, L.ev[1], L.ev[2]));
+            L.ms[0] += a;
+            L.ms[1] += b;
+            L.timed += 1;
+        }
         // every stream's results are checked and returned; a failure of one stream
         // does not hide the others' events.  The call then reports the failed
         // streams (their n_ev as emit left them), a timeout taking precedence.
@@ -1791,6 +1818,22 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
             }
         }
         if (code != LSS_OK) throw LssError(code, "group launch:" + failed);
+    });
+}
+
+int lss_group_stage_times(lss_group* g, int32_t lane, double* ms, int64_t* launches, int32_t reset) {
+    return guarded([&] {
+        if (!g || !ms || !launches) throw LssError(LSS_ERR_ARG, "null argument");
+        if (lane < 0 || lane >= GROUP_LANES) throw LssError(LSS_ERR_ARG, "no such lane");
+        if (!g->profile) throw LssError(LSS_ERR_STATE, "the group was created without profile");
+        GroupLane& L = g->lanes[lane];
+        ms[0] = L.ms[0];
+        ms[1] = L.ms[1];
+        *launches = L.timed;
+        if (reset) {
+            L.ms[0] = L.ms[1] = 0.0;
+            L.timed = 0;
+        }
     });
 }
 

@@ -23,6 +23,11 @@ constexpr int MAX_M = 63;
 constexpr int MAX_K = 8;
 constexpr int MAX_STAGE = 512;  // spikes of one iteration in a workgroup's neighbourhood
 constexpr int MAX_LIST = 512;   // spikes per workgroup over a whole batch
+constexpr int MAX_BAND = 512;   // FP32 rescoring: (column, template) pairs per round in a workgroup
+// BF16 error bound factor: |B_bf16 - B_fp32| <= BF16_ERR * ||U_t|| * ||Bpc_bf16 over the tile's
+// K-blocks||: two operand roundings of 2^-8 (product 2^-7 (1 + 2^-9)), the norm taken of the
+// rounded operand (1 / (1 - 2^-8)), and FP32 accumulation over <= 4096 terms (2^-12)
+constexpr float BF16_ERR = 0.0081f;
 
 // Grid barrier words: a top counter, per-group arrival counters and per-group
 // release words, each on its own 256-byte line.  Grids of at least BAR_HIER_MIN
@@ -52,6 +57,7 @@ constexpr int ERR_BARRIER = 1;
 constexpr int ERR_LIST = 2;
 constexpr int ERR_SLOT = 4;
 constexpr int ERR_STAGE = 8;
+constexpr int ERR_BAND = 16;
 
 struct alignas(16) Spike {   // one accepted spike inside the peel kernel
     int w;          // batch column
@@ -119,16 +125,23 @@ struct PeelParams {
     int half; int nt0min;
     float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
     unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch
-    // FP32 rescoring of a BF16 projection (rescore != 0): a column whose best
-    // score reaches lo2 gets that score recomputed as the FP32 dot product of U2
-    // row t with Bpc column w (over the listed K-blocks of t's 64-row tile) minus
-    // the BF16 value Binit[t][w], added to its current score before it is
-    // published.
+    // FP32 rescoring of a BF16 projection (rescore != 0): in a column whose best
+    // BF16 score reaches lo2, that template's score and the score of every
+    // template whose BF16 score lies within the BF16 error bound of it are
+    // recomputed as the FP32 dot product of U2 row t with Bpc column w (over the
+    // listed K-blocks of t's 64-row tile) minus the BF16 value Binit[t][w], added
+    // to the current score; the column publishes the FP32 maximum of those.
     int rescore;
     float lo2;
     const float* U2; int ldu;          // [TP][K*CP], FP32
     const int* kb_ptr; const int* kb_idx;
     const float* Binit;                // the BF16 projection before any subtraction
+    // Nsq[m][w] (row stride BS): squared norm of the BF16 PC projection of column w
+    // over the K-blocks listed for 64-template tile m.  |B_bf16 - B_fp32| of
+    // template t is at most BF16_ERR * sqrt(nm[t] * Nsq[t / 64][w]), so every
+    // template whose BF16 score could be the column's FP32 maximum is known and
+    // rescored too.
+    const float* Nsq;
     unsigned long long* n_rescored;    // running totals: FP32 rescores, threshold decisions flipped
     // proposal gate: score tile (64 columns, 64 templates) scored iff gate[(w/64) * gate_mt + t/64]
     const uint8_t* gate; int gate_mt;
@@ -182,6 +195,8 @@ struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
     const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
     float* C2;                            // optional second copy of the result
+    float* Nsq;                           // optional (BF16): per 64-row tile and column, the squared
+                                          // norm of the BF16 B operand over the tile's listed K-blocks
     const uint8_t* gate; int gate_mt;     // optional tile mask [ntile][gate_mt]: 0 = not computed
     const int16_t* const* active;         // optional: the job is skipped when *active is null
     int lda, ldb, ldc, mtiles, ntiles;

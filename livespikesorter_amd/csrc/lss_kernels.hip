@@ -355,6 +355,11 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    // squared norm of each column's BF16 operand over the tile's listed K-blocks
+    // (four threads per column, 16 of each step's 64 rows each): it bounds the
+    // BF16 error of every score of the tile (used by the peel's FP32 rescoring)
+    float nsq = 0.f;
+    const int ncol = tid >> 2, npart = tid & 3;
     const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
     for (int p = p0; p < p1; p += 4) {
         // gather: sub-block q of this step is K-block kb_idx[p + q] (zeros past the list end)
@@ -391,9 +396,21 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
             const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + (lane & 31)][kk * 16 + 8 * (lane >> 5)]);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
+        if (jb.Nsq) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const float v = (float)Bs[ncol][npart * 16 + e];
+                nsq += v * v;
+            }
+        }
     }
     store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
     if (jb.C2) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
+    if (jb.Nsq) {
+        nsq += __shfl_xor(nsq, 1);
+        nsq += __shfl_xor(nsq, 2);
+        if (npart == 0) jb.Nsq[(int64_t)mt * jb.ldc + n0 + ncol] = nsq;
+    }
 }
 
 // nm[t] = sum_{k,c} U[t,k,c]^2 accumulated in FP64, stored as FP64 and rounded FP32;
@@ -588,7 +605,9 @@ constexpr int PRE = 4;   // neighbour spikes fetched speculatively with the coun
 
 __device__ __forceinline__ uint64_t rt_now() { return __builtin_amdgcn_s_memrealtime(); }
 
-template <int R>
+// RS: the FP32 rescoring of a BF16 projection is compiled in (a separate
+// instance, so the FP32 matcher's register allocation does not carry it).
+template <int R, bool RS>
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
@@ -625,17 +644,21 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ int s_flag[CWX];
     __shared__ int s_rt[CWX];                  // FP32 rescoring: template whose correction is cached
     __shared__ float s_rd[CWX];                //   and the correction (FP32 score - BF16 score)
+    __shared__ float s_bcor[RS ? MAX_BAND : 1];  // FP32 rescoring: each listed pair's correction
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ __align__(16) int s_hcnt_w[PEEL_THREADS / 64];     // per-wave counts (read as int4)
     __shared__ __align__(16) int s_hcnt_o[PEEL_THREADS / 64];
-    __shared__ int s_qlo[R == 0 ? MAX_STAGE : 1], s_qhi[R == 0 ? MAX_STAGE : 1];   // staged spikes' row bands
-    __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];   // per-wave list of band-hitting spikes
-    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk;
+    // staged spikes' row bands (R == 0; the FP32 rescoring's pair list after phase A)
+    __shared__ int s_qlo[R == 0 ? MAX_STAGE : 1], s_qhi[R == 0 ? MAX_STAGE : 1];
+    // per-wave list of band-hitting spikes (R > 0; the FP32 rescoring's pair list after phase A)
+    __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];
+    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk, s_nband;
     __shared__ int s_wt[NACC];                 // local acceptance: winning template per column
     __shared__ float s_wb[NACC];               //   and its B value
     int* const s_rank = reinterpret_cast<int*>(s_val);
     static_assert(NWIN >= CWX + 2 * MAX_M + 2, "the LSS pooling window fits the score array");
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
+    static_assert(2 * MAX_BAND <= PEEL_THREADS && MAX_BAND <= MAX_STAGE, "rescoring pairs fit the reused arrays");
 
     const int g = blockIdx.x - bt.g_start[sj], tid = threadIdx.x;
     // Workgroup g owns column block cb = g / S and template subset sb = g % S:
@@ -1047,47 +1070,26 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     if (it == 0 && p.dbg_cfmax && S == 1) { p.dbg_cfmax[ww] = bv; p.dbg_imax[ww] = bI; }
                 }
             }
-            if (p.rescore) {
+            if (RS && p.rescore) {
                 // BF16 projection: every column whose best score reaches lo2 (10% under
                 // threshold) is rescored in FP32 before it is published, so pooling,
-                // acceptance and amplitudes see FP32 scores wherever they can matter
-                // (a column under lo2 in BF16 cannot beat an accepted one in FP32).
-                // The correction of (t, w) is fixed (subtractions are applied to both
-                // alike), so it is computed once per (column, template) and cached.
-                // The columns' own threads (tid < CW, written just above) list the
-                // columns that need a dot product; one barrier publishes the list.
+                // acceptance, the template choice and amplitudes see FP32 scores
+                // wherever they can matter (a column under lo2 in BF16 cannot beat an
+                // accepted one in FP32).  The correction of (t, w) is fixed
+                // (subtractions are applied to both alike).
                 const int lane = tid & 63, wv = tid >> 6;
-                bool need = false;
-                if (tid < CW && c0 + tid < c1) need = s_cf[tid] >= p.lo2 && s_rt[tid] != s_im[tid];
-                const uint64_t bal = __ballot(need);
-                if (tid < CW && lane == 0) s_hcnt_w[wv] = __popcll(bal);
-                if (CW > 64) __syncthreads();            // wave counts of other waves
-                int nres = 0, base = 0;
-                if (tid < CW) {
-                    for (int k = 0; k < (CW + 63) / 64; ++k) {
-                        if (k == wv) base = nres;
-                        nres += s_hcnt_w[k];
-                    }
-                    if (need) s_flag[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
-                    if (tid == 0) s_ncand = nres;
-                }
-                __syncthreads();
-                nres = s_ncand;
-                for (int q = wv; q < nres; q += PEEL_THREADS / 64) {
-                    const int cl = s_flag[q];
-                    const int t = s_im[cl], ww = c0 + cl;
-                    // FP32 dot of U2 row t with Bpc column ww over the K-blocks listed for
-                    // t's 64-row tile (the others are exact zeros)
+                // FP32 dot of U2 row t with Bpc column ww over the K-blocks listed for t's
+                // 64-row tile (the others are exact zeros); every lane of the wave ends with
+                // the same, order-fixed sum.  8 elements per lane in flight at once: the
+                // block index, then the template weight and the (scattered) Bpc entry.
+                auto dot32 = [&](int t, int ww) {
                     const int mt = t / GEMM_BM;
                     const int e0 = p.kb_ptr[mt] * GEMM_BK, e1 = p.kb_ptr[mt + 1] * GEMM_BK;
                     const float* urow = p.U2 + (int64_t)t * p.ldu;
-                    // 8 elements per lane in flight at once: the block index, then the
-                    // template weight and the (scattered) Bpc column entry
                     float acc = 0.f;
                     for (int e = e0 + lane; e < e1; e += 64 * 8) {
                         float uq[8], bq[8];
                         int kc[8];
-                        // the 8 block indices, then the 16 loads they address, each group in flight at once
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int ee = e + 64 * q;
@@ -1101,31 +1103,131 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
 #pragma unroll
                         for (int q = 0; q < 8; ++q) acc += uq[q] * bq[q];
                     }
-                    // butterfly: every lane ends with the same, order-fixed sum
                     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-                    if (lane == 0) {
-                        s_rd[cl] = acc - p.Binit[(int64_t)t * p.BS + ww];
-                        s_rt[cl] = t;
-                    }
-                }
-                if (nres > 0) __syncthreads();
-                int flip = 0;
+                    return acc;
+                };
+                // Every template whose score could be the column's FP32 maximum is
+                // rescored: a BF16 score b16 has |b16 - b32| <= BF16_ERR * sqrt(nm) * N (N:
+                // the norm of the column's BF16 operand over the template's tile), so in
+                // the sqrt(Cf) scale each template is within BF16_ERR * N of its FP32 value,
+                // and the FP32 maximum lies among the templates with sqrt(Cf16) +
+                // BF16_ERR * N >= sqrt(Cf16 of the BF16 best) - BF16_ERR * N(best).  Those
+                // (column, template) pairs are listed with their current BF16 score (at most
+                // MAX_BAND per workgroup and iteration; more is reported as an error),
+                // rescored one wave each (the correction of the column's last winner is
+                // cached), and the column keeps the FP32 maximum (larger score, then the
+                // smaller file index).
+                float cf16 = 0.f;                        // the column's best BF16 score
                 if (tid < CW && c0 + tid < c1) {
-                    float cf = s_cf[tid];
-                    if (cf >= p.lo2) {
-                        const int t = s_im[tid];
-                        const float b = s_bv[tid] + s_rd[tid];
-                        const float r = fmaxf(b, 0.f);
-                        const float c32 = (r * r) / p.nm[t];
-                        flip = (cf > p.Th2) != (c32 > p.Th2);
-                        s_bv[tid] = b;
-                        s_cf[tid] = c32;
+                    cf16 = s_cf[tid];
+                    float sl = -1.f;
+                    if (cf16 >= p.lo2) {
+                        const float nb = sqrtf(p.Nsq[(int64_t)(s_im[tid] / GEMM_BM) * p.BS + c0 + tid]);
+                        sl = fmaxf(sqrtf(cf16) - BF16_ERR * nb, 0x1p-60f);
+                    }
+                    s_cm[tid] = sl;                      // the band's floor in the sqrt(Cf) scale
+                }
+                int* const s_bi = R > 0 ? s_whit : s_qlo;        // column | template << 16
+                float* const s_bb = R > 0 ? reinterpret_cast<float*>(s_whit) + MAX_BAND
+                                          : reinterpret_cast<float*>(s_qhi);   // current BF16 score
+                if (tid == 0) s_nband = 0;
+                __syncthreads();
+                {
+                    const float sl = own ? s_cm[col] : -1.f;
+                    if (own && !edge && !woff && sl > 0.f) {
+                        if (R > 0 && rbase + rg * R < T) {     // (rows past T are padding)
+                            const int r0 = rbase + rg * R;
+                            const float nn = sqrtf(p.Nsq[(int64_t)(r0 / GEMM_BM) * p.BS + w]);
+                            const float e = sl - BF16_ERR * nn;
+                            const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
+                            const float* rnp = s_nm + rg * R;
+                            // the rows' candidate bits first (branch-free, 8 rows between
+                            // compiler barriers so the norm loads are not all hoisted), then
+                            // the listed ones, each row's value picked by a select chain (no
+                            // register-array indexing)
+                            uint64_t mask = 0;
+#pragma unroll
+                            for (int i0 = 0; i0 < (R > 0 ? R : 1); i0 += 8) {
+#pragma unroll
+                                for (int i = i0; i < i0 + 8 && i < (R > 0 ? R : 1); ++i) {
+                                    const float r = breg[i] > 0.f ? breg[i] : 0.f;
+                                    const float a = (r * r) * rnp[i];
+                                    mask |= (uint64_t)(a >= thr && r0 + i < T) << i;
+                                }
+                                asm volatile("" ::: "memory");
+                            }
+                            while (mask) {
+                                const int i = __builtin_ctzll(mask);
+                                mask &= mask - 1ull;
+                                const int k = atomicAdd(&s_nband, 1);
+                                if (k >= MAX_BAND) break;
+                                float v = 0.f;
+#pragma unroll
+                                for (int i2 = 0; i2 < (R > 0 ? R : 1); ++i2) v = i2 == i ? breg[i2] : v;
+                                s_bi[k] = col | ((r0 + i) << 16);
+                                s_bb[k] = v;
+                            }
+                        } else if (R == 0) {
+                            for (int t = rg; t < T; t += nrg) {
+                                if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) continue;
+                                const float b = p.B[(int64_t)t * p.BS + w];
+                                const float r = fmaxf(b, 0.f);
+                                const float a = (r * r) * p.rnm[t];
+                                const float nn = sqrtf(p.Nsq[(int64_t)(t / GEMM_BM) * p.BS + w]);
+                                const float e = sl - BF16_ERR * nn;
+                                const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
+                                if (a >= thr) {
+                                    const int k = atomicAdd(&s_nband, 1);
+                                    if (k >= MAX_BAND) break;
+                                    s_bi[k] = col | (t << 16);
+                                    s_bb[k] = b;
+                                }
+                            }
+                        }
                     }
                 }
-                const uint64_t fb = __ballot(flip);
-                if (lane == 0 && wv == 0) {
-                    if (nres > 0) atomicAdd(&p.n_rescored[0], (unsigned long long)nres);
+                __syncthreads();
+                const int nb = min(s_nband, MAX_BAND);
+                if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
+                for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
+                    const int cl = s_bi[q] & 0xffff, t = s_bi[q] >> 16, ww = c0 + cl;
+                    float corr;
+                    if (s_rt[cl] == t) {
+                        corr = s_rd[cl];
+                    } else {
+                        const float acc = dot32(t, ww);
+                        corr = acc - p.Binit[(int64_t)t * p.BS + ww];
+                    }
+                    if (lane == 0) s_bcor[q] = corr;
                 }
+                __syncthreads();
+                if (tid < CW && c0 + tid < c1 && s_cm[tid] > 0.f) {
+                    float bv = -1.f, bb = 0.f, bc = 0.f;
+                    int bt = 0, bo = 0x7fffffff;
+                    for (int q = 0; q < nb; ++q) {
+                        const int e = s_bi[q];
+                        if ((e & 0xffff) != tid) continue;
+                        const int t = e >> 16;
+                        const float c = s_bcor[q];
+                        const float b = s_bb[q] + c;
+                        const float r = fmaxf(b, 0.f);
+                        const float cf = (r * r) / p.nm[t];
+                        const int o = p.orig[t];
+                        if (better(cf, o, bv, bo)) { bv = cf; bo = o; bt = t; bb = b; bc = c; }
+                    }
+                    if (bo != 0x7fffffff) {
+                        s_cf[tid] = bv;
+                        s_im[tid] = bt;
+                        s_bv[tid] = bb;
+                        s_rt[tid] = bt;
+                        s_rd[tid] = bc;
+                    }
+                }
+                const int listed = nb;
+                int flip = 0;
+                if (tid < CW && c0 + tid < c1 && cf16 >= p.lo2) flip = (cf16 > p.Th2) != (s_cf[tid] > p.Th2);
+                const uint64_t fb = __ballot(flip);
+                if (tid == 0 && listed > 0) atomicAdd(&p.n_rescored[0], (unsigned long long)listed);
                 if (lane == 0 && fb) atomicAdd(&p.n_rescored[1], (unsigned long long)__popcll(fb));
             }
         }
@@ -1644,14 +1746,17 @@ void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* 
     hipLaunchKernelGGL(k_norms, dim3(T), dim3(256), 0, s, U, KC, nm, rnm, nm64);
 }
 
-static const void* peel_fn(int R) {
+static const void* peel_fn(int R, bool rescore) {
+#define LSS_PEEL_FN(RR)                                                                           \
+    return rescore ? reinterpret_cast<const void*>(k_peel<RR, true>) : reinterpret_cast<const void*>(k_peel<RR, false>)
     switch (R) {
-        case 8: return reinterpret_cast<const void*>(k_peel<8>);
-        case 16: return reinterpret_cast<const void*>(k_peel<16>);
-        case 32: return reinterpret_cast<const void*>(k_peel<32>);
-        case 64: return reinterpret_cast<const void*>(k_peel<64>);
-        default: return reinterpret_cast<const void*>(k_peel<0>);
+        case 8: LSS_PEEL_FN(8);
+        case 16: LSS_PEEL_FN(16);
+        case 32: LSS_PEEL_FN(32);
+        case 64: LSS_PEEL_FN(64);
+        default: LSS_PEEL_FN(0);
     }
+#undef LSS_PEEL_FN
 }
 
 int peel_rows(int T, int CW) {
@@ -1666,14 +1771,16 @@ hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t 
     PeelBatch copy = b;
     void* args[] = {&copy};
     const int G = b.g_start[b.n];
-    if (cooperative)
-        return hipLaunchCooperativeKernel(peel_fn(R), dim3(G), dim3(PEEL_THREADS), args, 0, s);
-    return hipLaunchKernel(peel_fn(R), dim3(G), dim3(PEEL_THREADS), args, 0, s);
+    const void* fn = peel_fn(R, b.p[0].rescore != 0);
+    if (cooperative) return hipLaunchCooperativeKernel(fn, dim3(G), dim3(PEEL_THREADS), args, 0, s);
+    return hipLaunchKernel(fn, dim3(G), dim3(PEEL_THREADS), args, 0, s);
 }
 
 int peel_max_blocks(int device, int R) {
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_fn(R), PEEL_THREADS, 0) != hipSuccess) return 0;
+    int per_cu = 0, per_cu2 = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_fn(R, false), PEEL_THREADS, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, peel_fn(R, true), PEEL_THREADS, 0) != hipSuccess) return 0;
+    per_cu = per_cu < per_cu2 ? per_cu : per_cu2;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return per_cu * cus;
 }

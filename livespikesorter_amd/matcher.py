@@ -204,12 +204,13 @@ class StreamGroup:
 
     def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
                  semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True,
-                 precision: int = PREC_FP32, preprocess: int = 0, peel_split: int = 0):
+                 precision: int = PREC_FP32, preprocess: int = 0, peel_split: int = 0, profile: bool = False):
         self._g = ct.c_void_p()
         self.n = len(oss_input_dirs)
         cfg = default_config(device=device, chunk=batch, max_batch=batch, max_iter=max_iter,
                              semantics=semantics, max_events=max_events, sparse=int(sparse),
-                             precision=precision, preprocess=preprocess, peel_split=peel_split)
+                             precision=precision, preprocess=preprocess, peel_split=peel_split,
+                             profile=int(profile))
         dirs = (ct.c_char_p * self.n)(*[os.fsencode(str(d)) for d in oss_input_dirs])
         check(lib().lss_group_create(self.n, dirs, ct.byref(cfg), ct.byref(self._g)))
         self.batch = batch
@@ -219,6 +220,13 @@ class StreamGroup:
         info = Info()
         check(lib().lss_group_query(self._g, stream, ct.byref(info)))
         return info
+
+    def stage_times(self, lane: int = 0, reset: bool = False):
+        """(front-stage ms, peel ms, launches) accumulated on one lane (needs ``profile=True``)."""
+        ms = (ct.c_double * 2)()
+        n = ct.c_int64(0)
+        check(lib().lss_group_stage_times(self._g, lane, ms, ct.byref(n), int(reset)))
+        return ms[0], ms[1], n.value
 
     def set_proposal(self, stream: int, net, lo: float = 0.25, hi: float = 0.75, reach: int = 4) -> None:
         """Gate one stream's score tiles with a proposal network (``None`` detaches)."""

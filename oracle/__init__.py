@@ -71,15 +71,17 @@ class OracleResult:
     B: np.ndarray | None = None
     Cfmax: np.ndarray | None = None
     imax: np.ndarray | None = None
+    margins: np.ndarray | None = None   # [3][W] decision margins (threshold, template, pooling)
 
 
 def run_batch(x: np.ndarray, Wr, wPCA, U, ctc, iCC, iU, *, Th: float = 8.0, nC: int = 10,
               max_iter: int = 100, semantics: int = SEM_KILOSORT, bits: int = 64,
               threads: int = 0, chan_map=None, stages: bool = False, cap: int | None = None,
-              features: bool = True, xf: np.ndarray | None = None) -> OracleResult:
+              features: bool = True, xf: np.ndarray | None = None, margins: bool = False) -> OracleResult:
     """Run one batch (``x`` int16 [W][ld]) through the oracle.  With ``xf``
     (preprocessed input [C][W], FP64, channel-gathered) ``x`` is ignored and
-    whitening starts from ``xf`` (see oracle.preprocess)."""
+    whitening starts from ``xf`` (see oracle.preprocess).  ``margins``: also
+    return the peel's decision margins per column (oracle.c, ``peel_m``)."""
     lib = _lib(bits)
     real = np.float64 if bits == 64 else np.float32
     creal = ct.c_double if bits == 64 else ct.c_float
@@ -111,8 +113,10 @@ def run_batch(x: np.ndarray, Wr, wPCA, U, ctc, iCC, iU, *, Th: float = 8.0, nC: 
     sB = np.zeros((T, W), real) if stages else None
     sCf = np.zeros(W, real) if stages else None
     sIm = np.zeros(W, np.int32) if stages else None
+    mg = np.zeros((3, W), real) if margins else None
     outs = (ct.c_long(cap), _p(ow, ct.c_long), _p(ot, ct.c_int), _p(oa, creal), _p(oi, ct.c_int), _p(of, creal),
-            ct.byref(nit), _p(sXw, creal), _p(sBpc, creal), _p(sB, creal), _p(sCf, creal), _p(sIm, ct.c_int))
+            ct.byref(nit), _p(sXw, creal), _p(sBpc, creal), _p(sB, creal), _p(sCf, creal), _p(sIm, ct.c_int),
+            _p(mg, creal))
     tabs = (_p(Wr, ct.c_float), _p(wPCA, ct.c_float), _p(U, ct.c_float), _p(ctc, ct.c_float),
             _p(iCC, ct.c_int64), _p(iU, ct.c_int64))
     if xf is not None:
@@ -129,7 +133,7 @@ def run_batch(x: np.ndarray, Wr, wPCA, U, ctc, iCC, iU, *, Th: float = 8.0, nC: 
         raise RuntimeError("oracle spike capacity exceeded")
     return OracleResult(ow[:n].copy(), ot[:n].copy(), oa[:n].copy(), oi[:n].copy(),
                         of[:n].copy() if features else np.zeros((n, nC, K), real),
-                        int(nit.value), sXw, sBpc, sB, sCf, sIm)
+                        int(nit.value), sXw, sBpc, sB, sCf, sIm, mg)
 
 
 def run_model_batch(model_dir_tables: dict, x: np.ndarray, **kw) -> OracleResult:
@@ -139,11 +143,15 @@ def run_model_batch(model_dir_tables: dict, x: np.ndarray, **kw) -> OracleResult
                      Th=t["Th"], nC=t["nC"], **kw)
 
 
-def load_tables(path: str | os.PathLike) -> dict:
-    """Read the oss_input arrays the oracle needs (plain numpy, no pickles)."""
+def load_tables(path: str | os.PathLike, ctc: np.ndarray | None = None) -> dict:
+    """Read the oss_input arrays the oracle needs (plain numpy, no pickles).
+    ``ctc``: use this table instead of the directory's ctc.npy (for a
+    directory written without it)."""
     p = Path(path)
-    out = {k: np.load(p / f"{k}.npy", allow_pickle=False)
-           for k in ("whiteningMat", "wPCA", "Wall3", "ctc", "iCC", "iU", "Ucc")}
+    names = ("whiteningMat", "wPCA", "Wall3", "iCC", "iU", "Ucc") + (("ctc",) if ctc is None else ())
+    out = {k: np.load(p / f"{k}.npy", allow_pickle=False) for k in names}
+    if ctc is not None:
+        out["ctc"] = np.asarray(ctc, np.float32)
     misc = {}
     for line in (p / "misc.txt").read_text().splitlines():
         if ":" in line:
@@ -204,10 +212,11 @@ def lit_scores(B, nm, M):
 
 def peel(B: np.ndarray, nm: np.ndarray, ctc: np.ndarray, U: np.ndarray, wPCA: np.ndarray, *,
          Th: float = 8.0, max_iter: int = 100, semantics: int = SEM_KILOSORT, bits: int = 64,
-         cap: int = 100000):
+         cap: int = 100000, margins: bool = False):
     """The peel loop alone on a given score matrix ``B [T][W]`` (modified copy returned).
 
-    Returns (w, t, amp, iter, n_iter, first_Cfmax, first_imax, B_after).
+    Returns (w, t, amp, iter, n_iter, first_Cfmax, first_imax, B_after), and the
+    decision margins [3][W] after them when ``margins`` is set.
     """
     lib = _lib(bits)
     real = np.float64 if bits == 64 else np.float32
@@ -228,15 +237,17 @@ def peel(B: np.ndarray, nm: np.ndarray, ctc: np.ndarray, U: np.ndarray, wPCA: np
     nit = ct.c_int(0)
     cf = np.zeros(W, real)
     im = np.zeros(W, np.int32)
-    fn = getattr(lib, f"orc{bits}_peel")
+    mg = np.zeros((3, W), real) if margins else None
+    fn = getattr(lib, f"orc{bits}_peel_m")
     fn.restype = ct.c_long
     n = fn(ct.byref(prm), _p(Bc, creal), None, ct.c_long(W), _p(nm, creal), _p(ctc, ct.c_float),
            _p(U, ct.c_float), _p(wPCA, ct.c_float), ct.c_long(cap), _p(ow, ct.c_long),
            _p(ot, ct.c_int), _p(oa, creal), _p(oi, ct.c_int), ct.byref(nit), _p(cf, creal),
-           _p(im, ct.c_int))
+           _p(im, ct.c_int), _p(mg, creal))
     if n < 0:
         raise RuntimeError("oracle spike capacity exceeded")
-    return ow[:n], ot[:n], oa[:n], oi[:n], int(nit.value), cf, im, Bc
+    out = (ow[:n], ot[:n], oa[:n], oi[:n], int(nit.value), cf, im, Bc)
+    return out + (mg,) if margins else out
 
 
 def stage_conv_project(x: np.ndarray, Wr, wPCA, U, *, bits: int = 64, threads: int = 0):

@@ -146,16 +146,43 @@ void ORC(norms)(const orc_params *p, const float *U, REAL *nm)
  * out_w (batch column), out_t, out_amp, out_iter.  Returns the spike count,
  * or -1 on capacity overflow.  *n_iter receives the number of iterations that
  * found spikes. */
+/* Decision margins (margins nullable, [3][W], each the minimum over the
+ * iterations run): how close the peel came, at column w, to deciding
+ * differently -- [0] threshold: |Cfmax - Th^2| / Th^2 where w is (within
+ * MARGIN_NEAR of) the peak of its pooling window; [1] template: (best - second
+ * best Cf) / best; [2] pooling: |Cfmax[w] - max of the other columns of the
+ * window| / the larger of the two.  [1] and [2] count only where the window's
+ * scores reach Th^2 (1 - MARGIN_NEAR); elsewhere they stay +inf.  Test support:
+ * a spike-list difference between two runs is a near tie when a margin around
+ * it is below the runs' score error. */
+#define MARGIN_NEAR 1e-3
+long ORC(peel_m)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
+                 const float *ctc, const float *U, const float *wPCA,
+                 long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, int *n_iter,
+                 REAL *first_Cfmax, int *first_imax, REAL *margins);
+
 long ORC(peel)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
                const float *ctc, const float *U, const float *wPCA,
                long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, int *n_iter,
                REAL *first_Cfmax, int *first_imax)
 {
+    return ORC(peel_m)(p, B, Xres, W, nm, ctc, U, wPCA, cap, out_w, out_t, out_amp, out_iter, n_iter,
+                       first_Cfmax, first_imax, NULL);
+}
+
+long ORC(peel_m)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
+                 const float *ctc, const float *U, const float *wPCA,
+                 long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, int *n_iter,
+                 REAL *first_Cfmax, int *first_imax, REAL *margins)
+{
     const int T = p->T, M = p->M, K = p->K, C = p->C, L = 2 * M + 1, h = M / 2;
     const int lss = p->semantics == SEM_LSS;
     const REAL Th2 = (REAL)(p->Th * p->Th);
     REAL *Cfmax = (REAL *)malloc(sizeof(REAL) * W);
+    REAL *Cf2 = (REAL *)malloc(sizeof(REAL) * W);     /* second largest Cf of each column */
     REAL *Cmax = (REAL *)malloc(sizeof(REAL) * W);
+    if (margins)
+        for (long i = 0; i < 3 * W; ++i) margins[i] = INFINITY;
     int *imax = (int *)malloc(sizeof(int) * W);
     REAL *rnm = (REAL *)malloc(sizeof(REAL) * T);
     long *pw = (long *)malloc(sizeof(long) * (W + 1) * (lss ? T : 1));
@@ -182,7 +209,7 @@ long ORC(peel)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
          * Cfmax, imax = max over templates, first index on ties (:167) */
         #pragma omp parallel for schedule(static) num_threads(NTHR(p))
         for (long w = 0; w < W; ++w) {
-            REAL best = lss ? -FLT_MAX : -INFINITY;
+            REAL best = lss ? -FLT_MAX : -INFINITY, second = -INFINITY;
             int bi = 0;
             const int edge = (w < M || w >= W - M);
             for (int t = 0; t < T; ++t) {
@@ -192,9 +219,11 @@ long ORC(peel)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
                     b = b > 0 ? b : 0;
                     v = lss ? (b * b) * rnm[t] : (b * b) / nm[t];
                 }
-                if (v > best) { best = v; bi = t; }
+                if (v > best) { second = best; best = v; bi = t; }
+                else if (v > second) second = v;
             }
             Cfmax[w] = best;
+            Cf2[w] = second;
             imax[w] = bi;
         }
         if (it == 0 && first_Cfmax) {
@@ -207,6 +236,33 @@ long ORC(peel)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
             const long lo = w - M < 0 ? 0 : w - M, hi = w + M >= W ? W - 1 : w + M;
             for (long j = lo; j <= hi; ++j) if (Cfmax[j] > m) m = Cfmax[j];
             Cmax[w] = m;
+        }
+        if (margins) {
+            #pragma omp parallel for schedule(static) num_threads(NTHR(p))
+            for (long w = M; w < W - M; ++w) {
+                const REAL cf = Cfmax[w], cm = Cmax[w];
+                REAL other = -INFINITY;     /* largest Cfmax of the window's other columns */
+                const long lo = w - M < 0 ? 0 : w - M, hi = w + M >= W ? W - 1 : w + M;
+                for (long j = lo; j <= hi; ++j) if (j != w && Cfmax[j] > other) other = Cfmax[j];
+                REAL *mg = margins;
+                if (cm > 0 && (cm - cf) <= (REAL)MARGIN_NEAR * cm) {
+                    REAL m0 = cf - Th2;
+                    m0 = (m0 < 0 ? -m0 : m0) / Th2;
+                    if (m0 < mg[w]) mg[w] = m0;
+                }
+                if (cm >= Th2 * (REAL)(1 - MARGIN_NEAR)) {
+                    if (cf > 0) {
+                        const REAL m1 = (cf - Cf2[w]) / cf;
+                        if (m1 < mg[W + w]) mg[W + w] = m1;
+                    }
+                    const REAL big = cf > other ? cf : other;
+                    if (big > 0) {
+                        REAL m2 = cf - other;
+                        m2 = (m2 < 0 ? -m2 : m2) / big;
+                        if (m2 < mg[2 * W + w]) mg[2 * W + w] = m2;
+                    }
+                }
+            }
         }
         /* acceptance */
         long n = 0;
@@ -280,7 +336,7 @@ long ORC(peel)(const orc_params *p, REAL *B, REAL *Xres, long W, const REAL *nm,
         }
     }
     *n_iter = it_found;
-    free(Cfmax); free(Cmax); free(imax); free(rnm); free(pw); free(pt); free(pa); free(wave);
+    free(Cfmax); free(Cf2); free(Cmax); free(imax); free(rnm); free(pw); free(pt); free(pa); free(wave);
     return nsp;
 }
 
@@ -318,19 +374,19 @@ static long ORC(match_whitened)(const orc_params *p, REAL *Xw, long W, const flo
                                 const float *ctc, const int64_t *iCC, const int64_t *iU,
                                 long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
                                 int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
-                                REAL *stage_Cfmax, int *stage_imax);
+                                REAL *stage_Cfmax, int *stage_imax, REAL *margins);
 
 long ORC(run_batch)(const orc_params *p, const int16_t *x, long W, long ld, const int *chan_map,
                     const float *Wr, const float *wPCA, const float *U, const float *ctc,
                     const int64_t *iCC, const int64_t *iU,
                     long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
                     int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
-                    REAL *stage_Cfmax, int *stage_imax)
+                    REAL *stage_Cfmax, int *stage_imax, REAL *margins)
 {
     REAL *Xw = (REAL *)malloc(sizeof(REAL) * (size_t)p->C * W);
     ORC(whiten)(p, x, W, ld, chan_map, Wr, Xw);
     return ORC(match_whitened)(p, Xw, W, wPCA, U, ctc, iCC, iU, cap, out_w, out_t, out_amp, out_iter, out_feat,
-                               n_iter, stage_Xw, stage_Bpc, stage_B, stage_Cfmax, stage_imax);
+                               n_iter, stage_Xw, stage_Bpc, stage_B, stage_Cfmax, stage_imax, margins);
 }
 
 /* The same batch from a preprocessed input xf [C][W] (FP64, channel-gathered):
@@ -340,7 +396,7 @@ long ORC(run_batch_f)(const orc_params *p, const double *xf, long W, const float
                       const float *U, const float *ctc, const int64_t *iCC, const int64_t *iU,
                       long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
                       int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
-                      REAL *stage_Cfmax, int *stage_imax)
+                      REAL *stage_Cfmax, int *stage_imax, REAL *margins)
 {
     const int C = p->C;
     REAL *Xw = (REAL *)malloc(sizeof(REAL) * (size_t)C * W);
@@ -356,7 +412,7 @@ long ORC(run_batch_f)(const orc_params *p, const double *xf, long W, const float
         }
     }
     return ORC(match_whitened)(p, Xw, W, wPCA, U, ctc, iCC, iU, cap, out_w, out_t, out_amp, out_iter, out_feat,
-                               n_iter, stage_Xw, stage_Bpc, stage_B, stage_Cfmax, stage_imax);
+                               n_iter, stage_Xw, stage_Bpc, stage_B, stage_Cfmax, stage_imax, margins);
 }
 
 /* Xw is taken over (freed here). */
@@ -364,7 +420,7 @@ static long ORC(match_whitened)(const orc_params *p, REAL *Xw, long W, const flo
                                 const float *ctc, const int64_t *iCC, const int64_t *iU,
                                 long cap, long *out_w, int *out_t, REAL *out_amp, int *out_iter, REAL *out_feat,
                                 int *n_iter, REAL *stage_Xw, REAL *stage_Bpc, REAL *stage_B,
-                                REAL *stage_Cfmax, int *stage_imax)
+                                REAL *stage_Cfmax, int *stage_imax, REAL *margins)
 {
     const int C = p->C, K = p->K, T = p->T, nC = p->nC;
     REAL *Bpc = (REAL *)malloc(sizeof(REAL) * (size_t)C * K * W);
@@ -380,8 +436,8 @@ static long ORC(match_whitened)(const orc_params *p, REAL *Xw, long W, const flo
     int *tt = (int *)malloc(sizeof(int) * (cap + 1));
     REAL *ta = (REAL *)malloc(sizeof(REAL) * (cap + 1));
     int *ti = (int *)malloc(sizeof(int) * (cap + 1));
-    long n = ORC(peel)(p, B, Xw, W, nm, ctc, U, wPCA, cap, tw, tt, ta, ti, n_iter,
-                       stage_Cfmax, stage_imax);
+    long n = ORC(peel_m)(p, B, Xw, W, nm, ctc, U, wPCA, cap, tw, tt, ta, ti, n_iter,
+                         stage_Cfmax, stage_imax, margins);
     if (n > 0) {
         /* stable counting sort by column */
         long *cnt = (long *)calloc((size_t)W + 1, sizeof(long));

@@ -14,41 +14,14 @@ import oracle
 from livespikesorter_amd import synth
 from livespikesorter_amd.matcher import (OnlineSpikesV2, STAGE_B, STAGE_BPC, STAGE_CFMAX, STAGE_IMAX,
                                          STAGE_NM, STAGE_XW)
-from tests.parity import agreement
+from tests.parity import agreement, check_events
 
 pytestmark = pytest.mark.gpu
 
-AMP_RTOL = 1e-3
-FEAT_RTOL = 1e-3
-
 
 def _oracle(tables, x, **kw):
+    kw.setdefault("margins", True)
     return oracle.run_model_batch(tables, x, **kw)
-
-
-def check_events(ref, ev, feat, *, M=61, nt0min=20, first_sample=0, label=""):
-    """Compare GPU events with an oracle result (columns -> Kilosort sample convention)."""
-    ref_s = ref.w + first_sample - M // 2 + nt0min
-    ident, within, pairs = agreement(ref_s, ref.t, ev["sample"], ev["template"], tol=1)
-    n = max(len(ref_s), len(ev))
-    print(f"[{label}] oracle {len(ref_s)} gpu {len(ev)} identical {ident:.5f} within1 {within:.5f}")
-    assert ident >= 0.995, f"{label}: identical fraction {ident}"
-    assert within >= 0.999, f"{label}: within-1 fraction {within}"
-    ri = np.array([p[0] for p in pairs])
-    gi = np.array([p[1] for p in pairs])
-    exact = ref_s[ri] == ev["sample"][gi]
-    ri, gi = ri[exact], gi[exact]
-    amp_err = np.abs(ev["amplitude"][gi] - ref.amp[ri]) / np.abs(ref.amp[ri])
-    print(f"[{label}] amp rel err max {amp_err.max():.3e}")
-    assert amp_err.max() <= AMP_RTOL
-    if feat is not None and len(ri):
-        fr = ref.feat[ri]
-        fg = feat[gi].astype(np.float64)
-        scale = np.abs(fr).reshape(len(ri), -1).max(1)
-        ferr = np.abs(fg - fr).reshape(len(ri), -1).max(1) / scale
-        print(f"[{label}] feature err max {ferr.max():.3e}")
-        assert ferr.max() <= FEAT_RTOL
-    return ident, within
 
 
 @pytest.fixture(scope="module")

@@ -16,7 +16,7 @@ import oracle
 from livespikesorter_amd import LssError, PRE_ALL, PRE_CAR, PRE_HIGHPASS, PRE_MEAN, SEM_KILOSORT, SEM_LSS, synth
 from livespikesorter_amd.matcher import STAGE_PRE, STAGE_PRE_MEAN, STAGE_PRE_MED, OnlineSpikesV2, StreamGroup
 from oracle import preprocess as pp
-from tests.test_gpu_matcher import check_events
+from tests.parity import check_events
 
 pytestmark = pytest.mark.gpu
 

@@ -326,3 +326,37 @@ def test_closest_cluster_brute_force_and_ties():
     for i in range(20):
         d = [((x[i] - cx) ** 2 + (y[i] - cy) ** 2) ** 0.5 for cx, cy in cent]
         assert got[i] == min(range(40), key=lambda k: (d[k], k))
+
+
+def test_peel_decision_margins_known_ties():
+    """The oracle's decision margins on a hand-built score matrix (no
+    subtraction: zero ctc, one iteration).  Column 150 sits exactly on the
+    threshold (Cf = 64 = Th^2), column 400 has two templates with equal scores,
+    columns 700 and 710 are equal peaks of one pooling window, column 900 is a
+    clean spike (Cf 144, nothing else near)."""
+    T, K, C, M, W = 2, 6, 4, 61, 1000
+    B = np.zeros((T, W))
+    B[0, 150] = 8.0
+    B[0, 400] = B[1, 400] = 10.0
+    B[0, 700] = B[0, 710] = 9.0
+    B[1, 900] = 12.0
+    B[1, 260] = 5.0
+    nm = np.ones(T)
+    ctc = np.zeros((T, T, 2 * M + 1), np.float32)
+    U = np.zeros((T, K, C), np.float32)
+    wPCA = np.zeros((K, M), np.float32)
+    w, t, amp, it, n_iter, cf, im, _, mg = oracle.peel(B, nm, ctc, U, wPCA, max_iter=1, margins=True)
+    assert mg.shape == (3, W)
+    assert mg[0, 150] == 0.0                              # threshold: |64 - 64| / 64
+    assert mg[1, 400] == 0.0                              # template: (100 - 100) / 100
+    assert mg[2, 700] == 0.0 and mg[2, 710] == 0.0        # pooling: equal peaks
+    assert mg[0, 900] == pytest.approx((144 - 64) / 64)
+    assert mg[1, 900] == 1.0 and mg[2, 900] == 1.0        # runner-up template and column at 0
+    # a window far below threshold (Cf 25 at 260) counts no template or pooling tie,
+    # and its column is no threshold decision either once Cf < Cmax
+    assert np.isinf(mg[1, 260]) and np.isinf(mg[2, 260])
+    assert np.isinf(mg[0, 240]) and mg[0, 260] == pytest.approx((64 - 25) / 64)
+    # Kilosort rules: 150 is not accepted (> Th^2 needed); the tie at 400 goes to template 0
+    got = sorted(zip(w.tolist(), t.tolist()))
+    assert (150, 0) not in got and (400, 0) in got and (900, 1) in got
+    assert (700, 0) in got and (710, 0) in got            # equal pooled maxima: both accepted
