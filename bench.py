@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
 """Headline benchmark of the MI355X matcher.
 
-Workload (one GPU): the plain FP32 matcher on a 384-channel, 512-template
-probe (Gaussian footprints sigma = 2 channels, rates log-uniform 0.5-20 Hz,
-amplitudes N(12, 3), banded-correlation whitening), streamed as
-3000-sample chunks framed by 61-sample margins (W = 3122 columns).  One
-step = one chunk through the whole path (int16 -> whitening -> PC conv ->
-projection -> peel loop -> events + PC features on the host).  The recording
-is 60 s of synthetic data held in HBM before the timed region.
+Default workload (one GPU): 8 probe streams of 384 channels x 1024 templates
+(Gaussian footprints, rates log-uniform 0.5-20 Hz, amplitudes N(12, 3),
+banded-correlation whitening), 1500-sample chunks framed by 61-sample
+margins (W = 1622 columns).  One step = one tick: a chunk of every stream
+matched by one graph launch (int16 -> whitening -> proposal network and gate
+-> PC conv -> BF16 projection -> peel loop with FP32 rescoring -> events + PC
+features on the host).  The recordings are held in HBM before the timed
+region.  Beside the gated BF16 group (``value``) the same ticks run through
+the plain FP32 group, the BF16 group without the gate and the plain FP32
+matcher one stream at a time, for agreement and speedups.
 
-Prints one JSON line: realtime factor (seconds of data per wall second),
-p50/p99 chunk latency, the dominant kernel's roofline figure and the CPU
-baseline (the FP32 CPU restatement of the same algorithm on the host cores,
-over a bounded sample of the same chunks).
+Prints one JSON line: aggregate realtime factor (seconds of data per wall
+second), p50/p99 chunk latency from an int16 hand-off on the host to events
+on the host, agreement with the plain FP32 matcher, the peel kernel's
+roofline figure and the CPU baseline (the FP32 C restatement of the same
+algorithm on the host cores, over a bounded sample of the same chunks).  The
+plain FP32 single-stream workload (384 ch x 512 templates, 3000-sample
+chunks) follows as secondary keys; ``--workload cfg2|group|bf16`` runs one of
+the other workloads alone.
 """
 from __future__ import annotations
 
@@ -184,16 +191,20 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
-def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed, lanes=2):
+def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed, lanes=2, host_recs=None):
     """Stream the group's recordings in real time: chunk k of stream s is handed
     over at k * N / FS plus a uniform +-5 ms jitter.  Whenever chunks are
     waiting and a launch lane is free, one launch on that lane matches the
     oldest waiting chunk of every stream that has one and is not in flight
     (the others idle); with two lanes, a chunk arriving during a launch starts
-    at once on the other lane.  Latency = events on the host - hand-over."""
+    at once on the other lane.  Latency = events on the host - hand-over.
+    With ``host_recs`` (the int16 recordings in host memory) the chunks are
+    handed over from the host, so the latency includes their copy to the GPU;
+    otherwise they are taken from ``bufs`` in device memory."""
     from concurrent.futures import ThreadPoolExecutor
     S = grp.n
     period = N / FS
+    W = grp.batch
     rng = np.random.default_rng(seed)
     n_chunks = min(n_ticks, int(seconds / period))
     arrive = np.arange(n_chunks)[:, None] * period + 0.005 + rng.uniform(-0.005, 0.005, size=(n_chunks, S))
@@ -204,9 +215,14 @@ def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed, lanes=2):
     inflight = {}
     t0 = time.perf_counter()
 
-    def work(lane, ptrs, first):
+    def work(lane, ks, first):
         ts = time.perf_counter()
-        grp.process(ptrs, first, device=True, lds=[C] * S, lane=lane)
+        if host_recs is not None:
+            grp.process([host_recs[s][k * N:k * N + W] if k is not None else None for s, k in enumerate(ks)],
+                        first, lane=lane)
+        else:
+            grp.process([bufs[s].ptr + k * N * C * 2 if k is not None else None for s, k in enumerate(ks)], first,
+                        device=True, lds=[C] * S, lane=lane)
         return ts, time.perf_counter()
 
     with ThreadPoolExecutor(lanes) as pool:
@@ -223,13 +239,13 @@ def paced_latency(grp, bufs, n_ticks, N, C, seconds, seed, lanes=2):
             ready = [s for s in range(S) if not busy_s[s] and nxt[s] < n_chunks and arrive[nxt[s], s] <= now]
             if ready and free:
                 lane = free.pop(0)
-                ptrs = [bufs[s].ptr + nxt[s] * N * C * 2 if s in ready else None for s in range(S)]
+                ks_all = [nxt[s] if s in ready else None for s in range(S)]
                 first = [nxt[s] * N for s in range(S)]
                 ks = [nxt[s] for s in ready]
                 for s in ready:
                     busy_s[s] = True
                     nxt[s] += 1
-                inflight[lane] = (pool.submit(work, lane, ptrs, first), ready, ks)
+                inflight[lane] = (pool.submit(work, lane, ks_all, first), ready, ks)
                 launches += 1
                 continue
             time.sleep(0.0002)
@@ -259,12 +275,61 @@ def make_target_data(cache: Path, seconds: float, n_streams: int = 8):
     return m, d, recs
 
 
+def peel_bytes(T, W, M, nC, iters, spikes):
+    """Algorithmic HBM bytes of one stream's peel over one batch: the score tile
+    read per iteration, the column maxima exchanged per iteration, the ctc rows
+    and score updates per spike, the feature reads per spike."""
+    return iters * 4.0 * T * W + iters * 8.0 * W + spikes * T * (2 * M + 1) * 12.0 + spikes * nC * M * 4.0
+
+
+def cpu_baseline_legs(tab, chunks, N, budget_s, threads):
+    """The FP32 C restatement of the matcher (oracle/liborc32) timed on host cores
+    over chunks of the workload: all threads and one thread, under Kilosort's
+    rules (100 iterations) and under the CUDA matcher's (50 iterations, integer
+    Th).  Each leg runs until its share of the budget is spent."""
+    import oracle
+    legs = {}
+    plan = [("kilosort", oracle.SEM_KILOSORT, 100, threads), ("kilosort_1thread", oracle.SEM_KILOSORT, 100, 1),
+            ("lss", oracle.SEM_LSS, 50, threads), ("lss_1thread", oracle.SEM_LSS, 50, 1)]
+    for name, sem, max_iter, th in plan:
+        spent, done = 0.0, 0
+        Th = float(int(tab["Th"])) if sem == oracle.SEM_LSS else tab["Th"]
+        for batch in chunks:
+            t1 = time.perf_counter()
+            oracle.run_batch(batch, tab["whiteningMat"], tab["wPCA"], tab["Wall3"], tab["ctc"], tab["iCC"], tab["iU"],
+                             Th=Th, nC=tab["nC"], max_iter=max_iter, semantics=sem, bits=32, threads=th)
+            spent += time.perf_counter() - t1
+            done += 1
+            if spent >= budget_s / len(plan):
+                break
+        legs[name] = {"x_realtime": round(done * N / FS / spent, 4), "threads": th, "chunks": done,
+                      "seconds": round(spent, 2), "max_iter": max_iter}
+        log(f"cpu baseline {name}: {legs[name]}")
+    return legs
+
+
+def host_cpu_info():
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        txt = Path("/proc/cpuinfo").read_text()
+        names = [l.split(":", 1)[1].strip() for l in txt.splitlines() if l.startswith("model name")]
+        info["model"] = names[0] if names else None
+        info["sockets"] = len({l for l in txt.splitlines() if l.startswith("physical id")}) or None
+    except OSError:
+        pass
+    return info
+
+
 def bench_target(args) -> dict:
-    """``--streams`` (default 8) probe streams of 384 channels x 1024 templates, 1500-sample chunks: the
-    plain FP32 matcher one stream at a time, and one graph launch per tick for
-    the plain FP32, BF16 and gated BF16 (trained proposal network) matchers;
-    throughput, paced latency under +-5 ms arrival jitter, and spike agreement
-    with the plain FP32 matcher."""
+    """``--streams`` (default 8) probe streams of 384 channels x 1024 templates,
+    1500-sample chunks with 61-sample margins (W = 1622): the plain FP32 matcher
+    one stream at a time (the reference's arrangement, one sorter per probe),
+    and one graph launch per tick for the plain FP32, BF16 and gated BF16
+    (trained proposal network) matchers.  Per mode: aggregate realtime factor
+    with the recordings resident in HBM, paced latency with chunks handed over
+    from host memory under +-5 ms arrival jitter, the peel kernel's device time
+    (HIP events inside the graph) and spike agreement with the plain FP32
+    matcher."""
     from livespikesorter_amd import OnlineSpikesV2, StreamGroup
     from livespikesorter_amd._hip import DeviceBuffer, synchronize
     from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_FP32
@@ -280,8 +345,9 @@ def bench_target(args) -> dict:
     log(f"target data ready in {time.time() - t0:.1f}s: {S} streams x {n_ticks} chunks")
     bufs = [DeviceBuffer.from_array(x) for x in recs]
     synchronize()
-    ptr = lambda s, j: bufs[s].ptr + j * N * C * 2
-    # plain FP32, one handle per stream, streams one after another (the reference's arrangement)
+    ptr = lambda s, j: bufs[s].ptr + (j % n_ticks) * N * C * 2
+    steps = args.steps
+    # plain FP32, one handle per stream, streams one after another: the agreement reference
     singles = [OnlineSpikesV2(d, chunk=N, max_batch=W) for _ in range(S)]
     ref = [[None] * n_ticks for _ in range(S)]
     for j in range(3):
@@ -303,21 +369,33 @@ def bench_target(args) -> dict:
     modes = {}
     all_modes = (("group_fp32", PREC_FP32, False), ("group_bf16", PREC_BF16_RESCORE, False),
                  ("group_bf16_gated", PREC_BF16_RESCORE, True))
-    for name, prec, gated in [m for m in all_modes if m[0] in args.modes.split(",")]:
-        grp = StreamGroup([d] * S, batch=W, precision=prec, peel_split=args.group_split)
+    headline = "group_bf16_gated"
+    outs_by_mode = {}
+    peel_by_mode = {}
+    for name, prec, gated in [mm for mm in all_modes if mm[0] in args.modes.split(",")]:
+        grp = StreamGroup([d] * S, batch=W, precision=prec, peel_split=args.group_split, profile=True)
         if gated:
             for s in range(S):
-                grp.set_proposal(s, net, 0.25, 0.75, 4)
-        for j in range(3):
-            grp.process([ptr(s, j) for s in range(S)], [j * N] * S, device=True, lds=[C] * S)
+                grp.set_proposal(s, net, args.gate_lo, args.gate_hi, args.gate_reach)
+        for j in range(args.warmup):
+            grp.process([ptr(s, j) for s in range(S)], [(j % n_ticks) * N] * S, device=True, lds=[C] * S)
         synchronize()
-        wall, outs = [], []
+        grp.stage_times(0, reset=True)
+        wall = []
         ts = time.perf_counter()
-        for j in range(n_ticks):
+        for j in range(steps):
             t1 = time.perf_counter()
-            outs.append(grp.process([ptr(s, j) for s in range(S)], [j * N] * S, device=True, lds=[C] * S))
+            grp.process([ptr(s, j) for s in range(S)], [(j % n_ticks) * N] * S, device=True, lds=[C] * S)
             wall.append((time.perf_counter() - t1) * 1e3)
         el = time.perf_counter() - ts
+        front_ms, peel_ms, launches = grp.stage_times(0, reset=True)
+        # untimed pass over the recording: outputs for the agreement, and the peel's
+        # iterations and spikes per stream and tick (the algorithmic work)
+        outs, work_b = [], []
+        for j in range(n_ticks):
+            outs.append(grp.process([ptr(s, j) for s in range(S)], [j * N] * S, device=True, lds=[C] * S))
+            qs = [grp.query(s) for s in range(S)]
+            work_b.append(sum(peel_bytes(q.T, W, M, q.nC, q.last_iters, q.last_spikes) for q in qs))
         tot, ident_n, within_n = 0, 0.0, 0.0
         for j, out in enumerate(outs):
             for s in range(S):
@@ -327,43 +405,125 @@ def bench_target(args) -> dict:
                 ident_n += i * nn
                 within_n += w_ * nn
                 tot += nn
-        lat, busy, nt = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205,
-                                      lanes=args.lanes)
         q = [grp.query(s) for s in range(S)]
         grp.close()
-        rtf = n_ticks * S * N / FS / el
-        res = {"x_realtime": round(rtf, 2), "tick_p50_ms": round(pct(wall, 50), 3),
-               "tick_p99_ms": round(pct(wall, 99), 3),
+        # paced run, chunks handed over from host memory (int16 hand-off -> events on the host)
+        grp = StreamGroup([d] * S, batch=W, precision=prec, peel_split=args.group_split)
+        if gated:
+            for s in range(S):
+                grp.set_proposal(s, net, args.gate_lo, args.gate_hi, args.gate_reach)
+        lat, busy, nt = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205,
+                                      lanes=args.lanes, host_recs=recs)
+        grp.close()
+        rtf = steps * S * N / FS / el
+        peel_avg = peel_ms / max(launches, 1)
+        res = {"x_realtime": round(rtf, 2), "ms_per_tick": round(el / steps * 1e3, 4),
+               "tick_p50_ms": round(pct(wall, 50), 3), "tick_p99_ms": round(pct(wall, 99), 3),
                "paced_latency_p50_ms": round(pct(lat, 50), 3), "paced_latency_p99_ms": round(pct(lat, 99), 3),
-               "paced_gpu_busy_fraction": round(busy, 4), "paced_launches": nt,
+               "paced_input": "host int16", "paced_gpu_busy_fraction": round(busy, 4), "paced_launches": nt,
+               "device_ms_per_tick": {"front_stages": round(front_ms / max(launches, 1), 4),
+                                      "peel": round(peel_avg, 4)},
                "agreement_vs_plain_fp32": {"identical": round(ident_n / max(tot, 1), 6),
                                            "within_1_sample": round(within_n / max(tot, 1), 6), "events": tot},
                "speedup_over_plain_fp32_sequential": round(rtf / seq_rtf, 3),
-               "peel_geometry": group_geometry(q)}
+               "peel_geometry": group_geometry(q),
+               "rescored_per_tick": round(sum(x.rescored for x in q) / max(1, q[0].batches), 1)}
         if gated:
             sc = sum(x.gate_tiles_scored for x in q)
             tt = sum(x.gate_tiles for x in q)
             res["tiles_scored_fraction"] = round(sc / max(tt, 1), 4)
         modes[name] = res
+        outs_by_mode[name] = outs
+        peel_by_mode[name] = (peel_avg, float(np.mean(work_b)))
         log(f"{name}: {res}")
     net.close()
     for b in bufs:
         b.free()
-    g = modes.get("group_bf16_gated") or next(iter(modes.values()))
+    if "group_fp32" in modes:
+        for name, res in modes.items():
+            res["speedup_over_plain_fp32_group"] = round(res["x_realtime"] / modes["group_fp32"]["x_realtime"], 3)
+    if headline not in modes:
+        headline = next(iter(modes))
+    g = modes[headline]
+    # roofline of the dominant kernel, the group's peel: all streams' algorithmic
+    # bytes per launch over its HIP-event-timed device time per launch
+    peel_avg, peel_b = peel_by_mode[headline]
+    achieved = peel_b / (peel_avg / 1e3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "target_traffic.json"
+    if tf.exists():
+        tr = json.loads(tf.read_text())
+        traffic = tr.get("bytes_per_launch", {}).get(f"lss::k_peel<{g['peel_geometry']['rows_per_thread']}, true>")
+        traffic = round(traffic) if traffic else None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": f"k_peel<{g['peel_geometry']['rows_per_thread']}> ({S} streams in one grid)",
+                "ms_per_launch": round(peel_avg, 5), "algorithmic_bytes": round(peel_b, 1)}
+    # spike-list differences of every mode against the FP64 oracle on a sample, each
+    # classified by the oracle's decision margins (near tie / cascade / unexplained)
+    explain = None
+    import oracle
+    tab = oracle.load_tables(d)
+    threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    if not args.no_cpu and args.explain_ticks > 0:
+        from tests.parity import explain_mismatches
+        explain = {}
+        refs64 = {}
+        for j in range(min(args.explain_ticks, n_ticks)):
+            for s in range(S):
+                refs64[(j, s)] = oracle.run_model_batch(tab, recs[s][j * N:j * N + W], margins=True, threads=threads,
+                                                        features=False)
+        for name, outs in outs_by_mode.items():
+            agg = {"events_fp64": 0, "identical": 0, "differing": 0, "near_tie": 0, "cascade": 0, "unexplained": 0}
+            for (j, s), r in refs64.items():
+                ev = outs[j][s][0]
+                i, _, _ = agreement(r.w + j * N - M // 2 + m.nt0min, r.t, ev["sample"], ev["template"])
+                ex = explain_mismatches(r, ev, first_sample=j * N, M=M, nt0min=m.nt0min)
+                agg["events_fp64"] += len(r.w)
+                agg["identical"] += int(round(i * max(len(r.w), len(ev))))
+                agg["differing"] += ex["events"]
+                for k in ("near_tie", "cascade", "unexplained"):
+                    agg[k] += ex[k]
+            agg["chunks"] = len(refs64)
+            explain[name] = agg
+            log(f"vs FP64 oracle, {name}: {agg}")
+    # the CPU baseline on the same workload: chunks of every stream, tick by tick
+    cpu = None
+    if not args.no_cpu:
+        chunks = [recs[s][j * N:j * N + W] for j in range(n_ticks) for s in range(S)]
+        legs = cpu_baseline_legs(tab, chunks, N, args.cpu_seconds, threads)
+        k = legs["kilosort"]
+        cpu = {"value": k["x_realtime"], "unit": "x realtime (one stream)", "cores": threads, "kind": "port",
+               "sample": f"{k['chunks']} chunks of the {S} streams' recordings (384 ch x 1024 templates, W={W}), "
+                         f"FP32 C restatement (oracle/liborc32, Kilosort rules, 100 iterations, features), "
+                         f"OpenMP {threads} threads, {k['seconds']} s",
+               "legs": legs, "host": host_cpu_info()}
     return {
-        "metric": METRIC + " [target: gated BF16 batched, 384 ch x 1024 templates]",
+        "metric": METRIC,
         "value": g["x_realtime"],
         "unit": f"x realtime (aggregate over {S} streams)",
-        "n_gpus": 1, "steps": n_ticks, "warmup": 3,
-        "ms_per_step": g["tick_p50_ms"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (projection, proposal), f32",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": g["ms_per_tick"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (projection, proposal net) + f32 (rescoring, peel)",
         "data": f"synthetic (seeded generator: {S} recordings of one 384-ch, 1024-template probe)",
         "config": {"workload": f"{S} streams x 384 ch x 1024 templates, 1500-sample chunks + 61-sample margins "
-                               "(W=1622), one graph launch per tick, BF16 projection with FP32 rescoring, "
-                               "proposal-network gate; paced run with +-5 ms arrival jitter",
-                   "streams": S, "templates": 1024, "chunk": N, "batch_columns": W},
+                               f"(W={W}), one graph launch per tick, BF16 projection with FP32 rescoring, "
+                               "proposal-network gate",
+                   "streams": S, "channels": C, "templates": 1024, "chunk": N, "batch_columns": W,
+                   "mode": headline},
+        "p50_ms": g["paced_latency_p50_ms"],
+        "p99_ms": g["paced_latency_p99_ms"],
+        "latency": "paced: chunks handed over as int16 in host memory at 30 kHz with +-5 ms jitter, "
+                   "to events and features on the host",
+        "agreement_vs_plain_fp32": g["agreement_vs_plain_fp32"],
+        "speedup_over_plain_fp32_group": g.get("speedup_over_plain_fp32_group"),
+        "speedup_over_plain_fp32_sequential": g["speedup_over_plain_fp32_sequential"],
         "plain_fp32_sequential_x_realtime": round(seq_rtf, 2),
         "modes": modes,
+        "vs_fp64_oracle": explain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
 
 
@@ -546,9 +706,17 @@ def main() -> int:
     ap.add_argument("--cache", default=os.environ.get("LSS_BENCH_CACHE", "/tmp/lss_bench_cache"))
     ap.add_argument("--no-autotune", action="store_true",
                     help="cfg2: keep the default peel geometry instead of lss_autotune's choice")
-    ap.add_argument("--workload", choices=("cfg2", "group", "bf16", "target"), default="cfg2",
-                    help="cfg2: the headline plain FP32 matcher; group: 8 streams per graph launch; "
-                         "bf16: BF16 projection with FP32 rescoring, 1024 templates, 60000-sample batches")
+    ap.add_argument("--workload", choices=("cfg2", "group", "bf16", "target"), default="target",
+                    help="target (default): 8 streams x 384 ch x 1024 templates per graph launch, gated BF16 as "
+                         "the headline beside plain FP32 and BF16; cfg2: the plain FP32 matcher on one 512-template "
+                         "stream; group: 8 streams of 256-1024 templates; bf16: BF16 projection with FP32 "
+                         "rescoring, 1024 templates, 60000-sample batches")
+    ap.add_argument("--no-secondary", action="store_true", help="target: skip the cfg2 secondary keys")
+    ap.add_argument("--explain-ticks", type=int, default=1,
+                    help="target: ticks whose spike lists are compared with the FP64 oracle (mismatch margins)")
+    ap.add_argument("--gate-lo", type=float, default=0.25, help="target: proposal gate fallback band, low edge")
+    ap.add_argument("--gate-hi", type=float, default=0.75, help="target: proposal gate acceptance threshold")
+    ap.add_argument("--gate-reach", type=int, default=4, help="target: proposal gate reach in columns")
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")
@@ -576,10 +744,27 @@ def main() -> int:
     if args.workload == "bf16":
         print(json.dumps(bench_bf16(args)), flush=True)
         return 0
-    if args.workload == "target":
-        print(json.dumps(bench_target(args)), flush=True)
+    if args.workload == "cfg2":
+        print(json.dumps(bench_cfg2(args)), flush=True)
         return 0
+    out = bench_target(args)
+    if not args.no_secondary:
+        # the plain FP32 matcher on one 384-channel, 512-template stream in 3000-sample
+        # chunks, as secondary keys of the same line
+        a2 = argparse.Namespace(**vars(args))
+        a2.no_cpu, a2.pre_steps, a2.inflight, a2.host_steps, a2.copy_calibration = True, 0, 1, 50, 0
+        c2 = bench_cfg2(a2)
+        out["secondary"] = {"cfg2_plain_fp32_single_stream": {
+            k: c2[k] for k in ("value", "unit", "config", "ms_per_step", "p50_ms", "p99_ms", "p99_ms_host_input",
+                               "mean_iterations", "mean_spikes_per_chunk", "peel_geometry", "roofline",
+                               "peel_phases_us", "stages")}}
+    print(json.dumps(out), flush=True)
+    return 0
 
+
+def bench_cfg2(args) -> dict:
+    """Plain FP32 matcher, one 384-channel, 512-template stream, 3000-sample
+    chunks with 61-sample margins (W = 3122), 60 s resident in HBM."""
     from livespikesorter_amd import OnlineSpikesV2
     from livespikesorter_amd._hip import DeviceBuffer, synchronize
     from livespikesorter_amd.matcher import STAGE_PHASES
@@ -838,8 +1023,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 if __name__ == "__main__":

@@ -273,7 +273,8 @@ int lss_group_process_lane(struct lss_group* g, int32_t lane, const int16_t* con
                            const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
 int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
 /* Device time of the launches on one lane of a group created with cfg->profile
- * set, measured by HIP events recorded inside the launch graph: ms[0] the
+ * set (a tick is then two graph launches, the stages in front of the peel and
+ * the peel, with HIP events recorded on the lane's stream around them): ms[0] the
  * stages in front of the peel (input pointers, preprocessing, whitening,
  * proposal network and gate, PC conv, projection), ms[1] the peel kernel
  * (all streams' peels in one grid); *launches = launches timed.  reset != 0

@@ -1565,6 +1565,8 @@ struct GroupLane {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph2 = nullptr;         // cfg.profile: the peel in a graph of its own
+    hipGraphExec_t exec2 = nullptr;
     const int16_t** h_xptr = nullptr;    // pinned: this launch's input pointers (null = idle)
     const int16_t** d_xptr = nullptr;    // device copy the kernels read
     std::vector<int32_t> ld_captured;
@@ -1573,16 +1575,19 @@ struct GroupLane {
     ProjectBatch pb{};
     PeelBatch kb{};
     ResetBatch rb{};
-    // cfg.profile: events recorded inside the graph at the tick's start and around the
-    // peel, and the device time they measured on this lane (front stages, peel)
+    // cfg.profile: the tick is two graphs, the front stages and the peel, with events
+    // recorded on the lane's stream before, between and after them; the device time
+    // they measured on this lane (front stages, peel)
     hipEvent_t ev[3] = {};
     double ms[2] = {};
     int64_t timed = 0;
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        if (exec2) (void)hipGraphExecDestroy(exec2);
+        if (graph2) (void)hipGraphDestroy(graph2);
+        exec = exec2 = nullptr;
+        graph = graph2 = nullptr;
     }
 };
 constexpr int GROUP_LANES = 4;
@@ -1642,7 +1647,6 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     L.drop_graph();
     hipStream_t st = L.stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    if (g->profile) HIPCHK(hipEventRecord(L.ev[0], st));
     HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
     launch_preprocess(pre, st);
     launch_whiten(L.wb, &L.rb, st);
@@ -1662,10 +1666,17 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     }
     launch_pcconv(L.cb, g->hs[0]->K, st);
     launch_project(L.pb, g->hs[0]->bf16, st);
-    if (g->profile) HIPCHK(hipEventRecord(L.ev[1], st));
+    if (g->profile) {
+        HIPCHK(hipStreamEndCapture(st, &L.graph));
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    }
     HIPCHK(launch_peel(L.kb, g->R, false, st));
-    if (g->profile) HIPCHK(hipEventRecord(L.ev[2], st));
-    HIPCHK(hipStreamEndCapture(st, &L.graph));
+    if (g->profile) {
+        HIPCHK(hipStreamEndCapture(st, &L.graph2));
+        HIPCHK(hipGraphInstantiate(&L.exec2, L.graph2, nullptr, nullptr, 0));
+    } else {
+        HIPCHK(hipStreamEndCapture(st, &L.graph));
+    }
     HIPCHK(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
     L.ld_captured.assign(ld, ld + n);
 }
@@ -1788,7 +1799,15 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         }
         if (!active) return;
         if (!L.exec || !std::equal(ld, ld + n, L.ld_captured.begin())) group_capture(g, L, ld);
-        HIPCHK(hipGraphLaunch(L.exec, L.stream));
+        if (g->profile) {
+            HIPCHK(hipEventRecord(L.ev[0], L.stream));
+            HIPCHK(hipGraphLaunch(L.exec, L.stream));
+            HIPCHK(hipEventRecord(L.ev[1], L.stream));
+            HIPCHK(hipGraphLaunch(L.exec2, L.stream));
+            HIPCHK(hipEventRecord(L.ev[2], L.stream));
+        } else {
+            HIPCHK(hipGraphLaunch(L.exec, L.stream));
+        }
         HIPCHK(hipStreamSynchronize(L.stream));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (g->profile) {

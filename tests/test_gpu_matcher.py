@@ -421,3 +421,29 @@ def test_two_handles_on_alternating_chunks(cfg2):
     finally:
         for h in hs:
             h.close()
+
+
+def test_stream_group_profile_timing(model_dirs):
+    """A group created with profile times its front stages and its peel with
+    HIP events around two graph launches per tick; the results are the ones of
+    the unprofiled group, bit for bit."""
+    from livespikesorter_amd import StreamGroup
+    dirs, recs = [], []
+    for k, T in enumerate((64, 160)):
+        m, d = model_dirs("grpp%d" % k, C=64, T=T, seed=51 + k, whitening="banded", rate=(2.0, 20.0),
+                          amp=(11.0, 3.0))
+        dirs.append(d)
+        recs.append(synth.make_recording(m, 1500 + 1622, seed=151 + k)[0])
+    W = 1622
+    with StreamGroup(dirs, batch=W) as g0, StreamGroup(dirs, batch=W, profile=True) as g1:
+        for t in range(2):
+            bs = [x[t * 1500:t * 1500 + W] for x in recs]
+            a = g0.process(bs, [t * 1500] * 2)
+            b = g1.process(bs, [t * 1500] * 2)
+            for (e0, f0), (e1, f1) in zip(a, b):
+                assert np.array_equal(e0, e1) and np.array_equal(f0, f1)
+        front, peel, n = g1.stage_times(0, reset=True)
+        assert n == 2 and front > 0 and peel > 0
+        assert g1.stage_times(0) == (0.0, 0.0, 0)
+        with pytest.raises(Exception):
+            g0.stage_times(0)
