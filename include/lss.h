@@ -171,6 +171,26 @@ int lss_process_batch_device(struct lss_handle* h, const int16_t* d_samples, int
                              int32_t ld, const int32_t* chan_map, int64_t first_sample,
                              lss_event* ev, int64_t ev_cap, int64_t* n_ev, float* feat);
 
+/* lss_process_batch_device without any device allocation: the batch's device
+ * buffers (whitened batch, PC projection, scores, the peel's exchange words and
+ * counters, the event staging) come from one static device array of
+ * LSS_STATIC_SCRATCH_BYTES per GPU, held from the library's first use of the
+ * array on that GPU until the process ends.  No counterpart in the reference:
+ * it is for callers that cannot allocate during a batch (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The handle
+ * (created beforehand, which allocates) supplies the tables.  The handle's
+ * per-batch needs (about 4*(C*W + 6*C*W + T*W) bytes, more in BF16 mode, plus
+ * the event staging of max_events) must fit the array: otherwise LSS_ERR_ARG
+ * with the sizes.  Handles with a proposal gate or keep_stages are rejected.
+ * Every call on one GPU shares that GPU's array, so calls that overlap in time
+ * on one GPU (from several threads) race: serialise them.  Positions
+ * (lss_last_positions) are not kept by this entry point.  Results are
+ * bit-identical to lss_process_batch_device. */
+#define LSS_STATIC_SCRATCH_BYTES (96ll << 20)
+int lss_process_batch_device_static(struct lss_handle* h, const int16_t* d_samples, int64_t n_samples, int32_t ld,
+                                    const int32_t* chan_map, int64_t first_sample, lss_event* ev, int64_t ev_cap,
+                                    int64_t* n_ev, float* feat);
+
 /* Streaming: hand over the next n_samples of the stream (first_sample = its
  * absolute index; chunks must be contiguous).  The library frames the batch
  * with the previous 2*M samples (edge-replicated before the first chunk, as

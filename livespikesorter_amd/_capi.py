@@ -23,6 +23,7 @@ LSS_ERR_CAPACITY = -4
 LSS_ERR_STATE = -5
 LSS_ERR_TIMEOUT = -6
 LSS_ERR_LIMIT = -7
+STATIC_SCRATCH_BYTES = 96 << 20     # LSS_STATIC_SCRATCH_BYTES
 
 SEM_KILOSORT = 0
 SEM_LSS = 1
@@ -32,6 +33,7 @@ PRE_MEAN, PRE_CAR, PRE_HIGHPASS = 1, 2, 4
 PRE_ALL = PRE_MEAN | PRE_CAR | PRE_HIGHPASS
 
 ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_process_batch_device",
+                 "lss_process_batch_device_static",
                  "lss_process_chunk", "lss_flush", "lss_query", "lss_debug_stage", "lss_stage_times",
                  "lss_destroy", "lss_last_error", "lss_group_create", "lss_group_process",
                  "lss_group_query", "lss_group_destroy", "lss_group_stage_times", "lss_proposal_n_params", "lss_proposal_create",
@@ -134,6 +136,7 @@ def lib() -> ct.CDLL:
                       ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64), ct.c_void_p]
         L.lss_process_batch.argtypes = batch_args
         L.lss_process_batch_device.argtypes = batch_args
+        L.lss_process_batch_device_static.argtypes = batch_args
         L.lss_process_chunk.argtypes = batch_args
         L.lss_flush.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64), ct.c_void_p]
         L.lss_query.argtypes = [ct.c_void_p, ct.POINTER(Info)]

@@ -14,6 +14,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -629,6 +630,7 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->d_iU = dalloc<int>(T); upload(h->d_iU, iUi);
     h->iU_int = iUi;
     h->d_cmap = dalloc<int>(CP);
+    h->cmap_cur.assign(CP, -1);            // nothing uploaded yet
     h->d_kbw_ptr = dalloc<int>(kbw_ptr.size()); upload(h->d_kbw_ptr, kbw_ptr);
     h->d_kbw_idx = dalloc<int>(kbw_idx.size()); upload(h->d_kbw_idx, kbw_idx);
     h->d_kbp_ptr = dalloc<int>(kbp_ptr.size()); upload(h->d_kbp_ptr, kbp_ptr);
@@ -778,24 +780,62 @@ void load(lss_handle* h, const std::string& dir_in) {
     h->last_spike.assign(std::max(T, h->n_cent), INT64_MIN / 2);   // indexed by template, or by cluster (relabel)
 }
 
+// Validate the channel map of a call and upload it when it changed.  cmap_cur
+// holds CP entries from load time on (-1: nothing uploaded yet), so a call with
+// the same map touches no heap memory.
 void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
-    std::vector<int> want(h->CP, 0);
+    bool same = true;
     for (int c = 0; c < h->C; ++c) {
         const int v = chan_map ? chan_map[c] : h->cmap_default[c];
         if (v < 0 || v >= ld) throw LssError(LSS_ERR_ARG, "channel map entry out of range of the row stride");
-        want[c] = v;
+        same = same && h->cmap_cur[c] == v;
     }
-    if (want != h->cmap_cur) {
-        HIPCHK(hipStreamSynchronize(h->stream));
-        HIPCHK(hipMemcpy(h->d_cmap, want.data(), want.size() * sizeof(int), hipMemcpyHostToDevice));
-        h->cmap_cur = want;
-        h->cmap_id = true;
-        for (int c = 0; c < h->C; ++c) h->cmap_id = h->cmap_id && want[c] == c;
-    }
+    if (same) return;
+    for (int c = 0; c < h->C; ++c) h->cmap_cur[c] = chan_map ? chan_map[c] : h->cmap_default[c];
+    for (int c = h->C; c < h->CP; ++c) h->cmap_cur[c] = 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(h->d_cmap, h->cmap_cur.data(), (size_t)h->CP * sizeof(int), hipMemcpyHostToDevice));
+    h->cmap_id = true;
+    for (int c = 0; c < h->C; ++c) h->cmap_id = h->cmap_id && h->cmap_cur[c] == c;
 }
 
 // Run one framed batch that is already on the device.
 }  // namespace
+
+// The per-batch device buffers a batch runs on: the handle's own (allocated
+// with it), or a region of the static scratch array (the allocation-free entry
+// point).  The tables (templates, ctc, norms, ...) always come from the handle.
+struct BatchBufs {
+    float *Xw = nullptr, *Bpc = nullptr, *B = nullptr, *Binit = nullptr, *Nsq = nullptr, *Cfmax = nullptr;
+    uint16_t* Bpcb = nullptr;
+    Spike *slots = nullptr, *list = nullptr;
+    int *slot_cnt = nullptr, *list_cnt = nullptr, *iter_cnt = nullptr, *err = nullptr;
+    unsigned* bar = nullptr;
+    unsigned long long *prof = nullptr, *ptag = nullptr;
+    int64_t ptag_words = 0;
+    unsigned* epoch = nullptr;
+    OutHeader* hdr = nullptr;          // device address of the output header, events, features
+    lss_event* ev = nullptr;
+    float *feat = nullptr, *pos = nullptr;
+    int *psum = nullptr, *pmin = nullptr, *pmax = nullptr;
+    float *pmean = nullptr, *pmed = nullptr, *X1 = nullptr, *Xp = nullptr;
+    float *dbgB = nullptr, *dbgCf = nullptr;
+    int* dbgIm = nullptr;
+};
+
+BatchBufs handle_bufs(const lss_handle* h) {
+    BatchBufs b;
+    b.Xw = h->d_Xw; b.Bpc = h->d_Bpc; b.B = h->d_B; b.Binit = h->d_Binit; b.Nsq = h->d_Nsq; b.Cfmax = h->d_Cfmax;
+    b.Bpcb = h->d_Bpcb;
+    b.slots = h->d_slots; b.list = h->d_list;
+    b.slot_cnt = h->d_slot_cnt; b.list_cnt = h->d_list_cnt; b.iter_cnt = h->d_iter_cnt; b.err = h->d_err;
+    b.bar = h->d_bar; b.prof = h->d_prof; b.ptag = h->d_ptag; b.ptag_words = h->ptag_words; b.epoch = h->d_epoch;
+    b.hdr = h->d_hdr; b.ev = h->d_ev; b.feat = h->d_feat; b.pos = h->d_pos;
+    b.psum = h->d_psum; b.pmin = h->d_pmin; b.pmax = h->d_pmax;
+    b.pmean = h->d_pmean; b.pmed = h->d_pmed; b.X1 = h->d_X1; b.Xp = h->d_Xp;
+    if (h->cfg.keep_stages) { b.dbgB = h->d_dbgB; b.dbgCf = h->d_dbgCf; b.dbgIm = h->d_dbgIm; }
+    return b;
+}
 
 struct BatchPlan {
     PreJob pre;
@@ -814,13 +854,13 @@ inline int peel_cap_index(int R) { return R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 
 
 // Job descriptors of one handle for one batch of W columns.  The input is read
 // from d_x, or, when xptr is set, from the device pointer stored at *xptr.
-void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, int W, int64_t ld, int CW, int R,
-                int S, BatchPlan& bp) {
+void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const int16_t* const* xptr, int W, int64_t ld,
+                int CW, int R, int S, BatchPlan& bp) {
     const int M = h->M;
     WhitenJob& wj = bp.wj;
     wj = WhitenJob{};
     wj.Wr = h->d_Wr; wj.x = d_x; wj.xptr = xptr; wj.cmap = h->d_cmap;
-    wj.kb_ptr = h->d_kbw_ptr; wj.kb_idx = h->d_kbw_idx; wj.Xw = h->d_Xw;
+    wj.kb_ptr = h->d_kbw_ptr; wj.kb_idx = h->d_kbw_idx; wj.Xw = bb.Xw;
     wj.ld = ld; wj.CP = h->CP; wj.W = W; wj.C = h->C; wj.XS = h->XS;
     wj.mtiles = h->CP / GEMM_BM;
     wj.cmap_id = h->cmap_id ? 1 : 0;
@@ -831,9 +871,9 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
         q = PreJob{};
         q.x = d_x; q.xptr = xptr; q.ld = ld; q.cmap = h->d_cmap;
         q.C = h->C; q.W = W; q.flags = h->pre; q.nslab = (W + PRE_SLAB - 1) / PRE_SLAB;
-        q.psum = h->d_psum; q.pmin = h->d_pmin; q.pmax = h->d_pmax;
-        q.mean = h->d_pmean; q.med = h->d_pmed; q.X1 = h->d_X1; q.Xp = h->d_Xp; q.PS = h->PS;
-        q.p2p = &h->d_hdr->p2p;
+        q.psum = bb.psum; q.pmin = bb.pmin; q.pmax = bb.pmax;
+        q.mean = bb.pmean; q.med = bb.pmed; q.X1 = bb.X1; q.Xp = bb.Xp; q.PS = h->PS;
+        q.p2p = &bb.hdr->p2p;
         if (h->pre & PRE_HP) {
             // taps h[m] = hp[m + crop], m in [0, W): the response cropped (F >= W) or zero-padded
             // to W (fft_highpass, KS/preprocessing.py:143-153); only m with hp index in
@@ -850,27 +890,27 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
             q.taps = h->d_hp + crop + m_lo;
             q.shift = (int)(((int64_t)s0 + std::max(0, m_lo)) % W);
         }
-        wj.xf = (h->pre & PRE_HP) ? h->d_Xp : h->d_X1;
+        wj.xf = (h->pre & PRE_HP) ? bb.Xp : bb.X1;
         wj.xfs = h->PS;
     }
     PcconvJob& cj = bp.cj;
     cj = PcconvJob{};
-    cj.Xw = h->d_Xw; cj.wpca = h->d_wpca; cj.Bpc = h->d_Bpc;
-    cj.Bpcb = reinterpret_cast<__bf16*>(h->d_Bpcb);
+    cj.Xw = bb.Xw; cj.wpca = h->d_wpca; cj.Bpc = bb.Bpc;
+    cj.Bpcb = reinterpret_cast<__bf16*>(bb.Bpcb);
     cj.active = xptr;
     cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP; cj.C = h->C;
     cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
     ProjectJob& pj = bp.pj;
     pj = ProjectJob{};
-    pj.A = h->d_U2; pj.Bm = h->d_Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = h->d_B;
+    pj.A = h->d_U2; pj.Bm = bb.Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = bb.B;
     pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
     if (h->prop) { pj.gate = h->d_gate; pj.gate_mt = h->TP / GEMM_BM; }
     pj.active = xptr;
     if (h->bf16) {
         pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
-        pj.Bmb = reinterpret_cast<const __bf16*>(h->d_Bpcb);
-        pj.C2 = h->d_Binit;
-        pj.Nsq = h->d_Nsq;
+        pj.Bmb = reinterpret_cast<const __bf16*>(bb.Bpcb);
+        pj.C2 = bb.Binit;
+        pj.Nsq = bb.Nsq;
     }
     pj.mtiles = h->TP / GEMM_BM;
     pj.ntiles = (W + GEMM_BN - 1) / GEMM_BN;
@@ -879,7 +919,7 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     const int NB = (W + CW - 1) / CW;
     const int G = S * NB;
     if (NB > h->max_groups || G > h->g_cap) throw LssError(LSS_ERR_ARG, "peel grid larger than the handle's flag space");
-    p.B = h->d_B; p.BS = h->BS;
+    p.B = bb.B; p.BS = h->BS;
     p.B_bytes = (int)std::min<int64_t>((int64_t)h->TP * h->BS * 4, INT32_MAX);
     p.ctc_bytes = (int)std::min<int64_t>((int64_t)h->T * h->ctc_rows * LP * 4, INT32_MAX);
     p.ctc_rows = h->ctc_rows;
@@ -891,23 +931,23 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
     p.max_iter = h->max_iter;
     p.CW = CW; p.G = G;
     p.S = S; p.NB = NB;
-    p.Pa = h->d_ptag; p.epoch = h->d_epoch;
+    p.Pa = bb.ptag; p.epoch = bb.epoch;
     p.lss = h->sem == LSS_SEM_LSS;
-    p.Cfmax = h->d_Cfmax;
-    p.slots = h->d_slots; p.slot_cnt = h->d_slot_cnt; p.slot_cap = h->slot_cap;
-    p.list = h->d_list; p.list_cnt = h->d_list_cnt; p.cap_g = h->cap_g;
-    p.bar = h->d_bar; p.iter_cnt = h->d_iter_cnt; p.err = h->d_err;
-    p.Bpc = h->d_Bpc; p.CP = h->CP;
+    p.Cfmax = bb.Cfmax;
+    p.slots = bb.slots; p.slot_cnt = bb.slot_cnt; p.slot_cap = h->slot_cap;
+    p.list = bb.list; p.list_cnt = bb.list_cnt; p.cap_g = h->cap_g;
+    p.bar = bb.bar; p.iter_cnt = bb.iter_cnt; p.err = bb.err;
+    p.Bpc = bb.Bpc; p.CP = h->CP;
     p.U = h->d_U; p.C = h->C; p.K = h->K; p.nC = h->nC;
     p.iCC = h->d_iCC; p.iU = h->d_iU; p.Gt = h->d_Gt;
     p.do_feat = 1;
-    p.hdr = h->d_hdr; p.ev = h->d_ev; p.feat = h->d_feat; p.ev_cap = h->ev_cap;
-    p.pos = h->d_pos; p.xc = h->d_xc; p.yc = h->d_yc;
+    p.hdr = bb.hdr; p.ev = bb.ev; p.feat = bb.feat; p.ev_cap = h->ev_cap;
+    p.pos = bb.pos; p.xc = h->d_xc; p.yc = h->d_yc;
     p.cent = h->d_cent; p.n_cent = h->n_cent;
     p.half = M / 2; p.nt0min = h->nt0min;
-    p.dbg_cfmax = h->cfg.keep_stages ? h->d_dbgCf : nullptr;
-    p.dbg_imax = h->cfg.keep_stages ? h->d_dbgIm : nullptr;
-    p.prof = h->d_prof;
+    p.dbg_cfmax = bb.dbgCf;
+    p.dbg_imax = bb.dbgIm;
+    p.prof = bb.prof;
     if (h->prop) { p.gate = h->d_gate; p.gate_mt = h->TP / GEMM_BM; }
     p.active = xptr;
     if (h->bf16) {
@@ -916,8 +956,8 @@ void plan_batch(lss_handle* h, const int16_t* d_x, const int16_t* const* xptr, i
         p.lo2 = 0.9f * p.Th2;
         p.U2 = h->d_U2; p.ldu = h->KCP;
         p.kb_ptr = h->d_kbp_ptr; p.kb_idx = h->d_kbp_idx;
-        p.Binit = h->d_Binit;
-        p.Nsq = h->d_Nsq;
+        p.Binit = bb.Binit;
+        p.Nsq = bb.Nsq;
         p.n_rescored = h->d_rescnt;
     }
     bp.R = R; bp.CW = CW; bp.G = G;
@@ -944,14 +984,14 @@ bool split_fits(const lss_handle* h, int W, int S, int R) {
 
 // The split sizes worth trying for a batch of W columns: none, then the fewest
 // subsets that reach 32, 16 and 8 rows per thread.
-std::vector<int> split_candidates(const lss_handle* h, int W) {
-    std::vector<int> out{1};
+int split_candidates(const lss_handle* h, int W, int out[4]) {
+    int n = 0;
+    out[n++] = 1;
     for (int r : {32, 16, 8}) {
         const int s2 = (h->T + REG_NRG * r - 1) / (REG_NRG * r);
-        if (split_fits(h, W, s2, split_rows(h->T, s2)) && std::find(out.begin(), out.end(), s2) == out.end())
-            out.push_back(s2);
+        if (split_fits(h, W, s2, split_rows(h->T, s2)) && std::find(out, out + n, s2) == out + n) out[n++] = s2;
     }
-    return out;
+    return n;
 }
 
 void peel_geometry(const lss_handle* h, int W, int& CW, int& R, int& S) {
@@ -969,11 +1009,12 @@ void peel_geometry(const lss_handle* h, int W, int& CW, int& R, int& S) {
     if (want == 0 && h->max_split > 1) {
         // default: 16 rows per thread when that split fits (measured fastest at 384 ch x
         // 512 templates, 3122 columns), else the fewest rows per thread that fit
-        const auto cand = split_candidates(h, W);
-        if (cand.size() > 1) {
-            S = cand.back();
-            for (int s2 : cand)
-                if (s2 > 1 && split_rows(h->T, s2) == 16) S = s2;
+        int cand[4];
+        const int nc = split_candidates(h, W, cand);
+        if (nc > 1) {
+            S = cand[nc - 1];
+            for (int i = 0; i < nc; ++i)
+                if (cand[i] > 1 && split_rows(h->T, cand[i]) == 16) S = cand[i];
             CW = MIN_CW; R = split_rows(h->T, S);
             return;
         }
@@ -990,8 +1031,7 @@ void peel_geometry(const lss_handle* h, int W, int& CW, int& R, int& S) {
 // Check the peel kernel's error flags of one handle after a batch.  A wait that
 // timed out (workgroups not co-resident) is LSS_ERR_TIMEOUT; a full internal
 // spike list is LSS_ERR_LIMIT.  Neither is about the caller's event buffer.
-void check_peel(lss_handle* h, int W) {
-    const OutHeader hd = *h->hdr;
+void check_peel(lss_handle* h, const OutHeader& hd, int W) {
     if (hd.err) {
         std::string why;
         if (hd.err & ERR_BARRIER) why += " a cross-workgroup wait timed out (peel grid not co-resident);";
@@ -1016,7 +1056,8 @@ void run_gate(lss_handle* h, int W, hipStream_t s, const int16_t* const* active 
 }
 
 // Run one framed batch that is already on the device.
-void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t first_sample) {
+void run_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, int W, int64_t ld, int64_t first_sample,
+               bool check = true) {
     if (W <= 0 || W > h->Wmax) throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(W) +
                                               " samples outside 1..max_batch " + std::to_string(h->Wmax));
     hipStream_t s = h->stream;
@@ -1025,7 +1066,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     int CW = 0, R = 0, S = 1;
     peel_geometry(h, W, CW, R, S);
     BatchPlan bp;
-    plan_batch(h, d_x, nullptr, W, ld, CW, R, S, bp);
+    plan_batch(h, bb, d_x, nullptr, W, ld, CW, R, S, bp);
     h->info.peel_cw = CW; h->info.peel_rows = R; h->info.peel_split = S; h->info.peel_groups = bp.G;
     WhitenBatch wb{};
     wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
@@ -1038,8 +1079,8 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     // the peel words of this batch are reset by the whitening launch
     ResetBatch rb{};
     rb.n = 1;
-    rb.bar[0] = h->d_bar; rb.iter_cnt[0] = h->d_iter_cnt; rb.n_iter[0] = h->sync_words; rb.err[0] = h->d_err;
-    rb.prof[0] = h->d_prof; rb.epoch[0] = h->d_epoch; rb.ptag[0] = h->d_ptag; rb.ptag_words[0] = h->ptag_words;
+    rb.bar[0] = bb.bar; rb.iter_cnt[0] = bb.iter_cnt; rb.n_iter[0] = h->sync_words; rb.err[0] = bb.err;
+    rb.prof[0] = bb.prof; rb.epoch[0] = bb.epoch; rb.ptag[0] = bb.ptag; rb.ptag_words[0] = bb.ptag_words;
     if (prof) HIPCHK(hipEventRecord(h->evt[1], s));
     if (bp.has_pre) {
         PreBatch pre{};
@@ -1055,8 +1096,8 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
     launch_pcconv(cb, h->K, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
     launch_project(pb, h->bf16, s);
-    if (h->cfg.keep_stages)
-        HIPCHK(hipMemcpyAsync(h->d_dbgB, h->d_B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
+    if (bb.dbgB)
+        HIPCHK(hipMemcpyAsync(bb.dbgB, bb.B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
     if (prof_peel) HIPCHK(hipEventRecord(h->evt[6], s));
     HIPCHK(hipGetLastError());
     HIPCHK(launch_peel(kb, R, h->coop, s));
@@ -1076,7 +1117,7 @@ void run_batch(lss_handle* h, const int16_t* d_x, int W, int64_t ld, int64_t fir
         h->copy_timed = false;
     }
     h->cur_first = first_sample;
-    check_peel(h, W);
+    if (check) check_peel(h, *h->hdr, W);
 }
 
 // Copy the kernel's events (already sorted by sample) to the caller, applying
@@ -1158,7 +1199,7 @@ void process_host_batch(lss_handle* h, const int16_t* samples, int64_t W, int32_
     if (h->cfg.profile == 1) HIPCHK(hipEventRecord(h->evt[0], h->stream));
     HIPCHK(hipMemcpyAsync(dx, samples, (size_t)W * ld * sizeof(int16_t), hipMemcpyHostToDevice, h->stream));
     h->copy_timed = h->cfg.profile == 1;
-    run_batch(h, dx, (int)W, ld, first_sample);
+    run_batch(h, handle_bufs(h), dx, (int)W, ld, first_sample);
 }
 
 bool dedup_on(const lss_handle* h) {
@@ -1261,7 +1302,7 @@ int lss_process_batch_device(lss_handle* h, const int16_t* d_samples, int64_t n_
         auto t0 = std::chrono::steady_clock::now();
         *n_ev = 0;
         set_cmap(h, chan_map, ld);
-        run_batch(h, d_samples, (int)n_samples, ld, first_sample);
+        run_batch(h, handle_bufs(h), d_samples, (int)n_samples, ld, first_sample);
         emit(h, ev, ev_cap, feat, INT64_MIN, INT64_MAX, false, n_ev);
         h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     });
@@ -1478,8 +1519,8 @@ int lss_autotune(lss_handle* h, const int16_t* d_samples, int64_t n, int32_t ld,
         const int32_t prof = h->cfg.profile;   // restored afterwards (the timings are host-side)
         h->cfg.profile = 0;
         *out = lss_tune_result{};
-        std::vector<int> cand = split_candidates(h, W);
-        if (cand.size() > 8) cand.resize(8);
+        int cand_a[4];
+        const std::vector<int> cand(cand_a, cand_a + split_candidates(h, W, cand_a));
         int best = 0;
         try {
             for (size_t i = 0; i < cand.size(); ++i) {
@@ -1489,7 +1530,7 @@ int lss_autotune(lss_handle* h, const int16_t* d_samples, int64_t n, int32_t ld,
                 std::vector<double> t;
                 for (int r = 0; r <= reps; ++r) {
                     const auto t0 = std::chrono::steady_clock::now();
-                    run_batch(h, d_samples, W, ld, 0);
+                    run_batch(h, handle_bufs(h), d_samples, W, ld, 0);
                     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                     if (r > 0) t.push_back(ms);
                 }
@@ -1630,7 +1671,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     for (int s2 = 0; s2 < n; ++s2) {
         lss_handle* h = g->hs[s2].get();
         BatchPlan bp;
-        plan_batch(h, nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, g->S[s2], bp);
+        plan_batch(h, handle_bufs(h), nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, g->S[s2], bp);
         if (bp.has_pre) pre.job[pre.n++] = bp.pre;
         L.wb.job[s2] = bp.wj;
         L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
@@ -1829,7 +1870,7 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
             h->cur_first = first_sample[s2];
             h->info.last_ms = ms;
             try {
-                check_peel(h, g->W);
+                check_peel(h, *h->hdr, g->W);
                 emit(h, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr, INT64_MIN, INT64_MAX, false, &n_ev[s2]);
             } catch (const LssError& e) {
                 failed += " stream " + std::to_string(s2) + ": " + e.what() + ";";
@@ -2014,7 +2055,7 @@ int lss_proposal_train_step(lss_proposal* net, lss_handle* h, const int16_t* d_s
         HIPCHK(hipSetDevice(net->dev));
         set_cmap(h, nullptr, ld);
         BatchPlan bp;
-        plan_batch(h, d_samples, nullptr, (int)n, ld, MIN_CW, 0, 1, bp);
+        plan_batch(h, handle_bufs(h), d_samples, nullptr, (int)n, ld, MIN_CW, 0, 1, bp);
         if (bp.has_pre) {
             PreBatch pre{};
             pre.n = 1;
@@ -2083,6 +2124,150 @@ int lss_set_proposal(lss_handle* h, lss_proposal* net, float lo, float hi, int32
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free batch entry point.  Its per-batch device buffers (whitened
+// batch, PC projection, scores, the peel's exchange words, counters and the
+// event staging) are carved from one static device array instead of the
+// handle's allocations; the tables stay the handle's.  The head of the array
+// holds the tag epoch (its own 256-byte line) and a fixed region for the
+// peel's tagged column maxima, so those words are only ever written as tagged
+// words, whichever handle used the array before.  Every other buffer is
+// written by the batch before it is read, and the per-batch words (counters,
+// flags, the epoch) are reset by the whitening launch, so a call needs no
+// clearing launch of its own.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[LSS_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t STATIC_TAG_BYTES = size_t(8) << 20;   // epoch line + tagged column maxima
+constexpr int STATIC_MAX_DEVICES = 64;
+
+struct StaticSlot {
+    std::once_flag once;
+    unsigned char* ptr = nullptr;
+    hipError_t rc = hipSuccess;
+};
+StaticSlot g_static[STATIC_MAX_DEVICES];
+
+// Device address of this device's copy of `scratch` (the host-side name is a
+// stand-in, never a device address).  Looked up once per device, with that
+// device current.
+unsigned char* static_scratch(int dev) {
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) throw LssError(LSS_ERR_ARG, "device ordinal beyond the static table");
+    StaticSlot& sl = g_static[dev];
+    std::call_once(sl.once, [&] {
+        void* p = nullptr;
+        sl.rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        sl.ptr = sl.rc == hipSuccess ? static_cast<unsigned char*>(p) : nullptr;
+    });
+    if (sl.rc != hipSuccess || !sl.ptr)
+        throw LssError(LSS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(sl.rc));
+    return sl.ptr;
+}
+
+// The handle's per-batch buffers laid out in the static array; *need = bytes used.
+BatchBufs static_bufs(const lss_handle* h, unsigned char* base, size_t* need) {
+    BatchBufs b;
+    b.epoch = reinterpret_cast<unsigned*>(base);
+    b.ptag = reinterpret_cast<unsigned long long*>(base + 256);
+    b.ptag_words = (int64_t)((STATIC_TAG_BYTES - 256) / sizeof(unsigned long long));
+    if (h->ptag_words > b.ptag_words)
+        throw LssError(LSS_ERR_ARG, "the handle's peel exchange needs " + std::to_string(h->ptag_words * 8) +
+                                        " bytes, more than the static array's " + std::to_string(b.ptag_words * 8));
+    size_t off = STATIC_TAG_BYTES;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) / 256 * 256;
+        unsigned char* p = base + off;
+        off += bytes;
+        return p;
+    };
+    const size_t CP = h->CP, TP = h->TP, KCP = h->KCP, BS = h->BS, XS = h->XS, Wmax = h->Wmax;
+    b.Xw = reinterpret_cast<float*>(take(CP * XS * 4));
+    b.Bpc = reinterpret_cast<float*>(take(KCP * BS * 4));
+    b.B = reinterpret_cast<float*>(take(TP * BS * 4));
+    if (h->bf16) {
+        b.Bpcb = reinterpret_cast<uint16_t*>(take(KCP * BS * 2));
+        b.Binit = reinterpret_cast<float*>(take(TP * BS * 4));
+        b.Nsq = reinterpret_cast<float*>(take(TP / GEMM_BM * BS * 4));
+    }
+    b.Cfmax = reinterpret_cast<float*>(take(Wmax * 4));
+    b.slots = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->slot_cap * sizeof(Spike)));
+    b.list = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->cap_g * sizeof(Spike)));
+    b.slot_cnt = reinterpret_cast<int*>(take((size_t)h->max_groups * 4));
+    b.list_cnt = reinterpret_cast<int*>(take((size_t)h->max_groups * 4));
+    b.iter_cnt = reinterpret_cast<int*>(take((size_t)h->sync_words * 4));
+    b.err = reinterpret_cast<int*>(take(4));
+    b.bar = reinterpret_cast<unsigned*>(take((size_t)BAR_WORDS * 4));
+    b.prof = reinterpret_cast<unsigned long long*>(take((size_t)PROF_WORDS * 8));
+    const size_t fw = (size_t)h->nC * h->K;
+    b.hdr = reinterpret_cast<OutHeader*>(take(sizeof(OutHeader)));
+    b.ev = reinterpret_cast<lss_event*>(take((size_t)h->ev_cap * sizeof(lss_event)));
+    b.feat = reinterpret_cast<float*>(take((size_t)h->ev_cap * fw * 4));
+    if (h->d_xc) b.pos = reinterpret_cast<float*>(take((size_t)h->ev_cap * 2 * 4));
+    if (h->pre) {
+        const size_t nslab = (Wmax + PRE_SLAB - 1) / PRE_SLAB, C = h->C, PS = h->PS;
+        b.psum = reinterpret_cast<int*>(take(nslab * C * 4));
+        b.pmin = reinterpret_cast<int*>(take(nslab * C * 4));
+        b.pmax = reinterpret_cast<int*>(take(nslab * C * 4));
+        b.pmean = reinterpret_cast<float*>(take(C * 4));
+        b.pmed = reinterpret_cast<float*>(take(Wmax * 4));
+        b.X1 = reinterpret_cast<float*>(take(C * PS * 4));
+        if (h->pre & PRE_HP) b.Xp = reinterpret_cast<float*>(take(C * PS * 4));
+    }
+    *need = off;
+    return b;
+}
+
+}  // namespace
+
+extern "C" int lss_process_batch_device_static(lss_handle* h, const int16_t* d_samples, int64_t n_samples, int32_t ld,
+                                               const int32_t* chan_map, int64_t first_sample, lss_event* ev,
+                                               int64_t ev_cap, int64_t* n_ev, float* feat) {
+    return guarded([&] {
+        if (!h || !n_ev || !d_samples || ev_cap < 0) throw LssError(LSS_ERR_ARG, "null argument");
+        *n_ev = 0;
+        if (n_samples <= 0 || n_samples > h->Wmax)
+            throw LssError(LSS_ERR_ARG, "batch of " + std::to_string(n_samples) + " samples outside 1..max_batch " +
+                                            std::to_string(h->Wmax));
+        if (h->prop) throw LssError(LSS_ERR_ARG, "the allocation-free entry point does not run a proposal gate");
+        if (h->cfg.keep_stages) throw LssError(LSS_ERR_ARG, "the allocation-free entry point keeps no stage copies");
+        HIPCHK(hipSetDevice(h->dev));
+        const auto t0 = std::chrono::steady_clock::now();
+        unsigned char* base = static_scratch(h->dev);
+        size_t need = 0;
+        const BatchBufs bb = static_bufs(h, base, &need);
+        if (need > (size_t)LSS_STATIC_SCRATCH_BYTES)
+            throw LssError(LSS_ERR_ARG, "this handle's batch needs " + std::to_string(need) +
+                                            " bytes of device scratch; the static array holds " +
+                                            std::to_string((size_t)LSS_STATIC_SCRATCH_BYTES) +
+                                            " (create the handle with a smaller max_batch or max_events)");
+        set_cmap(h, chan_map, ld);
+        run_batch(h, bb, d_samples, (int)n_samples, ld, first_sample, false);
+        OutHeader hd;
+        HIPCHK(hipMemcpy(&hd, bb.hdr, sizeof(hd), hipMemcpyDeviceToHost));
+        check_peel(h, hd, (int)n_samples);
+        if (hd.count > h->ev_cap)
+            throw LssError(LSS_ERR_LIMIT, "the batch has " + std::to_string(hd.count) +
+                                              " events, more than the handle's max_events " + std::to_string(h->ev_cap));
+        const int64_t n = hd.count, out = std::min<int64_t>(n, ev_cap);
+        *n_ev = n;
+        if (ev && out > 0) {
+            HIPCHK(hipMemcpy(ev, bb.ev, (size_t)out * sizeof(lss_event), hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < out; ++k) ev[k].sample += first_sample;   // the kernel writes column-0-relative samples
+        }
+        if (feat && out > 0)
+            HIPCHK(hipMemcpy(feat, bb.feat, (size_t)out * h->nC * h->K * sizeof(float), hipMemcpyDeviceToHost));
+        if (n > ev_cap)
+            throw LssError(LSS_ERR_CAPACITY, "event buffer holds " + std::to_string(ev_cap) + " of " +
+                                                 std::to_string(n) + " events");
+        if (h->pre) h->info.p2p = hd.p2p;
+        h->last_pos.clear();
+        h->info.events += n;
+        h->info.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
 
 namespace lss {
 void set_last_error(const std::string& msg) { g_last_error = msg; }

@@ -139,6 +139,21 @@ class OnlineSpikesV2:
         self.last_W = n_samples
         return ev[:n.value].copy(), feat[:n.value].copy()
 
+    def process_batch_device_static(self, dev_ptr: int, n_samples: int, ld: int, first_sample: int = 0, *,
+                                    chan_map=None, cap: int = 65536, features: bool = True):
+        """:meth:`process_batch_device` through the allocation-free entry point
+        (lss_process_batch_device_static: the batch's device buffers come from the
+        library's static scratch array)."""
+        cm = None if chan_map is None else np.ascontiguousarray(chan_map, dtype=np.int32)
+        ev, feat = self._out(cap)
+        n = ct.c_int64(0)
+        check(lib().lss_process_batch_device_static(self._h, ct.c_void_p(dev_ptr), n_samples, ld,
+                                                    None if cm is None else cm.ctypes.data,
+                                                    int(first_sample), ev.ctypes.data, cap, ct.byref(n),
+                                                    feat.ctypes.data if features else None))
+        self.last_W = n_samples
+        return ev[:n.value].copy(), feat[:n.value].copy()
+
     def process_chunk(self, chunk: np.ndarray, first_sample: int, *, chan_map=None,
                       cap: int = 65536):
         """Streaming form: next contiguous chunk ``[n][ld]`` of the stream."""
