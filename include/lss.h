@@ -174,7 +174,7 @@ int lss_process_batch_device(struct lss_handle* h, const int16_t* d_samples, int
 /* lss_process_batch_device without any device allocation: the batch's device
  * buffers (whitened batch, PC projection, scores, the peel's exchange words and
  * counters, the event staging) come from one static device array of
- * LSS_STATIC_SCRATCH_BYTES per GPU, held from the library's first use of the
+ * LSS_STATIC_SCRATCH_BYTES per GPU (128 MiB), held from the library's first use of the
  * array on that GPU until the process ends.  No counterpart in the reference:
  * it is for callers that cannot allocate during a batch (code under stream
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The handle
@@ -186,7 +186,7 @@ int lss_process_batch_device(struct lss_handle* h, const int16_t* d_samples, int
  * on one GPU (from several threads) race: serialise them.  Positions
  * (lss_last_positions) are not kept by this entry point.  Results are
  * bit-identical to lss_process_batch_device. */
-#define LSS_STATIC_SCRATCH_BYTES (96ll << 20)
+#define LSS_STATIC_SCRATCH_BYTES (128ll << 20)
 int lss_process_batch_device_static(struct lss_handle* h, const int16_t* d_samples, int64_t n_samples, int32_t ld,
                                     const int32_t* chan_map, int64_t first_sample, lss_event* ev, int64_t ev_cap,
                                     int64_t* n_ev, float* feat);

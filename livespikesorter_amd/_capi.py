@@ -23,7 +23,7 @@ LSS_ERR_CAPACITY = -4
 LSS_ERR_STATE = -5
 LSS_ERR_TIMEOUT = -6
 LSS_ERR_LIMIT = -7
-STATIC_SCRATCH_BYTES = 96 << 20     # LSS_STATIC_SCRATCH_BYTES
+STATIC_SCRATCH_BYTES = 128 << 20     # LSS_STATIC_SCRATCH_BYTES
 
 SEM_KILOSORT = 0
 SEM_LSS = 1

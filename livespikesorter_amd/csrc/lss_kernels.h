@@ -114,8 +114,8 @@ struct PeelParams {
     // iteration [max_iter], then per column block the iteration its owner finished
     // phase B [NB], then per workgroup the iteration it finished phase A [G]
     unsigned* bar; int* iter_cnt; int* err;
-    // features
-    const float* Bpc; int CP;
+    // features (and the FP32 rescoring): Bpc element (k*CP + c, w) at (k*CP + c) * bpc_sk + w * bpc_sw
+    const float* Bpc; int CP; int bpc_sk, bpc_sw;
     const float* U; int C, K, nC;
     const int* iCC; const int* iU;
     const float* Gt;                   // [K][K][2M-1]
@@ -142,6 +142,9 @@ struct PeelParams {
     // template whose BF16 score could be the column's FP32 maximum is known and
     // rescored too.
     const float* Nsq;
+    // Corr[t][w] (row stride BS): the FP32 correction of (t, w) once computed in this batch,
+    // NaN before (the BF16 projection fills it); corrections do not change during the peel
+    float* Corr;
     unsigned long long* n_rescored;    // running totals: FP32 rescores, threshold decisions flipped
     // proposal gate: score tile (64 columns, 64 templates) scored iff gate[(w/64) * gate_mt + t/64]
     const uint8_t* gate; int gate_mt;
@@ -195,6 +198,8 @@ struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
     const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
     float* C2;                            // optional second copy of the result
+    float* C3;                            // optional (BF16): filled with NaN, the FP32 correction cache
+                                          // of the peel (not yet computed)
     float* Nsq;                           // optional (BF16): per 64-row tile and column, the squared
                                           // norm of the BF16 B operand over the tile's listed K-blocks
     const uint8_t* gate; int gate_mt;     // optional tile mask [ntile][gate_mt]: 0 = not computed
@@ -204,6 +209,9 @@ struct ProjectJob {
 struct PcconvJob {
     const float* Xw; const float* wpca; float* Bpc;
     __bf16* Bpcb;                         // optional BF16 copy of the PC projection
+    int bpc_t;                            // nonzero: Bpc is written transposed, [BS][KCP] (BF16 mode:
+                                          // a column's K*C values contiguous, for the FP32 rescoring)
+    int KCP;
     const int16_t* const* active;         // optional: the job is skipped when *active is null
     int XS, M, BS, CP, wblocks;
     int C;                                // channels (rows PC_ROWS to a workgroup)

@@ -286,14 +286,14 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     }
     __syncthreads();
     const int rr = tid >> 6, l = tid & 63, c = cg * PC_ROWS + rr;
-    if (c >= C) return;
+    if (c >= C && !jb.bpc_t) return;
     float acc[4][KK];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
         for (int q = 0; q < KK; ++q) acc[jj][q] = 0.f;
     const float* xp = &xs[rr][4 * l];
-    for (int m0 = 0; m0 < M; m0 += 4) {
+    for (int m0 = 0; m0 < (c < C ? M : 0); m0 += 4) {
         const float4 a = *reinterpret_cast<const float4*>(xp + m0);
         const float4 b = *reinterpret_cast<const float4*>(xp + m0 + 4);
         const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -312,14 +312,37 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
         }
     }
     const int64_t o0 = (int64_t)c * jb.BS + w0 + 4 * l;
+    if (!jb.bpc_t) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) {
+            const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
+            *reinterpret_cast<float4*>(jb.Bpc + o) = make_float4(acc[0][q], acc[1][q], acc[2][q], acc[3][q]);
+            if (jb.Bpcb) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) jb.Bpcb[o + jj] = (__bf16)acc[jj][q];
+            }
+        }
+        return;
+    }
+    // transposed FP32 copy Bpc[w][k*CP + c]: the four channels of each (column, PC) go
+    // out as one 16-byte word through an LDS transpose (channels past C are zeros);
+    // the BF16 copy keeps the GEMM operand layout
+    __shared__ __align__(16) float tr[256][KK][PC_ROWS];
 #pragma unroll
     for (int q = 0; q < KK; ++q) {
-        const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
-        *reinterpret_cast<float4*>(jb.Bpc + o) = make_float4(acc[0][q], acc[1][q], acc[2][q], acc[3][q]);
-        if (jb.Bpcb) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) tr[4 * l + jj][q][rr] = acc[jj][q];
+        if (jb.Bpcb && c < C) {
+            const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) jb.Bpcb[o + jj] = (__bf16)acc[jj][q];
         }
+    }
+    __syncthreads();
+    for (int i = tid; i < 256 * KK; i += 256) {
+        const int ww = i / KK, q = i - ww * KK;
+        *reinterpret_cast<float4*>(jb.Bpc + (int64_t)(w0 + ww) * jb.KCP + q * jb.CP + cg * PC_ROWS) =
+            *reinterpret_cast<const float4*>(&tr[ww][q][0]);
     }
 }
 
@@ -406,6 +429,12 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     }
     store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
     if (jb.C2) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
+    if (jb.C3) {
+        f32x16 nan;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) nan[r] = __builtin_nanf("");
+        store_tile(jb.C3, jb.ldc, m0, n0, nan, lane, wm, wn);
+    }
     if (jb.Nsq) {
         nsq += __shfl_xor(nsq, 1);
         nsq += __shfl_xor(nsq, 2);
@@ -642,8 +671,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ float s_bv[CWX];
     __shared__ float s_cm[CWX];                // LSS pooled maxima
     __shared__ int s_flag[CWX];
-    __shared__ int s_rt[CWX];                  // FP32 rescoring: template whose correction is cached
-    __shared__ float s_rd[CWX];                //   and the correction (FP32 score - BF16 score)
     __shared__ float s_bcor[RS ? MAX_BAND : 1];  // FP32 rescoring: each listed pair's correction
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ __align__(16) int s_hcnt_w[PEEL_THREADS / 64];     // per-wave counts (read as int4)
@@ -690,7 +717,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     unsigned long long* const Pt = p.Pa + (int64_t)4 * S * W;   // templates
     unsigned bar_gen = 0;
     if (tid == 0) s_listn = 0;
-    for (int i = tid; i < CWX; i += PEEL_THREADS) s_rt[i] = -1;
 
     // Buffer descriptors: one scalar base per table plus a per-thread offset, so
     // the R rows of a tile need no address registers (row strides go in SGPRs).
@@ -1078,30 +1104,33 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 // accepted one in FP32).  The correction of (t, w) is fixed
                 // (subtractions are applied to both alike).
                 const int lane = tid & 63, wv = tid >> 6;
-                // FP32 dot of U2 row t with Bpc column ww over the K-blocks listed for t's
-                // 64-row tile (the others are exact zeros); every lane of the wave ends with
-                // the same, order-fixed sum.  8 elements per lane in flight at once: the
-                // block index, then the template weight and the (scattered) Bpc entry.
-                auto dot32 = [&](int t, int ww) {
+                // FP32 dot of U2 row t with Bpc column ww (a contiguous row of the transposed
+                // copy) over the K-blocks listed for t's 64-row tile (the others are exact
+                // zeros): four lanes per 16-wide block, one 16-byte load of each operand per
+                // lane and block, 32 blocks in flight per round; every lane of the wave ends
+                // with the same, order-fixed sum.
+                auto dot32 = [&](int t, int ww) __attribute__((always_inline)) {
                     const int mt = t / GEMM_BM;
-                    const int e0 = p.kb_ptr[mt] * GEMM_BK, e1 = p.kb_ptr[mt + 1] * GEMM_BK;
+                    const int j0 = p.kb_ptr[mt], j1 = p.kb_ptr[mt + 1];
                     const float* urow = p.U2 + (int64_t)t * p.ldu;
+                    const float* brow = p.Bpc + (int64_t)ww * p.bpc_sw;
+                    const int q4 = (lane & 3) * 4;
                     float acc = 0.f;
-                    for (int e = e0 + lane; e < e1; e += 64 * 8) {
-                        float uq[8], bq[8];
-                        int kc[8];
+                    for (int jr = j0; jr < j1; jr += 32) {
+                        int kb[2];
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int ee = e + 64 * q;
-                            kc[q] = ee < e1 ? p.kb_idx[ee / GEMM_BK] * GEMM_BK + ee % GEMM_BK : -1;
+                        for (int r = 0; r < 2; ++r) {
+                            const int j = jr + 16 * r + (lane >> 2);
+                            kb[r] = j < j1 ? p.kb_idx[j] * GEMM_BK + q4 : -1;
+                        }
+                        float4 u[2], b[2];
+#pragma unroll
+                        for (int r = 0; r < 2; ++r) {
+                            u[r] = kb[r] >= 0 ? *reinterpret_cast<const float4*>(urow + kb[r]) : make_float4(0.f, 0.f, 0.f, 0.f);
+                            b[r] = kb[r] >= 0 ? *reinterpret_cast<const float4*>(brow + kb[r]) : make_float4(0.f, 0.f, 0.f, 0.f);
                         }
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            uq[q] = kc[q] >= 0 ? urow[kc[q]] : 0.f;
-                            bq[q] = kc[q] >= 0 ? p.Bpc[(int64_t)kc[q] * p.BS + ww] : 0.f;
-                        }
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) acc += uq[q] * bq[q];
+                        for (int r = 0; r < 2; ++r) acc += ((u[r].x * b[r].x + u[r].y * b[r].y) + u[r].z * b[r].z) + u[r].w * b[r].w;
                     }
                     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
                     return acc;
@@ -1189,20 +1218,26 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 __syncthreads();
                 const int nb = min(s_nband, MAX_BAND);
                 if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
+                // corrections already computed in this batch (NaN: not yet), then one wave
+                // per missing pair; each (template, column) pair belongs to one workgroup
+                for (int q = tid; q < nb; q += PEEL_THREADS) {
+                    const int cl = s_bi[q] & 0xffff, t = s_bi[q] >> 16;
+                    s_bcor[q] = p.Corr[(int64_t)t * p.BS + c0 + cl];
+                }
+                __syncthreads();
                 for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
+                    const float cq = s_bcor[q];
+                    if (cq == cq) continue;                      // computed already (NaN: not yet); wave-uniform
                     const int cl = s_bi[q] & 0xffff, t = s_bi[q] >> 16, ww = c0 + cl;
-                    float corr;
-                    if (s_rt[cl] == t) {
-                        corr = s_rd[cl];
-                    } else {
-                        const float acc = dot32(t, ww);
-                        corr = acc - p.Binit[(int64_t)t * p.BS + ww];
+                    const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
+                    if (lane == 0) {
+                        s_bcor[q] = corr;
+                        p.Corr[(int64_t)t * p.BS + ww] = corr;
                     }
-                    if (lane == 0) s_bcor[q] = corr;
                 }
                 __syncthreads();
                 if (tid < CW && c0 + tid < c1 && s_cm[tid] > 0.f) {
-                    float bv = -1.f, bb = 0.f, bc = 0.f;
+                    float bv = -1.f, bb = 0.f;
                     int bt = 0, bo = 0x7fffffff;
                     for (int q = 0; q < nb; ++q) {
                         const int e = s_bi[q];
@@ -1213,14 +1248,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         const float r = fmaxf(b, 0.f);
                         const float cf = (r * r) / p.nm[t];
                         const int o = p.orig[t];
-                        if (better(cf, o, bv, bo)) { bv = cf; bo = o; bt = t; bb = b; bc = c; }
+                        if (better(cf, o, bv, bo)) { bv = cf; bo = o; bt = t; bb = b; }
                     }
                     if (bo != 0x7fffffff) {
                         s_cf[tid] = bv;
                         s_im[tid] = bt;
                         s_bv[tid] = bb;
-                        s_rt[tid] = bt;
-                        s_rd[tid] = bc;
                     }
                 }
                 const int listed = nb;
@@ -1596,7 +1629,8 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             const int ch = p.iCC[j * C + p.iU[si.t]];
             float f = 0.f;
             if (mine)
-                f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] + si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
+                f = p.Bpc[(int64_t)(k * p.CP + ch) * p.bpc_sk + (int64_t)si.w * p.bpc_sw] +
+                    si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
             // the spikes within +-(M-1) columns, in list order: a wave reads 64 at a time
             // and takes the near ones from the ballot, lowest index first
             for (int hh = flo; hh <= fhi; ++hh) {
