@@ -292,6 +292,9 @@ int lss_group_process_lane(struct lss_group* g, int32_t lane, const int16_t* con
                            int32_t device_input, const int64_t* first_sample, lss_event* const* ev,
                            const int64_t* ev_cap, int64_t* n_ev, float* const* feat);
 int lss_group_query(struct lss_group* g, int32_t stream, lss_info* info);
+/* lss_debug_stage stage 6 (the peel kernel's phase clock of the last launch)
+ * for one stream of a group.  No reference counterpart. */
+int lss_group_debug_stage(struct lss_group* g, int32_t stream, int32_t stage, void* out, int64_t* bytes);
 /* Device time of the launches on one lane of a group created with cfg->profile
  * set (a tick is then two graph launches, the stages in front of the peel and
  * the peel, with HIP events recorded on the lane's stream around them): ms[0] the

@@ -36,7 +36,7 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_process_batch_device_static",
                  "lss_process_chunk", "lss_flush", "lss_query", "lss_debug_stage", "lss_stage_times",
                  "lss_destroy", "lss_last_error", "lss_group_create", "lss_group_process",
-                 "lss_group_query", "lss_group_destroy", "lss_group_stage_times", "lss_proposal_n_params", "lss_proposal_create",
+                 "lss_group_query", "lss_group_destroy", "lss_group_stage_times", "lss_group_debug_stage", "lss_proposal_n_params", "lss_proposal_create",
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
                  "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions",
@@ -152,6 +152,7 @@ def lib() -> ct.CDLL:
         L.lss_group_query.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(Info)]
         L.lss_last_positions.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64)]
         L.lss_group_destroy.argtypes = [ct.c_void_p]
+        L.lss_group_debug_stage.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_void_p, ct.POINTER(ct.c_int64)]
         L.lss_group_stage_times.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_double),
                                             ct.POINTER(ct.c_int64), ct.c_int32]
         L.lss_proposal_n_params.restype = ct.c_int32

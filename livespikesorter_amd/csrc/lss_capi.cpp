@@ -1915,6 +1915,15 @@ int lss_group_stage_times(lss_group* g, int32_t lane, double* ms, int64_t* launc
     });
 }
 
+int lss_group_debug_stage(lss_group* g, int32_t stream, int32_t stage, void* out, int64_t* bytes) {
+    return guarded([&] {
+        if (!g || stream < 0 || stream >= g->n) throw LssError(LSS_ERR_ARG, "bad argument");
+        if (stage != 6) throw LssError(LSS_ERR_ARG, "groups report stage 6 (the peel's phase clock) only");
+        const int rc = lss_debug_stage(g->hs[stream].get(), stage, out, bytes);
+        if (rc != LSS_OK) throw LssError(rc, g_last_error);
+    });
+}
+
 int lss_group_query(lss_group* g, int32_t stream, lss_info* info) {
     return guarded([&] {
         if (!g || !info || stream < 0 || stream >= g->n) throw LssError(LSS_ERR_ARG, "bad argument");

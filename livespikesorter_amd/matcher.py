@@ -236,6 +236,13 @@ class StreamGroup:
         check(lib().lss_group_query(self._g, stream, ct.byref(info)))
         return info
 
+    def phases(self, stream: int) -> np.ndarray:
+        """The peel kernel's phase clock of the last launch for one stream (STAGE_PHASES layout)."""
+        out = np.zeros(16, np.uint64)
+        nb = ct.c_int64(out.nbytes)
+        check(lib().lss_group_debug_stage(self._g, stream, STAGE_PHASES, out.ctypes.data, ct.byref(nb)))
+        return out
+
     def stage_times(self, lane: int = 0, reset: bool = False):
         """(front-stage ms, peel ms, launches) accumulated on one lane (needs ``profile=True``)."""
         ms = (ct.c_double * 2)()
