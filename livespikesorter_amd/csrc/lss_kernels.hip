@@ -671,7 +671,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ float s_bv[CWX];
     __shared__ float s_cm[CWX];                // LSS pooled maxima
     __shared__ int s_flag[CWX];
-    __shared__ float s_bcor[RS ? MAX_BAND : 1];  // FP32 rescoring: each listed pair's correction
+    // FP32 rescoring: each listed pair's correction, then (register tiles) the columns'
+    // 64-bit selection keys
+    __shared__ __align__(8) float s_bcor[RS ? MAX_BAND + (R > 0 ? 2 * (PEEL_THREADS / REG_NRG) : 0) : 1];
+    __shared__ float s_nmx[RS && R > 0 ? PEEL_THREADS / REG_NRG : 1];   //   and each column's largest operand norm
     __shared__ int s_hcnt[MAX_NBR + 1];
     __shared__ __align__(16) int s_hcnt_w[PEEL_THREADS / 64];     // per-wave counts (read as int4)
     __shared__ __align__(16) int s_hcnt_o[PEEL_THREADS / 64];
@@ -723,6 +726,17 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     const __amdgpu_buffer_rsrc_t ctc_rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p.ctc), 0, p.ctc_bytes, BUF_DWORD3);
     float breg[R > 0 ? R : 1];
+    // the BF16 operand norm of this thread's tile at its column, and (column threads)
+    // the largest over the tiles: fixed for the batch
+    float nown = 0.f;
+    if (RS && R > 0 && p.rescore) {
+        if (w < W && rbase + rg * (R > 0 ? R : 1) < T) nown = sqrtf(p.Nsq[(int64_t)((rbase + rg * R) / GEMM_BM) * p.BS + w]);
+        if (tid < CW) {
+            float mx = 0.f;
+            for (int m = 0; m * GEMM_BM < T && c0 + tid < W; ++m) mx = fmaxf(mx, p.Nsq[(int64_t)m * p.BS + c0 + tid]);
+            s_nmx[R > 0 ? tid : 0] = sqrtf(mx);
+        }
+    }
     if (R > 0) {
         for (int t = tid; t < REG_NRG * R; t += PEEL_THREADS) s_nm[t] = p.rnm[rbase + t];
         for (int t = tid; t < min(p.ctc_rows, REG_MAX_T); t += PEEL_THREADS) {
@@ -895,6 +909,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     for (int h0 = 0; h0 < nhit; h0 += SPB) {
                         float am[SPB];
                         int vo[SPB];
+                        int blo = 1 << 30, bhi = 0;          // rows of the batch's nonzero ctc bands
     #pragma unroll
                         for (int j = 0; j < SPB; ++j) {
                             const bool in = h0 + j < nhit;
@@ -903,9 +918,15 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             const bool hit = in && l >= 0 && l <= 2 * M;
                             am[j] = hit ? sp.amp : 0.f;
                             vo[j] = ((sp.t * p.ctc_rows + r0) * LP + (hit ? l : 0)) * 4;
+                            const int bd = s_band[sp.t];
+                            blo = min(blo, bd & 0xffff);
+                            bhi = max(bhi, bd >> 16);
                         }
     #pragma unroll
                         for (int i0 = 0; i0 < R; i0 += RC) {
+                            // a load batch wholly outside the spikes' nonzero ctc rows changes
+                            // nothing (b - a * 0 is b): skipped (wave-uniform)
+                            if (RC < R && (r0 + i0 >= bhi || r0 + i0 + RC <= blo)) continue;
                             float c[SPB][RC];
     #pragma unroll
                             for (int j = 0; j < SPB; ++j)
@@ -1081,11 +1102,34 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 const int* pi = s_idx + vb;
                 const int* po = s_org + vb;
                 const float* pb = s_bvv + vb;
-                float bv = pv[0], bb = pb[0];
-                int bI = pi[0], bO = po[0];
-                for (int r = 1; r < nrg; ++r) {
-                    const int j = r * CW;
-                    if (better(pv[j], po[j], bv, bO)) { bv = pv[j]; bI = pi[j]; bO = po[j]; bb = pb[j]; }
+                float bv, bb;
+                int bI, bO;
+                if constexpr (R > 0) {
+                    // every row group's (score, file index) read at once, the winner chosen
+                    // in registers, then its template and B value: two LDS round trips
+                    float v[REG_NRG];
+                    int o[REG_NRG];
+#pragma unroll
+                    for (int r = 0; r < REG_NRG; ++r) { v[r] = pv[r * CW]; o[r] = po[r * CW]; }
+                    bv = v[0];
+                    bO = o[0];
+                    int br = 0;
+#pragma unroll
+                    for (int r = 1; r < REG_NRG; ++r) {
+                        const bool gt = better(v[r], o[r], bv, bO);
+                        bv = gt ? v[r] : bv;
+                        bO = gt ? o[r] : bO;
+                        br = gt ? r : br;
+                    }
+                    bI = pi[br * CW];
+                    bb = pb[br * CW];
+                } else {
+                    bv = pv[0]; bb = pb[0];
+                    bI = pi[0]; bO = po[0];
+                    for (int r = 1; r < nrg; ++r) {
+                        const int j = r * CW;
+                        if (better(pv[j], po[j], bv, bO)) { bv = pv[j]; bI = pi[j]; bO = po[j]; bb = pb[j]; }
+                    }
                 }
                 const int ww = c0 + tid;
                 if (ww < c1) {
@@ -1151,112 +1195,163 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     cf16 = s_cf[tid];
                     float sl = -1.f;
                     if (cf16 >= p.lo2) {
-                        const float nb = sqrtf(p.Nsq[(int64_t)(s_im[tid] / GEMM_BM) * p.BS + c0 + tid]);
+                        const float nb = R > 0 ? s_nmx[R > 0 ? tid : 0]
+                                               : sqrtf(p.Nsq[(int64_t)(s_im[tid] / GEMM_BM) * p.BS + c0 + tid]);
                         sl = fmaxf(sqrtf(cf16) - BF16_ERR * nb, 0x1p-60f);
                     }
                     s_cm[tid] = sl;                      // the band's floor in the sqrt(Cf) scale
                 }
-                int* const s_bi = R > 0 ? s_whit : s_qlo;        // column | template << 16
-                float* const s_bb = R > 0 ? reinterpret_cast<float*>(s_whit) + MAX_BAND
-                                          : reinterpret_cast<float*>(s_qhi);   // current BF16 score
-                if (tid == 0) s_nband = 0;
-                __syncthreads();
-                {
+                int listed = 0;
+                if constexpr (R > 0) {
+                    // Register tile.  Each thread lists its candidate rows (score plus the
+                    // BF16 bound reaching the floor) with their current BF16 score; their
+                    // corrections come from the batch's cache or, the first time, from an
+                    // FP32 dot (one wave per pair); then each listed pair offers its FP32
+                    // score to its column through one 64-bit LDS maximum of (score,
+                    // inverted file index, list position), and the column takes the winner.
+                    unsigned long long* const s_key = reinterpret_cast<unsigned long long*>(s_bcor + MAX_BAND);
+                    int* const s_pl = s_whit;                                        // column | row << 16
+                    float* const s_pv = reinterpret_cast<float*>(s_whit) + MAX_BAND;  // BF16 score, then FP32
+                    if (tid < CW) s_key[tid] = 0ull;
+                    if (tid == 0) s_nband = 0;
+                    __syncthreads();
+                    const float sl = own ? s_cm[col] : -1.f;
+                    const int r0 = rbase + rg * R;
+                    if (own && !edge && !woff && sl > 0.f && r0 < T) {
+                        const float e = sl - BF16_ERR * nown;
+                        const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
+                        const float* rnp = s_nm + rg * R;
+                        uint64_t mask = 0;
+#pragma unroll
+                        for (int i0 = 0; i0 < R; i0 += 8) {
+#pragma unroll
+                            for (int i = i0; i < i0 + 8 && i < R; ++i) {
+                                const float r = breg[i] > 0.f ? breg[i] : 0.f;
+                                const float a = (r * r) * rnp[i];
+                                mask |= (uint64_t)(a >= thr && r0 + i < T) << i;
+                            }
+                            asm volatile("" ::: "memory");
+                        }
+                        while (mask) {
+                            const int i = __builtin_ctzll(mask);
+                            mask &= mask - 1ull;
+                            const int k = atomicAdd(&s_nband, 1);
+                            if (k >= MAX_BAND) break;
+                            float v = 0.f;
+#pragma unroll
+                            for (int i2 = 0; i2 < R; ++i2) v = i2 == i ? breg[i2] : v;
+                            s_pl[k] = col | ((r0 + i) << 16);
+                            s_pv[k] = v;
+                        }
+                    }
+                    __syncthreads();
+                    const int nb = min(s_nband, MAX_BAND);
+                    if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
+                    for (int q = tid; q < nb; q += PEEL_THREADS) {
+                        const int e2 = s_pl[q];
+                        s_bcor[q] = p.Corr[(int64_t)(e2 >> 16) * p.BS + c0 + (e2 & 0xffff)];
+                    }
+                    __syncthreads();
+                    for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
+                        const float cq = s_bcor[q];
+                        if (cq == cq) continue;                  // cached (NaN: not yet); wave-uniform
+                        const int e2 = s_pl[q];
+                        const int cl = e2 & 0xffff, t = e2 >> 16, ww = c0 + cl;
+                        const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
+                        if (lane == 0) {
+                            s_bcor[q] = corr;
+                            p.Corr[(int64_t)t * p.BS + ww] = corr;
+                        }
+                    }
+                    __syncthreads();
+                    for (int q = tid; q < nb; q += PEEL_THREADS) {
+                        const int e2 = s_pl[q];
+                        const int t = e2 >> 16;
+                        const float b = s_pv[q] + s_bcor[q];
+                        const float r = fmaxf(b, 0.f);
+                        const float cf = (r * r) / s_nmf[t];
+                        s_pv[q] = b;
+                        atomicMax(&s_key[e2 & 0xffff], ((unsigned long long)__float_as_uint(cf) << 32) |
+                                                           ((unsigned long long)(0xffffu - (unsigned)s_or[t]) << 16) |
+                                                           (unsigned)q);
+                    }
+                    __syncthreads();
+                    if (tid < CW && c0 + tid < c1 && s_cm[tid] > 0.f && s_key[tid] != 0ull) {
+                        const unsigned long long kb = s_key[tid];
+                        const int q = (int)(kb & 0xffffull);
+                        s_cf[tid] = __uint_as_float((unsigned)(kb >> 32));
+                        s_im[tid] = s_pl[q] >> 16;
+                        s_bv[tid] = s_pv[q];
+                    }
+                    listed = nb;
+                } else {
+                    // Global tile: the same list / cache / dot steps over the rows of B in
+                    // global memory (each row's BF16 bound from its tile's operand norm);
+                    // every (template, column) pair belongs to one workgroup, and each
+                    // column scans the list for its FP32 maximum.
+                    int* const s_bi = s_qlo;                                   // column | template << 16
+                    float* const s_bb = reinterpret_cast<float*>(s_qhi);       // BF16 score, then FP32
+                    if (tid == 0) s_nband = 0;
+                    __syncthreads();
                     const float sl = own ? s_cm[col] : -1.f;
                     if (own && !edge && !woff && sl > 0.f) {
-                        if (R > 0 && rbase + rg * R < T) {     // (rows past T are padding)
-                            const int r0 = rbase + rg * R;
-                            const float nn = sqrtf(p.Nsq[(int64_t)(r0 / GEMM_BM) * p.BS + w]);
+                        for (int t = rg; t < T; t += nrg) {
+                            if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) continue;
+                            const float b = p.B[(int64_t)t * p.BS + w];
+                            const float r = fmaxf(b, 0.f);
+                            const float a = (r * r) * p.rnm[t];
+                            const float nn = sqrtf(p.Nsq[(int64_t)(t / GEMM_BM) * p.BS + w]);
                             const float e = sl - BF16_ERR * nn;
                             const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
-                            const float* rnp = s_nm + rg * R;
-                            // the rows' candidate bits first (branch-free, 8 rows between
-                            // compiler barriers so the norm loads are not all hoisted), then
-                            // the listed ones, each row's value picked by a select chain (no
-                            // register-array indexing)
-                            uint64_t mask = 0;
-#pragma unroll
-                            for (int i0 = 0; i0 < (R > 0 ? R : 1); i0 += 8) {
-#pragma unroll
-                                for (int i = i0; i < i0 + 8 && i < (R > 0 ? R : 1); ++i) {
-                                    const float r = breg[i] > 0.f ? breg[i] : 0.f;
-                                    const float a = (r * r) * rnp[i];
-                                    mask |= (uint64_t)(a >= thr && r0 + i < T) << i;
-                                }
-                                asm volatile("" ::: "memory");
-                            }
-                            while (mask) {
-                                const int i = __builtin_ctzll(mask);
-                                mask &= mask - 1ull;
+                            if (a >= thr) {
                                 const int k = atomicAdd(&s_nband, 1);
                                 if (k >= MAX_BAND) break;
-                                float v = 0.f;
-#pragma unroll
-                                for (int i2 = 0; i2 < (R > 0 ? R : 1); ++i2) v = i2 == i ? breg[i2] : v;
-                                s_bi[k] = col | ((r0 + i) << 16);
-                                s_bb[k] = v;
-                            }
-                        } else if (R == 0) {
-                            for (int t = rg; t < T; t += nrg) {
-                                if (p.gate && !p.gate[(w / GEMM_BN) * p.gate_mt + t / GEMM_BM]) continue;
-                                const float b = p.B[(int64_t)t * p.BS + w];
-                                const float r = fmaxf(b, 0.f);
-                                const float a = (r * r) * p.rnm[t];
-                                const float nn = sqrtf(p.Nsq[(int64_t)(t / GEMM_BM) * p.BS + w]);
-                                const float e = sl - BF16_ERR * nn;
-                                const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
-                                if (a >= thr) {
-                                    const int k = atomicAdd(&s_nband, 1);
-                                    if (k >= MAX_BAND) break;
-                                    s_bi[k] = col | (t << 16);
-                                    s_bb[k] = b;
-                                }
+                                s_bi[k] = col | (t << 16);
+                                s_bb[k] = b;
                             }
                         }
                     }
-                }
-                __syncthreads();
-                const int nb = min(s_nband, MAX_BAND);
-                if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
-                // corrections already computed in this batch (NaN: not yet), then one wave
-                // per missing pair; each (template, column) pair belongs to one workgroup
-                for (int q = tid; q < nb; q += PEEL_THREADS) {
-                    const int cl = s_bi[q] & 0xffff, t = s_bi[q] >> 16;
-                    s_bcor[q] = p.Corr[(int64_t)t * p.BS + c0 + cl];
-                }
-                __syncthreads();
-                for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
-                    const float cq = s_bcor[q];
-                    if (cq == cq) continue;                      // computed already (NaN: not yet); wave-uniform
-                    const int cl = s_bi[q] & 0xffff, t = s_bi[q] >> 16, ww = c0 + cl;
-                    const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
-                    if (lane == 0) {
-                        s_bcor[q] = corr;
-                        p.Corr[(int64_t)t * p.BS + ww] = corr;
+                    __syncthreads();
+                    const int nb = min(s_nband, MAX_BAND);
+                    if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
+                    for (int q = tid; q < nb; q += PEEL_THREADS) {
+                        const int e2 = s_bi[q];
+                        s_bcor[q] = p.Corr[(int64_t)(e2 >> 16) * p.BS + c0 + (e2 & 0xffff)];
                     }
-                }
-                __syncthreads();
-                if (tid < CW && c0 + tid < c1 && s_cm[tid] > 0.f) {
-                    float bv = -1.f, bb = 0.f;
-                    int bt = 0, bo = 0x7fffffff;
-                    for (int q = 0; q < nb; ++q) {
-                        const int e = s_bi[q];
-                        if ((e & 0xffff) != tid) continue;
-                        const int t = e >> 16;
-                        const float c = s_bcor[q];
-                        const float b = s_bb[q] + c;
-                        const float r = fmaxf(b, 0.f);
-                        const float cf = (r * r) / p.nm[t];
-                        const int o = p.orig[t];
-                        if (better(cf, o, bv, bo)) { bv = cf; bo = o; bt = t; bb = b; }
+                    __syncthreads();
+                    for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
+                        const float cq = s_bcor[q];
+                        if (cq == cq) continue;                  // cached (NaN: not yet); wave-uniform
+                        const int e2 = s_bi[q];
+                        const int cl = e2 & 0xffff, t = e2 >> 16, ww = c0 + cl;
+                        const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
+                        if (lane == 0) {
+                            s_bcor[q] = corr;
+                            p.Corr[(int64_t)t * p.BS + ww] = corr;
+                        }
                     }
-                    if (bo != 0x7fffffff) {
-                        s_cf[tid] = bv;
-                        s_im[tid] = bt;
-                        s_bv[tid] = bb;
+                    __syncthreads();
+                    if (tid < CW && c0 + tid < c1 && s_cm[tid] > 0.f) {
+                        float bv = -1.f, bb = 0.f;
+                        int bt = 0, bo = 0x7fffffff;
+                        for (int q = 0; q < nb; ++q) {
+                            const int e2 = s_bi[q];
+                            if ((e2 & 0xffff) != tid) continue;
+                            const int t = e2 >> 16;
+                            const float b = s_bb[q] + s_bcor[q];
+                            const float r = fmaxf(b, 0.f);
+                            const float cf = (r * r) / p.nm[t];
+                            const int o = p.orig[t];
+                            if (better(cf, o, bv, bo)) { bv = cf; bo = o; bt = t; bb = b; }
+                        }
+                        if (bo != 0x7fffffff) {
+                            s_cf[tid] = bv;
+                            s_im[tid] = bt;
+                            s_bv[tid] = bb;
+                        }
                     }
+                    listed = nb;
                 }
-                const int listed = nb;
                 int flip = 0;
                 if (tid < CW && c0 + tid < c1 && cf16 >= p.lo2) flip = (cf16 > p.Th2) != (s_cf[tid] > p.Th2);
                 const uint64_t fb = __ballot(flip);
