@@ -767,7 +767,7 @@ def bench_cfg2(args) -> dict:
     chunks with 61-sample margins (W = 3122), 60 s resident in HBM."""
     from livespikesorter_amd import OnlineSpikesV2
     from livespikesorter_amd._hip import DeviceBuffer, synchronize
-    from livespikesorter_amd.matcher import STAGE_PHASES
+    from livespikesorter_amd.matcher import NPHASE, PHASE_NAMES, STAGE_PHASES
 
     N, M = 3000, 61
     W = N + 2 * M
@@ -816,7 +816,7 @@ def bench_cfg2(args) -> dict:
     # and the same peel split, over the first chunks of the timed range: the other
     # stages' device times and the peel's phase clock (read back per chunk with a
     # synchronous copy).
-    phases = np.zeros(16)
+    phases = np.zeros(2 * NPHASE + 2)
     n_ph = min(args.steps, 100)
     with OnlineSpikesV2(d, chunk=N, profile=1, peel_split=int(q_last.peel_split)) as spd:
         for j in range(args.warmup, args.warmup + n_ph):
@@ -833,12 +833,11 @@ def bench_cfg2(args) -> dict:
     log("device ms per chunk by stage (peel: timed region; others: diagnostic pass): " +
         ", ".join(f"{k} {v:.4f}" for k, v in avg.items()))
     # the peel kernel's own clock (100 MHz ticks), workgroup 0, per chunk
-    phase_names = ("exchange", "scan", "publish", "accept", "wait", "epilogue", "subtract")
-    nwg = max(phases[15], 1.0)
+    nwg = max(phases[2 * NPHASE + 1], 1.0)
     peel_phases = {k: {"mean_wg": round(phases[i] / nwg / 100.0, 3),
-                       "max_wg": round(phases[7 + i] / n_ph / 100.0, 3)}
-                   for i, k in enumerate(phase_names)}
-    peel_phases["scan_passes_per_wg"] = round(phases[14] / nwg, 2)
+                       "max_wg": round(phases[NPHASE + i] / n_ph / 100.0, 3)}
+                   for i, k in enumerate(PHASE_NAMES)}
+    peel_phases["scan_passes_per_wg"] = round(phases[2 * NPHASE] / nwg, 2)
     log(f"peel phases, us per chunk (mean over workgroups / largest): {peel_phases}")
 
     # host-resident input: chunk latency from int16 on the host to events on the host

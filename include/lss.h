@@ -211,11 +211,12 @@ int lss_query(struct lss_handle* h, lss_info* info);
  * keep_stages).  stage: 0 Xw [C][W] f32, 1 Bpc [K][C][W] f32, 2 B before the
  * peel [T][W] f32, 3 Cfmax of the first iteration [W] f32, 4 imax [W] i32,
  * 5 template norms [T] f32, 6 peel-kernel time per phase of the last batch,
- * u64 ticks of the 100 MHz device clock: [0..6] summed over workgroups and
- * [7..13] the largest workgroup's (top-of-iteration exchange, scan, publishing
- * the column maxima, acceptance, waiting for and reading the neighbours'
- * maxima, events and features, subtraction), [14] scan passes that did work,
- * [15] workgroups, 7 the ctc table the matcher uses, file layout
+ * u64 ticks of the 100 MHz device clock: [0..8] summed over workgroups and
+ * [9..17] the largest workgroup's (top-of-iteration exchange, own scan,
+ * publishing the column maxima, acceptance, waiting for and reading the
+ * neighbours' maxima, events and features, own subtraction, the phase-A
+ * barrier (the other waves' subtraction and scan), column reduction and FP32
+ * rescoring), [18] scan passes that did work, [19] workgroups, 7 the ctc table the matcher uses, file layout
  * [T][T][2M+1] f32 (as ctc.npy, or as computed at load time without it; needs
  * neither keep_stages nor a batch); with preprocessing on: 8 the preprocessed
  * batch [C][W] f32 (the whitening input), 9 channel means [C] f32, 10 per-sample

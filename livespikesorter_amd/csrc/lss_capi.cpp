@@ -1657,6 +1657,10 @@ struct lss_group {
     bool profile = false;
     std::vector<int> S;   // per stream: template subsets per column block in the peel grid
     GroupLane lanes[GROUP_LANES];
+    // Lanes run on several host threads.  Allocations, synchronous copies and graph
+    // captures go under this lock, so no lane's capture sees another thread's
+    // allocating or synchronizing call in flight.
+    std::mutex mu;
     ~lss_group() {
         (void)hipSetDevice(dev);
         for (auto& L : lanes) {
@@ -1846,18 +1850,25 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
                 continue;
             }
             ++active;
-            set_cmap(h, nullptr, ld[s2]);
+            int16_t* dx = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(g->mu);
+                set_cmap(h, nullptr, ld[s2]);
+                if (!device_input) dx = ensure_dx(h, (size_t)g->W * ld[s2]);
+            }
             if (device_input) {
                 L.h_xptr[s2] = samples[s2];
             } else {
-                int16_t* dx = ensure_dx(h, (size_t)g->W * ld[s2]);
                 HIPCHK(hipMemcpyAsync(dx, samples[s2], (size_t)g->W * ld[s2] * sizeof(int16_t),
                                       hipMemcpyHostToDevice, L.stream));
                 L.h_xptr[s2] = dx;
             }
         }
         if (!active) return;
-        if (!L.exec || !std::equal(ld, ld + n, L.ld_captured.begin())) group_capture(g, L, ld);
+        if (!L.exec || !std::equal(ld, ld + n, L.ld_captured.begin())) {
+            std::lock_guard<std::mutex> lk(g->mu);
+            group_capture(g, L, ld);
+        }
         if (g->profile) {
             HIPCHK(hipEventRecord(L.ev[0], L.stream));
             HIPCHK(hipGraphLaunch(L.exec, L.stream));

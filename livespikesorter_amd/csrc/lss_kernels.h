@@ -81,7 +81,7 @@ struct OutHeader {  // written by the peel kernel into mapped host memory
 // for and the reading of the neighbours' maxima (LSS rules: for their spikes),
 // epilogue, phase A subtraction; then the number of phase-A passes that did
 // work and the number of workgroups (summed).
-constexpr int NPHASE = 7;
+constexpr int NPHASE = 9;
 constexpr int PROF_WORDS = 2 * NPHASE + 2;
 
 // word 3 of a raw buffer descriptor on gfx9-family parts (gfx950): 32-bit data format

@@ -682,7 +682,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     __shared__ int s_qlo[R == 0 ? MAX_STAGE : 1], s_qhi[R == 0 ? MAX_STAGE : 1];
     // per-wave list of band-hitting spikes (R > 0; the FP32 rescoring's pair list after phase A)
     __shared__ int s_whit[R > 0 ? PEEL_THREADS : 1];
-    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk, s_nband;
+    __shared__ int s_nsp, s_listn, s_off, s_n, s_tot, s_err, s_ncand, s_kchk, s_nband, s_nnan;
     __shared__ int s_wt[NACC];                 // local acceptance: winning template per column
     __shared__ float s_wb[NACC];               //   and its B value
     int* const s_rank = reinterpret_cast<int*>(s_val);
@@ -1090,8 +1090,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     }
                 }
             }
+            mark(1);
             if (wdirty) { s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo; s_bvv[tid] = bval; }
             __syncthreads();
+            mark(7);
             if (tid < CW) {
                 // per-thread base pointers: the row-group strides become immediate
                 // offsets.  The empty asm keeps the base from being treated as loop
@@ -1213,15 +1215,19 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     int* const s_pl = s_whit;                                        // column | row << 16
                     float* const s_pv = reinterpret_cast<float*>(s_whit) + MAX_BAND;  // BF16 score, then FP32
                     if (tid < CW) s_key[tid] = 0ull;
-                    if (tid == 0) s_nband = 0;
+                    if (tid == 0) { s_nband = 0; s_nnan = 0; }
                     __syncthreads();
                     const float sl = own ? s_cm[col] : -1.f;
                     const int r0 = rbase + rg * R;
-                    if (own && !edge && !woff && sl > 0.f && r0 < T) {
-                        const float e = sl - BF16_ERR * nown;
-                        const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
+                    // A thread's best score over its rows (this pass's scan, or the last
+                    // one's when its rows did not change) bounds every row's a = r^2 * (1/nm)
+                    // from above to within 2^-22, so a thread whose best is under the
+                    // threshold has no row in the band and skips the row tests.
+                    const float e = sl - BF16_ERR * nown;
+                    const float thr = e > 0.f ? e * e * (1.f - 0x1p-20f) : -1.f;
+                    uint64_t mask = 0;
+                    if (own && !edge && !woff && sl > 0.f && r0 < T && s_val[tid] >= thr * (1.f - 0x1p-18f)) {
                         const float* rnp = s_nm + rg * R;
-                        uint64_t mask = 0;
 #pragma unroll
                         for (int i0 = 0; i0 < R; i0 += 8) {
 #pragma unroll
@@ -1232,29 +1238,52 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             }
                             asm volatile("" ::: "memory");
                         }
-                        while (mask) {
-                            const int i = __builtin_ctzll(mask);
-                            mask &= mask - 1ull;
-                            const int k = atomicAdd(&s_nband, 1);
-                            if (k >= MAX_BAND) break;
-                            float v = 0.f;
+                    }
+                    // the wave's listed rows get consecutive slots (a prefix sum over the
+                    // lanes, one LDS atomic per wave); each lane then writes its rows in
+                    // row order, the register picked by the unrolled row index
+                    {
+                        const int cnt = __popcll(mask);
+                        int incl = cnt;
 #pragma unroll
-                            for (int i2 = 0; i2 < R; ++i2) v = i2 == i ? breg[i2] : v;
-                            s_pl[k] = col | ((r0 + i) << 16);
-                            s_pv[k] = v;
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const int y = __shfl_up(incl, o);
+                            incl += lane >= o ? y : 0;
+                        }
+                        const int wtot = __shfl(incl, 63);
+                        if (wtot > 0) {
+                            int base = 0;
+                            if (lane == 63) base = atomicAdd(&s_nband, wtot);
+                            int k = __shfl(base, 63) + incl - cnt;
+#pragma unroll
+                            for (int i = 0; i < R; ++i) {
+                                if ((mask >> i) & 1ull) {
+                                    if (k < MAX_BAND) {
+                                        s_pl[k] = col | ((r0 + i) << 16);
+                                        s_pv[k] = breg[i];
+                                    }
+                                    ++k;
+                                }
+                            }
                         }
                     }
                     __syncthreads();
                     const int nb = min(s_nband, MAX_BAND);
                     if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
+                    // cached corrections (NaN: not computed in this batch yet); the missing
+                    // ones are compacted (into the staged-spike array, dead until acceptance)
+                    // so the waves share them evenly
+                    int* const s_nanq = reinterpret_cast<int*>(s_sp);
                     for (int q = tid; q < nb; q += PEEL_THREADS) {
                         const int e2 = s_pl[q];
-                        s_bcor[q] = p.Corr[(int64_t)(e2 >> 16) * p.BS + c0 + (e2 & 0xffff)];
+                        const float c = p.Corr[(int64_t)(e2 >> 16) * p.BS + c0 + (e2 & 0xffff)];
+                        s_bcor[q] = c;
+                        if (c != c) s_nanq[atomicAdd(&s_nnan, 1)] = q;
                     }
                     __syncthreads();
-                    for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
-                        const float cq = s_bcor[q];
-                        if (cq == cq) continue;                  // cached (NaN: not yet); wave-uniform
+                    const int nnan = s_nnan;
+                    for (int j = wv; j < nnan; j += PEEL_THREADS / 64) {
+                        const int q = s_nanq[j];
                         const int e2 = s_pl[q];
                         const int cl = e2 & 0xffff, t = e2 >> 16, ww = c0 + cl;
                         const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
@@ -1359,7 +1388,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 if (lane == 0 && fb) atomicAdd(&p.n_rescored[1], (unsigned long long)__popcll(fb));
             }
         }
-        mark(1);
+        mark(8);
         if (local) {
             // ---------------- local acceptance (Kilosort rules) ----------------
             if (tid < CW && c0 + tid < c1) publish(tid, it & 1, ep | (unsigned)(it + 1));

@@ -24,6 +24,11 @@ __all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS
 (STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES, STAGE_CTC, STAGE_PRE,
  STAGE_PRE_MEAN, STAGE_PRE_MED) = range(11)
 
+# the peel kernel's phase clock (STAGE_PHASES): per phase the sum over workgroups,
+# then the largest workgroup's, then scan passes and workgroups
+PHASE_NAMES = ("exchange", "scan", "publish", "accept", "wait", "epilogue", "subtract", "barrier", "reduce")
+NPHASE = len(PHASE_NAMES)
+
 
 class OnlineSpikesV2:
     """One matcher instance (one probe stream) on one HIP device."""
@@ -189,7 +194,7 @@ class OnlineSpikesV2:
         shapes = {STAGE_XW: ((info.C, W), np.float32), STAGE_BPC: ((info.K, info.C, W), np.float32),
                   STAGE_B: ((info.T, W), np.float32), STAGE_CFMAX: ((W,), np.float32),
                   STAGE_IMAX: ((W,), np.int32), STAGE_NM: ((info.T,), np.float32),
-                  STAGE_PHASES: ((16,), np.uint64), STAGE_PRE: ((info.C, W), np.float32),
+                  STAGE_PHASES: ((2 * NPHASE + 2,), np.uint64), STAGE_PRE: ((info.C, W), np.float32),
                   STAGE_PRE_MEAN: ((info.C,), np.float32), STAGE_PRE_MED: ((W,), np.float32)}
         shape, dt = shapes[stage]
         out = np.zeros(shape, dt)
@@ -238,7 +243,7 @@ class StreamGroup:
 
     def phases(self, stream: int) -> np.ndarray:
         """The peel kernel's phase clock of the last launch for one stream (STAGE_PHASES layout)."""
-        out = np.zeros(16, np.uint64)
+        out = np.zeros(2 * NPHASE + 2, np.uint64)
         nb = ct.c_int64(out.nbytes)
         check(lib().lss_group_debug_stage(self._g, stream, STAGE_PHASES, out.ctypes.data, ct.byref(nb)))
         return out
