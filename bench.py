@@ -460,7 +460,8 @@ def bench_target(args) -> dict:
                 "kernel": f"k_peel<{g['peel_geometry']['rows_per_thread']}> ({S} streams in one grid)",
                 "ms_per_launch": round(peel_avg, 5), "algorithmic_bytes": round(peel_b, 1)}
     # spike-list differences of every mode against the FP64 oracle on a sample, each
-    # classified by the oracle's decision margins (near tie / cascade / unexplained)
+    # classified by the oracle's decision margins (near tie / cascade / unexplained; the gated
+    # mode's also against its ungated run: gate miss)
     explain = None
     import oracle
     tab = oracle.load_tables(d)
@@ -474,15 +475,20 @@ def bench_target(args) -> dict:
                 refs64[(j, s)] = oracle.run_model_batch(tab, recs[s][j * N:j * N + W], margins=True, threads=threads,
                                                         features=False)
         for name, outs in outs_by_mode.items():
+            # the gated mode's differences are also checked against the same matcher without the gate
+            ung = outs_by_mode.get("group_bf16") if name == "group_bf16_gated" else None
             agg = {"events_fp64": 0, "identical": 0, "differing": 0, "near_tie": 0, "cascade": 0, "unexplained": 0}
+            if ung is not None:
+                agg["gate_miss"] = 0
             for (j, s), r in refs64.items():
                 ev = outs[j][s][0]
                 i, _, _ = agreement(r.w + j * N - M // 2 + m.nt0min, r.t, ev["sample"], ev["template"])
-                ex = explain_mismatches(r, ev, first_sample=j * N, M=M, nt0min=m.nt0min)
+                ex = explain_mismatches(r, ev, first_sample=j * N, M=M, nt0min=m.nt0min,
+                                        ungated=None if ung is None else ung[j][s][0])
                 agg["events_fp64"] += len(r.w)
                 agg["identical"] += int(round(i * max(len(r.w), len(ev))))
                 agg["differing"] += ex["events"]
-                for k in ("near_tie", "cascade", "unexplained"):
+                for k in ("near_tie", "cascade", "unexplained") + (("gate_miss",) if ung is not None else ()):
                     agg[k] += ex[k]
             agg["chunks"] = len(refs64)
             explain[name] = agg
@@ -714,9 +720,9 @@ def main() -> int:
     ap.add_argument("--no-secondary", action="store_true", help="target: skip the cfg2 secondary keys")
     ap.add_argument("--explain-ticks", type=int, default=1,
                     help="target: ticks whose spike lists are compared with the FP64 oracle (mismatch margins)")
-    ap.add_argument("--gate-lo", type=float, default=0.25, help="target: proposal gate fallback band, low edge")
-    ap.add_argument("--gate-hi", type=float, default=0.75, help="target: proposal gate acceptance threshold")
-    ap.add_argument("--gate-reach", type=int, default=4, help="target: proposal gate reach in columns")
+    ap.add_argument("--gate-lo", type=float, default=0.1, help="target: proposal gate fallback band, low edge")
+    ap.add_argument("--gate-hi", type=float, default=0.5, help="target: proposal gate acceptance threshold")
+    ap.add_argument("--gate-reach", type=int, default=12, help="target: proposal gate reach in columns")
     ap.add_argument("--bf16-seconds", type=float, default=120.0, help="recording length (bf16)")
     ap.add_argument("--bf16-compare", type=int, default=12, help="batches compared with FP32 (bf16)")
     ap.add_argument("--group-seconds", type=float, default=6.0, help="recording length per stream (group)")

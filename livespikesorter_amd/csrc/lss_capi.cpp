@@ -1110,7 +1110,14 @@ void run_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, int W, in
         HIPCHK(hipMemcpyAsync(bb.dbgB, bb.B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
     if (prof_peel) HIPCHK(hipEventRecord(h->evt[6], s));
     HIPCHK(hipGetLastError());
-    HIPCHK(launch_peel(kb, R, h->coop, s));
+    {
+        const hipError_t e = launch_peel(kb, R, h->coop, s);
+        if (e == hipErrorIllegalState && h->coop)
+            // seen when another process on the same GPU holds its own cooperative launches
+            throw LssError(LSS_ERR_HIP, std::string("cooperative peel launch refused (") + hipGetErrorString(e) +
+                                            "): one process per GPU, or LSS_PEEL_LAUNCH=plain on an otherwise idle GPU");
+        HIPCHK(e);
+    }
     if (prof_peel) HIPCHK(hipEventRecord(h->evt[7], s));
     HIPCHK(hipStreamSynchronize(s));
     if (prof_peel) {

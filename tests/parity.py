@@ -61,16 +61,20 @@ TIE_RTOL = 1e-3
 
 
 def explain_mismatches(ref, ev, *, first_sample: int = 0, M: int = 61, nt0min: int = 20,
-                       tol: float = TIE_RTOL, tol_samples: int = 1):
+                       tol: float = TIE_RTOL, tol_samples: int = 1, ungated=None):
     """Classify every event that is not identical in (sample, template) between
     an oracle result ``ref`` (run with ``margins=True``) and GPU events ``ev``.
 
     An event at batch column c is a *near tie* when one of the oracle's decision
     margins (threshold, template, pooling; ``oracle.c`` peel_m) over columns
-    c - 1 .. c + 1 is below ``tol``; a *cascade* when it lies within 2M columns
-    of a near tie (a decision taken the other way changes the subtractions, and
-    so the later decisions, within 2M); otherwise *unexplained*.  Returns a
-    dict with the counts, the events' smallest margins and a list of the
+    c - 1 .. c + 1 is below ``tol``; with ``ungated`` (the events of the same
+    matcher on the same batch without the proposal gate) a *gate miss* when the
+    ungated run agrees with the oracle there (it has the oracle's event, or not
+    the gated run's extra one): the gate left that spike's score tile
+    uncomputed; a *cascade* when it lies within 2M columns of a near tie or a
+    gate miss (a decision taken the other way changes the subtractions, and so
+    the later decisions, within 2M); otherwise *unexplained*.  Returns a dict
+    with the counts, the events' smallest margins and a list of the
     unexplained ones."""
     mg = ref.margins
     if mg is None:
@@ -94,18 +98,31 @@ def explain_mismatches(ref, ev, *, first_sample: int = 0, M: int = 61, nt0min: i
 
     ties, out = [], {"near_tie": 0, "cascade": 0, "unexplained": 0, "events": len(cols), "kinds": {},
                      "margins": [], "unexplained_events": []}
+    ug = None
+    if ungated is not None:
+        out["gate_miss"] = 0
+        ug = {(int(a), int(b)) for a, b in zip(ungated["sample"] - shift, ungated["template"])}
+
+    def gate_caused(side, c, t):
+        if ug is None:
+            return False
+        return ((c, t) in ug) if side == "oracle" else ((c, t) not in ug)
+
     info = []
     for side, c, t in cols:
         m, k = local(c) if 0 <= c < W else (float("inf"), "outside")
-        info.append((side, c, t, m, k))
-        if m <= tol:
+        g = gate_caused(side, c, t)
+        info.append((side, c, t, m, k, g))
+        if m <= tol or g:
             ties.append(c)
     ties = np.asarray(sorted(ties), np.int64)
-    for side, c, t, m, k in info:
+    for side, c, t, m, k, g in info:
         out["margins"].append(m)
         if m <= tol:
             out["near_tie"] += 1
             out["kinds"][k] = out["kinds"].get(k, 0) + 1
+        elif g:
+            out["gate_miss"] += 1
         elif len(ties) and np.abs(ties - c).min() <= 2 * M:
             out["cascade"] += 1
         else:

@@ -11,6 +11,7 @@ policy, the fetch buffer's stale rows, saveSpikes, payload, spike lines) is
 compared on the CPU through all five window kinds; the GPU test runs the
 library's matcher in both loops.
 """
+import os
 import subprocess
 from pathlib import Path
 from types import SimpleNamespace
@@ -90,9 +91,13 @@ def _run_cpp(mode, oss, ap, n_chan_file, cfg, semantics, preprocess, counts, out
     out.mkdir(parents=True, exist_ok=True)
     cf = out / "counts.txt"
     cf.write_text("\n".join(str(c) for c in counts) + "\n")
+    # The driver is a second process on the GPU while this one holds its own HIP
+    # context: its cooperative peel launch is refused there (hipErrorIllegalState),
+    # so it launches the same co-resident grid plainly (this process is idle meanwhile).
+    env = dict(os.environ, LSS_PEEL_LAUNCH="plain")
     res = subprocess.run([_driver(), mode, str(oss), str(ap), str(n_chan_file), str(cfg.min_window),
                           str(cfg.max_window), str(cfg.time_behind_ms), str(int(cfg.small_skip)), str(semantics),
-                          str(preprocess), str(cf), str(out)], capture_output=True, text=True, timeout=300)
+                          str(preprocess), str(cf), str(out)], capture_output=True, text=True, timeout=300, env=env)
     assert res.returncode == 0, res.stderr
     data = (out / "payloads.bin").read_bytes()
     pays, i = [], 0

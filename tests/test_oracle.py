@@ -360,3 +360,27 @@ def test_peel_decision_margins_known_ties():
     got = sorted(zip(w.tolist(), t.tolist()))
     assert (150, 0) not in got and (400, 0) in got and (900, 1) in got
     assert (700, 0) in got and (710, 0) in got            # equal pooled maxima: both accepted
+
+
+def test_explain_mismatches_gate_miss():
+    """A spike the gated run lacks while its ungated run has it is a gate miss;
+    a difference near it is a cascade; one far from both is unexplained."""
+    from types import SimpleNamespace
+    from tests.parity import explain_mismatches
+    from livespikesorter_amd._capi import EVENT_DTYPE
+    W, M, nt0min = 1000, 61, 20
+    shift = -(M // 2) + nt0min
+    ref = SimpleNamespace(w=np.array([100, 300, 800]), t=np.array([1, 2, 3]), margins=np.ones((3, W)))
+
+    def evs(pairs):
+        e = np.zeros(len(pairs), EVENT_DTYPE)
+        e["sample"] = [c + shift for c, _ in pairs]
+        e["template"] = [t for _, t in pairs]
+        return e
+    gated = evs([(100, 1), (350, 4)])                     # misses 300 and 800, extra at 350
+    ungated = evs([(100, 1), (300, 2), (350, 4)])         # has 300 (the gate's miss), not 800
+    ex = explain_mismatches(ref, gated, M=M, nt0min=nt0min, ungated=ungated)
+    assert ex["events"] == 3 and ex["gate_miss"] == 1 and ex["cascade"] == 1 and ex["unexplained"] == 1
+    assert ex["unexplained_events"][0]["column"] == 800
+    plain = explain_mismatches(ref, gated, M=M, nt0min=nt0min)
+    assert "gate_miss" not in plain and plain["unexplained"] == 3
