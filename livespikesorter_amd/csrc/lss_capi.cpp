@@ -177,7 +177,7 @@ struct lss_handle {
     uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
     float* d_Binit = nullptr;
     float* d_Nsq = nullptr;          // per 64-template tile and column: squared norm of the BF16 operand
-    float* d_Corr = nullptr;         // FP32 correction cache of the rescoring [TP][BS]
+    uint8_t* d_act = nullptr;        // per score tile [BS/64][TP/64]: recomputed by the FP32 projection
     unsigned long long* d_rescnt = nullptr;
     // proposal gate (lss_set_proposal): the network, this handle's activations and the tile mask
     lss_proposal* prop = nullptr;
@@ -255,7 +255,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_Nsq, d_Corr, d_rescnt, d_ptag, d_epoch,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
@@ -647,7 +647,7 @@ void load(lss_handle* h, const std::string& dir_in) {
         h->d_Bpcb = dalloc<uint16_t>((size_t)KCP * h->BS);
         h->d_Binit = dalloc<float>((size_t)TP * h->BS);
         h->d_Nsq = dalloc<float>((size_t)(TP / GEMM_BM) * h->BS);
-        h->d_Corr = dalloc<float>((size_t)TP * h->BS);
+        h->d_act = dalloc<uint8_t>((size_t)(h->BS / GEMM_BN) * (TP / GEMM_BM));
         h->d_rescnt = dalloc<unsigned long long>(2);
         HIPCHK(hipMemset(h->d_rescnt, 0, 2 * sizeof(unsigned long long)));
     }
@@ -808,7 +808,8 @@ void set_cmap(lss_handle* h, const int32_t* chan_map, int32_t ld) {
 // with it), or a region of the static scratch array (the allocation-free entry
 // point).  The tables (templates, ctc, norms, ...) always come from the handle.
 struct BatchBufs {
-    float *Xw = nullptr, *Bpc = nullptr, *B = nullptr, *Binit = nullptr, *Nsq = nullptr, *Corr = nullptr;
+    float *Xw = nullptr, *Bpc = nullptr, *B = nullptr, *Binit = nullptr, *Nsq = nullptr;
+    uint8_t* act = nullptr;
     float* Cfmax = nullptr;
     uint16_t* Bpcb = nullptr;
     Spike *slots = nullptr, *list = nullptr;
@@ -828,7 +829,7 @@ struct BatchBufs {
 
 BatchBufs handle_bufs(const lss_handle* h) {
     BatchBufs b;
-    b.Xw = h->d_Xw; b.Bpc = h->d_Bpc; b.B = h->d_B; b.Binit = h->d_Binit; b.Nsq = h->d_Nsq; b.Corr = h->d_Corr;
+    b.Xw = h->d_Xw; b.Bpc = h->d_Bpc; b.B = h->d_B; b.Binit = h->d_Binit; b.Nsq = h->d_Nsq; b.act = h->d_act;
     b.Cfmax = h->d_Cfmax;
     b.Bpcb = h->d_Bpcb;
     b.slots = h->d_slots; b.list = h->d_list;
@@ -901,8 +902,6 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     cj = PcconvJob{};
     cj.Xw = bb.Xw; cj.wpca = h->d_wpca; cj.Bpc = bb.Bpc;
     cj.Bpcb = reinterpret_cast<__bf16*>(bb.Bpcb);
-    cj.bpc_t = h->bf16 ? 1 : 0;       // BF16 mode: the FP32 copy is the rescoring's, transposed
-    cj.KCP = h->KCP;
     cj.active = xptr;
     cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP; cj.C = h->C;
     cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
@@ -917,7 +916,12 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
         pj.Bmb = reinterpret_cast<const __bf16*>(bb.Bpcb);
         pj.C2 = bb.Binit;
         pj.Nsq = bb.Nsq;
-        pj.C3 = bb.Corr;
+        // tiles that could reach lo2 are recomputed by the FP32 projection
+        pj.act = bb.act;
+        pj.rnm = h->d_rnm;
+        pj.Th2 = h->Th * h->Th;
+        pj.lo2 = 0.9f * pj.Th2;             // as the peel's
+        pj.n_rescored = h->d_rescnt;
     }
     pj.mtiles = h->TP / GEMM_BM;
     pj.ntiles = (W + GEMM_BN - 1) / GEMM_BN;
@@ -945,8 +949,6 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     p.list = bb.list; p.list_cnt = bb.list_cnt; p.cap_g = h->cap_g;
     p.bar = bb.bar; p.iter_cnt = bb.iter_cnt; p.err = bb.err;
     p.Bpc = bb.Bpc; p.CP = h->CP;
-    p.bpc_sk = h->bf16 ? 1 : h->BS;
-    p.bpc_sw = h->bf16 ? h->KCP : 1;
     p.U = h->d_U; p.C = h->C; p.K = h->K; p.nC = h->nC;
     p.iCC = h->d_iCC; p.iU = h->d_iU; p.Gt = h->d_Gt;
     p.do_feat = 1;
@@ -967,7 +969,7 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
         p.kb_ptr = h->d_kbp_ptr; p.kb_idx = h->d_kbp_idx;
         p.Binit = bb.Binit;
         p.Nsq = bb.Nsq;
-        p.Corr = bb.Corr;
+        p.act = bb.act; p.act_mt = h->TP / GEMM_BM;
         p.n_rescored = h->d_rescnt;
     }
     bp.R = R; bp.CW = CW; bp.G = G;
@@ -1495,14 +1497,6 @@ int lss_debug_stage(lss_handle* h, int32_t stage, void* out, int64_t* bytes) {
         if (stage == 0) {
             HIPCHK(hipMemcpy2D(o, (size_t)W * 4, h->d_Xw + XOFF, (size_t)h->XS * 4, (size_t)W * 4, C,
                                hipMemcpyDeviceToHost));
-        } else if (stage == 1 && h->bf16) {
-            // BF16 mode keeps the FP32 PC projection transposed, [w][k*CP + c]
-            std::vector<float> tmp((size_t)W * h->KCP);
-            HIPCHK(hipMemcpy(tmp.data(), h->d_Bpc, tmp.size() * 4, hipMemcpyDeviceToHost));
-            float* of = reinterpret_cast<float*>(o);
-            for (int k = 0; k < K; ++k)
-                for (int c = 0; c < C; ++c)
-                    for (int w = 0; w < W; ++w) of[((size_t)k * C + c) * W + w] = tmp[(size_t)w * h->KCP + k * h->CP + c];
         } else if (stage == 1) {
             for (int k = 0; k < K; ++k)
                 HIPCHK(hipMemcpy2D(o + (size_t)k * C * W * 4, (size_t)W * 4, h->d_Bpc + (size_t)k * h->CP * h->BS,
@@ -1635,6 +1629,10 @@ struct GroupLane {
     hipGraphExec_t exec2 = nullptr;
     const int16_t** h_xptr = nullptr;    // pinned: this launch's input pointers (null = idle)
     const int16_t** d_xptr = nullptr;    // device copy the kernels read
+    // host input: per stream, a pinned staging copy of the caller's batch, so the
+    // copy to the device is a DMA from pinned memory (reused after the lane's sync)
+    std::vector<int16_t*> h_stage;
+    std::vector<size_t> h_stage_elems;
     std::vector<int32_t> ld_captured;
     WhitenBatch wb{};
     PcconvBatch cb{};
@@ -1675,6 +1673,8 @@ struct lss_group {
             for (auto& e : L.ev)
                 if (e) (void)hipEventDestroy(e);
             if (L.h_xptr) (void)hipHostFree(L.h_xptr);
+            for (int16_t* b : L.h_stage)
+                if (b) (void)hipHostFree(b);
             if (L.d_xptr) (void)hipFree(L.d_xptr);
             if (L.stream) (void)hipStreamDestroy(L.stream);
         }
@@ -1847,6 +1847,8 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         GroupLane& L = g->lanes[lane];
         HIPCHK(hipSetDevice(g->dev));
         auto t0 = std::chrono::steady_clock::now();
+        // a call that failed after queueing its copies may have left them in flight
+        HIPCHK(hipStreamSynchronize(L.stream));
         const int n = g->n;
         int active = 0;
         for (int s2 = 0; s2 < n; ++s2) {
@@ -1858,16 +1860,32 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
             }
             ++active;
             int16_t* dx = nullptr;
+            const size_t elems = (size_t)g->W * ld[s2];
             {
                 std::lock_guard<std::mutex> lk(g->mu);
                 set_cmap(h, nullptr, ld[s2]);
-                if (!device_input) dx = ensure_dx(h, (size_t)g->W * ld[s2]);
+                if (!device_input) {
+                    dx = ensure_dx(h, elems);
+                    if (L.h_stage.size() < (size_t)n) {
+                        L.h_stage.resize(n, nullptr);
+                        L.h_stage_elems.resize(n, 0);
+                    }
+                    if (L.h_stage_elems[s2] < elems) {
+                        if (L.h_stage[s2]) HIPCHK(hipHostFree(L.h_stage[s2]));
+                        L.h_stage[s2] = nullptr;
+                        L.h_stage_elems[s2] = 0;
+                        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&L.h_stage[s2]), elems * sizeof(int16_t),
+                                             hipHostMallocDefault));
+                        L.h_stage_elems[s2] = elems;
+                    }
+                }
             }
             if (device_input) {
                 L.h_xptr[s2] = samples[s2];
             } else {
-                HIPCHK(hipMemcpyAsync(dx, samples[s2], (size_t)g->W * ld[s2] * sizeof(int16_t),
-                                      hipMemcpyHostToDevice, L.stream));
+                // the lane's previous launch has synchronized, so its staging copy is free
+                std::memcpy(L.h_stage[s2], samples[s2], elems * sizeof(int16_t));
+                HIPCHK(hipMemcpyAsync(dx, L.h_stage[s2], elems * sizeof(int16_t), hipMemcpyHostToDevice, L.stream));
                 L.h_xptr[s2] = dx;
             }
         }
@@ -2236,7 +2254,7 @@ BatchBufs static_bufs(const lss_handle* h, unsigned char* base, size_t* need) {
         b.Bpcb = reinterpret_cast<uint16_t*>(take(KCP * BS * 2));
         b.Binit = reinterpret_cast<float*>(take(TP * BS * 4));
         b.Nsq = reinterpret_cast<float*>(take(TP / GEMM_BM * BS * 4));
-        b.Corr = reinterpret_cast<float*>(take(TP * BS * 4));
+        b.act = reinterpret_cast<uint8_t*>(take(BS / GEMM_BN * (TP / GEMM_BM)));
     }
     b.Cfmax = reinterpret_cast<float*>(take(Wmax * 4));
     b.slots = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->slot_cap * sizeof(Spike)));

@@ -114,8 +114,8 @@ struct PeelParams {
     // iteration [max_iter], then per column block the iteration its owner finished
     // phase B [NB], then per workgroup the iteration it finished phase A [G]
     unsigned* bar; int* iter_cnt; int* err;
-    // features (and the FP32 rescoring): Bpc element (k*CP + c, w) at (k*CP + c) * bpc_sk + w * bpc_sw
-    const float* Bpc; int CP; int bpc_sk, bpc_sw;
+    // features (and the FP32 rescoring): Bpc element (k*CP + c, w) at (k*CP + c) * BS + w
+    const float* Bpc; int CP;
     const float* U; int C, K, nC;
     const int* iCC; const int* iU;
     const float* Gt;                   // [K][K][2M-1]
@@ -125,26 +125,25 @@ struct PeelParams {
     int half; int nt0min;
     float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
     unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch
-    // FP32 rescoring of a BF16 projection (rescore != 0): in a column whose best
-    // BF16 score reaches lo2, that template's score and the score of every
-    // template whose BF16 score lies within the BF16 error bound of it are
-    // recomputed as the FP32 dot product of U2 row t with Bpc column w (over the
-    // listed K-blocks of t's 64-row tile) minus the BF16 value Binit[t][w], added
-    // to the current score; the column publishes the FP32 maximum of those.
+    // FP32 rescoring of a BF16 projection (rescore != 0).  Every 64 x 64 score
+    // tile holding a score that could reach lo2 in FP32 (its BF16 score plus the
+    // BF16 error bound) has been recomputed by the FP32 projection: act[(w/64) *
+    // act_mt + t/64] = 1, and those scores are the FP32 matcher's own.  The peel
+    // checks, per pass, that no score of the other tiles could reach lo2; where
+    // one could, the column's candidates (every template whose score lies within
+    // the BF16 error bound of the column's best) are decided on FP32 scores: an
+    // FP32 dot product of U2 row t with Bpc column w over the listed K-blocks of
+    // t's tile, minus the BF16 value Binit[t][w], added to the current score.
     int rescore;
     float lo2;
+    const uint8_t* act; int act_mt;
     const float* U2; int ldu;          // [TP][K*CP], FP32
     const int* kb_ptr; const int* kb_idx;
     const float* Binit;                // the BF16 projection before any subtraction
     // Nsq[m][w] (row stride BS): squared norm of the BF16 PC projection of column w
     // over the K-blocks listed for 64-template tile m.  |B_bf16 - B_fp32| of
-    // template t is at most BF16_ERR * sqrt(nm[t] * Nsq[t / 64][w]), so every
-    // template whose BF16 score could be the column's FP32 maximum is known and
-    // rescored too.
+    // template t is at most BF16_ERR * sqrt(nm[t] * Nsq[t / 64][w]).
     const float* Nsq;
-    // Corr[t][w] (row stride BS): the FP32 correction of (t, w) once computed in this batch,
-    // NaN before (the BF16 projection fills it); corrections do not change during the peel
-    float* Corr;
     unsigned long long* n_rescored;    // running totals: FP32 rescores, threshold decisions flipped
     // proposal gate: score tile (64 columns, 64 templates) scored iff gate[(w/64) * gate_mt + t/64]
     const uint8_t* gate; int gate_mt;
@@ -198,10 +197,16 @@ struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
     const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
     float* C2;                            // optional second copy of the result
-    float* C3;                            // optional (BF16): filled with NaN, the FP32 correction cache
-                                          // of the peel (not yet computed)
     float* Nsq;                           // optional (BF16): per 64-row tile and column, the squared
                                           // norm of the BF16 B operand over the tile's listed K-blocks
+    // BF16 mode: the BF16 projection marks act[nt * mtiles + mt] = 1 for every tile
+    // holding a score whose BF16 value plus the BF16 error bound reaches lo2
+    // (rnm: the templates' reciprocal norms); the FP32 projection then recomputes
+    // just those tiles, counting n_rescored[0] += scores, n_rescored[1] += threshold
+    // (Th2) decisions its FP32 scores change against the BF16 copy C2
+    uint8_t* act;
+    const float* rnm; float lo2, Th2;
+    unsigned long long* n_rescored;
     const uint8_t* gate; int gate_mt;     // optional tile mask [ntile][gate_mt]: 0 = not computed
     const int16_t* const* active;         // optional: the job is skipped when *active is null
     int lda, ldb, ldc, mtiles, ntiles;
@@ -209,9 +214,6 @@ struct ProjectJob {
 struct PcconvJob {
     const float* Xw; const float* wpca; float* Bpc;
     __bf16* Bpcb;                         // optional BF16 copy of the PC projection
-    int bpc_t;                            // nonzero: Bpc is written transposed, [BS][KCP] (BF16 mode:
-                                          // a column's K*C values contiguous, for the FP32 rescoring)
-    int KCP;
     const int16_t* const* active;         // optional: the job is skipped when *active is null
     int XS, M, BS, CP, wblocks;
     int C;                                // channels (rows PC_ROWS to a workgroup)

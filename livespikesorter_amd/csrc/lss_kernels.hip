@@ -218,6 +218,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
     if (jb.active && !*jb.active) return;                      // stream idle this launch
+    if (jb.act && !jb.act[nt * jb.mtiles + mt]) return;        // BF16 mode: tile kept in BF16
     const float* __restrict__ A = jb.A;
     const float* __restrict__ Bm = jb.Bm;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -249,6 +250,20 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
         acc = mfma_tile(s, acc, lane, wm, wn);
     }
     store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
+    if (jb.act && jb.C2 && jb.n_rescored) {
+        // threshold decisions of single scores that the FP32 values change
+        int flips = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+            const int64_t o = (int64_t)(m0 + wm * 32 + i) * jb.ldc + n0 + wn * 32 + (lane & 31);
+            const float rn = jb.rnm[m0 + wm * 32 + i];
+            const float b16 = fmaxf(jb.C2[o], 0.f), b32 = fmaxf(acc[r], 0.f);
+            flips += ((b16 * b16) * rn > jb.Th2) != ((b32 * b32) * rn > jb.Th2);
+        }
+        for (int o = 32; o > 0; o >>= 1) flips += __shfl_xor(flips, o);
+        if (lane == 0 && flips) atomicAdd(&jb.n_rescored[1], (unsigned long long)flips);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -285,8 +300,9 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
         wt[m][q] = q < KK ? jb.wpca[q * M + m] : 0.f;
     }
     __syncthreads();
+    // rows past C store zeros (their staged samples are zero), so the padded rows of
+    // the projection's operand never hold stale values
     const int rr = tid >> 6, l = tid & 63, c = cg * PC_ROWS + rr;
-    if (c >= C && !jb.bpc_t) return;
     float acc[4][KK];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj)
@@ -312,37 +328,17 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
         }
     }
     const int64_t o0 = (int64_t)c * jb.BS + w0 + 4 * l;
-    if (!jb.bpc_t) {
-#pragma unroll
-        for (int q = 0; q < KK; ++q) {
-            const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
-            *reinterpret_cast<float4*>(jb.Bpc + o) = make_float4(acc[0][q], acc[1][q], acc[2][q], acc[3][q]);
-            if (jb.Bpcb) {
-#pragma unroll
-                for (int jj = 0; jj < 4; ++jj) jb.Bpcb[o + jj] = (__bf16)acc[jj][q];
-            }
-        }
-        return;
-    }
-    // transposed FP32 copy Bpc[w][k*CP + c]: the four channels of each (column, PC) go
-    // out as one 16-byte word through an LDS transpose (channels past C are zeros);
-    // the BF16 copy keeps the GEMM operand layout
-    __shared__ __align__(16) float tr[256][KK][PC_ROWS];
 #pragma unroll
     for (int q = 0; q < KK; ++q) {
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) tr[4 * l + jj][q][rr] = acc[jj][q];
-        if (jb.Bpcb && c < C) {
-            const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) jb.Bpcb[o + jj] = (__bf16)acc[jj][q];
+        const int64_t o = (int64_t)q * jb.CP * jb.BS + o0;
+        *reinterpret_cast<float4*>(jb.Bpc + o) = make_float4(acc[0][q], acc[1][q], acc[2][q], acc[3][q]);
+        if (jb.Bpcb) {
+            // the four BF16 values as one 8-byte store
+            const __bf16 v[4] = {(__bf16)acc[0][q], (__bf16)acc[1][q], (__bf16)acc[2][q], (__bf16)acc[3][q]};
+            uint2 u;
+            __builtin_memcpy(&u, v, 8);
+            *reinterpret_cast<uint2*>(jb.Bpcb + o) = u;
         }
-    }
-    __syncthreads();
-    for (int i = tid; i < 256 * KK; i += 256) {
-        const int ww = i / KK, q = i - ww * KK;
-        *reinterpret_cast<float4*>(jb.Bpc + (int64_t)(w0 + ww) * jb.KCP + q * jb.CP + cg * PC_ROWS) =
-            *reinterpret_cast<const float4*>(&tr[ww][q][0]);
     }
 }
 
@@ -361,12 +357,17 @@ constexpr int BPAD = 8;
 __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     __shared__ __attribute__((aligned(16))) __bf16 As[GEMM_BM][BK16 + BPAD];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[GEMM_BN][BK16 + BPAD];
+    __shared__ float s_nq[GEMM_BN];
+    __shared__ int s_act;
     const int j = job_of(bt.start, bt.n, blockIdx.x);
     const ProjectJob& jb = bt.job[j];
     int mt, nt;
     if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
-    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
     if (jb.active && !*jb.active) return;                      // stream idle this launch
+    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) {           // tile not proposed: not computed
+        if (jb.act && threadIdx.x == 0) jb.act[nt * jb.mtiles + mt] = 0;
+        return;
+    }
     const __bf16* __restrict__ A = jb.Ab;
     const __bf16* __restrict__ Bm = jb.Bmb;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -429,16 +430,37 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     }
     store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
     if (jb.C2) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
-    if (jb.C3) {
-        f32x16 nan;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) nan[r] = __builtin_nanf("");
-        store_tile(jb.C3, jb.ldc, m0, n0, nan, lane, wm, wn);
-    }
     if (jb.Nsq) {
         nsq += __shfl_xor(nsq, 1);
         nsq += __shfl_xor(nsq, 2);
-        if (npart == 0) jb.Nsq[(int64_t)mt * jb.ldc + n0 + ncol] = nsq;
+        if (npart == 0) {
+            jb.Nsq[(int64_t)mt * jb.ldc + n0 + ncol] = nsq;
+            s_nq[ncol] = nsq;
+        }
+    }
+    if (jb.act) {
+        // Does any score of the tile reach lo2 within the BF16 error bound?  In
+        // the sqrt(Cf) scale a BF16 score is within BF16_ERR * N of its FP32 value
+        // (N: the column's operand norm over the tile), so the tile is marked when
+        // r * sqrt(1/nm) + BF16_ERR * N >= sqrt(lo2) for some score r (a 2^-10
+        // margin covers the rounding of these FP32 terms).
+        if (tid == 0) s_act = 0;
+        __syncthreads();
+        const float N = sqrtf(s_nq[wn * 32 + (lane & 31)]);
+        const float need = sqrtf(jb.lo2) * (1.f - 0x1p-10f) - BF16_ERR * N;
+        bool hit = need <= 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+            const float rn = jb.rnm[m0 + wm * 32 + i];            // 0 on padding rows
+            hit = hit || (acc[r] > 0.f && acc[r] * sqrtf(rn) >= need);
+        }
+        if (__ballot(hit) != 0ull && lane == 0) s_act = 1;
+        __syncthreads();
+        if (tid == 0) {
+            jb.act[nt * jb.mtiles + mt] = (uint8_t)s_act;
+            if (s_act && jb.n_rescored) atomicAdd(&jb.n_rescored[0], (unsigned long long)(GEMM_BM * GEMM_BN));
+        }
     }
 }
 
@@ -729,6 +751,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // the BF16 operand norm of this thread's tile at its column, and (column threads)
     // the largest over the tiles: fixed for the batch
     float nown = 0.f;
+    // BF16 mode: this wave's 64-row tile of the column block was recomputed in FP32
+    const bool wexact = RS && R > 0 && p.rescore && rbase + rg * (R > 0 ? R : 1) < T &&
+                        p.act[cb * p.act_mt + (rbase + rg * (R > 0 ? R : 1)) / GEMM_BM] != 0;
     if (RS && R > 0 && p.rescore) {
         if (w < W && rbase + rg * (R > 0 ? R : 1) < T) nown = sqrtf(p.Nsq[(int64_t)((rbase + rg * R) / GEMM_BM) * p.BS + w]);
         if (tid < CW) {
@@ -1091,8 +1116,18 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 }
             }
             mark(1);
+            // BF16 mode, register tile: whether some thread of a BF16 tile has a best
+            // score that could reach lo2 (its BF16 best plus the bound; the best bounds
+            // every row's a = r^2 * (1/nm) from above to within 2^-22), taken with the
+            // barrier below
+            int need = 0;
+            if (RS && R > 0 && p.rescore && own && !edge && !woff && !wexact && rbase + rg * (R > 0 ? R : 1) < T) {
+                const float e = sqrtf(p.lo2) - BF16_ERR * nown;
+                const float thr = e > 0.f ? e * e * (1.f - 0x1p-18f) : -1.f;
+                need = (wdirty ? best : s_val[tid]) >= thr;
+            }
             if (wdirty) { s_val[tid] = best; s_idx[tid] = bi; s_org[tid] = bo; s_bvv[tid] = bval; }
-            __syncthreads();
+            const int any_need = (RS && R > 0) ? __syncthreads_or(need) : (__syncthreads(), 0);
             mark(7);
             if (tid < CW) {
                 // per-thread base pointers: the row-group strides become immediate
@@ -1143,23 +1178,24 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 }
             }
             if (RS && p.rescore) {
-                // BF16 projection: every column whose best score reaches lo2 (10% under
-                // threshold) is rescored in FP32 before it is published, so pooling,
-                // acceptance, the template choice and amplitudes see FP32 scores
-                // wherever they can matter (a column under lo2 in BF16 cannot beat an
-                // accepted one in FP32).  The correction of (t, w) is fixed
+                // BF16 projection: the scores of every tile that could reach lo2 (10%
+                // under threshold) are the FP32 projection's already.  A score of the
+                // other tiles matters only if it could reach lo2 in FP32 (a column under
+                // lo2 cannot beat an accepted one, nor be accepted); when one could, the
+                // column's candidates are rescored in FP32 before it is published, so
+                // pooling, acceptance, the template choice and amplitudes see FP32 scores
+                // wherever they can matter.  The correction of (t, w) is fixed
                 // (subtractions are applied to both alike).
                 const int lane = tid & 63, wv = tid >> 6;
-                // FP32 dot of U2 row t with Bpc column ww (a contiguous row of the transposed
-                // copy) over the K-blocks listed for t's 64-row tile (the others are exact
-                // zeros): four lanes per 16-wide block, one 16-byte load of each operand per
-                // lane and block, 32 blocks in flight per round; every lane of the wave ends
-                // with the same, order-fixed sum.
+                // FP32 dot of U2 row t with Bpc column ww over the K-blocks listed for t's
+                // 64-row tile (the others are exact zeros): four lanes per 16-wide block,
+                // 32 blocks in flight per round; every lane of the wave ends with the
+                // same, order-fixed sum.
                 auto dot32 = [&](int t, int ww) __attribute__((always_inline)) {
                     const int mt = t / GEMM_BM;
                     const int j0 = p.kb_ptr[mt], j1 = p.kb_ptr[mt + 1];
                     const float* urow = p.U2 + (int64_t)t * p.ldu;
-                    const float* brow = p.Bpc + (int64_t)ww * p.bpc_sw;
+                    const float* bcol = p.Bpc + ww;               // element (k, ww) at k * BS
                     const int q4 = (lane & 3) * 4;
                     float acc = 0.f;
                     for (int jr = j0; jr < j1; jr += 32) {
@@ -1173,7 +1209,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
 #pragma unroll
                         for (int r = 0; r < 2; ++r) {
                             u[r] = kb[r] >= 0 ? *reinterpret_cast<const float4*>(urow + kb[r]) : make_float4(0.f, 0.f, 0.f, 0.f);
-                            b[r] = kb[r] >= 0 ? *reinterpret_cast<const float4*>(brow + kb[r]) : make_float4(0.f, 0.f, 0.f, 0.f);
+                            b[r] = kb[r] >= 0 ? make_float4(bcol[(int64_t)kb[r] * p.BS], bcol[(int64_t)(kb[r] + 1) * p.BS],
+                                                            bcol[(int64_t)(kb[r] + 2) * p.BS], bcol[(int64_t)(kb[r] + 3) * p.BS])
+                                              : make_float4(0.f, 0.f, 0.f, 0.f);
                         }
 #pragma unroll
                         for (int r = 0; r < 2; ++r) acc += ((u[r].x * b[r].x + u[r].y * b[r].y) + u[r].z * b[r].z) + u[r].w * b[r].w;
@@ -1192,8 +1230,12 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                 // rescored one wave each (the correction of the column's last winner is
                 // cached), and the column keeps the FP32 maximum (larger score, then the
                 // smaller file index).
+                // a score in an FP32 tile needs no correction
+                auto exact_t = [&](int t, int ww) { return p.act[(ww / GEMM_BN) * p.act_mt + t / GEMM_BM] != 0; };
+                // register tile: rescoring runs in a pass only when a thread asked for it
+                const bool go = R == 0 || any_need != 0;
                 float cf16 = 0.f;                        // the column's best BF16 score
-                if (tid < CW && c0 + tid < c1) {
+                if (go && tid < CW && c0 + tid < c1) {
                     cf16 = s_cf[tid];
                     float sl = -1.f;
                     if (cf16 >= p.lo2) {
@@ -1204,7 +1246,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     s_cm[tid] = sl;                      // the band's floor in the sqrt(Cf) scale
                 }
                 int listed = 0;
-                if constexpr (R > 0) {
+                if (!go) {
+                    // no candidate outside the FP32 tiles
+                } else if constexpr (R > 0) {
                     // Register tile.  Each thread lists its candidate rows (score plus the
                     // BF16 bound reaching the floor) with their current BF16 score; their
                     // corrections come from the batch's cache or, the first time, from an
@@ -1276,9 +1320,9 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     int* const s_nanq = reinterpret_cast<int*>(s_sp);
                     for (int q = tid; q < nb; q += PEEL_THREADS) {
                         const int e2 = s_pl[q];
-                        const float c = p.Corr[(int64_t)(e2 >> 16) * p.BS + c0 + (e2 & 0xffff)];
-                        s_bcor[q] = c;
-                        if (c != c) s_nanq[atomicAdd(&s_nnan, 1)] = q;
+                        const bool ex = exact_t(e2 >> 16, c0 + (e2 & 0xffff));
+                        s_bcor[q] = 0.f;
+                        if (!ex) s_nanq[atomicAdd(&s_nnan, 1)] = q;
                     }
                     __syncthreads();
                     const int nnan = s_nnan;
@@ -1287,10 +1331,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         const int e2 = s_pl[q];
                         const int cl = e2 & 0xffff, t = e2 >> 16, ww = c0 + cl;
                         const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
-                        if (lane == 0) {
-                            s_bcor[q] = corr;
-                            p.Corr[(int64_t)t * p.BS + ww] = corr;
-                        }
+                        if (lane == 0) s_bcor[q] = corr;
                     }
                     __syncthreads();
                     for (int q = tid; q < nb; q += PEEL_THREADS) {
@@ -1343,21 +1384,24 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                     __syncthreads();
                     const int nb = min(s_nband, MAX_BAND);
                     if (tid == 0 && s_nband > MAX_BAND) atomicOr(p.err, ERR_BAND);
+                    // pairs of BF16 tiles get their correction (compacted so the waves share
+                    // them evenly, in the staged-spike array, dead until acceptance)
+                    int* const s_nanq = reinterpret_cast<int*>(s_sp);
+                    if (tid == 0) s_nnan = 0;
+                    __syncthreads();
                     for (int q = tid; q < nb; q += PEEL_THREADS) {
                         const int e2 = s_bi[q];
-                        s_bcor[q] = p.Corr[(int64_t)(e2 >> 16) * p.BS + c0 + (e2 & 0xffff)];
+                        s_bcor[q] = 0.f;
+                        if (!exact_t(e2 >> 16, c0 + (e2 & 0xffff))) s_nanq[atomicAdd(&s_nnan, 1)] = q;
                     }
                     __syncthreads();
-                    for (int q = wv; q < nb; q += PEEL_THREADS / 64) {
-                        const float cq = s_bcor[q];
-                        if (cq == cq) continue;                  // cached (NaN: not yet); wave-uniform
+                    const int nnan = s_nnan;
+                    for (int j = wv; j < nnan; j += PEEL_THREADS / 64) {
+                        const int q = s_nanq[j];
                         const int e2 = s_bi[q];
                         const int cl = e2 & 0xffff, t = e2 >> 16, ww = c0 + cl;
                         const float corr = dot32(t, ww) - p.Binit[(int64_t)t * p.BS + ww];
-                        if (lane == 0) {
-                            s_bcor[q] = corr;
-                            p.Corr[(int64_t)t * p.BS + ww] = corr;
-                        }
+                        if (lane == 0) s_bcor[q] = corr;
                     }
                     __syncthreads();
                     if (tid < CW && c0 + tid < c1 && s_cm[tid] > 0.f) {
@@ -1753,7 +1797,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             const int ch = p.iCC[j * C + p.iU[si.t]];
             float f = 0.f;
             if (mine)
-                f = p.Bpc[(int64_t)(k * p.CP + ch) * p.bpc_sk + (int64_t)si.w * p.bpc_sw] +
+                f = p.Bpc[(int64_t)(k * p.CP + ch) * p.BS + si.w] +
                     si.amp * p.U[((int64_t)si.t * K + k) * C + ch];
             // the spikes within +-(M-1) columns, in list order: a wave reads 64 at a time
             // and takes the near ones from the ballot, lowest index first
@@ -1895,9 +1939,11 @@ void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {
     }
 }
 
+// BF16 mode: the BF16 projection of every tile, then the FP32 projection of the
+// tiles it marked (the same jobs; act selects the tiles).
 void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s) {
     if (bf16) hipLaunchKernelGGL(k_project_bf16, dim3(b.start[b.n]), dim3(256), 0, s, b);
-    else hipLaunchKernelGGL(k_project, dim3(b.start[b.n]), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_project, dim3(b.start[b.n]), dim3(256), 0, s, b);
 }
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s) {
