@@ -320,6 +320,21 @@ def host_cpu_info():
     return info
 
 
+def copy_calibration(mib: int) -> None:
+    """Three device-to-device copies of a known size: the reference kernel for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
+    from livespikesorter_amd._hip import DeviceBuffer, synchronize
+    nbytes = mib << 20
+    src = DeviceBuffer(nbytes)
+    dst = DeviceBuffer(nbytes)
+    for _ in range(3):
+        dst.copy_from(src, nbytes)
+    synchronize()
+    log(f"copy calibration: 3 x {mib} MiB device-to-device copies")
+    src.free()
+    dst.free()
+
+
 def bench_target(args) -> dict:
     """``--streams`` (default 8) probe streams of 384 channels x 1024 templates,
     1500-sample chunks with 61-sample margins (W = 1622): the plain FP32 matcher
@@ -493,6 +508,8 @@ def bench_target(args) -> dict:
             agg["chunks"] = len(refs64)
             explain[name] = agg
             log(f"vs FP64 oracle, {name}: {agg}")
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     # the CPU baseline on the same workload: chunks of every stream, tick by tick
     cpu = None
     if not args.no_cpu:
@@ -984,15 +1001,7 @@ def bench_cfg2(args) -> dict:
         log(f"cpu baseline {cpu['value']}x realtime over {done} chunks; parity {parity}")
     sp.close()
     if args.copy_calibration:
-        nbytes = args.copy_calibration << 20
-        src = DeviceBuffer(nbytes)
-        dst = DeviceBuffer(nbytes)
-        for _ in range(3):
-            dst.copy_from(src, nbytes)
-        synchronize()
-        log(f"copy calibration: 3 x {args.copy_calibration} MiB device-to-device copies")
-        src.free()
-        dst.free()
+        copy_calibration(args.copy_calibration)
     dx.free()
 
     out = {
