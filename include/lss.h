@@ -299,7 +299,7 @@ int lss_group_debug_stage(struct lss_group* g, int32_t stream, int32_t stage, vo
 /* Device time of the launches on one lane of a group created with cfg->profile
  * set (a tick is then two graph launches, the stages in front of the peel and
  * the peel, with HIP events recorded on the lane's stream around them): ms[0] the
- * stages in front of the peel (input pointers, preprocessing, whitening,
+ * stages in front of the peel (preprocessing, whitening,
  * proposal network and gate, PC conv, projection), ms[1] the peel kernel
  * (all streams' peels in one grid); *launches = launches timed.  reset != 0
  * clears the accumulators after reading.  No reference counterpart. */

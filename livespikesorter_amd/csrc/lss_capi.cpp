@@ -1717,7 +1717,6 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     L.drop_graph();
     hipStream_t st = L.stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, st));
     launch_preprocess(pre, st);
     launch_whiten(L.wb, &L.rb, st);
     {   // the proposal networks of all gated streams: one dispatch per layer
@@ -1894,6 +1893,9 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
             std::lock_guard<std::mutex> lk(g->mu);
             group_capture(g, L, ld);
         }
+        // this launch's input pointers (null: idle stream), read by every kernel of the
+        // graph; copied ahead of it on the lane's stream rather than as a graph node
+        HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, L.stream));
         if (g->profile) {
             HIPCHK(hipEventRecord(L.ev[0], L.stream));
             HIPCHK(hipGraphLaunch(L.exec, L.stream));
