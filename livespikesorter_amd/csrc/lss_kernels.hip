@@ -249,21 +249,22 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
         if (p + 1 < p1) gload(jb.kb_idx[p + 1]);
         acc = mfma_tile(s, acc, lane, wm, wn);
     }
-    store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
-    if (jb.act && jb.C2 && jb.n_rescored) {
-        // threshold decisions of single scores that the FP32 values change
+    if (jb.act && jb.n_rescored) {
+        // threshold decisions of single scores that the FP32 values change (the BF16
+        // scores are still in C: each lane reads the elements it overwrites next)
         int flips = 0;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
             const int64_t o = (int64_t)(m0 + wm * 32 + i) * jb.ldc + n0 + wn * 32 + (lane & 31);
             const float rn = jb.rnm[m0 + wm * 32 + i];
-            const float b16 = fmaxf(jb.C2[o], 0.f), b32 = fmaxf(acc[r], 0.f);
+            const float b16 = fmaxf(jb.C[o], 0.f), b32 = fmaxf(acc[r], 0.f);
             flips += ((b16 * b16) * rn > jb.Th2) != ((b32 * b32) * rn > jb.Th2);
         }
         for (int o = 32; o > 0; o >>= 1) flips += __shfl_xor(flips, o);
         if (lane == 0 && flips) atomicAdd(&jb.n_rescored[1], (unsigned long long)flips);
     }
+    store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -429,7 +430,6 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
         }
     }
     store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
-    if (jb.C2) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
     if (jb.Nsq) {
         nsq += __shfl_xor(nsq, 1);
         nsq += __shfl_xor(nsq, 2);
@@ -462,6 +462,9 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
             if (s_act && jb.n_rescored) atomicAdd(&jb.n_rescored[0], (unsigned long long)(GEMM_BM * GEMM_BN));
         }
     }
+    // the BF16 scores the peel's fallback starts from: needed only where the FP32
+    // projection does not overwrite them
+    if (jb.C2 && !(jb.act && s_act)) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
 }
 
 // nm[t] = sum_{k,c} U[t,k,c]^2 accumulated in FP64, stored as FP64 and rounded FP32;

@@ -79,10 +79,22 @@ __global__ __launch_bounds__(256) void k_pn_conv(PnConvTable tb) {
     __shared__ __attribute__((aligned(16))) __bf16 ws[PN_CH][KP + 8];
     const int g = blockIdx.y, w0 = blockIdx.x * PN_TILE;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the input rows from column w0 - 8 (a multiple of 8, as Wp is), 16 bytes per
+    // load; columns outside the batch read as zero.  xs[ci][c] holds column w0 - 8 + c.
     const __bf16* src = in + (int64_t)g * CI * Wp;
-    for (int i = tid; i < CI * TW; i += 256) {
-        const int ci = i / TW, c = i - ci * TW, w = w0 - PN_PAD + c;
-        xs[ci * XR + c] = (w >= 0 && w < W) ? src[(int64_t)ci * Wp + w] : (__bf16)0.f;
+    constexpr int N8 = XR / 8;
+    static_assert(XR % 8 == 0 && XR >= TW + PN_PAD, "the staged window covers the taps");
+    for (int i = tid; i < CI * N8; i += 256) {
+        const int ci = i / N8, c8 = i - ci * N8;
+        const int j = w0 - 8 + 8 * c8;
+        uint4 u = make_uint4(0u, 0u, 0u, 0u);
+        if (j >= 0 && j + 8 <= Wp) u = *reinterpret_cast<const uint4*>(src + (int64_t)ci * Wp + j);
+        __bf16 v[8];
+        __builtin_memcpy(v, &u, 16);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            if (j + e >= W) v[e] = (__bf16)0.f;
+        __builtin_memcpy(&xs[ci * XR + 8 * c8], v, 16);
     }
     for (int i = tid; i < PN_CH * (KP / 8); i += 256) {
         const int r = i / (KP / 8), c8 = i - r * (KP / 8);
@@ -106,7 +118,7 @@ __global__ __launch_bounds__(256) void k_pn_conv(PnConvTable tb) {
             const bool v0 = k0 < KR, v1 = k1 < KR;
             const bool valid = h ? v1 : v0;
             const int off = h ? (v1 ? o1 : 0) : (v0 ? o0 : 0);
-            const __bf16 x = xs[off + jc];
+            const __bf16 x = xs[off + jc + (8 - PN_PAD)];
             b[e] = valid ? x : (__bf16)0.f;
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
@@ -139,10 +151,12 @@ struct PnHeadArgs {
 struct PnHeadTable { int n; PnHeadArgs a[PN_MAX_JOBS]; };
 
 // A workgroup takes 256 columns of one group: the 32 hidden rows of its window
-// (256 + 8 columns, zero outside 0..W-1) are staged in LDS with coalesced loads,
-// then each thread sums its column's taps in the same order as a direct loop.
+// (columns w0 - 8 .. w0 + 263, zero outside 0..W-1) are staged in LDS as 16-byte
+// loads, then each thread sums its column's
+// taps in the same order as a direct loop (a tap outside the batch adds an exact
+// zero, which changes no partial sum).
 constexpr int PN_HEAD_COLS = 256;
-constexpr int PN_HEAD_WIN = PN_HEAD_COLS + 2 * PN_PAD;
+constexpr int PN_HEAD_WIN = PN_HEAD_COLS + 16;      // from w0 - 8, a multiple of 8 columns
 
 __global__ __launch_bounds__(256) void k_pn_head(PnHeadTable tb) {
     const PnHeadArgs& A = tb.a[blockIdx.z];
@@ -155,12 +169,22 @@ __global__ __launch_bounds__(256) void k_pn_head(PnHeadTable tb) {
     const int w0 = blockIdx.x * PN_HEAD_COLS, g = blockIdx.y;
     if (w0 >= Wp) return;                       // whole workgroup past this job's columns
     __shared__ float w4[PN_CH * PN_KW + 1];
-    __shared__ float s_a[PN_CH][PN_HEAD_WIN];
+    __shared__ __align__(16) float s_a[PN_CH][PN_HEAD_WIN];
     for (int i = threadIdx.x; i < PN_CH * PN_KW + 1; i += 256) w4[i] = A.params[PN_W4 + i];
-    for (int i = threadIdx.x; i < PN_CH * PN_HEAD_WIN; i += 256) {
-        const int ci = i / PN_HEAD_WIN, c = i - ci * PN_HEAD_WIN;
-        const int j = w0 - PN_PAD + c;
-        s_a[ci][c] = (j >= 0 && j < W) ? (float)a3[((int64_t)g * PN_CH + ci) * Wp + j] : 0.f;
+    constexpr int N8 = PN_HEAD_WIN / 8;
+    for (int i = threadIdx.x; i < PN_CH * N8; i += 256) {
+        const int ci = i / N8, c8 = i - ci * N8;
+        const int j = w0 - 8 + 8 * c8;          // a multiple of 8, as Wp is
+        uint4 u = make_uint4(0u, 0u, 0u, 0u);
+        if (j >= 0 && j + 8 <= Wp) u = *reinterpret_cast<const uint4*>(a3 + ((int64_t)g * PN_CH + ci) * Wp + j);
+        __bf16 v[8];
+        __builtin_memcpy(v, &u, 16);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            if (j + e >= W) v[e] = (__bf16)0.f;     // columns past the batch read as zero
+        float4* dst = reinterpret_cast<float4*>(&s_a[ci][8 * c8]);
+        dst[0] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        dst[1] = make_float4((float)v[4], (float)v[5], (float)v[6], (float)v[7]);
     }
     __syncthreads();
     const int w = w0 + threadIdx.x;
@@ -169,10 +193,7 @@ __global__ __launch_bounds__(256) void k_pn_head(PnHeadTable tb) {
     if (w < W) {
         for (int ci = 0; ci < PN_CH; ++ci) {
 #pragma unroll
-            for (int k = 0; k < PN_KW; ++k) {
-                const int j = w + k - PN_PAD;
-                if (j >= 0 && j < W) acc += w4[ci * PN_KW + k] * s_a[ci][threadIdx.x + k];
-            }
+            for (int k = 0; k < PN_KW; ++k) acc += w4[ci * PN_KW + k] * s_a[ci][threadIdx.x + 4 + k];
         }
     }
     const int64_t o = (int64_t)g * Wp + w;
