@@ -347,7 +347,7 @@ def bench_target(args) -> dict:
     matcher."""
     from livespikesorter_amd import OnlineSpikesV2, StreamGroup
     from livespikesorter_amd._hip import DeviceBuffer, synchronize
-    from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_FP32
+    from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_BF16X3, PREC_FP32
     from livespikesorter_amd.proposal import ProposalNet
     from tests.parity import agreement
     N, M = 1500, 61
@@ -383,8 +383,9 @@ def bench_target(args) -> dict:
     net.load(weights)
     modes = {}
     all_modes = (("group_fp32", PREC_FP32, False), ("group_bf16", PREC_BF16_RESCORE, False),
-                 ("group_bf16_gated", PREC_BF16_RESCORE, True))
-    headline = "group_bf16_gated"
+                 ("group_bf16_gated", PREC_BF16_RESCORE, True), ("group_bf16x3", PREC_BF16X3, False),
+                 ("group_bf16x3_gated", PREC_BF16X3, True))
+    headline = "group_bf16x3_gated" if "group_bf16x3_gated" in args.modes.split(",") else "group_bf16_gated"
     outs_by_mode = {}
     peel_by_mode = {}
     for name, prec, gated in [mm for mm in all_modes if mm[0] in args.modes.split(",")]:
@@ -491,7 +492,7 @@ def bench_target(args) -> dict:
                                                         features=False)
         for name, outs in outs_by_mode.items():
             # the gated mode's differences are also checked against the same matcher without the gate
-            ung = outs_by_mode.get("group_bf16") if name == "group_bf16_gated" else None
+            ung = outs_by_mode.get(name[:-len("_gated")]) if name.endswith("_gated") else None
             agg = {"events_fp64": 0, "identical": 0, "differing": 0, "near_tie": 0, "cascade": 0, "unexplained": 0}
             if ung is not None:
                 agg["gate_miss"] = 0

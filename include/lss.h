@@ -44,6 +44,10 @@ extern "C" {
 
 #define LSS_PREC_FP32 0          /* FP32 projection (FP32 MFMA) */
 #define LSS_PREC_BF16_RESCORE 1  /* BF16 MFMA projection, FP32 accumulate, near-threshold FP32 rescore */
+#define LSS_PREC_BF16X3 2        /* split-BF16 MFMA projection: each FP32 operand as a BF16 hi + lo
+                                    pair, three BF16 products (lo*hi + hi*lo + hi*hi) accumulated in
+                                    FP32; within 3 * 2^-16 * sum|U||Bpc| of the exact scores, so every
+                                    score is decided at FP32-equivalent accuracy without rescoring */
 
 typedef struct lss_config {
     int32_t device;        /* HIP device ordinal (OnlineSpikesV2.cu:356-357) */
@@ -116,7 +120,7 @@ typedef struct lss_info {
     int64_t white_kblocks;  /* 64x16 whitening operand blocks computed per 64-sample column tile */
     int64_t proj_kblocks;   /* 64x16 projection operand blocks computed per 64-sample column tile */
     int64_t proj_kblocks_dense;  /* the same without exact-zero skipping */
-    int32_t precision;      /* LSS_PREC_FP32 or LSS_PREC_BF16_RESCORE */
+    int32_t precision;      /* LSS_PREC_FP32, LSS_PREC_BF16_RESCORE or LSS_PREC_BF16X3 */
     float p2p;                   /* peak-to-peak of the last batch's raw samples (preprocessing on) */
     int64_t rescored;       /* BF16 mode: (column, template) scores rescored in FP32 (running total) */
     int64_t rescore_flips;  /* BF16 mode: of those, accept/reject decisions the rescore changed */

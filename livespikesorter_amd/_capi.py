@@ -29,6 +29,7 @@ SEM_KILOSORT = 0
 SEM_LSS = 1
 PREC_FP32 = 0
 PREC_BF16_RESCORE = 1
+PREC_BF16X3 = 2
 PRE_MEAN, PRE_CAR, PRE_HIGHPASS = 1, 2, 4
 PRE_ALL = PRE_MEAN | PRE_CAR | PRE_HIGHPASS
 

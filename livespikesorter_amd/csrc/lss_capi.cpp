@@ -53,6 +53,13 @@ inline uint16_t bf16_bits(float f) {
     return (uint16_t)(u >> 16);
 }
 
+inline float bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 template <class F>
 int guarded(F&& f) {
     try {
@@ -134,7 +141,8 @@ struct lss_proposal {
     int step = 0;
     ~lss_proposal() {
         (void)hipSetDevice(dev);
-        void* ptrs[] = {wt.params, wt.wf1, wt.wf2, wt.wf3, wt.wb2, wt.wb3, d_m, d_v, d_grad, d_xw, d_y};
+        void* ptrs[] = {wt.params, wt.wf1, wt.wf2, wt.wf3, wt.wb2, wt.wb3, wt.wt1, wt.wt2, wt.wt3, d_m, d_v, d_grad,
+                        d_xw, d_y};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         free_pn_buffers(buf);
@@ -158,6 +166,7 @@ struct lss_handle {
     int64_t cur_first = 0;   // absolute sample of column 0 of the last batch
     bool coop = true;        // cooperative launch of the peel kernel
     bool bf16 = false;       // BF16 projection with FP32 rescoring (LSS_PREC_BF16_RESCORE)
+    bool x3 = false;         // split-BF16 projection (LSS_PREC_BF16X3)
 
     // static device tables (internal template order)
     float *d_Wr = nullptr, *d_U2 = nullptr, *d_U = nullptr, *d_wpca = nullptr;
@@ -175,6 +184,7 @@ struct lss_handle {
     unsigned long long* d_prof = nullptr;
     // BF16 projection operands, its unsubtracted result, and the rescore counters
     uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
+    uint16_t* d_U2l = nullptr;       // split-BF16: the lo part of U2 (d_U2b holds the hi part)
     float* d_Binit = nullptr;
     float* d_Nsq = nullptr;          // per 64-template tile and column: squared norm of the BF16 operand
     uint8_t* d_act = nullptr;        // per score tile [BS/64][TP/64]: recomputed by the FP32 projection
@@ -255,7 +265,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_U2l, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
@@ -580,6 +590,16 @@ void load(lss_handle* h, const std::string& dir_in) {
         std::vector<uint16_t> U2b(U2.size());
         for (size_t i = 0; i < U2.size(); ++i) U2b[i] = bf16_bits(U2[i]);
         h->d_U2b = dalloc<uint16_t>(U2b.size()); upload(h->d_U2b, U2b);
+    }
+    if (h->x3) {
+        // U2 = hi + lo + e: hi = bf16(u), lo = bf16(u - hi), both rounded to nearest even
+        std::vector<uint16_t> hi(U2.size()), lo(U2.size());
+        for (size_t i = 0; i < U2.size(); ++i) {
+            hi[i] = bf16_bits(U2[i]);
+            lo[i] = bf16_bits(U2[i] - bf16_value(hi[i]));
+        }
+        h->d_U2b = dalloc<uint16_t>(hi.size()); upload(h->d_U2b, hi);
+        h->d_U2l = dalloc<uint16_t>(lo.size()); upload(h->d_U2l, lo);
     }
     h->d_U = dalloc<float>(Ui.size()); upload(h->d_U, Ui);
     h->d_wpca = dalloc<float>(wpf.size()); upload(h->d_wpca, wpf);
@@ -911,6 +931,10 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
     if (h->prop) { pj.gate = h->d_gate; pj.gate_mt = h->TP / GEMM_BM; }
     pj.active = xptr;
+    if (h->x3) {
+        pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
+        pj.Al = reinterpret_cast<const __bf16*>(h->d_U2l);
+    }
     if (h->bf16) {
         pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
         pj.Bmb = reinterpret_cast<const __bf16*>(bb.Bpcb);
@@ -1063,7 +1087,7 @@ namespace {
 
 // Proposal network on the whitened batch, then the tile mask.
 void run_gate(lss_handle* h, int W, hipStream_t s, const int16_t* const* active = nullptr) {
-    pn_forward(h->prop->wt, h->pnb, h->d_Xw, h->XS, XOFF, h->C, W, s, active);
+    pn_infer(h->prop->wt, h->pnb, h->d_Xw, h->XS, XOFF, h->C, W, s, active);
     pn_gate(h->pnb, h->gate, W, s, active);
 }
 
@@ -1107,7 +1131,7 @@ void run_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, int W, in
     if (prof) HIPCHK(hipEventRecord(h->evt[4], s));
     launch_pcconv(cb, h->K, s);
     if (prof) HIPCHK(hipEventRecord(h->evt[5], s));
-    launch_project(pb, h->bf16, s);
+    launch_project(pb, h->cfg.precision, s);
     if (bb.dbgB)
         HIPCHK(hipMemcpyAsync(bb.dbgB, bb.B, sizeof(float) * (size_t)h->TP * h->BS, hipMemcpyDeviceToDevice, s));
     if (prof_peel) HIPCHK(hipEventRecord(h->evt[6], s));
@@ -1231,9 +1255,11 @@ std::unique_ptr<lss_handle> create_handle(const char* oss_input_dir, const lss_c
     auto h = std::make_unique<lss_handle>();
     if (keep) h->keep.assign(keep, keep + n_keep);
     if (cfg) h->cfg = *cfg; else lss_config_init(&h->cfg);
-    if (h->cfg.precision != LSS_PREC_FP32 && h->cfg.precision != LSS_PREC_BF16_RESCORE)
+    if (h->cfg.precision != LSS_PREC_FP32 && h->cfg.precision != LSS_PREC_BF16_RESCORE &&
+        h->cfg.precision != LSS_PREC_BF16X3)
         throw LssError(LSS_ERR_ARG, "unknown precision mode");
     h->bf16 = h->cfg.precision == LSS_PREC_BF16_RESCORE;
+    h->x3 = h->cfg.precision == LSS_PREC_BF16X3;
     if (h->bf16 && h->cfg.semantics != LSS_SEM_KILOSORT)
         throw LssError(LSS_ERR_ARG, "BF16 projection with FP32 rescoring is defined for Kilosort semantics");
     if (h->cfg.semantics != LSS_SEM_KILOSORT && h->cfg.semantics != LSS_SEM_LSS)
@@ -1413,7 +1439,7 @@ int lss_query(lss_handle* h, lss_info* info) {
         i.C = h->C; i.T = h->T; i.M = h->M; i.K = h->K; i.nC = h->nC;
         i.nt0min = h->nt0min; i.dt = h->dt; i.semantics = h->sem; i.Th = h->Th;
         i.max_batch = h->Wmax; i.chunk = h->cfg.chunk; i.max_iter = h->max_iter;
-        i.precision = h->bf16 ? LSS_PREC_BF16_RESCORE : LSS_PREC_FP32;
+        i.precision = h->cfg.precision;
         if (h->d_gate_stats) {
             unsigned long long c[4];
             HIPCHK(hipSetDevice(h->dev));
@@ -1730,11 +1756,11 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
             gj[ng] = PnGateJob{h->pnb, h->gate, g->W, &L.d_xptr[s2]};
             ++ng;
         }
-        pn_forward_batch(fj, ng, st);
+        pn_infer_batch(fj, ng, st);
         pn_gate_batch(gj, ng, st);
     }
     launch_pcconv(L.cb, g->hs[0]->K, st);
-    launch_project(L.pb, g->hs[0]->bf16, st);
+    launch_project(L.pb, g->hs[0]->cfg.precision, st);
     if (g->profile) {
         HIPCHK(hipStreamEndCapture(st, &L.graph));
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2010,6 +2036,9 @@ int lss_proposal_create(int32_t device, int32_t channels, int32_t max_batch, uin
         net->wt.wf3 = dalloc<__bf16>(PN_CH * PN_KP);
         net->wt.wb2 = dalloc<__bf16>(PN_CH * PN_KP);
         net->wt.wb3 = dalloc<__bf16>(PN_CH * PN_KP);
+        net->wt.wt1 = dalloc<__bf16>(PN_CH * PN_KT1);
+        net->wt.wt2 = dalloc<__bf16>(PN_CH * PN_KP);
+        net->wt.wt3 = dalloc<__bf16>(PN_CH * PN_KP);
         net->d_m = dalloc<float>(PN_NPARAMS);
         net->d_v = dalloc<float>(PN_NPARAMS);
         net->d_grad = dalloc<float>(PN_NPARAMS);
@@ -2085,7 +2114,7 @@ int lss_proposal_forward(lss_proposal* net, const float* xw, int32_t n, float* p
         pn_check_width(net, n);
         HIPCHK(hipSetDevice(net->dev));
         pn_upload_xw(net, xw, n);
-        pn_forward(net->wt, net->buf, net->d_xw, n, 0, net->C, n, net->stream);
+        pn_infer(net->wt, net->buf, net->d_xw, n, 0, net->C, n, net->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy2DAsync(prob, sizeof(float) * n, net->buf.p, sizeof(float) * net->buf.Wp, sizeof(float) * n,
                                 net->G, hipMemcpyDeviceToHost, net->stream));

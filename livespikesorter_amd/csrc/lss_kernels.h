@@ -195,7 +195,8 @@ struct PreBatch {
 void launch_preprocess(PreBatch& b, hipStream_t s);
 struct ProjectJob {
     const float* A; const float* Bm; const int* kb_ptr; const int* kb_idx; float* C;
-    const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16)
+    const __bf16* Ab; const __bf16* Bmb;  // BF16 operands (precision mode BF16); split-BF16 mode: Ab = hi
+    const __bf16* Al;                     //   part of A, Al = lo part (B is split from Bm as it is staged)
     float* C2;                            // optional second copy of the result
     float* Nsq;                           // optional (BF16): per 64-row tile and column, the squared
                                           // norm of the BF16 B operand over the tile's listed K-blocks
@@ -254,7 +255,9 @@ struct WhitenLaunch {
 static_assert(sizeof(WhitenLaunch) <= 4096, "whitening launch must fit the kernel-argument segment");
 void launch_whiten(const WhitenBatch& b, const ResetBatch* rb, hipStream_t s);
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s);
-void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s);
+// projection kernels of a precision mode (LSS_PREC_*): FP32 MFMA; BF16 MFMA then the FP32
+// recomputation of the tiles it marked; split-BF16 (three BF16 products per FP32 product)
+void launch_project(const ProjectBatch& b, int precision, hipStream_t s);
 
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
 // load-time ctc table on the GPU (FP64 accumulation): U [T][K][C] internal order,

@@ -467,6 +467,106 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     if (jb.C2 && !(jb.act && s_act)) store_tile(jb.C2, jb.ldc, m0, n0, acc, lane, wm, wn);
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 projection: every FP32 operand x is carried as a BF16 pair
+// x = hi + lo + e (hi = bf16(x), lo = bf16(x - hi), |e| <= 2^-16 |x|), and each
+// product as the three BF16 MFMA products lo_A*hi_B + hi_A*lo_B + hi_A*hi_B
+// accumulated in FP32 (the dropped lo_A*lo_B is <= 2^-16 |A||B|), so a score is
+// within 3 * 2^-16 * sum |U||Bpc| of the exact dot product plus the FP32
+// accumulation error: the FP32 projection's accuracy at a sixth of its MFMA time.
+// U's pair is split once at load (Ah, Al); Bpc is read as FP32 and split while
+// it is staged.  A step stages four listed 16-deep K-blocks (64 of the
+// reduction) in LDS as [row][k] for both operands, so each MFMA operand is one
+// 16-byte LDS read; the next step's global loads are in flight while the
+// current step's 48 MFMAs run.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_project_x3(ProjectBatch bt) {
+    __shared__ __attribute__((aligned(16))) __bf16 Ash[GEMM_BM][BK16 + BPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Asl[GEMM_BM][BK16 + BPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bsh[GEMM_BN][BK16 + BPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bsl[GEMM_BN][BK16 + BPAD];
+    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    const ProjectJob& jb = bt.job[j];
+    int mt, nt;
+    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
+    if (jb.active && !*jb.active) return;                      // stream idle this launch
+    const __bf16* __restrict__ Ah = jb.Ab;
+    const __bf16* __restrict__ Al = jb.Al;
+    const float* __restrict__ Bm = jb.Bm;
+    const int lda = jb.lda, ldb = jb.ldb;
+    const int m0 = mt * GEMM_BM, n0 = nt * GEMM_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int ar = tid >> 2, aq = tid & 3;               // A: row, listed block of the step
+    const int kq = tid >> 4, nq = tid & 15;              // B: rows 4 kq .. 4 kq + 3 of the step, columns 4 nq ..
+    const int bq = kq >> 2, br = (kq & 3) * 4;           //    = block bq of the step, its rows br .. br + 3
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    // this thread's share of a step, in registers (no arrays: they would go to scratch)
+    uint4 ah0, ah1, al0, al1;       // A row segment of block aq: hi, then lo (two 16-byte words each)
+    float4 b0, b1, b2, b3;          // B: four rows of block bq, four columns each
+    auto gload = [&](int p) __attribute__((always_inline)) {
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        ah0 = ah1 = al0 = al1 = z;
+        if (p + aq < p1) {
+            const int64_t ao = (int64_t)(m0 + ar) * lda + jb.kb_idx[p + aq] * GEMM_BK;
+            const uint4* sh = reinterpret_cast<const uint4*>(Ah + ao);
+            const uint4* sl = reinterpret_cast<const uint4*>(Al + ao);
+            ah0 = sh[0]; ah1 = sh[1]; al0 = sl[0]; al1 = sl[1];
+        }
+        const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
+        b0 = b1 = b2 = b3 = zf;
+        if (p + bq < p1) {
+            const float* src = Bm + (int64_t)(jb.kb_idx[p + bq] * GEMM_BK + br) * ldb + n0 + 4 * nq;
+            b0 = *reinterpret_cast<const float4*>(src);
+            b1 = *reinterpret_cast<const float4*>(src + ldb);
+            b2 = *reinterpret_cast<const float4*>(src + 2 * (int64_t)ldb);
+            b3 = *reinterpret_cast<const float4*>(src + 3 * (int64_t)ldb);
+        }
+    };
+    // column c of the four staged rows as hi and lo BF16 parts, one 8-byte store each
+    auto split4 = [&](float x0, float x1, float x2, float x3, int c) __attribute__((always_inline)) {
+        const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1, h2 = (__bf16)x2, h3 = (__bf16)x3;
+        const __bf16 hi[4] = {h0, h1, h2, h3};
+        const __bf16 lo[4] = {(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1), (__bf16)(x2 - (float)h2),
+                              (__bf16)(x3 - (float)h3)};
+        uint2 uh, ul;
+        __builtin_memcpy(&uh, hi, 8);
+        __builtin_memcpy(&ul, lo, 8);
+        *reinterpret_cast<uint2*>(&Bsh[4 * nq + c][bq * 16 + br]) = uh;
+        *reinterpret_cast<uint2*>(&Bsl[4 * nq + c][bq * 16 + br]) = ul;
+    };
+    if (p0 < p1) gload(p0);
+    for (int p = p0; p < p1; p += 4) {
+        __syncthreads();
+        *reinterpret_cast<uint4*>(&Ash[ar][aq * 16]) = ah0;
+        *reinterpret_cast<uint4*>(&Ash[ar][aq * 16 + 8]) = ah1;
+        *reinterpret_cast<uint4*>(&Asl[ar][aq * 16]) = al0;
+        *reinterpret_cast<uint4*>(&Asl[ar][aq * 16 + 8]) = al1;
+        split4(b0.x, b1.x, b2.x, b3.x, 0);
+        split4(b0.y, b1.y, b2.y, b3.y, 1);
+        split4(b0.z, b1.z, b2.z, b3.z, 2);
+        split4(b0.w, b1.w, b2.w, b3.w, 3);
+        __syncthreads();
+        if (p + 4 < p1) gload(p + 4);
+        const int nk = min(4, p1 - p);
+        for (int kk = 0; kk < nk; ++kk) {
+            const int ko = kk * 16 + 8 * (lane >> 5);
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&Ash[wm * 32 + (lane & 31)][ko]);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(&Asl[wm * 32 + (lane & 31)][ko]);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bsh[wn * 32 + (lane & 31)][ko]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bsl[wn * 32 + (lane & 31)][ko]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        }
+    }
+    store_tile(jb.C, jb.ldc, m0, n0, acc, lane, wm, wn);
+}
+
 // nm[t] = sum_{k,c} U[t,k,c]^2 accumulated in FP64, stored as FP64 and rounded FP32;
 // rnm = FP32 reciprocal (the LSS matcher multiplies by 1/nm, reciprocal_kernel).
 __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ U, int KC,
@@ -1943,9 +2043,13 @@ void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {
 }
 
 // BF16 mode: the BF16 projection of every tile, then the FP32 projection of the
-// tiles it marked (the same jobs; act selects the tiles).
-void launch_project(const ProjectBatch& b, bool bf16, hipStream_t s) {
-    if (bf16) hipLaunchKernelGGL(k_project_bf16, dim3(b.start[b.n]), dim3(256), 0, s, b);
+// tiles it marked (the same jobs; act selects the tiles).  Split-BF16: one launch.
+void launch_project(const ProjectBatch& b, int precision, hipStream_t s) {
+    if (precision == LSS_PREC_BF16X3) {
+        hipLaunchKernelGGL(k_project_x3, dim3(b.start[b.n]), dim3(256), 0, s, b);
+        return;
+    }
+    if (precision == LSS_PREC_BF16_RESCORE) hipLaunchKernelGGL(k_project_bf16, dim3(b.start[b.n]), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_project, dim3(b.start[b.n]), dim3(256), 0, s, b);
 }
 

@@ -40,7 +40,11 @@ struct PnWeights {            // device pointers
     float* params;            // [PN_NPARAMS] FP32 master
     __bf16* wf1; __bf16* wf2; __bf16* wf3;   // forward, [32][KP]
     __bf16* wb2; __bf16* wb3;                // data gradient of layers 2 and 3, [32][KP]
+    __bf16* wt1; __bf16* wt2; __bf16* wt3;   // inference, tap-major [32][k * CI + ci] (PN_KT1 / PN_KP wide)
 };
+
+// tap-major reduction of layer 1 (9 taps x 24 channels), padded to a multiple of 16
+constexpr int PN_KT1 = (PN_KW * PN_CIN + 15) / 16 * 16;   // 224
 
 struct PnBuffers {            // per-batch activations, [G][rows][Wp]
     __bf16* a0; __bf16* a1; __bf16* a2; __bf16* a3;
@@ -77,6 +81,9 @@ struct PnGateJob {
     const int16_t* const* active;
 };
 void pn_forward_batch(const PnFwdJob* jobs, int n, hipStream_t s);
+// P(spike) of each job in one fused launch (input, three convolutions and the head per
+// column tile with the activations in LDS); writes b.p and b.z only (no activations)
+void pn_infer_batch(const PnFwdJob* jobs, int n, hipStream_t s);
 void pn_gate_batch(const PnGateJob* jobs, int n, hipStream_t s);
 
 int pn_groups(int C);
@@ -87,6 +94,8 @@ size_t pn_part_floats(int G, int Wp);
 void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
                 hipStream_t s, const int16_t* const* active = nullptr);
 void pn_gate(const PnBuffers& b, const PnGate& gt, int W, hipStream_t s, const int16_t* const* active = nullptr);
+void pn_infer(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
+              hipStream_t s, const int16_t* const* active = nullptr);
 // loss and gradients (into grad [PN_NPARAMS]) for labels y [G][Wp] (0/1 bytes)
 void pn_backward(const PnWeights& wt, const PnBuffers& b, const uint8_t* y, int W, float pos_weight,
                  float* grad, hipStream_t s);

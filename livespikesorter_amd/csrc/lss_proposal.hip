@@ -395,6 +395,18 @@ __global__ __launch_bounds__(256) void k_pn_pack(PnWeights wt) {
         const int co = i / PN_KP1, kk = i - co * PN_KP1;
         wt.wf1[i] = (__bf16)(kk < PN_CIN * PN_KW ? P[PN_W1 + co * PN_CIN * PN_KW + kk] : 0.f);
     }
+    // inference, tap-major: column k * CI + ci holds W[co][ci][k]; zero past 9 * CI
+    if (i < PN_CH * PN_KT1) {
+        const int co = i / PN_KT1, kk = i - co * PN_KT1;
+        const int k = kk / PN_CIN, ci = kk - k * PN_CIN;
+        wt.wt1[i] = (__bf16)(k < PN_KW ? P[PN_W1 + (co * PN_CIN + ci) * PN_KW + k] : 0.f);
+    }
+    if (i < PN_CH * PN_KP) {
+        const int co = i / PN_KP, kk = i - co * PN_KP;
+        const int k = kk / PN_CH, ci = kk - k * PN_CH;
+        wt.wt2[i] = (__bf16)P[PN_W2 + (co * PN_CH + ci) * PN_KW + k];
+        wt.wt3[i] = (__bf16)P[PN_W3 + (co * PN_CH + ci) * PN_KW + k];
+    }
     if (i < PN_CH * PN_KP) {
         const int r = i / PN_KP, kk = i - r * PN_KP;
         wt.wf2[i] = (__bf16)P[PN_W2 + r * PN_CH * PN_KW + kk];
@@ -404,6 +416,175 @@ __global__ __launch_bounds__(256) void k_pn_pack(PnWeights wt) {
         const int src = (co * PN_CH + r) * PN_KW + (PN_KW - 1 - k);
         wt.wb2[i] = (__bf16)P[PN_W2 + src];
         wt.wb3[i] = (__bf16)P[PN_W3 + src];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: one workgroup (8 waves) takes one group and PN_FT output
+// columns and runs the input gather, the three convolutions and the head with
+// every activation in LDS.  Each layer is computed over 256 columns (a 32-column
+// MFMA tile per wave), 4 columns further left than the layer before and 8
+// columns narrower in use, so layer l covers the taps layer l + 1 reads:
+// layer l's column jc is batch column w0 - 16 + 4 l + jc, and its tap k of
+// column jc reads column jc + k of the layer below.  Activations are stored
+// [column][channel] with the reduction in tap-major order (k' = k * CI + ci), so
+// the eight consecutive k' of an MFMA operand are eight channels of one column,
+// one 16-byte LDS read.  Activations of columns outside the batch are zero (the
+// 'same' zero padding of every layer), as in the per-layer kernels; the sums
+// run in another order than there (tap-major), so results agree to FP32
+// rounding and the rare BF16 storage flip it causes.  The next layer's weights
+// are loaded into registers while the current layer runs.
+// ---------------------------------------------------------------------------
+constexpr int PN_FT = 224;                 // output columns per fused workgroup
+constexpr int PN_FX = 8 * 32 + PN_KW - 1;  // staged columns of each layer's input (264)
+constexpr int PN_FCP = PN_CH + 8;          // LDS pitch (channels) of a staged column
+constexpr int PN_FWP = PN_KP + 8;          // LDS pitch of a weight row (the widest layer)
+
+struct PnInferArgs {
+    const float* xw; int64_t xs; int xoff; int C; int W; int Wp; int G;
+    const __bf16* wt1; const __bf16* wt2; const __bf16* wt3; const float* params;
+    float* z; float* p;
+    const int16_t* const* active;
+};
+struct PnInferTable { int n; PnInferArgs a[PN_MAX_JOBS]; };
+
+// one layer: out[jc][co] = relu(bias[co] + sum_{k, ci} w[co][k * CI + ci] in[jc + k][ci]),
+// zero for batch columns base + jc outside [0, W)
+template <int CI>
+__device__ __forceinline__ void pn_fused_layer(const __bf16* in, __bf16* out, const __bf16* ws, const float* bias,
+                                               int base, int W, int lane, int wv) {
+    constexpr int KR = CI * PN_KW;
+    constexpr int KP = (KR + 15) / 16 * 16;
+    const int h = lane >> 5, j = lane & 31, jc = wv * 32 + j;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int s = 0; s < KP / 16; ++s) {
+        const int kq = 16 * s + 8 * h;              // first reduction index of this lane's eight
+        const int k = kq / CI, ci0 = kq - k * CI;   // one tap, eight channels (CI is a multiple of 8)
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&ws[j * PN_FWP + kq]);
+        bf16x8 b;
+        if (kq < KR) {
+            b = *reinterpret_cast<const bf16x8*>(&in[(jc + k) * PN_FCP + ci0]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) b[e] = (__bf16)0.f;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int w = base + jc;
+    const bool ok = w >= 0 && w < W;
+#pragma unroll
+    for (int g4 = 0; g4 < 4; ++g4) {
+        __bf16 v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int co = 8 * g4 + 4 * h + e;           // D element 4 g4 + e of this lane
+            v[e] = ok ? (__bf16)fmaxf(acc[4 * g4 + e] + bias[co], 0.f) : (__bf16)0.f;
+        }
+        uint2 u;
+        __builtin_memcpy(&u, v, 8);
+        *reinterpret_cast<uint2*>(&out[jc * PN_FCP + 8 * g4 + 4 * h]) = u;
+    }
+}
+
+__global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
+    const PnInferArgs& A = tb.a[blockIdx.z];
+    if (A.active && !*A.active) return;
+    const int g = blockIdx.y, w0 = blockIdx.x * PN_FT;
+    if (g >= A.G || w0 >= A.W) return;
+    __shared__ __attribute__((aligned(16))) __bf16 bx[PN_FX * PN_FCP];
+    __shared__ __attribute__((aligned(16))) __bf16 by[PN_FX * PN_FCP];
+    __shared__ __attribute__((aligned(16))) __bf16 ws[PN_CH * PN_FWP];
+    __shared__ float s_bias[3 * PN_CH];
+    __shared__ __attribute__((aligned(16))) float s_w4[PN_KW * PN_CH + 1];   // [k][ci], then b4
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int W = A.W, C = A.C;
+    // input: rows ch = 16 g - 4 + ci of the whitened batch from column w0 - 16, eight
+    // columns per thread and chunk, as BF16 into bx[column][ci]
+    {
+        constexpr int NCH = PN_FX / 8;              // 33 chunks per row
+        for (int i = tid; i < PN_CIN * NCH; i += 512) {
+            const int ci = i / NCH, c0 = (i - ci * NCH) * 8;
+            const int ch = g * PN_GROUP - PN_HALO + ci;
+            const float* row = A.xw + (int64_t)(ch >= 0 && ch < C ? ch : 0) * A.xs + A.xoff;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int w = w0 - 16 + c0 + e;
+                const float v = (ch >= 0 && ch < C && w >= 0 && w < W) ? row[w] : 0.f;
+                bx[(c0 + e) * PN_FCP + ci] = (__bf16)v;
+            }
+        }
+    }
+    for (int i = tid; i < PN_CH * (PN_KT1 / 8); i += 512) {
+        const int r = i / (PN_KT1 / 8), c8 = i - r * (PN_KT1 / 8);
+        *reinterpret_cast<uint4*>(&ws[r * PN_FWP + c8 * 8]) = *reinterpret_cast<const uint4*>(A.wt1 + r * PN_KT1 + c8 * 8);
+    }
+    for (int i = tid; i < 3 * PN_CH; i += 512) {
+        const int l = i / PN_CH, co = i - l * PN_CH;
+        s_bias[i] = A.params[(l == 0 ? PN_B1 : l == 1 ? PN_B2 : PN_B3) + co];
+    }
+    for (int i = tid; i < PN_KW * PN_CH; i += 512) {
+        const int k = i / PN_CH, ci = i - k * PN_CH;
+        s_w4[i] = A.params[PN_W4 + ci * PN_KW + k];
+    }
+    if (tid == 0) s_w4[PN_KW * PN_CH] = A.params[PN_B4];
+    // the next layer's weights, in flight while the current layer runs
+    constexpr int NW = PN_CH * PN_KP / 8;           // 16-byte words of a 32 x 288 layer
+    constexpr int NWT = (NW + 511) / 512;
+    uint4 wreg[NWT];
+    auto wload = [&](const __bf16* src) {
+#pragma unroll
+        for (int q = 0; q < NWT; ++q) {
+            const int i = tid + 512 * q;
+            wreg[q] = i < NW ? reinterpret_cast<const uint4*>(src)[i] : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto wstore = [&]() {
+#pragma unroll
+        for (int q = 0; q < NWT; ++q) {
+            const int i = tid + 512 * q;
+            if (i < NW) {
+                const int r = i / (PN_KP / 8), c8 = i - r * (PN_KP / 8);
+                *reinterpret_cast<uint4*>(&ws[r * PN_FWP + c8 * 8]) = wreg[q];
+            }
+        }
+    };
+    __syncthreads();
+    wload(A.wt2);
+    pn_fused_layer<PN_CIN>(bx, by, ws, s_bias, w0 - 12, W, lane, wv);
+    __syncthreads();
+    wstore();
+    __syncthreads();
+    wload(A.wt3);
+    pn_fused_layer<PN_CH>(by, bx, ws, s_bias + PN_CH, w0 - 8, W, lane, wv);
+    __syncthreads();
+    wstore();
+    __syncthreads();
+    pn_fused_layer<PN_CH>(bx, by, ws, s_bias + 2 * PN_CH, w0 - 4, W, lane, wv);
+    __syncthreads();
+    // head: z = b4 + sum_k sum_ci W4[ci][k] a3[w + k - 4][ci] (layer 3's column jh + k)
+    if (tid < PN_FT) {
+        const int w = w0 + tid;
+        if (w < A.Wp) {
+            float acc = s_w4[PN_KW * PN_CH];
+            if (w < W) {
+#pragma unroll
+                for (int k = 0; k < PN_KW; ++k) {
+                    const __bf16* col = &by[(tid + k) * PN_FCP];
+#pragma unroll
+                    for (int c8 = 0; c8 < PN_CH / 8; ++c8) {
+                        const bf16x8 v = *reinterpret_cast<const bf16x8*>(col + 8 * c8);
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) acc += s_w4[k * PN_CH + 8 * c8 + e] * (float)v[e];
+                    }
+                }
+            }
+            const int64_t o = (int64_t)g * A.Wp + w;
+            A.z[o] = w < W ? acc : 0.f;
+            A.p[o] = w < W ? 1.f / (1.f + expf(-acc)) : 0.f;
+        }
     }
 }
 
@@ -494,6 +675,27 @@ void pn_forward_batch(const PnFwdJob* jobs, int n, hipStream_t s) {
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, t2);
     hipLaunchKernelGGL((k_pn_conv<PN_CH, 0>), grid, dim3(256), 0, s, t3);
     hipLaunchKernelGGL(k_pn_head, dim3((Wpm + PN_HEAD_COLS - 1) / PN_HEAD_COLS, Gm, n), dim3(256), 0, s, th);
+}
+
+void pn_infer_batch(const PnFwdJob* jobs, int n, hipStream_t s) {
+    if (n <= 0) return;
+    PnInferTable t{};
+    t.n = n;
+    int Gm = 0, Wm = 0;
+    for (int j = 0; j < n; ++j) {
+        const PnFwdJob& J = jobs[j];
+        t.a[j] = PnInferArgs{J.xw, J.xs, J.xoff, J.C, J.W, J.b.Wp, J.b.G, J.wt.wt1, J.wt.wt2, J.wt.wt3,
+                             J.wt.params, J.b.z, J.b.p, J.active};
+        Gm = max(Gm, J.b.G);
+        Wm = max(Wm, J.W);
+    }
+    hipLaunchKernelGGL(k_pn_fused, dim3((Wm + PN_FT - 1) / PN_FT, Gm, n), dim3(512), 0, s, t);
+}
+
+void pn_infer(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
+              hipStream_t s, const int16_t* const* active) {
+    const PnFwdJob j{xw, xs, xoff, C, W, wt, b, active};
+    pn_infer_batch(&j, 1, s);
 }
 
 void pn_gate_batch(const PnGateJob* jobs, int n, hipStream_t s) {

@@ -15,11 +15,11 @@ import os
 
 import numpy as np
 
-from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_BF16_RESCORE, PREC_FP32, SEM_KILOSORT, SEM_LSS, check,
+from ._capi import (EVENT_DTYPE, Config, Info, LssError, PREC_BF16_RESCORE, PREC_BF16X3, PREC_FP32, SEM_KILOSORT, SEM_LSS, check,
                     default_config, lib)
 
 __all__ = ["OnlineSpikesV2", "StreamGroup", "LssError", "SEM_KILOSORT", "SEM_LSS", "PREC_FP32",
-           "PREC_BF16_RESCORE"]
+           "PREC_BF16_RESCORE", "PREC_BF16X3"]
 
 (STAGE_XW, STAGE_BPC, STAGE_B, STAGE_CFMAX, STAGE_IMAX, STAGE_NM, STAGE_PHASES, STAGE_CTC, STAGE_PRE,
  STAGE_PRE_MEAN, STAGE_PRE_MED) = range(11)
