@@ -335,6 +335,24 @@ def copy_calibration(mib: int) -> None:
     dst.free()
 
 
+HEADLINE_DTYPE = {
+    "group_bf16x3_gated": "bf16 (whitening and projection as split-BF16 pairs on MFMA, FP32 accumulation; "
+                          "proposal net) + f32 (peel)",
+    "group_bf16x3": "bf16 (whitening and projection as split-BF16 pairs on MFMA, FP32 accumulation) + f32 (peel)",
+    "group_bf16_gated": "bf16 (projection, proposal net) + f32 (rescoring, peel)",
+    "group_bf16": "bf16 (projection) + f32 (rescoring, peel)",
+    "group_fp32": "f32",
+}
+HEADLINE_FORM = {
+    "group_bf16x3_gated": "split-BF16 whitening and projection (three BF16 MFMA products per FP32 product, FP32 "
+                          "accumulation), proposal-network gate",
+    "group_bf16x3": "split-BF16 whitening and projection (three BF16 MFMA products per FP32 product, FP32 accumulation)",
+    "group_bf16_gated": "BF16 projection with FP32 rescoring, proposal-network gate",
+    "group_bf16": "BF16 projection with FP32 rescoring",
+    "group_fp32": "plain FP32 matcher",
+}
+
+
 def bench_target(args) -> dict:
     """``--streams`` (default 8) probe streams of 384 channels x 1024 templates,
     1500-sample chunks with 61-sample margins (W = 1622): the plain FP32 matcher
@@ -469,7 +487,8 @@ def bench_target(args) -> dict:
     tf = ROOT / "profiles" / "target_traffic.json"
     if tf.exists():
         tr = json.loads(tf.read_text())
-        traffic = tr.get("bytes_per_launch", {}).get(f"lss::k_peel<{g['peel_geometry']['rows_per_thread']}, true>")
+        rs = "true" if headline in ("group_bf16", "group_bf16_gated") else "false"   # the headline's peel instance
+        traffic = tr.get("bytes_per_launch", {}).get(f"lss::k_peel<{g['peel_geometry']['rows_per_thread']}, {rs}>")
         traffic = round(traffic) if traffic else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
@@ -529,11 +548,10 @@ def bench_target(args) -> dict:
         "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": g["ms_per_tick"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16 (projection, proposal net) + f32 (rescoring, peel)",
+        "dtype": HEADLINE_DTYPE.get(headline, "f32"),
         "data": f"synthetic (seeded generator: {S} recordings of one 384-ch, 1024-template probe)",
         "config": {"workload": f"{S} streams x 384 ch x 1024 templates, 1500-sample chunks + 61-sample margins "
-                               f"(W={W}), one graph launch per tick, BF16 projection with FP32 rescoring, "
-                               "proposal-network gate",
+                               f"(W={W}), one graph launch per tick, {HEADLINE_FORM.get(headline, headline)}",
                    "streams": S, "channels": C, "templates": 1024, "chunk": N, "batch_columns": W,
                    "mode": headline},
         "p50_ms": g["paced_latency_p50_ms"],
@@ -595,7 +613,7 @@ def bench_bf16(args) -> dict:
     import oracle
     from livespikesorter_amd import OnlineSpikesV2
     from livespikesorter_amd._hip import DeviceBuffer, synchronize
-    from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_FP32
+    from livespikesorter_amd.matcher import PREC_BF16_RESCORE, PREC_BF16X3, PREC_FP32
     from tests.parity import agreement
     N, M = 60000, 61
     W = N + 2 * M
@@ -631,24 +649,33 @@ def bench_bf16(args) -> dict:
 
     sb, el_b, lat_b, ev_b, it_b, sk_b, st_b = run(PREC_BF16_RESCORE)
     sf, el_f, lat_f, ev_f, it_f, sk_f, st_f = run(PREC_FP32)
+    sx, el_x, lat_x, ev_x, it_x, sk_x, st_x = run(PREC_BF16X3)
+    sx.close()
     rtf_b = steps * N / FS / el_b
     rtf_f = steps * N / FS / el_f
+    rtf_x = steps * N / FS / el_x
+    log(f"split-bf16: {rtf_x:.1f}x realtime, p99 {pct(lat_x, 99):.2f} ms; stages ms: "
+        + ", ".join(f"{k} {v:.3f}" for k, v in st_x.items()))
     log(f"bf16: {rtf_b:.1f}x realtime, p99 {pct(lat_b, 99):.2f} ms; fp32: {rtf_f:.1f}x, p99 {pct(lat_f, 99):.2f} ms")
     log("bf16 stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st_b.items()))
     log("fp32 stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st_f.items()))
     # spike agreement with the plain FP32 GPU matcher over the compared batches
     n_cmp = min(steps, args.bf16_compare)
-    tot = ident_n = within_n = 0
-    for j in range(n_cmp):
-        a, b = ev_f[j], ev_b[j]
-        i, w, _ = agreement(a["sample"], a["template"], b["sample"], b["template"])
-        n = max(len(a), len(b))
-        ident_n += i * n
-        within_n += w * n
-        tot += n
-    vs_fp32 = {"batches": n_cmp, "events": tot, "identical": round(ident_n / max(tot, 1), 6),
-               "within_1_sample": round(within_n / max(tot, 1), 6)}
-    log(f"bf16 vs fp32 GPU: {vs_fp32}")
+
+    def vs_fp32_of(evs):
+        tot = ident_n = within_n = 0
+        for j in range(n_cmp):
+            a, b = ev_f[j], evs[j]
+            i, w, _ = agreement(a["sample"], a["template"], b["sample"], b["template"])
+            n = max(len(a), len(b))
+            ident_n += i * n
+            within_n += w * n
+            tot += n
+        return {"batches": n_cmp, "events": tot, "identical": round(ident_n / max(tot, 1), 6),
+                "within_1_sample": round(within_n / max(tot, 1), 6)}
+    vs_fp32 = vs_fp32_of(ev_b)
+    vs_fp32_x = vs_fp32_of(ev_x)
+    log(f"bf16 vs fp32 GPU: {vs_fp32}; split-bf16 vs fp32 GPU: {vs_fp32_x}")
     info = sb.query()
     # the FP64 oracle (and the FP32 CPU restatement, the CPU baseline) on the first batch
     tab = oracle.load_tables(d)
@@ -664,7 +691,7 @@ def bench_bf16(args) -> dict:
         t32 = time.perf_counter() - t1
         rs = ref.w - M // 2 + m.nt0min
         res = {}
-        for name, ev in (("bf16", ev_b[0]), ("fp32", ev_f[0])):
+        for name, ev in (("bf16", ev_b[0]), ("fp32", ev_f[0]), ("split_bf16", ev_x[0])):
             i, w, pairs = agreement(rs, ref.t, ev["sample"], ev["template"])
             ri = np.array([p[0] for p in pairs])
             gi = np.array([p[1] for p in pairs])
@@ -702,6 +729,9 @@ def bench_bf16(args) -> dict:
         "p50_ms": round(pct(lat_b, 50), 3), "p99_ms": round(pct(lat_b, 99), 3),
         "fp32_x_realtime": round(rtf_f, 3), "fp32_p99_ms": round(pct(lat_f, 99), 3),
         "speedup_over_fp32": round(rtf_b / rtf_f, 4),
+        "split_bf16": {"x_realtime": round(rtf_x, 3), "p99_ms": round(pct(lat_x, 99), 3),
+                       "speedup_over_fp32": round(rtf_x / rtf_f, 4), "vs_fp32_gpu": vs_fp32_x,
+                       "stages_ms": {k: round(v, 4) for k, v in st_x.items()}},
         "project_ms": {"bf16": round(st_b["project"], 4), "fp32": round(st_f["project"], 4)},
         "project_tflops": {"bf16": round(proj[1] / (st_b["project"] / 1e3), 2),
                            "fp32": round(proj[1] / (st_f["project"] / 1e3), 2)},
@@ -751,7 +781,7 @@ def main() -> int:
     ap.add_argument("--group-split", type=int, default=0,
                     help="group and target workloads: peel template split (0 automatic, 1 none, n ways)")
     ap.add_argument("--lanes", type=int, default=4, help="concurrent launch lanes of the paced group runs (1-4)")
-    ap.add_argument("--modes", default="group_fp32,group_bf16,group_bf16_gated",
+    ap.add_argument("--modes", default="group_fp32,group_bf16,group_bf16_gated,group_bf16x3,group_bf16x3_gated",
                     help="target workload: which group modes to run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized buffer three times (a known-size kernel "

@@ -60,7 +60,12 @@ typedef struct lss_config {
                               (saveSpikes, OnlineSpikesV2.cu:1192-1194); -1 = on for LSS only */
     int32_t max_events;    /* event capacity per batch on the device (default 65536) */
     int32_t keep_stages;   /* nonzero: keep per-stage device copies for lss_debug_stage */
-    int32_t sparse;        /* nonzero (default): skip exact-zero template/channel blocks */
+    int32_t sparse;        /* 0: dense.  1: skip exact-zero template/channel blocks and exact-zero
+                              ctc rows (bit-identical to dense).  2 (default): also treat as zero a
+                              projection weight at most 2^-40 of its template's largest and a ctc row
+                              of s whose entries are all at most 2^-40 sqrt(nm[s] nm[t]) (the far
+                              tails of spatially local templates): a score moves by under 2^-40 of
+                              its scale, far below one FP32 rounding of a score near threshold */
     int32_t profile;       /* 1: time each device stage with HIP events (lss_stage_times);
                               2: the peel stage only (two events per batch) */
     int32_t peel_split;    /* template subsets per peel column block: 0 = automatic,

@@ -185,6 +185,7 @@ struct lss_handle {
     // BF16 projection operands, its unsubtracted result, and the rescore counters
     uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
     uint16_t* d_U2l = nullptr;       // split-BF16: the lo part of U2 (d_U2b holds the hi part)
+    uint16_t *d_Wrh = nullptr, *d_Wrl = nullptr;   // split-BF16: the whitening matrix's hi and lo parts
     float* d_Binit = nullptr;
     float* d_Nsq = nullptr;          // per 64-template tile and column: squared norm of the BF16 operand
     uint8_t* d_act = nullptr;        // per score tile [BS/64][TP/64]: recomputed by the FP32 projection
@@ -265,7 +266,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_U2l, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_U2l, d_Wrh, d_Wrl, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
@@ -493,6 +494,24 @@ void load(lss_handle* h, const std::string& dir_in) {
                 Ui[((size_t)tp * K + k) * C + c] = u;
             }
     }
+    // template norms (FP64, internal order): the scale of the negligible-value tests below
+    std::vector<double> nmh(T, 0.0);
+    for (int tp = 0; tp < T; ++tp)
+        for (int i = 0; i < K * C; ++i) nmh[tp] += (double)Ui[(size_t)tp * K * C + i] * Ui[(size_t)tp * K * C + i];
+    // Sparse level 2: a projection weight at most 2^-40 of its template's largest is
+    // dropped (its products sum to under 2^-40 * umax * sum|Bpc|, far under the FP32
+    // resolution of any score near threshold), so the K-block lists skip the far tails
+    // of spatially local templates.  The internal U (features, ctc) keeps every value.
+    const double tau = h->cfg.sparse >= 2 ? NEGLIGIBLE : 0.0;
+    if (tau > 0.0)
+        for (int tp = 0; tp < T; ++tp) {
+            float* row = &U2[(size_t)tp * KCP];
+            float umax = 0.f;
+            for (int i = 0; i < KCP; ++i) umax = std::max(umax, std::fabs(row[i]));
+            const float cut = (float)(tau * umax);
+            for (int i = 0; i < KCP; ++i)
+                if (std::fabs(row[i]) <= cut) row[i] = 0.f;
+        }
     std::vector<int> kbw_ptr, kbw_idx, kbp_ptr, kbp_idx;
     build_kblocks(Wrp, CP, CP, h->cfg.sparse != 0, kbw_ptr, kbw_idx);
     build_kblocks(U2, TP, KCP, h->cfg.sparse != 0, kbp_ptr, kbp_idx);
@@ -532,10 +551,12 @@ void load(lss_handle* h, const std::string& dir_in) {
                 const int t = keep.empty() ? h->orig[tp] : keep[h->orig[tp]];
                 const float* src = &ctc[((size_t)t * T_file + s) * L];
                 float* dst = &ctc_i[((size_t)sp * TS + tp) * LP];
+                // rows that matter: nonzero, or (sparse level 2) above 2^-40 sqrt(nm_s nm_t)
+                const double cut = tau * std::sqrt(nmh[sp] * nmh[tp]);
                 bool nz = false;
                 for (int l = 0; l < L; ++l) {
                     dst[l] = src[l];
-                    nz |= src[l] != 0.f;
+                    nz |= std::fabs((double)src[l]) > cut;
                 }
                 if (nz) { lo = std::min(lo, tp); hi = tp + 1; }
             }
@@ -592,14 +613,22 @@ void load(lss_handle* h, const std::string& dir_in) {
         h->d_U2b = dalloc<uint16_t>(U2b.size()); upload(h->d_U2b, U2b);
     }
     if (h->x3) {
-        // U2 = hi + lo + e: hi = bf16(u), lo = bf16(u - hi), both rounded to nearest even
-        std::vector<uint16_t> hi(U2.size()), lo(U2.size());
-        for (size_t i = 0; i < U2.size(); ++i) {
-            hi[i] = bf16_bits(U2[i]);
-            lo[i] = bf16_bits(U2[i] - bf16_value(hi[i]));
-        }
+        // u = hi + lo + e: hi = bf16(u), lo = bf16(u - hi), both rounded to nearest even
+        auto split = [](const std::vector<float>& u, std::vector<uint16_t>& hi, std::vector<uint16_t>& lo) {
+            hi.resize(u.size());
+            lo.resize(u.size());
+            for (size_t i = 0; i < u.size(); ++i) {
+                hi[i] = bf16_bits(u[i]);
+                lo[i] = bf16_bits(u[i] - bf16_value(hi[i]));
+            }
+        };
+        std::vector<uint16_t> hi, lo;
+        split(U2, hi, lo);
         h->d_U2b = dalloc<uint16_t>(hi.size()); upload(h->d_U2b, hi);
         h->d_U2l = dalloc<uint16_t>(lo.size()); upload(h->d_U2l, lo);
+        split(Wrp, hi, lo);
+        h->d_Wrh = dalloc<uint16_t>(hi.size()); upload(h->d_Wrh, hi);
+        h->d_Wrl = dalloc<uint16_t>(lo.size()); upload(h->d_Wrl, lo);
     }
     h->d_U = dalloc<float>(Ui.size()); upload(h->d_U, Ui);
     h->d_wpca = dalloc<float>(wpf.size()); upload(h->d_wpca, wpf);
@@ -621,17 +650,20 @@ void load(lss_handle* h, const std::string& dir_in) {
         }
         double* d_wtw = dalloc<double>(wtw.size());
         int* d_span = dalloc<int>(span.size());
+        double* d_nmh = dalloc<double>(nmh.size());
         upload(d_wtw, wtw);
         upload(d_span, span);
+        upload(d_nmh, nmh);
         HIPCHK(hipMemset(h->d_ctc, 0, (size_t)T * TS * LP * sizeof(float)));
         launch_ctc(h->d_U, d_wtw, reinterpret_cast<const int2*>(d_span), T, K, C, M, TS, h->d_ctc, h->d_tlo,
-                   h->d_thi, nullptr);
+                   h->d_thi, d_nmh, tau, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMemcpy(tlo.data(), h->d_tlo, T * sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(thi.data(), h->d_thi, T * sizeof(int), hipMemcpyDeviceToHost));
         (void)hipFree(d_wtw);
         (void)hipFree(d_span);
+        (void)hipFree(d_nmh);
     }
     for (int sp = 0; sp < T; ++sp) {
         if (!h->cfg.sparse) { tlo[sp] = 0; thi[sp] = T; }
@@ -889,6 +921,10 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     wj.ld = ld; wj.CP = h->CP; wj.W = W; wj.C = h->C; wj.XS = h->XS;
     wj.mtiles = h->CP / GEMM_BM;
     wj.cmap_id = h->cmap_id ? 1 : 0;
+    if (h->x3) {
+        wj.Wrh = reinterpret_cast<const __bf16*>(h->d_Wrh);
+        wj.Wrl = reinterpret_cast<const __bf16*>(h->d_Wrl);
+    }
     wj.ntiles = (W + XOFF + GEMM_BN - 1) / GEMM_BN;          // cover the right zero pad
     bp.has_pre = h->pre != 0;
     if (bp.has_pre) {
@@ -1299,7 +1335,7 @@ int lss_config_init(lss_config* cfg) {
     cfg->dedup = -1;
     cfg->max_events = 65536;
     cfg->keep_stages = 0;
-    cfg->sparse = 1;
+    cfg->sparse = 2;
     cfg->profile = 0;
     return LSS_OK;
 }

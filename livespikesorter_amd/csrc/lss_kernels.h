@@ -165,6 +165,7 @@ struct WhitenJob {
     const float* xf; int64_t xfs;      // when set, the input is the preprocessed batch xf[c][w]
     int64_t ld; int CP, W, C, XS, mtiles, ntiles;
     int cmap_id;                       // the channel map is the identity (c -> c)
+    const __bf16* Wrh; const __bf16* Wrl;   // split-BF16 mode: Wr as hi and lo BF16 parts (else null)
 };
 
 // ---- preprocessing in front of whitening (lss_preprocess.hip) ----
@@ -262,9 +263,14 @@ void launch_project(const ProjectBatch& b, int precision, hipStream_t s);
 void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* nm64, hipStream_t s);
 // load-time ctc table on the GPU (FP64 accumulation): U [T][K][C] internal order,
 // wtw [K*K][2M+1] FP64, span [T] = [first, last+1) nonzero channel of each template;
-// ctc [T][TS][LP] zero on entry; tlo/thi = nonzero t' range of each s' (thi = 0 if none)
+// ctc [T][TS][LP] zero on entry; tlo/thi = the t' range of each s' whose rows matter (thi = 0
+// if none): nonzero rows, or with tau > 0 rows holding an entry above tau * sqrt(nm[s'] nm[t'])
+// (nm: FP64 template norms, internal order)
 void launch_ctc(const float* U, const double* wtw, const int2* span, int T, int K, int C, int M, int TS, float* ctc,
-                int* tlo, int* thi, hipStream_t s);
+                int* tlo, int* thi, const double* nm, double tau, hipStream_t s);
+// Sparse level 2: template entries and ctc rows this far below their scale are treated as
+// zero (2^-40; see lss_config.sparse)
+constexpr double NEGLIGIBLE = 0x1p-40;
 
 // Rows of the score tile each peel thread keeps in registers (0: tile stays in global memory).
 int peel_rows(int T, int CW);

@@ -210,6 +210,129 @@ __global__ __launch_bounds__(256) void k_whiten(WhitenLaunch wl) {
     store_tile(jb.Xw + XOFF, jb.XS, m0, n0, acc, lane, wm, wn);
 }
 
+// Split-BF16 whitening (the handle's precision is LSS_PREC_BF16X3): Wr carried as a
+// BF16 pair (hi, lo) split at load, the input split as it is staged (an int16
+// sample is exactly hi + lo), three BF16 MFMA products per FP32 product with
+// FP32 accumulation.  Tile order, K-block lists, input forms and the output
+// layout are k_whiten's; LDS holds both operands as [row][k].
+__global__ __launch_bounds__(256) void k_whiten_x3(WhitenLaunch wl) {
+    typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+    constexpr int KD = 4 * GEMM_BK, KPAD = KD + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 Ash[GEMM_BM][KPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Asl[GEMM_BM][KPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bsh[GEMM_BN][KPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bsl[GEMM_BN][KPAD];
+    __shared__ int s_cm[WCM];
+    __shared__ int s_kb[WCM / GEMM_BK];
+    if ((int)blockIdx.x < wl.r.n) reset_stream(wl.r, blockIdx.x);
+    const WhitenBatch& bt = wl.w;
+    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    const WhitenJob& jb = bt.job[j];
+    int mt, nt;
+    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
+    const int16_t* __restrict__ x = jb.xptr ? *jb.xptr : jb.x;
+    const float* __restrict__ xf = jb.xf;
+    if (!x) return;                                             // stream idle this launch
+    const int CP = jb.CP, W = jb.W, C = jb.C;
+    const int64_t ld = jb.ld;
+    const int m0 = mt * GEMM_BM, n0 = nt * GEMM_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int q = tid & 3, ar = tid >> 2;        // listed block of the step, A row / B sample
+    const bool cm_lds = !xf && CP <= WCM;
+    if (cm_lds)
+        for (int i = tid; i < CP; i += 256) s_cm[i] = jb.cmap[i];
+    const int* __restrict__ cm = cm_lds ? s_cm : jb.cmap;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    const bool kb_lds = p1 - p0 <= WCM / GEMM_BK;
+    if (kb_lds)
+        for (int i = tid; i < p1 - p0; i += 256) s_kb[i] = jb.kb_idx[p0 + i];
+    auto kbi = [&](int p) { return kb_lds ? s_kb[p - p0] : jb.kb_idx[p]; };
+    uint4 ah0, ah1, al0, al1;      // this thread's A row segment of block q: hi, lo
+    float rb[16];                  // 16 input values: sample n0 + ar, channels of block q
+    auto gload = [&](int pb) __attribute__((always_inline)) {
+        const bool have = pb + q < p1;
+        const int k0 = have ? kbi(pb + q) * GEMM_BK : 0;
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        ah0 = ah1 = al0 = al1 = z;
+        if (have) {
+            const int64_t ao = (int64_t)(m0 + ar) * CP + k0;
+            const uint4* sh = reinterpret_cast<const uint4*>(jb.Wrh + ao);
+            const uint4* sl = reinterpret_cast<const uint4*>(jb.Wrl + ao);
+            ah0 = sh[0]; ah1 = sh[1]; al0 = sl[0]; al1 = sl[1];
+        }
+        const int n = n0 + ar;
+        if (xf) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = k0 + i;
+                rb[i] = (have && n < W && c < C) ? xf[(int64_t)c * jb.xfs + n] : 0.f;
+            }
+        } else {
+            const int16_t* row = x + (int64_t)n * ld;
+            const bool vec = have && n < W && jb.cmap_id && k0 + GEMM_BK <= C &&
+                             ((reinterpret_cast<uintptr_t>(row + k0) & 15) == 0);
+            if (vec) {
+                const int4 v0 = reinterpret_cast<const int4*>(row + k0)[0];
+                const int4 v1 = reinterpret_cast<const int4*>(row + k0)[1];
+                const int w8[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    rb[2 * i] = (float)(int16_t)(w8[i] & 0xffff);
+                    rb[2 * i + 1] = (float)(int16_t)((unsigned)w8[i] >> 16);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int c = k0 + i;
+                    rb[i] = (have && n < W && c < C) ? (float)row[cm[c]] : 0.f;
+                }
+            }
+        }
+    };
+    __syncthreads();                             // the staged channel map and block list
+    if (p0 < p1) gload(p0);
+    for (int pb = p0; pb < p1; pb += 4) {
+        __syncthreads();
+        *reinterpret_cast<uint4*>(&Ash[ar][q * 16]) = ah0;
+        *reinterpret_cast<uint4*>(&Ash[ar][q * 16 + 8]) = ah1;
+        *reinterpret_cast<uint4*>(&Asl[ar][q * 16]) = al0;
+        *reinterpret_cast<uint4*>(&Asl[ar][q * 16 + 8]) = al1;
+#pragma unroll
+        for (int h8 = 0; h8 < 2; ++h8) {
+            __bf16 hi[8], lo[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = rb[8 * h8 + e];
+                hi[e] = (__bf16)v;
+                lo[e] = (__bf16)(v - (float)hi[e]);
+            }
+            uint4 uh, ul;
+            __builtin_memcpy(&uh, hi, 16);
+            __builtin_memcpy(&ul, lo, 16);
+            *reinterpret_cast<uint4*>(&Bsh[ar][q * 16 + 8 * h8]) = uh;
+            *reinterpret_cast<uint4*>(&Bsl[ar][q * 16 + 8 * h8]) = ul;
+        }
+        __syncthreads();
+        if (pb + 4 < p1) gload(pb + 4);
+        const int nk = min(4, p1 - pb);
+        for (int kk = 0; kk < nk; ++kk) {
+            const int ko = kk * 16 + 8 * (lane >> 5);
+            const bf16x8_t a_h = *reinterpret_cast<const bf16x8_t*>(&Ash[wm * 32 + (lane & 31)][ko]);
+            const bf16x8_t a_l = *reinterpret_cast<const bf16x8_t*>(&Asl[wm * 32 + (lane & 31)][ko]);
+            const bf16x8_t b_h = *reinterpret_cast<const bf16x8_t*>(&Bsh[wn * 32 + (lane & 31)][ko]);
+            const bf16x8_t b_l = *reinterpret_cast<const bf16x8_t*>(&Bsl[wn * 32 + (lane & 31)][ko]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l, b_h, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, b_l, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, b_h, acc, 0, 0, 0);
+        }
+    }
+    store_tile(jb.Xw + XOFF, jb.XS, m0, n0, acc, lane, wm, wn);
+}
+
 __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     __shared__ GemmTile s;
     const int j = job_of(bt.start, bt.n, blockIdx.x);
@@ -2028,7 +2151,11 @@ void launch_whiten(const WhitenBatch& b, const ResetBatch* rb, hipStream_t s) {
     WhitenLaunch wl{};
     wl.w = b;
     if (rb) wl.r = *rb;                          // else n = 0: nothing to reset
-    hipLaunchKernelGGL(k_whiten, dim3(b.start[b.n]), dim3(256), 0, s, wl);
+    // a launch's streams share one precision mode
+    if (b.n > 0 && b.job[0].Wrh)
+        hipLaunchKernelGGL(k_whiten_x3, dim3(b.start[b.n]), dim3(256), 0, s, wl);
+    else
+        hipLaunchKernelGGL(k_whiten, dim3(b.start[b.n]), dim3(256), 0, s, wl);
 }
 
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {

@@ -16,12 +16,14 @@ constexpr int CTC_WAVES = 4;
 // nonzero channel spans only (disjoint spans give an exact-zero row, left as
 // the zero the output was cleared to), lanes striding over channels, then a
 // butterfly sum leaves every product in every lane; lanes then own lags
-// l = lane and lane + 64.  tlo/thi collect the nonzero t' range of each s'.
+// l = lane and lane + 64.  tlo/thi collect the t' range of each s' whose rows
+// matter: nonzero, or (tau > 0) with an entry above tau * sqrt(nm[s'] nm[t']).
 template <int K>
 __global__ __launch_bounds__(64 * CTC_WAVES) void k_ctc(const float* __restrict__ U, const double* __restrict__ wtw,
                                                         const int2* __restrict__ span, int T, int C, int L, int TS,
                                                         float* __restrict__ ctc, int* __restrict__ tlo,
-                                                        int* __restrict__ thi) {
+                                                        int* __restrict__ thi, const double* __restrict__ nm,
+                                                        double tau) {
     extern __shared__ double s_wtw[];   // [K*K][L]
     const int sp = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -59,6 +61,7 @@ __global__ __launch_bounds__(64 * CTC_WAVES) void k_ctc(const float* __restrict_
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) acc[a][b] += __shfl_xor(acc[a][b], o, 64);
         bool nz = false;
+        const double cut = tau > 0.0 ? tau * sqrt(nm[sp] * nm[tp]) : 0.0;
         float* out = ctc + ((int64_t)sp * TS + tp) * LP;
         for (int l = lane; l < L; l += 64) {
             double v = 0.0;
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(64 * CTC_WAVES) void k_ctc(const float* __restrict_
                 for (int b = 0; b < K; ++b) v = fma(acc[a][b], s_wtw[(a * K + b) * L + l], v);
             const float f = (float)v;
             out[l] = f;
-            nz |= f != 0.f;
+            nz |= fabs((double)f) > cut;
         }
         if (__any(nz)) {
             lo = min(lo, tp);
@@ -94,7 +97,7 @@ __global__ __launch_bounds__(64 * CTC_WAVES) void k_ctc(const float* __restrict_
 // U [T][K][C] FP32 (internal order), wtw [K*K][L] FP64, span [T] nonzero channel
 // range of each template; ctc [T][TS][LP] must be zero on entry.
 void launch_ctc(const float* U, const double* wtw, const int2* span, int T, int K, int C, int M, int TS, float* ctc,
-                int* tlo, int* thi, hipStream_t s) {
+                int* tlo, int* thi, const double* nm, double tau, hipStream_t s) {
     const int L = 2 * M + 1;
     const size_t lds = sizeof(double) * K * K * L;
     const dim3 grid(T), block(64 * CTC_WAVES);
@@ -102,7 +105,7 @@ void launch_ctc(const float* U, const double* wtw, const int2* span, int T, int 
 #define LSS_CTC_CASE(k) \
     case k:                                                                                    \
         (void)hipFuncSetAttribute((const void*)k_ctc<k>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-        k_ctc<k><<<grid, block, lds, s>>>(U, wtw, span, T, C, L, TS, ctc, tlo, thi);           \
+        k_ctc<k><<<grid, block, lds, s>>>(U, wtw, span, T, C, L, TS, ctc, tlo, thi, nm, tau);  \
         break;
         LSS_CTC_CASE(1) LSS_CTC_CASE(2) LSS_CTC_CASE(3) LSS_CTC_CASE(4)
         LSS_CTC_CASE(5) LSS_CTC_CASE(6) LSS_CTC_CASE(7) LSS_CTC_CASE(8)

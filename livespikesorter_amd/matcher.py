@@ -36,7 +36,7 @@ class OnlineSpikesV2:
     def __init__(self, oss_input_dir: str | os.PathLike, *, device: int = 0, chunk: int = 3000,
                  max_batch: int = 0, max_iter: int = 0, semantics: int = SEM_KILOSORT,
                  precision: int = PREC_FP32, dedup: int = -1, max_events: int = 65536,
-                 keep_stages: bool = False, sparse: bool = True, profile: int = 0,
+                 keep_stages: bool = False, sparse: int = 2, profile: int = 0,
                  peel_split: int = 0, relabel: bool = False, preprocess: int = 0, keep=None):
         """``keep``: ascending Wall3 template indices to load (a template subset,
         lss_create_subset); event template ids are then positions in ``keep``."""
@@ -223,7 +223,7 @@ class StreamGroup:
     """
 
     def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
-                 semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: bool = True,
+                 semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: int = 2,
                  precision: int = PREC_FP32, preprocess: int = 0, peel_split: int = 0, profile: bool = False):
         self._g = ct.c_void_p()
         self.n = len(oss_input_dirs)

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_config_defaults():
     cfg = _capi.default_config()
-    assert cfg.chunk == 3000 and cfg.max_events == 65536 and cfg.sparse == 1
+    assert cfg.chunk == 3000 and cfg.max_events == 65536 and cfg.sparse == 2
     assert cfg.semantics == _capi.SEM_KILOSORT and cfg.dedup == -1 and cfg.profile == 0
     assert ct.sizeof(_capi.Config) == 16 * 4
 

@@ -1,4 +1,4 @@
-"""Split-BF16 projection (precision LSS_PREC_BF16X3).
+"""Split-BF16 whitening and projection (precision LSS_PREC_BF16X3).
 
 Each FP32 operand is carried as a BF16 pair x = hi + lo + e with
 hi = bf16(x), lo = bf16(x - hi), |e| <= 2^-16 |x|, and each product as the
@@ -6,11 +6,13 @@ three BF16 MFMA products lo*hi + hi*lo + hi*hi with FP32 accumulation.  The
 dropped lo*lo term and the two residuals e bound the product error by
 3 * 2^-16 |U||Bpc| (plus 2^-24-order terms), so a score is within
 3 * 2^-16 * sum|U||Bpc| of the exact dot product, plus FP32 accumulation
-error; the peel then runs on those scores as on FP32 ones.
+error; the peel then runs on those scores as on FP32 ones.  The whitening
+GEMM runs the same way (an int16 sample is exactly a BF16 hi + lo pair, so its
+bound is 2 * 2^-16 * sum|Wr||x|).
 
 Model: 384 channels, 1024 templates, bursty gamma(0.5) ISIs, banded whitening
-(cfg3).  Bars: the projection stage within that bound of the FP64 oracle's
-scores; events against the FP64 oracle at the plain matcher's bar
+(cfg3).  Bars: the whitening and projection stages within those bounds of the
+FP64 oracle's; events against the FP64 oracle at the plain matcher's bar
 (tests/parity.py check_events: identical >= 99.5%, the rest +-1 sample and each
 a near tie of the oracle's decision margins, amplitudes 1e-3, features 1e-3),
 on 12000-sample batches (register-tile peel) and on one 60000-sample batch
@@ -22,7 +24,7 @@ import pytest
 
 import oracle
 from livespikesorter_amd import StreamGroup, synth
-from livespikesorter_amd.matcher import PREC_BF16X3, PREC_FP32, STAGE_B, STAGE_BPC, OnlineSpikesV2
+from livespikesorter_amd.matcher import PREC_BF16X3, PREC_FP32, STAGE_B, STAGE_BPC, STAGE_XW, OnlineSpikesV2
 from tests.parity import agreement, check_events
 
 pytestmark = pytest.mark.gpu
@@ -40,7 +42,7 @@ def cfg3(model_dirs):
     return m, d, oracle.load_tables(d), x
 
 
-def test_x3_projection_error_bound(cfg3):
+def test_x3_whitening_and_projection_error_bounds(cfg3):
     m, d, tab, x = cfg3
     batch = x[:3122]
     print("[x3] projection stage vs FP64, W=3122", flush=True)
@@ -49,8 +51,18 @@ def test_x3_projection_error_bound(cfg3):
     for prec in (PREC_BF16X3, PREC_FP32):
         with OnlineSpikesV2(d, precision=prec, keep_stages=True, max_batch=3122) as sp:
             sp.process_batch(batch)
-            out[prec] = (sp.debug_stage(STAGE_B).astype(np.float64), sp.debug_stage(STAGE_BPC).astype(np.float64))
-    B, Bpc = out[PREC_BF16X3]
+            out[prec] = (sp.debug_stage(STAGE_B).astype(np.float64), sp.debug_stage(STAGE_BPC).astype(np.float64),
+                         sp.debug_stage(STAGE_XW).astype(np.float64))
+    B, Bpc, Xw = out[PREC_BF16X3]
+    # whitening: Wr split into hi + lo, each int16 sample exactly hi + lo, so a whitened
+    # value is within 2 * 2^-16 * sum |Wr||x| of the exact one, plus FP32 accumulation
+    Wr = tab["whiteningMat"].astype(np.float64)
+    absX = np.abs(Wr) @ np.abs(batch.astype(np.float64).T)
+    errw = np.abs(Xw - ref.Xw)
+    boundw = (2 * 2.0 ** -16 + 3 * 384 * 2.0 ** -24) * absX + 1e-7 * np.abs(ref.Xw).max()
+    print(f"split-BF16 whitening: max |err|/sum|Wr||x| {(errw / np.maximum(absX, 1e-30)).max():.2e}, "
+          f"max |err|/bound {(errw / boundw).max():.3f}")
+    assert (errw <= boundw).all()
     U = tab["Wall3"].astype(np.float64)                          # [T][K][C]
     T, K, C = U.shape
     absB = np.abs(U).reshape(T, K * C) @ np.abs(Bpc).reshape(K * C, -1)

@@ -87,11 +87,33 @@ def test_sparse_skipping_is_exact(cfg2):
     """Skipping exact-zero blocks and reordering templates changes no bit of the output."""
     m, d, tab, x = cfg2
     batch = x[:3122]
-    with OnlineSpikesV2(d, sparse=True) as a, OnlineSpikesV2(d, sparse=False) as b:
+    with OnlineSpikesV2(d, sparse=1) as a, OnlineSpikesV2(d, sparse=0) as b:
         ea, fa = a.process_batch(batch)
         eb, fb = b.process_batch(batch)
     assert np.array_equal(ea, eb)
     assert np.array_equal(fa, fb)
+
+
+def test_negligible_tails(cfg2):
+    """Sparse level 2 (the default) also drops projection weights at most 2^-40 of their
+    template's largest and ctc rows at most 2^-40 sqrt(nm_s nm_t): every spike keeps its
+    sample and template, amplitudes move by less than 1e-6 relative (a score near
+    threshold moves by far less than one FP32 rounding), and so do the features (they
+    read the amplitudes, not the tables).  The projection's K-block lists shrink."""
+    m, d, tab, x = cfg2
+    with OnlineSpikesV2(d, sparse=2) as a, OnlineSpikesV2(d, sparse=1) as b:
+        ia, ib = a.query(), b.query()
+        for j in range(3):
+            batch = x[j * 3000:j * 3000 + 3122]
+            ea, fa = a.process_batch(batch, first_sample=j * 3000)
+            eb, fb = b.process_batch(batch, first_sample=j * 3000)
+            assert np.array_equal(ea["sample"], eb["sample"]) and np.array_equal(ea["template"], eb["template"])
+            rel = np.abs(ea["amplitude"] - eb["amplitude"]) / np.abs(eb["amplitude"])
+            assert rel.max() <= 1e-6, rel.max()
+            mag = np.abs(fb).reshape(len(fb), -1).max(1)
+            assert (np.abs(fa - fb).reshape(len(fb), -1).max(1) <= 1e-6 * np.maximum(mag, 1e-30)).all()
+    print(f"projection K-blocks: {ib.proj_kblocks} exact zeros, {ia.proj_kblocks} with negligible tails")
+    assert ia.proj_kblocks <= ib.proj_kblocks
 
 
 def test_template_split_is_exact(cfg2, model_dirs):
