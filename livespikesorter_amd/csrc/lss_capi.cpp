@@ -1685,6 +1685,10 @@ const char* lss_last_error(void) { return g_last_error.c_str(); }
 // lane's launch is untouched by it).
 struct GroupLane {
     hipStream_t stream = nullptr;
+    // the proposal networks' branch of the captured tick (beside the PC conv), and the
+    // events that fork it off and join it back
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph2 = nullptr;         // cfg.profile: the peel in a graph of its own
@@ -1738,6 +1742,9 @@ struct lss_group {
             for (int16_t* b : L.h_stage)
                 if (b) (void)hipHostFree(b);
             if (L.d_xptr) (void)hipFree(L.d_xptr);
+            if (L.fork) (void)hipEventDestroy(L.fork);
+            if (L.join) (void)hipEventDestroy(L.join);
+            if (L.side) (void)hipStreamDestroy(L.side);
             if (L.stream) (void)hipStreamDestroy(L.stream);
         }
     }
@@ -1776,12 +1783,26 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         L.rb.err[s2] = h->d_err; L.rb.prof[s2] = h->d_prof; L.rb.active[s2] = &L.d_xptr[s2];
         L.rb.epoch[s2] = h->d_epoch; L.rb.ptag[s2] = h->d_ptag; L.rb.ptag_words[s2] = h->ptag_words;
     }
+    // GEMM launches of 2, 4 or 8 streams: each stream's tiles on its own share of the
+    // XCDs, so an XCD's L2 keeps the template and whitening operands of 8 / n streams
+    if (n > 1 && 8 % n == 0) {
+        int wmax = 0, pmax = 0;
+        for (int s2 = 0; s2 < n; ++s2) {
+            wmax = std::max(wmax, gemm_blocks_ilv(L.wb.job[s2].mtiles, L.wb.job[s2].ntiles, n));
+            pmax = std::max(pmax, gemm_blocks_ilv(L.pb.job[s2].mtiles, L.pb.job[s2].ntiles, n));
+        }
+        L.wb.ilv = L.pb.ilv = 1;
+        L.wb.start[n] = n * wmax;
+        L.pb.start[n] = n * pmax;
+    }
     L.drop_graph();
     hipStream_t st = L.stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     launch_preprocess(pre, st);
     launch_whiten(L.wb, &L.rb, st);
-    {   // the proposal networks of all gated streams: one dispatch per layer
+    {   // the proposal networks of all gated streams (one fused dispatch, then the gate),
+        // on a branch of the graph that runs beside the PC conv: both read only the
+        // whitened batch, and the projection waits for both
         PnFwdJob fj[PN_MAX_JOBS];
         PnGateJob gj[PN_MAX_JOBS];
         int ng = 0;
@@ -1792,10 +1813,16 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
             gj[ng] = PnGateJob{h->pnb, h->gate, g->W, &L.d_xptr[s2]};
             ++ng;
         }
-        pn_infer_batch(fj, ng, st);
-        pn_gate_batch(gj, ng, st);
+        if (ng > 0) {
+            HIPCHK(hipEventRecord(L.fork, st));
+            HIPCHK(hipStreamWaitEvent(L.side, L.fork, 0));
+            pn_infer_batch(fj, ng, L.side);
+            pn_gate_batch(gj, ng, L.side);
+            HIPCHK(hipEventRecord(L.join, L.side));
+        }
+        launch_pcconv(L.cb, g->hs[0]->K, st);
+        if (ng > 0) HIPCHK(hipStreamWaitEvent(st, L.join, 0));
     }
-    launch_pcconv(L.cb, g->hs[0]->K, st);
     launch_project(L.pb, g->hs[0]->cfg.precision, st);
     if (g->profile) {
         HIPCHK(hipStreamEndCapture(st, &L.graph));
@@ -1885,6 +1912,9 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
             if (g->profile)
                 for (auto& e : L.ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+            HIPCHK(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&L.fork, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&L.join, hipEventDisableTiming));
             HIPCHK(hipHostMalloc((void**)&L.h_xptr, sizeof(void*) * MAX_JOBS, hipHostMallocDefault));
             HIPCHK(hipMalloc((void**)&L.d_xptr, sizeof(void*) * MAX_JOBS));
         }

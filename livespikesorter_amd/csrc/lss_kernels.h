@@ -224,6 +224,11 @@ constexpr int PC_ROWS_PER_BLOCK = 4;      // channel rows of one PC conv workgro
 template <class J> struct JobBatch {
     int n;
     int start[MAX_JOBS + 1];       // first block of each job; start[n] = blocks in the launch
+    // GEMM launches: nonzero when the n jobs (n dividing 8) are interleaved over the
+    // XCDs, job j on the XCDs x = j (mod n), so each XCD's L2 holds the operands of
+    // 8 / n ... 1 jobs instead of all of them (the launch then has n * gemm_blocks_ilv
+    // blocks and start[] is not read)
+    int ilv;
     J job[MAX_JOBS];
 };
 using WhitenBatch = JobBatch<WhitenJob>;
@@ -232,6 +237,11 @@ using PcconvBatch = JobBatch<PcconvJob>;
 
 // Blocks a GEMM job occupies: M-tiles x N-tiles padded to a multiple of 8 (XCD order).
 inline int gemm_blocks(int mtiles, int ntiles) { return mtiles * ((ntiles + 7) / 8 * 8); }
+// Interleaved launches: blocks per job when n jobs share the 8 XCDs (8 / n XCDs each).
+inline int gemm_blocks_ilv(int mtiles, int ntiles, int n) {
+    const int per = 8 / n;
+    return mtiles * ((ntiles + per - 1) / per * per);
+}
 
 
 struct ResetBatch {

@@ -80,6 +80,24 @@ __device__ __forceinline__ bool xcd_tile(int k, int mtiles, int ntiles, int& m, 
     return nt < ntiles;
 }
 
+// The job and tile of this block.  Interleaved launches (bt.ilv): job j = block % n
+// runs on the XCDs x = j (mod n) and deals its tiles over them as xcd_tile does over
+// all eight (the M-tiles of one N-tile on one XCD).
+template <class J>
+__device__ __forceinline__ bool gemm_tile(const JobBatch<J>& bt, int& j, int& mt, int& nt) {
+    if (bt.ilv) {
+        const int n = bt.n, per = 8 / n;
+        j = (int)blockIdx.x % n;
+        const int k = (int)blockIdx.x / n;
+        const int slot = k / per;
+        nt = k % per + per * (slot / bt.job[j].mtiles);
+        mt = slot % bt.job[j].mtiles;
+        return nt < bt.job[j].ntiles;
+    }
+    j = job_of(bt.start, bt.n, blockIdx.x);
+    return xcd_tile(blockIdx.x - bt.start[j], bt.job[j].mtiles, bt.job[j].ntiles, mt, nt);
+}
+
 // Xw[c][XOFF + w] = sum_c' Wr[c][c'] * x[w][cmap[c']]   (rows c < CP)
 // Columns w >= W read zero input and so store zero: they are the right-hand
 // zero padding the PC convolution reads.
@@ -119,10 +137,9 @@ __global__ __launch_bounds__(256) void k_whiten(WhitenLaunch wl) {
     // the first workgroups also reset the batch's peel words, one stream each
     if ((int)blockIdx.x < wl.r.n) reset_stream(wl.r, blockIdx.x);
     const WhitenBatch& bt = wl.w;
-    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    int j, mt, nt;
+    if (!gemm_tile(bt, j, mt, nt)) return;
     const WhitenJob& jb = bt.job[j];
-    int mt, nt;
-    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     const float* __restrict__ Wr = jb.Wr;
     const int16_t* __restrict__ x = jb.xptr ? *jb.xptr : jb.x;
     const float* __restrict__ xf = jb.xf;
@@ -226,10 +243,9 @@ __global__ __launch_bounds__(256) void k_whiten_x3(WhitenLaunch wl) {
     __shared__ int s_kb[WCM / GEMM_BK];
     if ((int)blockIdx.x < wl.r.n) reset_stream(wl.r, blockIdx.x);
     const WhitenBatch& bt = wl.w;
-    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    int j, mt, nt;
+    if (!gemm_tile(bt, j, mt, nt)) return;
     const WhitenJob& jb = bt.job[j];
-    int mt, nt;
-    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     const int16_t* __restrict__ x = jb.xptr ? *jb.xptr : jb.x;
     const float* __restrict__ xf = jb.xf;
     if (!x) return;                                             // stream idle this launch
@@ -335,10 +351,9 @@ __global__ __launch_bounds__(256) void k_whiten_x3(WhitenLaunch wl) {
 
 __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     __shared__ GemmTile s;
-    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    int j, mt, nt;
+    if (!gemm_tile(bt, j, mt, nt)) return;
     const ProjectJob& jb = bt.job[j];
-    int mt, nt;
-    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
     if (jb.active && !*jb.active) return;                      // stream idle this launch
     if (jb.act && !jb.act[nt * jb.mtiles + mt]) return;        // BF16 mode: tile kept in BF16
@@ -483,10 +498,9 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     __shared__ __attribute__((aligned(16))) __bf16 Bs[GEMM_BN][BK16 + BPAD];
     __shared__ float s_nq[GEMM_BN];
     __shared__ int s_act;
-    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    int j, mt, nt;
+    if (!gemm_tile(bt, j, mt, nt)) return;
     const ProjectJob& jb = bt.job[j];
-    int mt, nt;
-    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     if (jb.active && !*jb.active) return;                      // stream idle this launch
     if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) {           // tile not proposed: not computed
         if (jb.act && threadIdx.x == 0) jb.act[nt * jb.mtiles + mt] = 0;
@@ -608,10 +622,9 @@ __global__ __launch_bounds__(256) void k_project_x3(ProjectBatch bt) {
     __shared__ __attribute__((aligned(16))) __bf16 Asl[GEMM_BM][BK16 + BPAD];
     __shared__ __attribute__((aligned(16))) __bf16 Bsh[GEMM_BN][BK16 + BPAD];
     __shared__ __attribute__((aligned(16))) __bf16 Bsl[GEMM_BN][BK16 + BPAD];
-    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    int j, mt, nt;
+    if (!gemm_tile(bt, j, mt, nt)) return;
     const ProjectJob& jb = bt.job[j];
-    int mt, nt;
-    if (!xcd_tile(blockIdx.x - bt.start[j], jb.mtiles, jb.ntiles, mt, nt)) return;
     if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
     if (jb.active && !*jb.active) return;                      // stream idle this launch
     const __bf16* __restrict__ Ah = jb.Ab;

@@ -501,21 +501,22 @@ __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
     __shared__ __attribute__((aligned(16))) float s_w4[PN_KW * PN_CH + 1];   // [k][ci], then b4
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int W = A.W, C = A.C;
-    // input: rows ch = 16 g - 4 + ci of the whitened batch from column w0 - 16, eight
-    // columns per thread and chunk, as BF16 into bx[column][ci]
-    {
-        constexpr int NCH = PN_FX / 8;              // 33 chunks per row
-        for (int i = tid; i < PN_CIN * NCH; i += 512) {
-            const int ci = i / NCH, c0 = (i - ci * NCH) * 8;
-            const int ch = g * PN_GROUP - PN_HALO + ci;
-            const float* row = A.xw + (int64_t)(ch >= 0 && ch < C ? ch : 0) * A.xs + A.xoff;
+    // input: rows ch = 16 g - 4 + ci of the whitened batch from column w0 - 16, as BF16
+    // into bx[column][ci]: a thread takes one column and eight consecutive rows (a wave's
+    // loads of one row are 64 consecutive columns) and stores them as one 16-byte word
+    static_assert(PN_CIN % 8 == 0 && PN_CH % 8 == 0, "channels come in 16-byte words of eight");
+    for (int i = tid; i < PN_FX * (PN_CIN / 8); i += 512) {
+        const int q = i / PN_FX, c = i - q * PN_FX;
+        const int w = w0 - 16 + c;
+        __bf16 v[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int w = w0 - 16 + c0 + e;
-                const float v = (ch >= 0 && ch < C && w >= 0 && w < W) ? row[w] : 0.f;
-                bx[(c0 + e) * PN_FCP + ci] = (__bf16)v;
-            }
+        for (int e = 0; e < 8; ++e) {
+            const int ch = g * PN_GROUP - PN_HALO + 8 * q + e;
+            v[e] = (__bf16)((ch >= 0 && ch < C && w >= 0 && w < W) ? A.xw[(int64_t)ch * A.xs + A.xoff + w] : 0.f);
         }
+        uint4 u;
+        __builtin_memcpy(&u, v, 16);
+        *reinterpret_cast<uint4*>(&bx[c * PN_FCP + 8 * q]) = u;
     }
     for (int i = tid; i < PN_CH * (PN_KT1 / 8); i += 512) {
         const int r = i / (PN_KT1 / 8), c8 = i - r * (PN_KT1 / 8);

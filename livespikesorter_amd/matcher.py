@@ -261,10 +261,13 @@ class StreamGroup:
         self._props = getattr(self, "_props", {})
         self._props[stream] = net
 
-    def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536, lane: int = 0):
+    def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536, lane: int = 0,
+                copy: bool = True):
         """``batches``: per stream an int16 array ``[batch][ld]`` (or, with
         ``device=True``, a device pointer and ``lds`` the row strides); ``None``
-        for a stream with no batch this call (it returns no events)."""
+        for a stream with no batch this call (it returns no events).  With
+        ``copy=False`` the returned arrays are views of the lane's buffers, valid
+        until the lane's next call."""
         n = self.n
         if device:
             ptrs = [int(b) if b is not None else None for b in batches]
@@ -283,20 +286,22 @@ class StreamGroup:
             self._ld = ld
         bufs = getattr(self, "_bufs", {})                 # reused across calls, one set per lane
         if lane not in bufs or bufs[lane][0] != cap:
-            bufs[lane] = (cap, [np.zeros(cap, EVENT_DTYPE) for _ in range(n)],
-                          [np.zeros((cap, self.info[s].nC, self.info[s].K), np.float32) for s in range(n)])
+            evs = [np.zeros(cap, EVENT_DTYPE) for _ in range(n)]
+            feats = [np.zeros((cap, self.info[s].nC, self.info[s].K), np.float32) for s in range(n)]
+            # the output arguments of the call, fixed per lane
+            bufs[lane] = (cap, evs, feats, (ct.c_void_p * n)(*[e.ctypes.data for e in evs]),
+                          (ct.c_int64 * n)(*([cap] * n)), (ct.c_void_p * n)(*[f.ctypes.data for f in feats]))
         self._bufs = bufs
-        evs, feats = bufs[lane][1], bufs[lane][2]
+        _, evs, feats, c_ev, c_cap, c_feat = bufs[lane]
         c_ptrs = (ct.c_void_p * n)(*ptrs)
         c_ld = (ct.c_int32 * n)(*ld)
         c_first = (ct.c_int64 * n)(*[int(f) for f in first_samples])
-        c_ev = (ct.c_void_p * n)(*[e.ctypes.data for e in evs])
-        c_cap = (ct.c_int64 * n)(*([cap] * n))
         c_n = (ct.c_int64 * n)()
-        c_feat = (ct.c_void_p * n)(*[f.ctypes.data for f in feats])
         check(lib().lss_group_process_lane(self._g, lane, c_ptrs, c_ld, int(device), c_first, c_ev, c_cap, c_n,
                                            c_feat))
         del keep
+        if not copy:
+            return [(evs[s][:c_n[s]], feats[s][:c_n[s]]) for s in range(n)]
         return [(evs[s][:c_n[s]].copy(), feats[s][:c_n[s]].copy()) for s in range(n)]
 
     def close(self) -> None:
