@@ -1236,7 +1236,8 @@ void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sam
     }
     const int64_t total = (int64_t)pick.size();
     *n_out = total;
-    // pass 2: copy out (the first cap of them when they do not all fit)
+    // pass 2: copy out (the first cap of them when they do not all fit); when every event
+    // is reported, the features are one block copy
     const int64_t out = std::min(total, cap);
     for (int64_t k = 0; k < out; ++k) {
         const int64_t i = pick[k];
@@ -1244,8 +1245,9 @@ void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sam
             ev[k] = h->ev[i];
             ev[k].sample += h->cur_first;
         }
-        if (feat) std::memcpy(feat + k * fw, h->feat + i * fw, sizeof(float) * fw);
+        if (feat && total != n) std::memcpy(feat + k * fw, h->feat + i * fw, sizeof(float) * fw);
     }
+    if (feat && total == n && out > 0) std::memcpy(feat, h->feat, sizeof(float) * fw * out);
     if (total > cap)
         throw LssError(LSS_ERR_CAPACITY, "event buffer holds " + std::to_string(cap) + " of " +
                                              std::to_string(total) + " events");
@@ -1685,10 +1687,6 @@ const char* lss_last_error(void) { return g_last_error.c_str(); }
 // lane's launch is untouched by it).
 struct GroupLane {
     hipStream_t stream = nullptr;
-    // the proposal networks' branch of the captured tick (beside the PC conv), and the
-    // events that fork it off and join it back
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph2 = nullptr;         // cfg.profile: the peel in a graph of its own
@@ -1742,9 +1740,6 @@ struct lss_group {
             for (int16_t* b : L.h_stage)
                 if (b) (void)hipHostFree(b);
             if (L.d_xptr) (void)hipFree(L.d_xptr);
-            if (L.fork) (void)hipEventDestroy(L.fork);
-            if (L.join) (void)hipEventDestroy(L.join);
-            if (L.side) (void)hipStreamDestroy(L.side);
             if (L.stream) (void)hipStreamDestroy(L.stream);
         }
     }
@@ -1800,9 +1795,9 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     launch_preprocess(pre, st);
     launch_whiten(L.wb, &L.rb, st);
-    {   // the proposal networks of all gated streams (one fused dispatch, then the gate),
-        // on a branch of the graph that runs beside the PC conv: both read only the
-        // whitened batch, and the projection waits for both
+    {   // the proposal networks of all gated streams: one fused dispatch, then the gate.
+        // (On a branch of the graph beside the PC conv they ran slower than one after the
+        // other: the PC conv's small workgroups hold the CUs the network's need.)
         PnFwdJob fj[PN_MAX_JOBS];
         PnGateJob gj[PN_MAX_JOBS];
         int ng = 0;
@@ -1813,15 +1808,9 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
             gj[ng] = PnGateJob{h->pnb, h->gate, g->W, &L.d_xptr[s2]};
             ++ng;
         }
-        if (ng > 0) {
-            HIPCHK(hipEventRecord(L.fork, st));
-            HIPCHK(hipStreamWaitEvent(L.side, L.fork, 0));
-            pn_infer_batch(fj, ng, L.side);
-            pn_gate_batch(gj, ng, L.side);
-            HIPCHK(hipEventRecord(L.join, L.side));
-        }
+        pn_infer_batch(fj, ng, st);
+        pn_gate_batch(gj, ng, st);
         launch_pcconv(L.cb, g->hs[0]->K, st);
-        if (ng > 0) HIPCHK(hipStreamWaitEvent(st, L.join, 0));
     }
     launch_project(L.pb, g->hs[0]->cfg.precision, st);
     if (g->profile) {
@@ -1912,9 +1901,6 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
             if (g->profile)
                 for (auto& e : L.ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
-            HIPCHK(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
-            HIPCHK(hipEventCreateWithFlags(&L.fork, hipEventDisableTiming));
-            HIPCHK(hipEventCreateWithFlags(&L.join, hipEventDisableTiming));
             HIPCHK(hipHostMalloc((void**)&L.h_xptr, sizeof(void*) * MAX_JOBS, hipHostMallocDefault));
             HIPCHK(hipMalloc((void**)&L.d_xptr, sizeof(void*) * MAX_JOBS));
         }
