@@ -1781,14 +1781,16 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     // GEMM launches of 2, 4 or 8 streams: each stream's tiles on its own share of the
     // XCDs, so an XCD's L2 keeps the template and whitening operands of 8 / n streams
     if (n > 1 && 8 % n == 0) {
-        int wmax = 0, pmax = 0;
+        int wmax = 0, pmax = 0, cmax = 0;
         for (int s2 = 0; s2 < n; ++s2) {
             wmax = std::max(wmax, gemm_blocks_ilv(L.wb.job[s2].mtiles, L.wb.job[s2].ntiles, n));
             pmax = std::max(pmax, gemm_blocks_ilv(L.pb.job[s2].mtiles, L.pb.job[s2].ntiles, n));
+            cmax = std::max(cmax, L.cb.start[s2 + 1] - L.cb.start[s2]);
         }
-        L.wb.ilv = L.pb.ilv = 1;
+        L.wb.ilv = L.pb.ilv = L.cb.ilv = 1;
         L.wb.start[n] = n * wmax;
         L.pb.start[n] = n * pmax;
+        L.cb.start[n] = n * cmax;
     }
     L.drop_graph();
     hipStream_t st = L.stream;

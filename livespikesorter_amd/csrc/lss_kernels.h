@@ -224,10 +224,10 @@ constexpr int PC_ROWS_PER_BLOCK = 4;      // channel rows of one PC conv workgro
 template <class J> struct JobBatch {
     int n;
     int start[MAX_JOBS + 1];       // first block of each job; start[n] = blocks in the launch
-    // GEMM launches: nonzero when the n jobs (n dividing 8) are interleaved over the
-    // XCDs, job j on the XCDs x = j (mod n), so each XCD's L2 holds the operands of
-    // 8 / n ... 1 jobs instead of all of them (the launch then has n * gemm_blocks_ilv
-    // blocks and start[] is not read)
+    // nonzero when the n jobs (n dividing 8) are interleaved over the XCDs, job j on
+    // the XCDs x = j (mod n), so each XCD's L2 holds the operands of 8 / n ... 1 jobs
+    // instead of all of them (GEMM launches then have n * gemm_blocks_ilv blocks, the
+    // PC conv n times the largest job's blocks, and start[] is not read)
     int ilv;
     J job[MAX_JOBS];
 };

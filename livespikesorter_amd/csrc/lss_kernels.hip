@@ -349,14 +349,30 @@ __global__ __launch_bounds__(256) void k_whiten_x3(WhitenLaunch wl) {
     store_tile(jb.Xw + XOFF, jb.XS, m0, n0, acc, lane, wm, wn);
 }
 
+// A projection tile's K-block list entries p0 .. p1 - 1, staged in LDS (indexed
+// from p0) so that no operand load waits on a list entry; a longer list is read
+// from global memory.  Ends with a barrier.
+constexpr int PROJ_KB_LDS = 512;
+__device__ __forceinline__ bool stage_kblocks(const int* __restrict__ kb_idx, int p0, int p1, int* s_kb, int tid) {
+    const bool lds = p1 - p0 <= PROJ_KB_LDS;
+    if (lds)
+        for (int i = tid; i < p1 - p0; i += 256) s_kb[i] = kb_idx[p0 + i];
+    __syncthreads();
+    return lds;
+}
+
 __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     __shared__ GemmTile s;
+    __shared__ int s_kb[PROJ_KB_LDS];
     int j, mt, nt;
     if (!gemm_tile(bt, j, mt, nt)) return;
     const ProjectJob& jb = bt.job[j];
-    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
-    if (jb.active && !*jb.active) return;                      // stream idle this launch
-    if (jb.act && !jb.act[nt * jb.mtiles + mt]) return;        // BF16 mode: tile kept in BF16
+    // the tile's flags and list bounds are loaded together (one memory latency)
+    const bool gated = jb.gate && !jb.gate[nt * jb.gate_mt + mt];   // tile not proposed: not computed
+    const bool idle = jb.active && !*jb.active;                     // stream idle this launch
+    const bool kept = jb.act && !jb.act[nt * jb.mtiles + mt];       // BF16 mode: tile kept in BF16
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    if (gated || idle || kept) return;
     const float* __restrict__ A = jb.A;
     const float* __restrict__ Bm = jb.Bm;
     const int lda = jb.lda, ldb = jb.ldb;
@@ -368,14 +384,15 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    const bool kb_lds = stage_kblocks(jb.kb_idx, p0, p1, s_kb, tid);
+    auto kbi = [&](int p) { return kb_lds ? s_kb[p - p0] : jb.kb_idx[p]; };
     float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
     auto gload = [&](int kb) {
         const int k0 = kb * GEMM_BK;
         ra = *reinterpret_cast<const float4*>(A + (int64_t)(m0 + ar) * lda + k0 + ak);
         rb = *reinterpret_cast<const float4*>(Bm + (int64_t)(k0 + br) * ldb + n0 + bc);
     };
-    if (p0 < p1) gload(jb.kb_idx[p0]);
+    if (p0 < p1) gload(kbi(p0));
     for (int p = p0; p < p1; ++p) {
         __syncthreads();
         s.As[ak + 0][ar] = ra.x;
@@ -384,7 +401,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectBatch bt) {
         s.As[ak + 3][ar] = ra.w;
         *reinterpret_cast<float4*>(&s.Bs[br][bc]) = rb;
         __syncthreads();
-        if (p + 1 < p1) gload(jb.kb_idx[p + 1]);
+        if (p + 1 < p1) gload(kbi(p + 1));
         acc = mfma_tile(s, acc, lane, wm, wn);
     }
     if (jb.act && jb.n_rescored) {
@@ -423,10 +440,12 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     constexpr int XW = 256 + 2 * XOFF;           // staged columns per row (>= 256 + M - 1 + 4)
     __shared__ __align__(16) float xs[PC_ROWS][XW];
     __shared__ __align__(16) float wt[MAX_M + 1][8];   // wt[m][k] = wpca[k][m]: a tap's weights, two 16-byte reads
-    const int j = job_of(bt.start, bt.n, blockIdx.x);
+    // interleaved launches: job j = block mod n, on the XCDs where the job's whitening ran
+    const int j = bt.ilv ? (int)blockIdx.x % bt.n : job_of(bt.start, bt.n, blockIdx.x);
+    const int k = bt.ilv ? (int)blockIdx.x / bt.n : (int)blockIdx.x - bt.start[j];
     const PcconvJob& jb = bt.job[j];
+    if (bt.ilv && k >= jb.wblocks * ((jb.C + PC_ROWS - 1) / PC_ROWS)) return;
     if (jb.active && !*jb.active) return;                      // stream idle this launch
-    const int k = blockIdx.x - bt.start[j];
     const int cg = k / jb.wblocks, w0 = (k % jb.wblocks) * 256, tid = threadIdx.x;
     const int M = jb.M, h = M / 2, C = jb.C;
     for (int i = tid; i < PC_ROWS * XW; i += 256) {
@@ -498,11 +517,15 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     __shared__ __attribute__((aligned(16))) __bf16 Bs[GEMM_BN][BK16 + BPAD];
     __shared__ float s_nq[GEMM_BN];
     __shared__ int s_act;
+    __shared__ int s_kb[PROJ_KB_LDS];
     int j, mt, nt;
     if (!gemm_tile(bt, j, mt, nt)) return;
     const ProjectJob& jb = bt.job[j];
-    if (jb.active && !*jb.active) return;                      // stream idle this launch
-    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) {           // tile not proposed: not computed
+    const bool idle = jb.active && !*jb.active;
+    const bool gated = jb.gate && !jb.gate[nt * jb.gate_mt + mt];
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    if (idle) return;                                          // stream idle this launch
+    if (gated) {                                               // tile not proposed: not computed
         if (jb.act && threadIdx.x == 0) jb.act[nt * jb.mtiles + mt] = 0;
         return;
     }
@@ -522,14 +545,15 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
     // BF16 error of every score of the tile (used by the peel's FP32 rescoring)
     float nsq = 0.f;
     const int ncol = tid >> 2, npart = tid & 3;
-    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    const bool kb_lds = stage_kblocks(jb.kb_idx, p0, p1, s_kb, tid);
+    auto kbi = [&](int p) { return kb_lds ? s_kb[p - p0] : jb.kb_idx[p]; };
     for (int p = p0; p < p1; p += 4) {
         // gather: sub-block q of this step is K-block kb_idx[p + q] (zeros past the list end)
         {
             const int pa = p + aq;
             uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0;
             if (pa < p1) {
-                const __bf16* src = A + (int64_t)(m0 + ar) * lda + jb.kb_idx[pa] * GEMM_BK;
+                const __bf16* src = A + (int64_t)(m0 + ar) * lda + kbi(pa) * GEMM_BK;
                 v0 = *reinterpret_cast<const uint4*>(src);
                 v1 = *reinterpret_cast<const uint4*>(src + 8);
             }
@@ -539,7 +563,7 @@ __global__ __launch_bounds__(256) void k_project_bf16(ProjectBatch bt) {
             const int pb = p + bq;
             __bf16 tmp[16];
             if (pb < p1) {
-                const __bf16* src = Bm + (int64_t)(jb.kb_idx[pb] * GEMM_BK + brow) * ldb + n0 + bc;
+                const __bf16* src = Bm + (int64_t)(kbi(pb) * GEMM_BK + brow) * ldb + n0 + bc;
                 const uint4 u0 = *reinterpret_cast<const uint4*>(src);
                 const uint4 u1 = *reinterpret_cast<const uint4*>(src + 8);
                 __builtin_memcpy(tmp, &u0, 16);
@@ -622,11 +646,15 @@ __global__ __launch_bounds__(256) void k_project_x3(ProjectBatch bt) {
     __shared__ __attribute__((aligned(16))) __bf16 Asl[GEMM_BM][BK16 + BPAD];
     __shared__ __attribute__((aligned(16))) __bf16 Bsh[GEMM_BN][BK16 + BPAD];
     __shared__ __attribute__((aligned(16))) __bf16 Bsl[GEMM_BN][BK16 + BPAD];
+    __shared__ int s_kb[PROJ_KB_LDS];
     int j, mt, nt;
     if (!gemm_tile(bt, j, mt, nt)) return;
     const ProjectJob& jb = bt.job[j];
-    if (jb.gate && !jb.gate[nt * jb.gate_mt + mt]) return;     // tile not proposed: not computed
-    if (jb.active && !*jb.active) return;                      // stream idle this launch
+    // the tile's flags and list bounds are loaded together (one memory latency)
+    const bool gated = jb.gate && !jb.gate[nt * jb.gate_mt + mt];   // tile not proposed: not computed
+    const bool idle = jb.active && !*jb.active;                     // stream idle this launch
+    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    if (gated || idle) return;
     const __bf16* __restrict__ Ah = jb.Ab;
     const __bf16* __restrict__ Al = jb.Al;
     const float* __restrict__ Bm = jb.Bm;
@@ -635,45 +663,55 @@ __global__ __launch_bounds__(256) void k_project_x3(ProjectBatch bt) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const int ar = tid >> 2, aq = tid & 3;               // A: row, listed block of the step
-    const int kq = tid >> 4, nq = tid & 15;              // B: rows 4 kq .. 4 kq + 3 of the step, columns 4 nq ..
-    const int bq = kq >> 2, br = (kq & 3) * 4;           //    = block bq of the step, its rows br .. br + 3
+    const int bn = tid & 63, bg = tid >> 6;              // B: column bn, step rows 8 bg .. and 32 + 8 bg ..
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    const int p0 = jb.kb_ptr[mt], p1 = jb.kb_ptr[mt + 1];
+    const bool kb_lds = stage_kblocks(jb.kb_idx, p0, p1, s_kb, tid);
+    auto kbi = [&](int p) { return kb_lds ? s_kb[p - p0] : jb.kb_idx[p]; };
     // this thread's share of a step, in registers (no arrays: they would go to scratch)
     uint4 ah0, ah1, al0, al1;       // A row segment of block aq: hi, then lo (two 16-byte words each)
-    float4 b0, b1, b2, b3;          // B: four rows of block bq, four columns each
+    float4 b0, b1, b2, b3;          // B column bn: step rows 8 bg .. 8 bg + 7 (b0, b1), 32 + 8 bg .. (b2, b3)
     auto gload = [&](int p) __attribute__((always_inline)) {
         const uint4 z = make_uint4(0u, 0u, 0u, 0u);
         ah0 = ah1 = al0 = al1 = z;
         if (p + aq < p1) {
-            const int64_t ao = (int64_t)(m0 + ar) * lda + jb.kb_idx[p + aq] * GEMM_BK;
+            const int64_t ao = (int64_t)(m0 + ar) * lda + kbi(p + aq) * GEMM_BK;
             const uint4* sh = reinterpret_cast<const uint4*>(Ah + ao);
             const uint4* sl = reinterpret_cast<const uint4*>(Al + ao);
             ah0 = sh[0]; ah1 = sh[1]; al0 = sl[0]; al1 = sl[1];
         }
+        // step rows 8 g .. 8 g + 7 are rows 8 (g & 1) .. of listed block g >> 1; a wave
+        // reads 64 consecutive columns of one row per load
         const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
         b0 = b1 = b2 = b3 = zf;
-        if (p + bq < p1) {
-            const float* src = Bm + (int64_t)(jb.kb_idx[p + bq] * GEMM_BK + br) * ldb + n0 + 4 * nq;
-            b0 = *reinterpret_cast<const float4*>(src);
-            b1 = *reinterpret_cast<const float4*>(src + ldb);
-            b2 = *reinterpret_cast<const float4*>(src + 2 * (int64_t)ldb);
-            b3 = *reinterpret_cast<const float4*>(src + 3 * (int64_t)ldb);
+        const int64_t l = ldb;
+        if (p + (bg >> 1) < p1) {
+            const float* src = Bm + (int64_t)(kbi(p + (bg >> 1)) * GEMM_BK + 8 * (bg & 1)) * l + n0 + bn;
+            b0 = make_float4(src[0], src[l], src[2 * l], src[3 * l]);
+            b1 = make_float4(src[4 * l], src[5 * l], src[6 * l], src[7 * l]);
+        }
+        if (p + 2 + (bg >> 1) < p1) {
+            const float* src = Bm + (int64_t)(kbi(p + 2 + (bg >> 1)) * GEMM_BK + 8 * (bg & 1)) * l + n0 + bn;
+            b2 = make_float4(src[0], src[l], src[2 * l], src[3 * l]);
+            b3 = make_float4(src[4 * l], src[5 * l], src[6 * l], src[7 * l]);
         }
     };
-    // column c of the four staged rows as hi and lo BF16 parts, one 8-byte store each
-    auto split4 = [&](float x0, float x1, float x2, float x3, int c) __attribute__((always_inline)) {
-        const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1, h2 = (__bf16)x2, h3 = (__bf16)x3;
-        const __bf16 hi[4] = {h0, h1, h2, h3};
-        const __bf16 lo[4] = {(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1), (__bf16)(x2 - (float)h2),
-                              (__bf16)(x3 - (float)h3)};
-        uint2 uh, ul;
-        __builtin_memcpy(&uh, hi, 8);
-        __builtin_memcpy(&ul, lo, 8);
-        *reinterpret_cast<uint2*>(&Bsh[4 * nq + c][bq * 16 + br]) = uh;
-        *reinterpret_cast<uint2*>(&Bsl[4 * nq + c][bq * 16 + br]) = ul;
+    // eight consecutive k of column bn as hi and lo BF16 parts, one 16-byte store each
+    // (the 64 columns of a wave's stores fall on distinct banks in groups of eight)
+    auto split8 = [&](float4 u, float4 v, int k0) __attribute__((always_inline)) {
+        const float x[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+        __bf16 hi[8], lo[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            hi[e] = (__bf16)x[e];
+            lo[e] = (__bf16)(x[e] - (float)hi[e]);
+        }
+        uint4 uh, ul;
+        __builtin_memcpy(&uh, hi, 16);
+        __builtin_memcpy(&ul, lo, 16);
+        *reinterpret_cast<uint4*>(&Bsh[bn][k0]) = uh;
+        *reinterpret_cast<uint4*>(&Bsl[bn][k0]) = ul;
     };
     if (p0 < p1) gload(p0);
     for (int p = p0; p < p1; p += 4) {
@@ -682,10 +720,8 @@ __global__ __launch_bounds__(256) void k_project_x3(ProjectBatch bt) {
         *reinterpret_cast<uint4*>(&Ash[ar][aq * 16 + 8]) = ah1;
         *reinterpret_cast<uint4*>(&Asl[ar][aq * 16]) = al0;
         *reinterpret_cast<uint4*>(&Asl[ar][aq * 16 + 8]) = al1;
-        split4(b0.x, b1.x, b2.x, b3.x, 0);
-        split4(b0.y, b1.y, b2.y, b3.y, 1);
-        split4(b0.z, b1.z, b2.z, b3.z, 2);
-        split4(b0.w, b1.w, b2.w, b3.w, 3);
+        split8(b0, b1, 8 * bg);
+        split8(b2, b3, 32 + 8 * bg);
         __syncthreads();
         if (p + 4 < p1) gload(p + 4);
         const int nk = min(4, p1 - p);

@@ -490,10 +490,16 @@ __device__ __forceinline__ void pn_fused_layer(const __bf16* in, __bf16* out, co
 }
 
 __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
-    const PnInferArgs& A = tb.a[blockIdx.z];
-    if (A.active && !*A.active) return;
-    const int g = blockIdx.y, w0 = blockIdx.x * PN_FT;
+    // grid (stream, column tile, group): consecutive workgroups are consecutive streams,
+    // so with n | 8 streams stream j runs on the XCDs x = j mod n, where an interleaved
+    // whitening launch left its whitened batch (in that XCD's L2)
+    const PnInferArgs& A = tb.a[blockIdx.x];
+    const int g = blockIdx.z, w0 = blockIdx.y * PN_FT;
     if (g >= A.G || w0 >= A.W) return;
+    // the stream's active flag is read beside the prologue's loads (every address
+    // below is valid for an idle stream too); an idle stream's workgroups leave
+    // after the prologue's barrier, where the flag has arrived with the rest
+    const bool idle = A.active && !*A.active;
     __shared__ __attribute__((aligned(16))) __bf16 bx[PN_FX * PN_FCP];
     __shared__ __attribute__((aligned(16))) __bf16 by[PN_FX * PN_FCP];
     __shared__ __attribute__((aligned(16))) __bf16 ws[PN_CH * PN_FWP];
@@ -503,34 +509,58 @@ __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
     const int W = A.W, C = A.C;
     // input: rows ch = 16 g - 4 + ci of the whitened batch from column w0 - 16, as BF16
     // into bx[column][ci]: a thread takes one column and eight consecutive rows (a wave's
-    // loads of one row are 64 consecutive columns) and stores them as one 16-byte word
+    // loads of one row are 64 consecutive columns) and stores them as one 16-byte word.
+    // Every global load of the prologue (input, layer 1's weights, the biases, the
+    // head) is issued before the first LDS store, so they take one memory latency.
     static_assert(PN_CIN % 8 == 0 && PN_CH % 8 == 0, "channels come in 16-byte words of eight");
-    for (int i = tid; i < PN_FX * (PN_CIN / 8); i += 512) {
+    constexpr int NIN = PN_FX * (PN_CIN / 8);            // items of eight input rows
+    constexpr int NW1 = PN_CH * (PN_KT1 / 8);            // 16-byte words of layer 1's weights
+    static_assert(NIN <= 1024 && NW1 <= 1024 && PN_KW * PN_CH < 512 && 3 * PN_CH <= 512,
+                  "the prologue's items fit two per thread");
+    float xv[2][8];
+    uint4 w1v[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = tid + 512 * h;
         const int q = i / PN_FX, c = i - q * PN_FX;
         const int w = w0 - 16 + c;
-        __bf16 v[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int ch = g * PN_GROUP - PN_HALO + 8 * q + e;
-            v[e] = (__bf16)((ch >= 0 && ch < C && w >= 0 && w < W) ? A.xw[(int64_t)ch * A.xs + A.xoff + w] : 0.f);
+            xv[h][e] = (i < NIN && ch >= 0 && ch < C && w >= 0 && w < W) ? A.xw[(int64_t)ch * A.xs + A.xoff + w] : 0.f;
         }
-        uint4 u;
-        __builtin_memcpy(&u, v, 16);
-        *reinterpret_cast<uint4*>(&bx[c * PN_FCP + 8 * q]) = u;
+        w1v[h] = i < NW1 ? reinterpret_cast<const uint4*>(A.wt1)[i] : make_uint4(0u, 0u, 0u, 0u);
     }
-    for (int i = tid; i < PN_CH * (PN_KT1 / 8); i += 512) {
-        const int r = i / (PN_KT1 / 8), c8 = i - r * (PN_KT1 / 8);
-        *reinterpret_cast<uint4*>(&ws[r * PN_FWP + c8 * 8]) = *reinterpret_cast<const uint4*>(A.wt1 + r * PN_KT1 + c8 * 8);
+    float bias_v = 0.f, w4_v = 0.f;
+    if (tid < 3 * PN_CH) {
+        const int l = tid / PN_CH, co = tid - l * PN_CH;
+        bias_v = A.params[(l == 0 ? PN_B1 : l == 1 ? PN_B2 : PN_B3) + co];
     }
-    for (int i = tid; i < 3 * PN_CH; i += 512) {
-        const int l = i / PN_CH, co = i - l * PN_CH;
-        s_bias[i] = A.params[(l == 0 ? PN_B1 : l == 1 ? PN_B2 : PN_B3) + co];
+    if (tid < PN_KW * PN_CH) {
+        const int k = tid / PN_CH, ci = tid - k * PN_CH;
+        w4_v = A.params[PN_W4 + ci * PN_KW + k];
+    } else if (tid == PN_KW * PN_CH) {
+        w4_v = A.params[PN_B4];
     }
-    for (int i = tid; i < PN_KW * PN_CH; i += 512) {
-        const int k = i / PN_CH, ci = i - k * PN_CH;
-        s_w4[i] = A.params[PN_W4 + ci * PN_KW + k];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = tid + 512 * h;
+        if (i < NIN) {
+            const int q = i / PN_FX, c = i - q * PN_FX;
+            __bf16 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (__bf16)xv[h][e];
+            uint4 u;
+            __builtin_memcpy(&u, v, 16);
+            *reinterpret_cast<uint4*>(&bx[c * PN_FCP + 8 * q]) = u;
+        }
+        if (i < NW1) {
+            const int r = i / (PN_KT1 / 8), c8 = i - r * (PN_KT1 / 8);
+            *reinterpret_cast<uint4*>(&ws[r * PN_FWP + c8 * 8]) = w1v[h];
+        }
     }
-    if (tid == 0) s_w4[PN_KW * PN_CH] = A.params[PN_B4];
+    if (tid < 3 * PN_CH) s_bias[tid] = bias_v;
+    if (tid <= PN_KW * PN_CH) s_w4[tid] = w4_v;
     // the next layer's weights, in flight while the current layer runs
     constexpr int NW = PN_CH * PN_KP / 8;           // 16-byte words of a 32 x 288 layer
     constexpr int NWT = (NW + 511) / 512;
@@ -553,6 +583,7 @@ __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
         }
     };
     __syncthreads();
+    if (idle) return;                                 // uniform over the workgroup
     wload(A.wt2);
     pn_fused_layer<PN_CIN>(bx, by, ws, s_bias, w0 - 12, W, lane, wv);
     __syncthreads();
@@ -603,32 +634,30 @@ struct PnGateArgs {
 struct PnGateTable { int n; PnGateArgs a[PN_MAX_JOBS]; };
 
 __global__ __launch_bounds__(256) void k_pn_gate(PnGateTable tb) {
-    const PnGateArgs& A = tb.a[blockIdx.z];
-    if (A.active && !*A.active) return;
+    // grid (stream, column block): stream j on the XCDs x = j mod n, as its network's
+    // workgroups and its projection tiles
+    const PnGateArgs& A = tb.a[blockIdx.x];
+    const bool idle = A.active && !*A.active;
     const float* __restrict__ p = A.p;
     const int G = A.G, W = A.W, Wp = A.Wp;
     const PnGate& gt = A.gt;
-    if ((int)blockIdx.x * 64 >= W) return;
+    if ((int)blockIdx.y * 64 >= W) return;
     __shared__ int s_state[64];                       // per group: 0 off, 1 uncertain, 2 on
-    __shared__ int s_unc;
-    const int n = blockIdx.x;
+    const int n = blockIdx.y;
     const int lo = max(0, n * 64 - gt.reach), hi = min(W, n * 64 + 64 + gt.reach);
-    if (threadIdx.x == 0) s_unc = 0;
-    __syncthreads();
     for (int g = threadIdx.x >> 6; g < G; g += 4) {  // one wave per group
         float mx = 0.f;
         for (int w = lo + (threadIdx.x & 63); w < hi; w += 64) mx = fmaxf(mx, p[(int64_t)g * Wp + w]);
         for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-        if ((threadIdx.x & 63) == 0) {
-            const int st = mx >= gt.hi ? 2 : (mx > gt.lo ? 1 : 0);
-            s_state[g] = st;
-            if (st == 1) atomicOr(&s_unc, 1);
-        }
+        if ((threadIdx.x & 63) == 0) s_state[g] = mx >= gt.hi ? 2 : (mx > gt.lo ? 1 : 0);
     }
     __syncthreads();
+    if (idle) return;                                 // stream idle this launch (uniform)
+    bool unc = false;                                 // some group uncertain: the fallback
+    for (int g = 0; g < G; ++g) unc = unc || s_state[g] == 1;
     int scored = 0;
     for (int m = threadIdx.x; m < gt.MT; m += 256) {
-        bool on = s_unc != 0;
+        bool on = unc;
         if (!on) {
             const int a = max(0, gt.mt_glo[m] - 1), b = min(G - 1, gt.mt_ghi[m] + 1);
             for (int g = a; g <= b && !on; ++g) on = s_state[g] == 2;
@@ -640,7 +669,7 @@ __global__ __launch_bounds__(256) void k_pn_gate(PnGateTable tb) {
     if ((threadIdx.x & 63) == 0 && scored) atomicAdd(&gt.stats[0], (unsigned long long)scored);
     if (threadIdx.x == 0) {
         atomicAdd(&gt.stats[1], (unsigned long long)gt.MT);
-        if (s_unc) atomicAdd(&gt.stats[2], 1ull);
+        if (unc) atomicAdd(&gt.stats[2], 1ull);
         atomicAdd(&gt.stats[3], 1ull);
     }
 }
@@ -690,7 +719,7 @@ void pn_infer_batch(const PnFwdJob* jobs, int n, hipStream_t s) {
         Gm = max(Gm, J.b.G);
         Wm = max(Wm, J.W);
     }
-    hipLaunchKernelGGL(k_pn_fused, dim3((Wm + PN_FT - 1) / PN_FT, Gm, n), dim3(512), 0, s, t);
+    hipLaunchKernelGGL(k_pn_fused, dim3(n, (Wm + PN_FT - 1) / PN_FT, Gm), dim3(512), 0, s, t);
 }
 
 void pn_infer(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
@@ -708,7 +737,7 @@ void pn_gate_batch(const PnGateJob* jobs, int n, hipStream_t s) {
         tg.a[j] = PnGateArgs{jobs[j].b.p, jobs[j].b.G, jobs[j].W, jobs[j].b.Wp, jobs[j].gt, jobs[j].active};
         Wm = max(Wm, jobs[j].W);
     }
-    hipLaunchKernelGGL(k_pn_gate, dim3((Wm + 63) / 64, 1, n), dim3(256), 0, s, tg);
+    hipLaunchKernelGGL(k_pn_gate, dim3(n, (Wm + 63) / 64), dim3(256), 0, s, tg);
 }
 
 void pn_forward(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
