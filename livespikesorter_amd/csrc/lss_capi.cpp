@@ -186,6 +186,7 @@ struct lss_handle {
     uint16_t *d_U2b = nullptr, *d_Bpcb = nullptr;
     uint16_t* d_U2l = nullptr;       // split-BF16: the lo part of U2 (d_U2b holds the hi part)
     uint16_t *d_Wrh = nullptr, *d_Wrl = nullptr;   // split-BF16: the whitening matrix's hi and lo parts
+    uint16_t* d_pcq = nullptr;                     //   and the PC conv's banded PCA operand
     float* d_Binit = nullptr;
     float* d_Nsq = nullptr;          // per 64-template tile and column: squared norm of the BF16 operand
     uint8_t* d_act = nullptr;        // per score tile [BS/64][TP/64]: recomputed by the FP32 projection
@@ -266,7 +267,7 @@ struct lss_handle {
         void* ptrs[] = {d_Wr, d_U2, d_U, d_wpca, d_nm, d_rnm, d_ctc, d_Gt, d_nm64, d_orig, d_tlo, d_thi,
                         d_iCC, d_iU, d_cmap, d_kbw_ptr, d_kbw_idx, d_kbp_ptr, d_kbp_idx, d_Xw, d_Bpc, d_B,
                         d_Cfmax, d_slots, d_list, d_slot_cnt, d_list_cnt, d_iter_cnt, d_err, d_bar,
-                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_U2l, d_Wrh, d_Wrl, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
+                        d_dbgB, d_dbgCf, d_dbgIm, d_x, d_prof, d_U2b, d_U2l, d_Wrh, d_Wrl, d_pcq, d_Bpcb, d_Binit, d_Nsq, d_act, d_rescnt, d_ptag, d_epoch,
                         d_gate, d_mt_glo, d_mt_ghi, d_gate_stats, d_xc, d_yc, d_cent, d_hp, d_psum, d_pmin,
                         d_pmax, d_pmean, d_pmed, d_X1, d_Xp};
         (void)hipSetDevice(dev);
@@ -629,6 +630,18 @@ void load(lss_handle* h, const std::string& dir_in) {
         split(Wrp, hi, lo);
         h->d_Wrh = dalloc<uint16_t>(hi.size()); upload(h->d_Wrh, hi);
         h->d_Wrl = dalloc<uint16_t>(lo.size()); upload(h->d_Wrl, lo);
+        // the PC conv's B operand Q[n][j'] = wpca[n / 16][j' - n % 16 + M/2 - 32] (zero outside
+        // 0 .. M - 1 and for n / 16 >= K), n < 32 ceil(K / 2), j' < PCX_K: hi rows, then lo rows
+        const int nq = 32 * ((K + 1) / 2);
+        std::vector<float> q((size_t)nq * PCX_K, 0.f);
+        for (int n = 0; n < nq; ++n)
+            for (int jp = 0; jp < PCX_K; ++jp) {
+                const int kq = n / 16, m = jp - n % 16 + M / 2 - 32;
+                if (kq < K && m >= 0 && m < M) q[(size_t)n * PCX_K + jp] = wpf[(size_t)kq * M + m];
+            }
+        split(q, hi, lo);
+        hi.insert(hi.end(), lo.begin(), lo.end());
+        h->d_pcq = dalloc<uint16_t>(hi.size()); upload(h->d_pcq, hi);
     }
     h->d_U = dalloc<float>(Ui.size()); upload(h->d_U, Ui);
     h->d_wpca = dalloc<float>(wpf.size()); upload(h->d_wpca, wpf);
@@ -905,7 +918,6 @@ struct BatchPlan {
 };
 
 // PC conv workgroups per 256-column block: channel rows PC_ROWS_PER_BLOCK at a time.
-inline int pc_groups(int C) { return (C + PC_ROWS_PER_BLOCK - 1) / PC_ROWS_PER_BLOCK; }
 
 inline int peel_cap_index(int R) { return R == 0 ? 0 : R == 8 ? 1 : R == 16 ? 2 : R == 32 ? 3 : 4; }
 
@@ -959,7 +971,9 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     cj.Xw = bb.Xw; cj.wpca = h->d_wpca; cj.Bpc = bb.Bpc;
     cj.Bpcb = reinterpret_cast<__bf16*>(bb.Bpcb);
     cj.active = xptr;
-    cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP; cj.C = h->C;
+    cj.XS = h->XS; cj.M = M; cj.BS = h->BS; cj.CP = h->CP; cj.C = h->C; cj.x3 = h->x3 ? 1 : 0;
+    cj.q = reinterpret_cast<const __bf16*>(h->d_pcq);
+    cj.ncols = (W + GEMM_BN - 1) / GEMM_BN * GEMM_BN;
     cj.wblocks = ((W + GEMM_BN - 1) / GEMM_BN * GEMM_BN + 255) / 256;
     ProjectJob& pj = bp.pj;
     pj = ProjectJob{};
@@ -1143,7 +1157,7 @@ void run_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, int W, in
     WhitenBatch wb{};
     wb.n = 1; wb.start[0] = 0; wb.start[1] = gemm_blocks(bp.wj.mtiles, bp.wj.ntiles); wb.job[0] = bp.wj;
     PcconvBatch cb{};
-    cb.n = 1; cb.start[0] = 0; cb.start[1] = bp.cj.wblocks * pc_groups(h->C); cb.job[0] = bp.cj;
+    cb.n = 1; cb.start[0] = 0; cb.start[1] = pc_job_blocks(bp.cj); cb.job[0] = bp.cj;
     ProjectBatch pb{};
     pb.n = 1; pb.start[0] = 0; pb.start[1] = gemm_blocks(bp.pj.mtiles, bp.pj.ntiles); pb.job[0] = bp.pj;
     PeelBatch kb{};
@@ -1769,7 +1783,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         L.wb.job[s2] = bp.wj;
         L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
         L.cb.job[s2] = bp.cj;
-        L.cb.start[s2 + 1] = L.cb.start[s2] + bp.cj.wblocks * pc_groups(h->C);
+        L.cb.start[s2 + 1] = L.cb.start[s2] + pc_job_blocks(bp.cj);
         L.pb.job[s2] = bp.pj;
         L.pb.start[s2 + 1] = L.pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
         L.kb.p[s2] = bp.pp;

@@ -219,8 +219,18 @@ struct PcconvJob {
     const int16_t* const* active;         // optional: the job is skipped when *active is null
     int XS, M, BS, CP, wblocks;
     int C;                                // channels (rows PC_ROWS to a workgroup)
+    int x3;                               // split-BF16 products on MFMA (32 channel rows to a workgroup)
+    const __bf16* q;                      //   its banded PCA operand, hi rows then lo rows ([n][PCX_K])
+    int ncols;                            //   and the columns it computes (the batch rounded up to 64)
 };
 constexpr int PC_ROWS_PER_BLOCK = 4;      // channel rows of one PC conv workgroup
+constexpr int PCX_ROWS = 32;              //   of one split-BF16 PC conv workgroup
+constexpr int PCX_K = 80;                 // samples of its 16-column sub-tile (M <= 63: M/2 + 48 <= 80)
+// Workgroups of a PC conv job: 256-column blocks x channel-row blocks.
+__host__ __device__ inline int pc_job_blocks(const PcconvJob& j) {
+    const int rows = j.x3 ? PCX_ROWS : PC_ROWS_PER_BLOCK;
+    return j.wblocks * ((j.C + rows - 1) / rows);
+}
 template <class J> struct JobBatch {
     int n;
     int start[MAX_JOBS + 1];       // first block of each job; start[n] = blocks in the launch

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
     const int j = bt.ilv ? (int)blockIdx.x % bt.n : job_of(bt.start, bt.n, blockIdx.x);
     const int k = bt.ilv ? (int)blockIdx.x / bt.n : (int)blockIdx.x - bt.start[j];
     const PcconvJob& jb = bt.job[j];
-    if (bt.ilv && k >= jb.wblocks * ((jb.C + PC_ROWS - 1) / PC_ROWS)) return;
+    if (bt.ilv && k >= pc_job_blocks(jb)) return;
     if (jb.active && !*jb.active) return;                      // stream idle this launch
     const int cg = k / jb.wblocks, w0 = (k % jb.wblocks) * 256, tid = threadIdx.x;
     const int M = jb.M, h = M / 2, C = jb.C;
@@ -501,6 +501,114 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
 }
 
 // ---------------------------------------------------------------------------
+// The PC conv as split-BF16 products on MFMA (precision LSS_PREC_BF16X3).  Over
+// 16 output columns w0 .. w0 + 15 and the 80 samples j = w0 - 32 + j' the conv is
+// the GEMM
+//   Bpc[k][c][w0 + w'] = sum_j' Xw[c][w0 - 32 + j'] * Q[j'][k * 16 + w'],
+//   Q[j'][k * 16 + w'] = wpca[k][j' - w' + M/2 - 32]   (zero outside 0 .. M - 1),
+// whose B operand Q depends on neither the data nor w0: it is split into BF16 hi
+// and lo once per handle and staged in LDS as [n][j'] by each workgroup.  The A operand is the
+// whitened rows themselves (eight consecutive samples of one channel per lane),
+// split while loaded.  Each product is lo_A*hi_B + hi_A*lo_B + hi_A*hi_B with FP32
+// accumulation, so an output is within 3 * 2^-16 * sum |wpca||Xw| of the exact
+// sum plus the FP32 accumulation error.  A wave owns 32 channels and 64 columns
+// (four 16-column sub-tiles); consecutive sub-tiles share four of their five
+// 16-sample blocks, which stay in registers.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int PCX_SUB = 4;         // 16-column sub-tiles of a wave
+
+template <int KK>
+__global__ __launch_bounds__(256) void k_pcconv_x3(PcconvBatch bt) {
+    constexpr int NB = (KK + 1) / 2;             // 32-wide blocks of n = k * 16 + w'
+    constexpr int QP = PCX_K + 8;                // LDS pitch of a Q row (conflict-free 16-byte reads)
+    __shared__ __attribute__((aligned(16))) __bf16 Qh[NB * 32][QP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ql[NB * 32][QP];
+    const int j = bt.ilv ? (int)blockIdx.x % bt.n : job_of(bt.start, bt.n, blockIdx.x);
+    const int k = bt.ilv ? (int)blockIdx.x / bt.n : (int)blockIdx.x - bt.start[j];
+    const PcconvJob& jb = bt.job[j];
+    if (bt.ilv && k >= pc_job_blocks(jb)) return;
+    if (jb.active && !*jb.active) return;                      // stream idle this launch
+    const int C = jb.C;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int cb = (k / jb.wblocks) * PCX_ROWS;                 // the workgroup's channel rows
+    const int wb0 = (k % jb.wblocks) * 256 + wv * (16 * PCX_SUB);   // the wave's first column
+    static_assert(PCX_K % 8 == 0, "Q rows are copied as 16-byte words");
+    for (int i = tid; i < 2 * NB * 32 * (PCX_K / 8); i += 256) {
+        const int r = i / (PCX_K / 8), c8 = i - r * (PCX_K / 8);      // r < NB * 32: hi, else lo
+        const uint4 v = reinterpret_cast<const uint4*>(jb.q)[i];
+        __bf16* dst = r < NB * 32 ? &Qh[r][8 * c8] : &Ql[r - NB * 32][8 * c8];
+        *reinterpret_cast<uint4*>(dst) = v;
+    }
+    // A: channel cb + lane % 32, samples wb0 - 32 + 16 b + 8 (lane / 32) + e of block b
+    // (Xw rows hold zeros from column -XOFF and past the batch)
+    const int c = cb + (lane & 31);
+    const bool cv = c < C;
+    const float* xr = jb.Xw + (int64_t)(cv ? c : 0) * jb.XS + XOFF + wb0 - 32 + 8 * (lane >> 5);
+    bf16x8 ah[PCX_SUB + 4], al[PCX_SUB + 4];
+    auto aload = [&](int b) __attribute__((always_inline)) {
+        float4 u = make_float4(0.f, 0.f, 0.f, 0.f), v = u;
+        if (cv) {
+            u = *reinterpret_cast<const float4*>(xr + 16 * b);
+            v = *reinterpret_cast<const float4*>(xr + 16 * b + 4);
+        }
+        const float x[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const __bf16 hi = (__bf16)x[e];
+            ah[b][e] = hi;
+            al[b][e] = (__bf16)(x[e] - (float)hi);
+        }
+    };
+    // columns past the batch rounded up to 64 are not computed (a wave's 64 columns are
+    // all inside or all outside)
+    const bool live = wb0 < jb.ncols;
+    if (live) {
+#pragma unroll
+        for (int b = 0; b < 5; ++b) aload(b);
+    }
+    __syncthreads();                             // Q staged
+    if (!live) return;
+#pragma unroll
+    for (int t = 0; t < PCX_SUB; ++t) {
+        if (t + 5 < PCX_SUB + 4) aload(t + 5);   // the next sub-tile's new block
+        // Q is re-read from LDS per sub-tile (held in registers over all four it took
+        // 210 VGPRs: one wave per SIMD)
+        asm volatile("" ::: "memory");
+        f32x16 acc[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[nb][r] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Qh[nb * 32 + (lane & 31)][16 * s + 8 * (lane >> 5)]);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Ql[nb * 32 + (lane & 31)][16 * s + 8 * (lane >> 5)]);
+                acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[t + s], bh, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t + s], bl, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t + s], bh, acc[nb], 0, 0, 0);
+            }
+        }
+        // D[i][n]: channel cb + i, PC k = n / 16, column wb0 + 16 t + n % 16 (rows past C are
+        // zero, so the padded rows of the projection's operand never hold stale values)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int n = nb * 32 + (lane & 31), kq = n >> 4;
+            if (kq < KK) {
+                float* dst = jb.Bpc + ((int64_t)kq * jb.CP + cb) * jb.BS + wb0 + 16 * t + (n & 15);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+                    if (cb + i < jb.CP) dst[(int64_t)i * jb.BS] = acc[nb][r];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // BF16 projection with FP32 accumulation (v_mfma_f32_32x32x16_bf16; lane l holds
 // A[l % 32][8 * (l / 32) + e] and B[8 * (l / 32) + e][l % 32], e = 0..7).  Each
 // pipeline step gathers four listed 16-wide K-blocks into a 64-deep LDS tile
@@ -508,7 +616,6 @@ __global__ __launch_bounds__(256) void k_pcconv(PcconvBatch bt) {
 // granularity.  Optionally writes a second copy of the result (the initial
 // scores that the peel's FP32 rescoring compares against).
 // ---------------------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int BK16 = 64;           // K depth of one BF16 step (four 16-blocks)
 constexpr int BPAD = 8;
 
@@ -2208,9 +2315,13 @@ void launch_whiten(const WhitenBatch& b, const ResetBatch* rb, hipStream_t s) {
 }
 
 void launch_pcconv(const PcconvBatch& b, int K, hipStream_t s) {
+    const bool x3 = b.job[0].x3 != 0;           // the jobs of a launch share the precision
     switch (K) {
 #define LSS_PC_CASE(KK) \
-    case KK: hipLaunchKernelGGL(k_pcconv<KK>, dim3(b.start[b.n]), dim3(256), 0, s, b); break;
+    case KK: \
+        if (x3) hipLaunchKernelGGL(k_pcconv_x3<KK>, dim3(b.start[b.n]), dim3(256), 0, s, b); \
+        else hipLaunchKernelGGL(k_pcconv<KK>, dim3(b.start[b.n]), dim3(256), 0, s, b); \
+        break;
         LSS_PC_CASE(1) LSS_PC_CASE(2) LSS_PC_CASE(3) LSS_PC_CASE(4)
         LSS_PC_CASE(5) LSS_PC_CASE(6) LSS_PC_CASE(7) LSS_PC_CASE(8)
 #undef LSS_PC_CASE

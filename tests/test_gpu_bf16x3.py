@@ -1,4 +1,4 @@
-"""Split-BF16 whitening and projection (precision LSS_PREC_BF16X3).
+"""Split-BF16 whitening, PC conv and projection (precision LSS_PREC_BF16X3).
 
 Each FP32 operand is carried as a BF16 pair x = hi + lo + e with
 hi = bf16(x), lo = bf16(x - hi), |e| <= 2^-16 |x|, and each product as the
@@ -8,11 +8,13 @@ dropped lo*lo term and the two residuals e bound the product error by
 3 * 2^-16 * sum|U||Bpc| of the exact dot product, plus FP32 accumulation
 error; the peel then runs on those scores as on FP32 ones.  The whitening
 GEMM runs the same way (an int16 sample is exactly a BF16 hi + lo pair, so its
-bound is 2 * 2^-16 * sum|Wr||x|).
+bound is 2 * 2^-16 * sum|Wr||x|), and so does the PC conv, as a GEMM of the
+whitened rows with a banded matrix of the PCA basis (3 * 2^-16 * sum|wPCA||Xw|).
 
 Model: 384 channels, 1024 templates, bursty gamma(0.5) ISIs, banded whitening
 (cfg3).  Bars: the whitening and projection stages within those bounds of the
-FP64 oracle's; events against the FP64 oracle at the plain matcher's bar
+FP64 oracle's, the PC conv within its bound of the FP64 conv of the same
+whitened rows; events against the FP64 oracle at the plain matcher's bar
 (tests/parity.py check_events: identical >= 99.5%, the rest +-1 sample and each
 a near tie of the oracle's decision margins, amplitudes 1e-3, features 1e-3),
 on 12000-sample batches (register-tile peel) and on one 60000-sample batch
@@ -63,6 +65,22 @@ def test_x3_whitening_and_projection_error_bounds(cfg3):
     print(f"split-BF16 whitening: max |err|/sum|Wr||x| {(errw / np.maximum(absX, 1e-30)).max():.2e}, "
           f"max |err|/bound {(errw / boundw).max():.3f}")
     assert (errw <= boundw).all()
+    # PC conv: split-BF16 products of wPCA and the GPU's own whitened rows with FP32
+    # accumulation of 3 * 80 products, so within (3 * 2^-16 + 240 * 2^-24) * sum |wPCA||Xw|
+    # of the FP64 conv of the same rows (zero outside the batch)
+    P = tab["wPCA"].astype(np.float64)                           # [K][M]
+    Kp, Mp = P.shape
+    hp = Mp // 2
+    Wb = Xw.shape[1]
+    Xpad = np.zeros((Xw.shape[0], Wb + 2 * hp))
+    Xpad[:, hp:hp + Wb] = Xw
+    conv = np.stack([sum(P[k, m] * Xpad[:, m:m + Wb] for m in range(Mp)) for k in range(Kp)])
+    aconv = np.stack([sum(abs(P[k, m]) * np.abs(Xpad[:, m:m + Wb]) for m in range(Mp)) for k in range(Kp)])
+    errc = np.abs(Bpc - conv)
+    boundc = (3 * 2.0 ** -16 + 240 * 2.0 ** -24) * aconv + 1e-30
+    print(f"split-BF16 PC conv: max |err|/sum|wPCA||Xw| {(errc / np.maximum(aconv, 1e-30)).max():.2e}, "
+          f"max |err|/bound {(errc / boundc).max():.3f}")
+    assert (errc <= boundc).all()
     U = tab["Wall3"].astype(np.float64)                          # [T][K][C]
     T, K, C = U.shape
     absB = np.abs(U).reshape(T, K * C) @ np.abs(Bpc).reshape(K * C, -1)
