@@ -989,9 +989,9 @@ def bench_cfg2(args) -> dict:
     if tf.exists():
         # the peel's entry is the kernel of the timed chunks (its tile height)
         kname = {"whiten": "lss::k_whiten", "pcconv": "lss::k_pcconv", "project": "lss::k_project",
-                 "peel": f"lss::k_peel<{int(q_last.peel_rows)}>"}[dom]
+                 "peel": f"lss::k_peel<{int(q_last.peel_rows)}"}[dom]
         for k, v in json.loads(tf.read_text())["bytes_per_launch"].items():
-            if k.startswith(kname):
+            if k.startswith(kname) and (dom != "peel" or k[len(kname)] in ">,"):
                 traffic = round(v)
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 5), "traffic": traffic, "kernel": dom,
