@@ -110,7 +110,10 @@ __device__ __forceinline__ bool gemm_tile(const JobBatch<J>& bt, int& j, int& mt
 // sample load never waits for a map entry; with the identity map and aligned
 // rows a thread reads its 16 channels as two 16-byte loads.
 constexpr int WBK = 4 * GEMM_BK;     // reduction depth staged per step
-constexpr int WCM = 2048;            // channel maps up to this size are staged in LDS
+// Channel maps up to this size are staged in LDS (larger ones are read from global
+// memory).  Kept small: with it the whitening tiles take 37-39 KB of LDS, four
+// workgroups per CU (a 2048-entry map left room for three).
+constexpr int WCM = 512;
 
 struct WhitenTile {
     float As[WBK][GEMM_BM + LDS_PAD];
@@ -1221,6 +1224,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
         s_tot = !stop;
         s_err = ld_c(p.err);
     };
+    // a wave whose 64-template tile the gate did not propose for this column block
+    // scores zero everywhere (its stale rows are neither updated nor read); the mask
+    // is fixed for the batch, so it is read once here, not in every iteration
+    const bool woff = R > 0 && p.gate && !p.gate[cb * p.gate_mt + (rbase + rg * R) / GEMM_BM];
     for (int it = 0; it <= p.max_iter; ++it) {
         int nsp = 0;
         if (local && it == p.max_iter) break;
@@ -1277,9 +1284,6 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
             // A wave whose rows no spike touched keeps last pass's per-thread
             // maxima in LDS (its registers did not change): it skips the scan.
             bool wdirty = it == 0;
-            // a wave whose 64-template tile the gate did not propose for this column
-            // block scores zero everywhere (its stale rows are neither updated nor read)
-            const bool woff = R > 0 && p.gate && !p.gate[cb * p.gate_mt + (rbase + rg * R) / GEMM_BM];
             if (R > 0) {
                 // Runs with the whole wave active (every lane tests spikes for the ballot);
                 // edge columns and columns past W are updated too but never read.
