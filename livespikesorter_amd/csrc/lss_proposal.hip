@@ -490,15 +490,16 @@ __device__ __forceinline__ void pn_fused_layer(const __bf16* in, __bf16* out, co
 }
 
 __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
-    // grid (stream, column tile, group): consecutive workgroups are consecutive streams,
-    // so with n | 8 streams stream j runs on the XCDs x = j mod n, where an interleaved
-    // whitening launch left its whitened batch (in that XCD's L2)
+    // grid (stream, column tile, group slot): consecutive workgroups are consecutive
+    // streams, so with n | 8 streams stream j runs on the XCDs x = j mod n, where an
+    // interleaved whitening launch left its whitened batch (in that XCD's L2).  A
+    // workgroup runs the groups g = slot, slot + gridDim.z, ... in turn; the next
+    // group's input rows and layer 1's weights are loaded into registers while the
+    // current group's last layer runs.
     const PnInferArgs& A = tb.a[blockIdx.x];
-    const int g = blockIdx.z, w0 = blockIdx.y * PN_FT;
-    if (g >= A.G || w0 >= A.W) return;
-    // the stream's active flag is read beside the prologue's loads (every address
-    // below is valid for an idle stream too); an idle stream's workgroups leave
-    // after the prologue's barrier, where the flag has arrived with the rest
+    const int w0 = blockIdx.y * PN_FT, G = A.G, GZ = gridDim.z;
+    int g = blockIdx.z;
+    if (g >= G || w0 >= A.W) return;
     const bool idle = A.active && !*A.active;
     __shared__ __attribute__((aligned(16))) __bf16 bx[PN_FX * PN_FCP];
     __shared__ __attribute__((aligned(16))) __bf16 by[PN_FX * PN_FCP];
@@ -507,60 +508,42 @@ __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
     __shared__ __attribute__((aligned(16))) float s_w4[PN_KW * PN_CH + 1];   // [k][ci], then b4
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int W = A.W, C = A.C;
-    // input: rows ch = 16 g - 4 + ci of the whitened batch from column w0 - 16, as BF16
-    // into bx[column][ci]: a thread takes one column and eight consecutive rows (a wave's
-    // loads of one row are 64 consecutive columns) and stores them as one 16-byte word.
-    // Every global load of the prologue (input, layer 1's weights, the biases, the
-    // head) is issued before the first LDS store, so they take one memory latency.
     static_assert(PN_CIN % 8 == 0 && PN_CH % 8 == 0, "channels come in 16-byte words of eight");
     constexpr int NIN = PN_FX * (PN_CIN / 8);            // items of eight input rows
     constexpr int NW1 = PN_CH * (PN_KT1 / 8);            // 16-byte words of layer 1's weights
     static_assert(NIN <= 1024 && NW1 <= 1024 && PN_KW * PN_CH < 512 && 3 * PN_CH <= 512,
                   "the prologue's items fit two per thread");
+    // input rows ch = 16 g - 4 + ci of the whitened batch from column w0 - 16 (a thread
+    // takes one column and eight consecutive rows; a wave's loads of one row are 64
+    // consecutive columns), and layer 1's weights, into registers
     float xv[2][8];
     uint4 w1v[2];
+    auto in_load = [&](int gg) __attribute__((always_inline)) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int i = tid + 512 * h;
-        const int q = i / PN_FX, c = i - q * PN_FX;
-        const int w = w0 - 16 + c;
+        for (int h = 0; h < 2; ++h) {
+            const int i = tid + 512 * h;
+            const int q = i / PN_FX, c = i - q * PN_FX;
+            const int w = w0 - 16 + c;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int ch = g * PN_GROUP - PN_HALO + 8 * q + e;
-            xv[h][e] = (i < NIN && ch >= 0 && ch < C && w >= 0 && w < W) ? A.xw[(int64_t)ch * A.xs + A.xoff + w] : 0.f;
+            for (int e = 0; e < 8; ++e) {
+                const int ch = gg * PN_GROUP - PN_HALO + 8 * q + e;
+                xv[h][e] = (i < NIN && ch >= 0 && ch < C && w >= 0 && w < W) ? A.xw[(int64_t)ch * A.xs + A.xoff + w]
+                                                                             : 0.f;
+            }
+            w1v[h] = i < NW1 ? reinterpret_cast<const uint4*>(A.wt1)[i] : make_uint4(0u, 0u, 0u, 0u);
         }
-        w1v[h] = i < NW1 ? reinterpret_cast<const uint4*>(A.wt1)[i] : make_uint4(0u, 0u, 0u, 0u);
-    }
-    float bias_v = 0.f, w4_v = 0.f;
+    };
+    in_load(g);
     if (tid < 3 * PN_CH) {
         const int l = tid / PN_CH, co = tid - l * PN_CH;
-        bias_v = A.params[(l == 0 ? PN_B1 : l == 1 ? PN_B2 : PN_B3) + co];
+        s_bias[tid] = A.params[(l == 0 ? PN_B1 : l == 1 ? PN_B2 : PN_B3) + co];
     }
     if (tid < PN_KW * PN_CH) {
         const int k = tid / PN_CH, ci = tid - k * PN_CH;
-        w4_v = A.params[PN_W4 + ci * PN_KW + k];
+        s_w4[tid] = A.params[PN_W4 + ci * PN_KW + k];
     } else if (tid == PN_KW * PN_CH) {
-        w4_v = A.params[PN_B4];
+        s_w4[tid] = A.params[PN_B4];
     }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int i = tid + 512 * h;
-        if (i < NIN) {
-            const int q = i / PN_FX, c = i - q * PN_FX;
-            __bf16 v[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = (__bf16)xv[h][e];
-            uint4 u;
-            __builtin_memcpy(&u, v, 16);
-            *reinterpret_cast<uint4*>(&bx[c * PN_FCP + 8 * q]) = u;
-        }
-        if (i < NW1) {
-            const int r = i / (PN_KT1 / 8), c8 = i - r * (PN_KT1 / 8);
-            *reinterpret_cast<uint4*>(&ws[r * PN_FWP + c8 * 8]) = w1v[h];
-        }
-    }
-    if (tid < 3 * PN_CH) s_bias[tid] = bias_v;
-    if (tid <= PN_KW * PN_CH) s_w4[tid] = w4_v;
     // the next layer's weights, in flight while the current layer runs
     constexpr int NW = PN_CH * PN_KP / 8;           // 16-byte words of a 32 x 288 layer
     constexpr int NWT = (NW + 511) / 512;
@@ -582,41 +565,66 @@ __global__ __launch_bounds__(512) void k_pn_fused(PnInferTable tb) {
             }
         }
     };
-    __syncthreads();
-    if (idle) return;                                 // uniform over the workgroup
-    wload(A.wt2);
-    pn_fused_layer<PN_CIN>(bx, by, ws, s_bias, w0 - 12, W, lane, wv);
-    __syncthreads();
-    wstore();
-    __syncthreads();
-    wload(A.wt3);
-    pn_fused_layer<PN_CH>(by, bx, ws, s_bias + PN_CH, w0 - 8, W, lane, wv);
-    __syncthreads();
-    wstore();
-    __syncthreads();
-    pn_fused_layer<PN_CH>(bx, by, ws, s_bias + 2 * PN_CH, w0 - 4, W, lane, wv);
-    __syncthreads();
-    // head: z = b4 + sum_k sum_ci W4[ci][k] a3[w + k - 4][ci] (layer 3's column jh + k)
-    if (tid < PN_FT) {
-        const int w = w0 + tid;
-        if (w < A.Wp) {
-            float acc = s_w4[PN_KW * PN_CH];
-            if (w < W) {
+    for (;;) {
 #pragma unroll
-                for (int k = 0; k < PN_KW; ++k) {
-                    const __bf16* col = &by[(tid + k) * PN_FCP];
+        for (int h = 0; h < 2; ++h) {
+            const int i = tid + 512 * h;
+            if (i < NIN) {
+                const int q = i / PN_FX, c = i - q * PN_FX;
+                __bf16 v[8];
 #pragma unroll
-                    for (int c8 = 0; c8 < PN_CH / 8; ++c8) {
-                        const bf16x8 v = *reinterpret_cast<const bf16x8*>(col + 8 * c8);
+                for (int e = 0; e < 8; ++e) v[e] = (__bf16)xv[h][e];
+                uint4 u;
+                __builtin_memcpy(&u, v, 16);
+                *reinterpret_cast<uint4*>(&bx[c * PN_FCP + 8 * q]) = u;
+            }
+            if (i < NW1) {
+                const int r = i / (PN_KT1 / 8), c8 = i - r * (PN_KT1 / 8);
+                *reinterpret_cast<uint4*>(&ws[r * PN_FWP + c8 * 8]) = w1v[h];
+            }
+        }
+        __syncthreads();
+        if (idle) return;                             // uniform over the workgroup
+        wload(A.wt2);
+        pn_fused_layer<PN_CIN>(bx, by, ws, s_bias, w0 - 12, W, lane, wv);
+        __syncthreads();
+        wstore();
+        __syncthreads();
+        wload(A.wt3);
+        pn_fused_layer<PN_CH>(by, bx, ws, s_bias + PN_CH, w0 - 8, W, lane, wv);
+        __syncthreads();
+        wstore();
+        __syncthreads();
+        const int gn = g + GZ;
+        if (gn < G) in_load(gn);                      // the next group's input, during layer 3
+        pn_fused_layer<PN_CH>(bx, by, ws, s_bias + 2 * PN_CH, w0 - 4, W, lane, wv);
+        __syncthreads();
+        // head: z = b4 + sum_k sum_ci W4[ci][k] a3[w + k - 4][ci] (layer 3's column jh + k)
+        if (tid < PN_FT) {
+            const int w = w0 + tid;
+            if (w < A.Wp) {
+                float acc = s_w4[PN_KW * PN_CH];
+                if (w < W) {
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) acc += s_w4[k * PN_CH + 8 * c8 + e] * (float)v[e];
+                    for (int k = 0; k < PN_KW; ++k) {
+                        const __bf16* col = &by[(tid + k) * PN_FCP];
+#pragma unroll
+                        for (int c8 = 0; c8 < PN_CH / 8; ++c8) {
+                            const bf16x8 v = *reinterpret_cast<const bf16x8*>(col + 8 * c8);
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) acc += s_w4[k * PN_CH + 8 * c8 + e] * (float)v[e];
+                        }
                     }
                 }
+                const int64_t o = (int64_t)g * A.Wp + w;
+                A.z[o] = w < W ? acc : 0.f;
+                A.p[o] = w < W ? 1.f / (1.f + expf(-acc)) : 0.f;
             }
-            const int64_t o = (int64_t)g * A.Wp + w;
-            A.z[o] = w < W ? acc : 0.f;
-            A.p[o] = w < W ? 1.f / (1.f + expf(-acc)) : 0.f;
         }
+        g = gn;
+        if (g >= G) break;
+        // the head reads by; the next group writes bx and ws, then the barrier above
+        // its layer 1 (which writes by) orders it after every head read
     }
 }
 
@@ -719,7 +727,9 @@ void pn_infer_batch(const PnFwdJob* jobs, int n, hipStream_t s) {
         Gm = max(Gm, J.b.G);
         Wm = max(Wm, J.W);
     }
-    hipLaunchKernelGGL(k_pn_fused, dim3(n, (Wm + PN_FT - 1) / PN_FT, Gm), dim3(512), 0, s, t);
+    // group slots: one round of two workgroups per CU for the 8-stream, 24-group shape
+    const int gz = std::min(Gm, 8);
+    hipLaunchKernelGGL(k_pn_fused, dim3(n, (Wm + PN_FT - 1) / PN_FT, gz), dim3(512), 0, s, t);
 }
 
 void pn_infer(const PnWeights& wt, const PnBuffers& b, const float* xw, int64_t xs, int xoff, int C, int W,
