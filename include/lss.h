@@ -316,6 +316,16 @@ int lss_group_stage_times(struct lss_group* g, int32_t lane, double* ms, int64_t
 /* lss_set_proposal for one stream of a group (the launch graph is rebuilt). */
 int lss_group_set_proposal(struct lss_group* g, int32_t stream, struct lss_proposal* net, float lo, float hi,
                            int32_t reach);
+/* Pipelined lanes (on != 0): lanes 1..3 get per-batch buffers of their own, so
+ * consecutive batches of the same streams may be in flight on two lanes at once
+ * (driven from two host threads): the next batch's stages in front of the peel
+ * run on the CUs the current peel leaves free, and every lane's peel waits for
+ * the peel enqueued before it, so one peel grid is co-resident at a time.
+ * Batches must be device-resident.  Results are those of one lane taking the
+ * batches in turn; the handles' host state (event counts, the last batch) is
+ * updated as each call returns.  No reference counterpart: for callers with a
+ * backlog of ticks (replays, offline sorting of several probes). */
+int lss_group_set_pipelined(struct lss_group* g, int32_t on);
 int lss_group_destroy(struct lss_group* g);
 
 /* ---- spike-proposal network (a learned gate; no reference counterpart) ----

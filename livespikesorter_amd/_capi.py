@@ -40,7 +40,8 @@ ABI_FUNCTIONS = ("lss_config_init", "lss_create", "lss_process_batch", "lss_proc
                  "lss_group_query", "lss_group_destroy", "lss_group_stage_times", "lss_group_debug_stage", "lss_proposal_n_params", "lss_proposal_create",
                  "lss_proposal_set_params", "lss_proposal_get_params", "lss_proposal_forward",
                  "lss_proposal_grad", "lss_proposal_train_step", "lss_proposal_destroy", "lss_set_proposal",
-                 "lss_autotune", "lss_group_set_proposal", "lss_group_process_lane", "lss_last_positions",
+                 "lss_autotune", "lss_group_set_proposal", "lss_group_set_pipelined", "lss_group_process_lane",
+                 "lss_last_positions",
                  "lss_create_subset", "lss_window_plan", "lss_save_spikes", "lss_format_spikes",
                  "lss_payload_encode", "lss_payload_decode", "lss_sorter_params_encode",
                  "lss_sorter_params_decode", "lss_replay_open", "lss_replay_samples", "lss_replay_fetch",
@@ -172,6 +173,7 @@ def lib() -> ct.CDLL:
         L.lss_group_set_proposal.argtypes = [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_float, ct.c_float,
                                              ct.c_int32]
         L.lss_set_proposal.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_float, ct.c_int32]
+        L.lss_group_set_pipelined.argtypes = [ct.c_void_p, ct.c_int32]
         P64 = ct.POINTER(ct.c_int64)
         L.lss_window_plan.argtypes = [ct.POINTER(WindowCfg), ct.c_int64, ct.c_int64, ct.POINTER(WindowStep)]
         L.lss_save_spikes.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int64,

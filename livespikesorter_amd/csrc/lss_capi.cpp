@@ -890,6 +890,7 @@ struct BatchBufs {
     float *pmean = nullptr, *pmed = nullptr, *X1 = nullptr, *Xp = nullptr;
     float *dbgB = nullptr, *dbgCf = nullptr;
     int* dbgIm = nullptr;
+    uint8_t* gate = nullptr;           // proposal gate: the score tiles to compute
 };
 
 BatchBufs handle_bufs(const lss_handle* h) {
@@ -904,6 +905,60 @@ BatchBufs handle_bufs(const lss_handle* h) {
     b.psum = h->d_psum; b.pmin = h->d_pmin; b.pmax = h->d_pmax;
     b.pmean = h->d_pmean; b.pmed = h->d_pmed; b.X1 = h->d_X1; b.Xp = h->d_Xp;
     if (h->cfg.keep_stages) { b.dbgB = h->d_dbgB; b.dbgCf = h->d_dbgCf; b.dbgIm = h->d_dbgIm; }
+    b.gate = h->d_gate;
+    return b;
+}
+
+// The handle's per-batch device buffers laid out from base: the tag epoch at base,
+// the tagged column maxima after it (tag_bytes in all), then the rest; *need = bytes
+// used.  base may be null to size the layout.
+BatchBufs layout_bufs(const lss_handle* h, unsigned char* base, size_t tag_bytes, size_t* need) {
+    BatchBufs b;
+    b.epoch = reinterpret_cast<unsigned*>(base);
+    b.ptag = reinterpret_cast<unsigned long long*>(base + 256);
+    b.ptag_words = (int64_t)((tag_bytes - 256) / sizeof(unsigned long long));
+    size_t off = tag_bytes;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) / 256 * 256;
+        unsigned char* p = base + off;
+        off += bytes;
+        return p;
+    };
+    const size_t CP = h->CP, TP = h->TP, KCP = h->KCP, BS = h->BS, XS = h->XS, Wmax = h->Wmax;
+    b.Xw = reinterpret_cast<float*>(take(CP * XS * 4));
+    b.Bpc = reinterpret_cast<float*>(take(KCP * BS * 4));
+    b.B = reinterpret_cast<float*>(take(TP * BS * 4));
+    if (h->bf16) {
+        b.Bpcb = reinterpret_cast<uint16_t*>(take(KCP * BS * 2));
+        b.Binit = reinterpret_cast<float*>(take(TP * BS * 4));
+        b.Nsq = reinterpret_cast<float*>(take(TP / GEMM_BM * BS * 4));
+        b.act = reinterpret_cast<uint8_t*>(take(BS / GEMM_BN * (TP / GEMM_BM)));
+    }
+    b.Cfmax = reinterpret_cast<float*>(take(Wmax * 4));
+    b.slots = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->slot_cap * sizeof(Spike)));
+    b.list = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->cap_g * sizeof(Spike)));
+    b.slot_cnt = reinterpret_cast<int*>(take((size_t)h->max_groups * 4));
+    b.list_cnt = reinterpret_cast<int*>(take((size_t)h->max_groups * 4));
+    b.iter_cnt = reinterpret_cast<int*>(take((size_t)h->sync_words * 4));
+    b.err = reinterpret_cast<int*>(take(4));
+    b.bar = reinterpret_cast<unsigned*>(take((size_t)BAR_WORDS * 4));
+    b.prof = reinterpret_cast<unsigned long long*>(take((size_t)PROF_WORDS * 8));
+    const size_t fw = (size_t)h->nC * h->K;
+    b.hdr = reinterpret_cast<OutHeader*>(take(sizeof(OutHeader)));
+    b.ev = reinterpret_cast<lss_event*>(take((size_t)h->ev_cap * sizeof(lss_event)));
+    b.feat = reinterpret_cast<float*>(take((size_t)h->ev_cap * fw * 4));
+    if (h->d_xc) b.pos = reinterpret_cast<float*>(take((size_t)h->ev_cap * 2 * 4));
+    if (h->pre) {
+        const size_t nslab = (Wmax + PRE_SLAB - 1) / PRE_SLAB, C = h->C, PS = h->PS;
+        b.psum = reinterpret_cast<int*>(take(nslab * C * 4));
+        b.pmin = reinterpret_cast<int*>(take(nslab * C * 4));
+        b.pmax = reinterpret_cast<int*>(take(nslab * C * 4));
+        b.pmean = reinterpret_cast<float*>(take(C * 4));
+        b.pmed = reinterpret_cast<float*>(take(Wmax * 4));
+        b.X1 = reinterpret_cast<float*>(take(C * PS * 4));
+        if (h->pre & PRE_HP) b.Xp = reinterpret_cast<float*>(take(C * PS * 4));
+    }
+    *need = off;
     return b;
 }
 
@@ -979,7 +1034,7 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     pj = ProjectJob{};
     pj.A = h->d_U2; pj.Bm = bb.Bpc; pj.kb_ptr = h->d_kbp_ptr; pj.kb_idx = h->d_kbp_idx; pj.C = bb.B;
     pj.lda = h->KCP; pj.ldb = h->BS; pj.ldc = h->BS;
-    if (h->prop) { pj.gate = h->d_gate; pj.gate_mt = h->TP / GEMM_BM; }
+    if (h->prop) { pj.gate = bb.gate; pj.gate_mt = h->TP / GEMM_BM; }
     pj.active = xptr;
     if (h->x3) {
         pj.Ab = reinterpret_cast<const __bf16*>(h->d_U2b);
@@ -1033,7 +1088,7 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     p.dbg_cfmax = bb.dbgCf;
     p.dbg_imax = bb.dbgIm;
     p.prof = bb.prof;
-    if (h->prop) { p.gate = h->d_gate; p.gate_mt = h->TP / GEMM_BM; }
+    if (h->prop) { p.gate = bb.gate; p.gate_mt = h->TP / GEMM_BM; }
     p.active = xptr;
     if (h->bf16) {
         // candidates within 10% of threshold on the score scale are decided in FP32
@@ -1219,14 +1274,15 @@ void run_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, int W, in
 // the positions of the last call) changes only when they all fit; otherwise
 // the buffer holds the first `cap` of them, the call fails with
 // LSS_ERR_CAPACITY and can be repeated with a larger buffer.
-void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample, bool dedup,
-          int64_t* n_out) {
+void emit_from(lss_handle* h, const OutHeader& hd, const lss_event* src_ev, const float* src_feat,
+               const float* src_pos, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample,
+               bool dedup, int64_t* n_out) {
     *n_out = 0;
-    if (h->hdr->count > h->ev_cap)
-        throw LssError(LSS_ERR_LIMIT, "the batch has " + std::to_string(h->hdr->count) +
+    if (hd.count > h->ev_cap)
+        throw LssError(LSS_ERR_LIMIT, "the batch has " + std::to_string(hd.count) +
                                           " events, more than the handle's max_events " + std::to_string(h->ev_cap));
-    const int64_t n = h->hdr->count;
-    if (h->pre) h->info.p2p = h->hdr->p2p;
+    const int64_t n = hd.count;
+    if (h->pre) h->info.p2p = hd.p2p;
     const int64_t fw = (int64_t)h->nC * h->K;
     // pass 1: which events are reported (dedup against a scratch copy of the last times)
     std::vector<int64_t> pick;
@@ -1235,10 +1291,10 @@ void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sam
     {
         std::map<int, int64_t> last;
         for (int64_t i = 0; i < n; ++i) {
-            const int64_t s = h->ev[i].sample + h->cur_first;   // the kernel writes samples relative to column 0
+            const int64_t s = src_ev[i].sample + h->cur_first;   // the kernel writes samples relative to column 0
             if (s < lo_sample || s >= hi_sample) continue;
             if (dedup) {
-                const int t = h->ev[i].template_id;
+                const int t = src_ev[i].template_id;
                 auto it = last.find(t);
                 const int64_t prev = it != last.end() ? it->second : h->last_spike[t];
                 if (s - prev < h->dt) continue;
@@ -1256,23 +1312,28 @@ void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sam
     for (int64_t k = 0; k < out; ++k) {
         const int64_t i = pick[k];
         if (ev) {
-            ev[k] = h->ev[i];
+            ev[k] = src_ev[i];
             ev[k].sample += h->cur_first;
         }
-        if (feat && total != n) std::memcpy(feat + k * fw, h->feat + i * fw, sizeof(float) * fw);
+        if (feat && total != n) std::memcpy(feat + k * fw, src_feat + i * fw, sizeof(float) * fw);
     }
-    if (feat && total == n && out > 0) std::memcpy(feat, h->feat, sizeof(float) * fw * out);
+    if (feat && total == n && out > 0) std::memcpy(feat, src_feat, sizeof(float) * fw * out);
     if (total > cap)
         throw LssError(LSS_ERR_CAPACITY, "event buffer holds " + std::to_string(cap) + " of " +
                                              std::to_string(total) + " events");
     for (const auto& tl : touched) h->last_spike[tl.first] = tl.second;
     h->last_pos.clear();
-    if (h->pos)
+    if (src_pos)
         for (int64_t k = 0; k < total; ++k) {
-            h->last_pos.push_back(h->pos[2 * pick[k]]);
-            h->last_pos.push_back(h->pos[2 * pick[k] + 1]);
+            h->last_pos.push_back(src_pos[2 * pick[k]]);
+            h->last_pos.push_back(src_pos[2 * pick[k] + 1]);
         }
     h->info.events += total;
+}
+
+void emit(lss_handle* h, lss_event* ev, int64_t cap, float* feat, int64_t lo_sample, int64_t hi_sample, bool dedup,
+          int64_t* n_out) {
+    emit_from(h, *h->hdr, h->ev, h->feat, h->pos, ev, cap, feat, lo_sample, hi_sample, dedup, n_out);
 }
 
 int16_t* ensure_dx(lss_handle* h, size_t elems) {
@@ -1697,9 +1758,80 @@ const char* lss_last_error(void) { return g_last_error.c_str(); }
 // group's register tile height so one kernel instance serves all template counts.
 // ---------------------------------------------------------------------------
 // A launch lane: its own HIP stream, captured graph and input-pointer table.
+// A lane's own per-batch buffers for one stream, so that a pipelined group can
+// have a stream in flight on two lanes at once: the device buffers laid out as
+// the handle's, the events and features in mapped host memory the peel writes
+// into, the proposal network's outputs and the gate's tile mask.
+struct LaneBufs {
+    unsigned char* dev = nullptr;
+    void* host = nullptr;
+    BatchBufs b;
+    OutHeader* hdr = nullptr;            // host views of the outputs
+    lss_event* ev = nullptr;
+    float* feat = nullptr;
+    float* pos = nullptr;
+    PnBuffers pnb{};                     // the handle's, with this lane's z and p
+    float* pz = nullptr;
+    float* pp = nullptr;
+    void release() {
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
+        if (pz) (void)hipFree(pz);
+        if (pp) (void)hipFree(pp);
+        if (b.gate) (void)hipFree(b.gate);
+        dev = nullptr; host = nullptr; pz = pp = nullptr; b.gate = nullptr;
+    }
+};
+
+// Allocate a lane's buffers for a handle (all zero: no tag of any epoch).
+static void alloc_lane_bufs(const lss_handle* h, LaneBufs& lb) {
+    const size_t tag_bytes = 256 + ((size_t)h->ptag_words * sizeof(unsigned long long) + 255) / 256 * 256;
+    size_t need = 0;
+    (void)layout_bufs(h, nullptr, tag_bytes, &need);
+    HIPCHK(hipMalloc(reinterpret_cast<void**>(&lb.dev), need));
+    HIPCHK(hipMemset(lb.dev, 0, need));
+    lb.b = layout_bufs(h, lb.dev, tag_bytes, &need);
+    // the outputs: mapped, coherent host memory, as the handle's
+    const size_t fw = (size_t)h->nC * h->K;
+    const size_t out_bytes = sizeof(OutHeader) + sizeof(lss_event) * (size_t)h->ev_cap +
+                             sizeof(float) * (size_t)h->ev_cap * fw + sizeof(float) * 2 * (size_t)h->ev_cap;
+    HIPCHK(hipHostMalloc(&lb.host, out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(lb.host, 0, out_bytes);
+    void* dout = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&dout, lb.host, 0));
+    lb.hdr = static_cast<OutHeader*>(lb.host);
+    lb.ev = reinterpret_cast<lss_event*>(lb.hdr + 1);
+    lb.feat = reinterpret_cast<float*>(lb.ev + h->ev_cap);
+    lb.b.hdr = static_cast<OutHeader*>(dout);
+    lb.b.ev = reinterpret_cast<lss_event*>(lb.b.hdr + 1);
+    lb.b.feat = reinterpret_cast<float*>(lb.b.ev + h->ev_cap);
+    lb.b.pos = nullptr;
+    if (h->d_xc) {
+        lb.pos = lb.feat + (size_t)h->ev_cap * fw;
+        lb.b.pos = lb.b.feat + (size_t)h->ev_cap * fw;
+    }
+    if (h->cfg.keep_stages) { lb.b.dbgB = h->d_dbgB; lb.b.dbgCf = h->d_dbgCf; lb.b.dbgIm = h->d_dbgIm; }
+    if (h->prop) {
+        lb.pnb = h->pnb;
+        const size_t zp = (size_t)h->pnb.G * h->pnb.Wp;
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&lb.pz), zp * sizeof(float)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&lb.pp), zp * sizeof(float)));
+        lb.pnb.z = lb.pz;
+        lb.pnb.p = lb.pp;
+        const size_t nm = (size_t)((h->Wmax + 63) / 64) * (h->TP / GEMM_BM);
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&lb.b.gate), nm));
+        HIPCHK(hipMemset(lb.b.gate, 0, nm));
+    }
+}
+
 // Lanes run concurrently on disjoint sets of streams (a stream idle in a
-// lane's launch is untouched by it).
+// lane's launch is untouched by it).  In a pipelined group lanes 1.. have
+// their own per-batch buffers, so consecutive batches of the same streams can
+// be in flight on two lanes: the next batch's front stages run beside the
+// current peel, and the peels run one after another.
 struct GroupLane {
+    int id = 0;
+    std::vector<LaneBufs> own;           // pipelined groups, lanes 1..: per stream
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1738,6 +1870,11 @@ struct lss_group {
     std::vector<std::unique_ptr<lss_handle>> hs;
     int n = 0, W = 0, R = 0, CW = 0, dev = 0;
     bool profile = false;
+    // pipelined: every lane's peel waits for the peel enqueued before it (one peel grid
+    // co-resident at a time); the order of enqueueing is taken under `seq`
+    bool pipelined = false;
+    hipEvent_t peel_ev = nullptr;
+    std::mutex seq;
     std::vector<int> S;   // per stream: template subsets per column block in the peel grid
     GroupLane lanes[GROUP_LANES];
     // Lanes run on several host threads.  Allocations, synchronous copies and graph
@@ -1755,7 +1892,9 @@ struct lss_group {
                 if (b) (void)hipHostFree(b);
             if (L.d_xptr) (void)hipFree(L.d_xptr);
             if (L.stream) (void)hipStreamDestroy(L.stream);
+            for (auto& lb : L.own) lb.release();
         }
+        if (peel_ev) (void)hipEventDestroy(peel_ev);
     }
 };
 
@@ -1775,10 +1914,17 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     L.wb = WhitenBatch{}; L.cb = PcconvBatch{}; L.pb = ProjectBatch{}; L.kb = PeelBatch{}; L.rb = ResetBatch{};
     L.wb.n = L.cb.n = L.pb.n = L.kb.n = L.rb.n = n;
     PreBatch pre{};
+    // lanes 1.. of a pipelined group work in buffers of their own
+    const bool mine = g->pipelined && L.id > 0;
+    if (mine && (int)L.own.size() < n) {
+        L.own.resize(n);
+        for (int s2 = 0; s2 < n; ++s2) alloc_lane_bufs(g->hs[s2].get(), L.own[s2]);
+    }
     for (int s2 = 0; s2 < n; ++s2) {
         lss_handle* h = g->hs[s2].get();
+        const BatchBufs bb = mine ? L.own[s2].b : handle_bufs(h);
         BatchPlan bp;
-        plan_batch(h, handle_bufs(h), nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, g->S[s2], bp);
+        plan_batch(h, bb, nullptr, &L.d_xptr[s2], g->W, ld[s2], g->CW, g->R, g->S[s2], bp);
         if (bp.has_pre) pre.job[pre.n++] = bp.pre;
         L.wb.job[s2] = bp.wj;
         L.wb.start[s2 + 1] = L.wb.start[s2] + gemm_blocks(bp.wj.mtiles, bp.wj.ntiles);
@@ -1788,9 +1934,9 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         L.pb.start[s2 + 1] = L.pb.start[s2] + gemm_blocks(bp.pj.mtiles, bp.pj.ntiles);
         L.kb.p[s2] = bp.pp;
         L.kb.g_start[s2 + 1] = L.kb.g_start[s2] + bp.G;
-        L.rb.bar[s2] = h->d_bar; L.rb.iter_cnt[s2] = h->d_iter_cnt; L.rb.n_iter[s2] = h->sync_words;
-        L.rb.err[s2] = h->d_err; L.rb.prof[s2] = h->d_prof; L.rb.active[s2] = &L.d_xptr[s2];
-        L.rb.epoch[s2] = h->d_epoch; L.rb.ptag[s2] = h->d_ptag; L.rb.ptag_words[s2] = h->ptag_words;
+        L.rb.bar[s2] = bb.bar; L.rb.iter_cnt[s2] = bb.iter_cnt; L.rb.n_iter[s2] = h->sync_words;
+        L.rb.err[s2] = bb.err; L.rb.prof[s2] = bb.prof; L.rb.active[s2] = &L.d_xptr[s2];
+        L.rb.epoch[s2] = bb.epoch; L.rb.ptag[s2] = bb.ptag; L.rb.ptag_words[s2] = h->ptag_words;
     }
     // GEMM launches of 2, 4 or 8 streams: each stream's tiles on its own share of the
     // XCDs, so an XCD's L2 keeps the template and whitening operands of 8 / n streams
@@ -1820,8 +1966,11 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         for (int s2 = 0; s2 < n; ++s2) {
             lss_handle* h = g->hs[s2].get();
             if (!h->prop) continue;
-            fj[ng] = PnFwdJob{h->d_Xw, h->XS, XOFF, h->C, g->W, h->prop->wt, h->pnb, &L.d_xptr[s2]};
-            gj[ng] = PnGateJob{h->pnb, h->gate, g->W, &L.d_xptr[s2]};
+            const PnBuffers& pnb = mine ? L.own[s2].pnb : h->pnb;
+            PnGate gt = h->gate;
+            if (mine) gt.mask = L.own[s2].b.gate;
+            fj[ng] = PnFwdJob{mine ? L.own[s2].b.Xw : h->d_Xw, h->XS, XOFF, h->C, g->W, h->prop->wt, pnb, &L.d_xptr[s2]};
+            gj[ng] = PnGateJob{pnb, gt, g->W, &L.d_xptr[s2]};
             ++ng;
         }
         pn_infer_batch(fj, ng, st);
@@ -1829,12 +1978,14 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
         launch_pcconv(L.cb, g->hs[0]->K, st);
     }
     launch_project(L.pb, g->hs[0]->cfg.precision, st);
-    if (g->profile) {
+    // profiled and pipelined groups: the front stages and the peel are two graphs
+    const bool split = g->profile || g->pipelined;
+    if (split) {
         HIPCHK(hipStreamEndCapture(st, &L.graph));
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     }
     HIPCHK(launch_peel(L.kb, g->R, false, st));
-    if (g->profile) {
+    if (split) {
         HIPCHK(hipStreamEndCapture(st, &L.graph2));
         HIPCHK(hipGraphInstantiate(&L.exec2, L.graph2, nullptr, nullptr, 0));
     } else {
@@ -1913,6 +2064,7 @@ int lss_group_create(int32_t n_streams, const char* const* oss_input_dirs, const
                                             " workgroups) is not co-resident on this device");
         HIPCHK(hipSetDevice(g->dev));
         g->profile = cfg->profile != 0;
+        for (int li = 0; li < GROUP_LANES; ++li) g->lanes[li].id = li;
         for (auto& L : g->lanes) {
             if (g->profile)
                 for (auto& e : L.ev) HIPCHK(hipEventCreate(&e));
@@ -1937,6 +2089,8 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         if (!g || !samples || !ld || !first_sample || !ev || !ev_cap || !n_ev)
             throw LssError(LSS_ERR_ARG, "null argument");
         if (lane < 0 || lane >= GROUP_LANES) throw LssError(LSS_ERR_ARG, "no such lane");
+        if (g->pipelined && !device_input)
+            throw LssError(LSS_ERR_ARG, "a pipelined group takes device-resident batches");
         GroupLane& L = g->lanes[lane];
         HIPCHK(hipSetDevice(g->dev));
         auto t0 = std::chrono::steady_clock::now();
@@ -1990,7 +2144,18 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         // this launch's input pointers (null: idle stream), read by every kernel of the
         // graph; copied ahead of it on the lane's stream rather than as a graph node
         HIPCHK(hipMemcpyAsync(L.d_xptr, L.h_xptr, sizeof(void*) * n, hipMemcpyHostToDevice, L.stream));
-        if (g->profile) {
+        if (g->pipelined) {
+            // the peel waits for the peel enqueued before it on any lane, so one peel grid
+            // is co-resident at a time; the front stages run beside the other lane's peel
+            std::lock_guard<std::mutex> lk(g->seq);
+            if (g->profile) HIPCHK(hipEventRecord(L.ev[0], L.stream));
+            HIPCHK(hipGraphLaunch(L.exec, L.stream));
+            if (g->profile) HIPCHK(hipEventRecord(L.ev[1], L.stream));
+            HIPCHK(hipStreamWaitEvent(L.stream, g->peel_ev, 0));
+            HIPCHK(hipGraphLaunch(L.exec2, L.stream));
+            HIPCHK(hipEventRecord(g->peel_ev, L.stream));
+            if (g->profile) HIPCHK(hipEventRecord(L.ev[2], L.stream));
+        } else if (g->profile) {
             HIPCHK(hipEventRecord(L.ev[0], L.stream));
             HIPCHK(hipGraphLaunch(L.exec, L.stream));
             HIPCHK(hipEventRecord(L.ev[1], L.stream));
@@ -2014,14 +2179,26 @@ int lss_group_process_lane(lss_group* g, int32_t lane, const int16_t* const* sam
         // streams (their n_ev as emit left them), a timeout taking precedence.
         std::string failed;
         int code = LSS_OK;
+        // pipelined lanes may emit for the same handles at once: the handles' host state
+        // (event counts, last batch) is updated under the group lock
+        std::unique_lock<std::mutex> lk(g->mu, std::defer_lock);
+        if (g->pipelined) lk.lock();
+        const bool mine = g->pipelined && L.id > 0;
         for (int s2 = 0; s2 < n; ++s2) {
             if (!samples[s2]) continue;
             lss_handle* h = g->hs[s2].get();
             h->cur_first = first_sample[s2];
             h->info.last_ms = ms;
             try {
-                check_peel(h, *h->hdr, g->W);
-                emit(h, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr, INT64_MIN, INT64_MAX, false, &n_ev[s2]);
+                if (mine) {
+                    const LaneBufs& lb = L.own[s2];
+                    check_peel(h, *lb.hdr, g->W);
+                    emit_from(h, *lb.hdr, lb.ev, lb.feat, lb.pos, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr,
+                              INT64_MIN, INT64_MAX, false, &n_ev[s2]);
+                } else {
+                    check_peel(h, *h->hdr, g->W);
+                    emit(h, ev[s2], ev_cap[s2], feat ? feat[s2] : nullptr, INT64_MIN, INT64_MAX, false, &n_ev[s2]);
+                }
             } catch (const LssError& e) {
                 failed += " stream " + std::to_string(s2) + ": " + e.what() + ";";
                 if (code != LSS_ERR_TIMEOUT) code = e.code;
@@ -2072,6 +2249,18 @@ int lss_group_set_proposal(lss_group* g, int32_t stream, lss_proposal* net, floa
         const int rc = lss_set_proposal(g->hs[stream].get(), net, lo, hi, reach);
         if (rc != LSS_OK) throw LssError(rc, g_last_error);
         for (auto& L : g->lanes) L.drop_graph();
+    });
+}
+
+int lss_group_set_pipelined(lss_group* g, int32_t on) {
+    return guarded([&] {
+        if (!g) throw LssError(LSS_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(g->dev));
+        std::lock_guard<std::mutex> lk(g->mu);
+        for (auto& L : g->lanes) HIPCHK(hipStreamSynchronize(L.stream));
+        if (on && !g->peel_ev) HIPCHK(hipEventCreateWithFlags(&g->peel_ev, hipEventDisableTiming));
+        g->pipelined = on != 0;
+        for (auto& L : g->lanes) L.drop_graph();    // recaptured on the next launch (split graphs, own buffers)
     });
 }
 
@@ -2331,56 +2520,11 @@ unsigned char* static_scratch(int dev) {
 
 // The handle's per-batch buffers laid out in the static array; *need = bytes used.
 BatchBufs static_bufs(const lss_handle* h, unsigned char* base, size_t* need) {
-    BatchBufs b;
-    b.epoch = reinterpret_cast<unsigned*>(base);
-    b.ptag = reinterpret_cast<unsigned long long*>(base + 256);
-    b.ptag_words = (int64_t)((STATIC_TAG_BYTES - 256) / sizeof(unsigned long long));
-    if (h->ptag_words > b.ptag_words)
+    if (h->ptag_words > (int64_t)((STATIC_TAG_BYTES - 256) / sizeof(unsigned long long)))
         throw LssError(LSS_ERR_ARG, "the handle's peel exchange needs " + std::to_string(h->ptag_words * 8) +
-                                        " bytes, more than the static array's " + std::to_string(b.ptag_words * 8));
-    size_t off = STATIC_TAG_BYTES;
-    auto take = [&](size_t bytes) {
-        off = (off + 255) / 256 * 256;
-        unsigned char* p = base + off;
-        off += bytes;
-        return p;
-    };
-    const size_t CP = h->CP, TP = h->TP, KCP = h->KCP, BS = h->BS, XS = h->XS, Wmax = h->Wmax;
-    b.Xw = reinterpret_cast<float*>(take(CP * XS * 4));
-    b.Bpc = reinterpret_cast<float*>(take(KCP * BS * 4));
-    b.B = reinterpret_cast<float*>(take(TP * BS * 4));
-    if (h->bf16) {
-        b.Bpcb = reinterpret_cast<uint16_t*>(take(KCP * BS * 2));
-        b.Binit = reinterpret_cast<float*>(take(TP * BS * 4));
-        b.Nsq = reinterpret_cast<float*>(take(TP / GEMM_BM * BS * 4));
-        b.act = reinterpret_cast<uint8_t*>(take(BS / GEMM_BN * (TP / GEMM_BM)));
-    }
-    b.Cfmax = reinterpret_cast<float*>(take(Wmax * 4));
-    b.slots = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->slot_cap * sizeof(Spike)));
-    b.list = reinterpret_cast<Spike*>(take((size_t)h->max_groups * h->cap_g * sizeof(Spike)));
-    b.slot_cnt = reinterpret_cast<int*>(take((size_t)h->max_groups * 4));
-    b.list_cnt = reinterpret_cast<int*>(take((size_t)h->max_groups * 4));
-    b.iter_cnt = reinterpret_cast<int*>(take((size_t)h->sync_words * 4));
-    b.err = reinterpret_cast<int*>(take(4));
-    b.bar = reinterpret_cast<unsigned*>(take((size_t)BAR_WORDS * 4));
-    b.prof = reinterpret_cast<unsigned long long*>(take((size_t)PROF_WORDS * 8));
-    const size_t fw = (size_t)h->nC * h->K;
-    b.hdr = reinterpret_cast<OutHeader*>(take(sizeof(OutHeader)));
-    b.ev = reinterpret_cast<lss_event*>(take((size_t)h->ev_cap * sizeof(lss_event)));
-    b.feat = reinterpret_cast<float*>(take((size_t)h->ev_cap * fw * 4));
-    if (h->d_xc) b.pos = reinterpret_cast<float*>(take((size_t)h->ev_cap * 2 * 4));
-    if (h->pre) {
-        const size_t nslab = (Wmax + PRE_SLAB - 1) / PRE_SLAB, C = h->C, PS = h->PS;
-        b.psum = reinterpret_cast<int*>(take(nslab * C * 4));
-        b.pmin = reinterpret_cast<int*>(take(nslab * C * 4));
-        b.pmax = reinterpret_cast<int*>(take(nslab * C * 4));
-        b.pmean = reinterpret_cast<float*>(take(C * 4));
-        b.pmed = reinterpret_cast<float*>(take(Wmax * 4));
-        b.X1 = reinterpret_cast<float*>(take(C * PS * 4));
-        if (h->pre & PRE_HP) b.Xp = reinterpret_cast<float*>(take(C * PS * 4));
-    }
-    *need = off;
-    return b;
+                                        " bytes, more than the static array's " +
+                                        std::to_string(STATIC_TAG_BYTES - 256));
+    return layout_bufs(h, base, STATIC_TAG_BYTES, need);
 }
 
 }  // namespace

@@ -261,6 +261,12 @@ class StreamGroup:
         self._props = getattr(self, "_props", {})
         self._props[stream] = net
 
+    def set_pipelined(self, on: bool = True) -> None:
+        """Pipelined lanes (lss_group_set_pipelined): consecutive batches of the same
+        streams may be in flight on two lanes at once, from two host threads; the
+        next batch's front stages run beside the current peel.  Device input only."""
+        check(lib().lss_group_set_pipelined(self._g, int(on)))
+
     def process(self, batches, first_samples, *, device: bool = False, lds=None, cap: int = 65536, lane: int = 0,
                 copy: bool = True):
         """``batches``: per stream an int16 array ``[batch][ld]`` (or, with
