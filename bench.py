@@ -5,12 +5,15 @@ Default workload (one GPU): 8 probe streams of 384 channels x 1024 templates
 (Gaussian footprints, rates log-uniform 0.5-20 Hz, amplitudes N(12, 3),
 banded-correlation whitening), 1500-sample chunks framed by 61-sample
 margins (W = 1622 columns).  One step = one tick: a chunk of every stream
-matched by one graph launch (int16 -> whitening -> proposal network and gate
--> PC conv -> BF16 projection -> peel loop with FP32 rescoring -> events + PC
+matched by one graph launch (int16 -> split-BF16 whitening -> proposal network
+and gate -> split-BF16 PC conv and projection -> peel loop -> events + PC
 features on the host).  The recordings are held in HBM before the timed
-region.  Beside the gated BF16 group (``value``) the same ticks run through
-the plain FP32 group, the BF16 group without the gate and the plain FP32
-matcher one stream at a time, for agreement and speedups.
+region.  Beside the gated split-BF16 group (``value``, one tick at a time) the
+same ticks run through the plain FP32 group, BF16 with FP32 rescoring (gated
+and not), split-BF16 without the gate and the plain FP32 matcher one stream at
+a time, for agreement and speedups; each group mode is also timed with two
+ticks in flight on two pipelined lanes (``pipelined``: the next tick's front
+stages run beside the current peel).
 
 Prints one JSON line: aggregate realtime factor (seconds of data per wall
 second), p50/p99 chunk latency from an int16 hand-off on the host to events
@@ -27,6 +30,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -428,6 +432,27 @@ def bench_target(args) -> dict:
                         copy=False)
             wall.append((time.perf_counter() - t1) * 1e3)
         el = time.perf_counter() - ts
+        # pipelined: the same ticks on two lanes from two host threads (consecutive ticks in
+        # flight at once: the next tick's front stages beside the current peel)
+        el_pipe = None
+        if args.pipeline:
+            grp.set_pipelined(True)
+
+            def lane_run(lane, lo, hi):
+                for jj in range(lo + lane, hi, 2):
+                    grp.process([ptr(s, jj) for s in range(S)], [(jj % n_ticks) * N] * S, device=True,
+                                lds=[C] * S, lane=lane, copy=False)
+
+            for lo_, hi_ in ((0, args.warmup), (0, steps)):
+                synchronize()
+                tp = time.perf_counter()
+                th = [threading.Thread(target=lane_run, args=(lane, lo_, hi_)) for lane in (0, 1)]
+                for t_ in th:
+                    t_.start()
+                for t_ in th:
+                    t_.join()
+                el_pipe = time.perf_counter() - tp
+            grp.set_pipelined(False)
         # paced run, chunks handed over from host memory (int16 hand-off -> events on the host)
         lat, busy, nt = paced_latency(grp, bufs, n_ticks, N, C, args.group_seconds, seed=20261205,
                                       lanes=args.lanes, host_recs=recs)
@@ -460,6 +485,9 @@ def bench_target(args) -> dict:
         rtf = steps * S * N / FS / el
         peel_avg = peel_ms / max(launches, 1)
         res = {"x_realtime": round(rtf, 2), "ms_per_tick": round(el / steps * 1e3, 4),
+               "pipelined": (None if el_pipe is None else
+                             {"x_realtime": round(steps * S * N / FS / el_pipe, 2),
+                              "ms_per_tick": round(el_pipe / steps * 1e3, 4)}),
                "tick_p50_ms": round(pct(wall, 50), 3), "tick_p99_ms": round(pct(wall, 99), 3),
                "paced_latency_p50_ms": round(pct(lat, 50), 3), "paced_latency_p99_ms": round(pct(lat, 99), 3),
                "paced_input": "host int16", "paced_gpu_busy_fraction": round(busy, 4), "paced_launches": nt,
@@ -570,6 +598,9 @@ def bench_target(args) -> dict:
         "speedup_over_plain_fp32_group": g.get("speedup_over_plain_fp32_group"),
         "speedup_over_plain_fp32_sequential": g["speedup_over_plain_fp32_sequential"],
         "plain_fp32_sequential_x_realtime": round(seq_rtf, 2),
+        # the same ticks with two in flight (two pipelined lanes): the capacity with a
+        # backlog, where every mode is bound by the peel alone
+        "pipelined": g.get("pipelined"),
         "modes": modes,
         "vs_fp64_oracle": explain,
         "roofline": roofline,
@@ -789,6 +820,8 @@ def main() -> int:
     ap.add_argument("--group-split", type=int, default=0,
                     help="group and target workloads: peel template split (0 automatic, 1 none, n ways)")
     ap.add_argument("--lanes", type=int, default=4, help="concurrent launch lanes of the paced group runs (1-4)")
+    ap.add_argument("--pipeline", type=int, default=1,
+                    help="target: also time the ticks on two pipelined lanes from two host threads (0 = skip)")
     ap.add_argument("--modes", default="group_fp32,group_bf16,group_bf16_gated,group_bf16x3,group_bf16x3_gated",
                     help="target workload: which group modes to run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
