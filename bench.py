@@ -412,7 +412,10 @@ def bench_target(args) -> dict:
     peel_by_mode = {}
     for name, prec, gated in [mm for mm in all_modes if mm[0] in args.modes.split(",")]:
         def make(profile):
-            gr = StreamGroup([d] * S, batch=W, precision=prec, peel_split=args.group_split, profile=profile)
+            # profiled groups time the front stages and the peel with events (level 2:
+            # without the peel's phase clock, so the peel runs as in production)
+            gr = StreamGroup([d] * S, batch=W, precision=prec, peel_split=args.group_split,
+                             profile=2 if profile else 0)
             if gated:
                 for s in range(S):
                     gr.set_proposal(s, net, args.gate_lo, args.gate_hi, args.gate_reach)
@@ -524,7 +527,10 @@ def bench_target(args) -> dict:
     if tf.exists():
         tr = json.loads(tf.read_text())
         rs = "true" if headline in ("group_bf16", "group_bf16_gated") else "false"   # the headline's peel instance
-        traffic = tr.get("bytes_per_launch", {}).get(f"lss::k_peel<{g['peel_geometry']['rows_per_thread']}, {rs}>")
+        bpl = tr.get("bytes_per_launch", {})
+        rows = g['peel_geometry']['rows_per_thread']
+        # the production instance (no phase clock); older profiles name it without the clock flag
+        traffic = bpl.get(f"lss::k_peel<{rows}, {rs}, false>", bpl.get(f"lss::k_peel<{rows}, {rs}>"))
         traffic = round(traffic) if traffic else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,

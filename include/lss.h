@@ -66,8 +66,11 @@ typedef struct lss_config {
                               of s whose entries are all at most 2^-40 sqrt(nm[s] nm[t]) (the far
                               tails of spatially local templates): a score moves by under 2^-40 of
                               its scale, far below one FP32 rounding of a score near threshold */
-    int32_t profile;       /* 1: time each device stage with HIP events (lss_stage_times);
-                              2: the peel stage only (two events per batch) */
+    int32_t profile;       /* 1: time each device stage with HIP events (lss_stage_times)
+                              and run the peel kernel's phase clock (lss_debug_stage
+                              stage 6; it costs the peel ~5%); 2: the peel stage only
+                              (two events per batch), no phase clock.  Groups: nonzero
+                              times the stages in front of the peel and the peel */
     int32_t peel_split;    /* template subsets per peel column block: 0 = automatic,
                               1 = none, n > 1 = n when that geometry fits, else at
                               most n (Kilosort semantics only) */

@@ -1088,6 +1088,7 @@ void plan_batch(lss_handle* h, const BatchBufs& bb, const int16_t* d_x, const in
     p.dbg_cfmax = bb.dbgCf;
     p.dbg_imax = bb.dbgIm;
     p.prof = bb.prof;
+    p.clock = h->cfg.profile == 1;     // the phase clock: profile level 1 only
     if (h->prop) { p.gate = bb.gate; p.gate_mt = h->TP / GEMM_BM; }
     p.active = xptr;
     if (h->bf16) {

@@ -120,11 +120,12 @@ struct PeelParams {
     const int* iCC; const int* iU;
     const float* Gt;                   // [K][K][2M-1]
     int do_feat;
+    int clock;                         // the phase clock (PROF_WORDS counters) runs: the profiling instance
     // outputs
     OutHeader* hdr; lss_event* ev; float* feat; int ev_cap;
     int half; int nt0min;
     float* dbg_cfmax; int* dbg_imax;   // first-iteration copies, nullable
-    unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch
+    unsigned long long* prof;          // PROF_WORDS counters, zeroed per batch (written when clock != 0)
     // FP32 rescoring of a BF16 projection (rescore != 0).  Every 64 x 64 score
     // tile holding a score that could reach lo2 in FP32 (its BF16 score plus the
     // BF16 error bound) has been recomputed by the FP32 projection: act[(w/64) *

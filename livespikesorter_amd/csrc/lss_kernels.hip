@@ -1043,7 +1043,7 @@ __device__ __forceinline__ uint64_t rt_now() { return __builtin_amdgcn_s_memreal
 
 // RS: the FP32 rescoring of a BF16 projection is compiled in (a separate
 // instance, so the FP32 matcher's register allocation does not carry it).
-template <int R, bool RS>
+template <int R, bool RS, bool CK>
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
@@ -1196,7 +1196,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     int* const flagB = p.iter_cnt + 2 * p.max_iter;
     int* const flagA = flagB + p.NB;
     // phase clock: thread 0 accumulates in LDS (no registers held across the loop)
-    const bool timing = (tid == 0);
+    const bool timing = CK && tid == 0;     // profiling instance only (the clock cost ~5% of the peel)
     __shared__ uint64_t s_ph[NPHASE + 2];      // phases, then the previous mark and the pass count
     if (tid < NPHASE + 2) s_ph[tid] = 0;
     __syncthreads();
@@ -2348,9 +2348,12 @@ void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* 
     hipLaunchKernelGGL(k_norms, dim3(T), dim3(256), 0, s, U, KC, nm, rnm, nm64);
 }
 
-static const void* peel_fn(int R, bool rescore) {
-#define LSS_PEEL_FN(RR)                                                                           \
-    return rescore ? reinterpret_cast<const void*>(k_peel<RR, true>) : reinterpret_cast<const void*>(k_peel<RR, false>)
+static const void* peel_fn(int R, bool rescore, bool clock) {
+#define LSS_PEEL_FN(RR)                                                                                         \
+    return clock ? (rescore ? reinterpret_cast<const void*>(k_peel<RR, true, true>)                             \
+                            : reinterpret_cast<const void*>(k_peel<RR, false, true>))                           \
+                 : (rescore ? reinterpret_cast<const void*>(k_peel<RR, true, false>)                            \
+                            : reinterpret_cast<const void*>(k_peel<RR, false, false>))
     switch (R) {
         case 8: LSS_PEEL_FN(8);
         case 16: LSS_PEEL_FN(16);
@@ -2373,16 +2376,19 @@ hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t 
     PeelBatch copy = b;
     void* args[] = {&copy};
     const int G = b.g_start[b.n];
-    const void* fn = peel_fn(R, b.p[0].rescore != 0);
+    const void* fn = peel_fn(R, b.p[0].rescore != 0, b.p[0].clock != 0);
     if (cooperative) return hipLaunchCooperativeKernel(fn, dim3(G), dim3(PEEL_THREADS), args, 0, s);
     return hipLaunchKernel(fn, dim3(G), dim3(PEEL_THREADS), args, 0, s);
 }
 
 int peel_max_blocks(int device, int R) {
-    int per_cu = 0, per_cu2 = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_fn(R, false), PEEL_THREADS, 0) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, peel_fn(R, true), PEEL_THREADS, 0) != hipSuccess) return 0;
-    per_cu = per_cu < per_cu2 ? per_cu : per_cu2;
+    int per_cu = 1 << 30, cus = 0;
+    for (int v = 0; v < 4; ++v) {
+        int k = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, peel_fn(R, v & 1, v >> 1), PEEL_THREADS, 0) != hipSuccess)
+            return 0;
+        per_cu = k < per_cu ? k : per_cu;
+    }
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return per_cu * cus;
 }

@@ -220,11 +220,14 @@ class StreamGroup:
     split each stream's templates over several peel workgroups per column
     block when that grid is co-resident, 1 keeps one, n > 1 asks for n ways
     for the largest stream (``info[s].peel_split`` reports the choice).
+    ``profile``: 1 (or True) times the stages in front of the peel and the peel
+    with HIP events and runs the peel's phase clock (:meth:`phases`); 2 times
+    them without the clock.
     """
 
     def __init__(self, oss_input_dirs, *, batch: int, device: int = 0, max_iter: int = 0,
                  semantics: int = SEM_KILOSORT, max_events: int = 65536, sparse: int = 2,
-                 precision: int = PREC_FP32, preprocess: int = 0, peel_split: int = 0, profile: bool = False):
+                 precision: int = PREC_FP32, preprocess: int = 0, peel_split: int = 0, profile: int = 0):
         self._g = ct.c_void_p()
         self.n = len(oss_input_dirs)
         cfg = default_config(device=device, chunk=batch, max_batch=batch, max_iter=max_iter,

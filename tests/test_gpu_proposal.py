@@ -192,7 +192,7 @@ def test_trained_gate_gain(tmp_path):
     assert off[-1] + ticks * N + W - N <= len(x)
 
     def group(prec, net):
-        g = StreamGroup([d] * S, batch=W, precision=prec, profile=True)
+        g = StreamGroup([d] * S, batch=W, precision=prec, profile=2)
         if net is not None:
             for k in range(S):
                 g.set_proposal(k, net, 0.1, 0.5, 12)
@@ -245,7 +245,7 @@ def test_gate_at_low_spike_density(tmp_path):
     synchronize()
 
     def group(net):
-        g = StreamGroup([d] * S, batch=W, precision=PREC_BF16X3, profile=True)
+        g = StreamGroup([d] * S, batch=W, precision=PREC_BF16X3, profile=2)
         if net is not None:
             for k in range(S):
                 g.set_proposal(k, net, 0.1, 0.5, 12)
