@@ -1043,7 +1043,7 @@ __device__ __forceinline__ uint64_t rt_now() { return __builtin_amdgcn_s_memreal
 
 // RS: the FP32 rescoring of a BF16 projection is compiled in (a separate
 // instance, so the FP32 matcher's register allocation does not carry it).
-template <int R, bool RS, bool CK>
+template <int R, bool RS, bool CK, bool LS>
 __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
@@ -1121,7 +1121,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // Kilosort rules: every workgroup decides acceptance itself for the columns
     // within M of its block, from the published maxima of the columns within 2M
     // (blocks hlo2..hhi2).  LSS rules (one subset) exchange accepted spikes instead.
-    const bool local = !p.lss;
+    const bool local = !LS;                 // LS: the instance for LSS rules
     unsigned long long* const Pa = p.Pa;                   // scores
     unsigned long long* const Pb = p.Pa + (int64_t)2 * S * W;   // B values
     unsigned long long* const Pt = p.Pa + (int64_t)4 * S * W;   // templates
@@ -1413,10 +1413,10 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                         }
                         asm volatile("" ::: "memory");
                     }
-                    const float thr = p.lss ? amax : amax * (1.f - 0x1p-20f);
+                    const float thr = LS ? amax : amax * (1.f - 0x1p-20f);
                     if (a2 < thr) {
                         const float r = fmaxf(bval, 0.f);
-                        best = p.lss ? amax : (r * r) / nmp[bslot];
+                        best = LS ? amax : (r * r) / nmp[bslot];
                         bo = orp[bslot];
                     } else {
                         best = -1.f;
@@ -1430,7 +1430,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             for (int i = i0; i < i0 + 8; ++i) {
                                 const float r = fmaxf(breg[i], 0.f);
                                 const float a = (r * r) * rnp[i];
-                                const float cf = p.lss ? a : (r * r) / nmp[i];
+                                const float cf = LS ? a : (r * r) / nmp[i];
                                 const int o = orp[i];
                                 const bool gt = a >= thr && (cf > best || (cf == best && o < bo));
                                 best = gt ? cf : best;
@@ -1480,7 +1480,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             in[k] = t < T;
                             on[k] = in[k] && !gated(t);
                             bv[k] = on[k] ? p.B[(int64_t)t * p.BS + w] : 0.f;
-                            nv[k] = on[k] ? (p.lss ? p.rnm[t] : p.nm[t]) : 1.f;
+                            nv[k] = on[k] ? (LS ? p.rnm[t] : p.nm[t]) : 1.f;
                             ov[k] = in[k] ? p.orig[t] : 0;
                         }
 #pragma unroll
@@ -1494,7 +1494,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
                             float cf = 0.f;
                             if (!edge) {
                                 const float r = fmaxf(bv[k], 0.f);
-                                cf = p.lss ? (r * r) * nv[k] : (r * r) / nv[k];
+                                cf = LS ? (r * r) * nv[k] : (r * r) / nv[k];
                             }
                             if (better(cf, ov[k], best, bo)) { best = cf; bi = t; bo = ov[k]; bval = bv[k]; }
                         }
@@ -2348,12 +2348,16 @@ void launch_norms(const float* U, int KC, int T, float* nm, float* rnm, double* 
     hipLaunchKernelGGL(k_norms, dim3(T), dim3(256), 0, s, U, KC, nm, rnm, nm64);
 }
 
-static const void* peel_fn(int R, bool rescore, bool clock) {
+// The peel instance: tile height, FP32 rescoring compiled in (Kilosort rules only),
+// the phase clock (profiling), LSS rules.  Each leaves the code of the others out.
+static const void* peel_fn(int R, bool rescore, bool clock, bool lss) {
 #define LSS_PEEL_FN(RR)                                                                                         \
-    return clock ? (rescore ? reinterpret_cast<const void*>(k_peel<RR, true, true>)                             \
-                            : reinterpret_cast<const void*>(k_peel<RR, false, true>))                           \
-                 : (rescore ? reinterpret_cast<const void*>(k_peel<RR, true, false>)                            \
-                            : reinterpret_cast<const void*>(k_peel<RR, false, false>))
+    if (lss) return clock ? reinterpret_cast<const void*>(k_peel<RR, false, true, true>)                        \
+                          : reinterpret_cast<const void*>(k_peel<RR, false, false, true>);                      \
+    return clock ? (rescore ? reinterpret_cast<const void*>(k_peel<RR, true, true, false>)                      \
+                            : reinterpret_cast<const void*>(k_peel<RR, false, true, false>))                    \
+                 : (rescore ? reinterpret_cast<const void*>(k_peel<RR, true, false, false>)                     \
+                            : reinterpret_cast<const void*>(k_peel<RR, false, false, false>))
     switch (R) {
         case 8: LSS_PEEL_FN(8);
         case 16: LSS_PEEL_FN(16);
@@ -2376,16 +2380,18 @@ hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t 
     PeelBatch copy = b;
     void* args[] = {&copy};
     const int G = b.g_start[b.n];
-    const void* fn = peel_fn(R, b.p[0].rescore != 0, b.p[0].clock != 0);
+    const void* fn = peel_fn(R, b.p[0].rescore != 0, b.p[0].clock != 0, b.p[0].lss != 0);
     if (cooperative) return hipLaunchCooperativeKernel(fn, dim3(G), dim3(PEEL_THREADS), args, 0, s);
     return hipLaunchKernel(fn, dim3(G), dim3(PEEL_THREADS), args, 0, s);
 }
 
 int peel_max_blocks(int device, int R) {
     int per_cu = 1 << 30, cus = 0;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < 6; ++v) {          // rescore x clock (Kilosort rules), clock (LSS rules)
         int k = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, peel_fn(R, v & 1, v >> 1), PEEL_THREADS, 0) != hipSuccess)
+        const bool lss = v >= 4, rescore = !lss && (v & 1), clock = lss ? v == 5 : (v >> 1) != 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, peel_fn(R, rescore, clock, lss), PEEL_THREADS, 0) !=
+            hipSuccess)
             return 0;
         per_cu = k < per_cu ? k : per_cu;
     }
