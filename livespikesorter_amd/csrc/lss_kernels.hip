@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // rows [sb * REG_NRG * R, (sb + 1) * REG_NRG * R) of the block's columns.
     // Subset 0 is the block's owner: it accepts spikes, publishes them and
     // writes the events; the other subsets publish per-column partial maxima.
-    const int S = p.S;
+    const int S = R == 64 ? 1 : p.S;        // a 64-row tile holds every template (T <= 1024): one subset
     const int cb = g / S, sb = g - cb * S;
     const int rbase = R > 0 ? sb * REG_NRG * R : 0;
     const bool owner = sb == 0;
@@ -2377,6 +2377,9 @@ int peel_rows(int T, int CW) {
 }
 
 hipError_t launch_peel(const PeelBatch& b, int R, bool cooperative, hipStream_t s) {
+    if (R == 64)                            // the 64-row instance assumes one template subset
+        for (int j = 0; j < b.n; ++j)
+            if (b.p[j].S != 1) return hipErrorInvalidValue;
     PeelBatch copy = b;
     void* args[] = {&copy};
     const int G = b.g_start[b.n];
