@@ -529,8 +529,8 @@ def bench_target(args) -> dict:
         rs = "true" if headline in ("group_bf16", "group_bf16_gated") else "false"   # the headline's peel instance
         bpl = tr.get("bytes_per_launch", {})
         rows = g['peel_geometry']['rows_per_thread']
-        # the production instance (no phase clock); older profiles name it without the clock flag
-        traffic = bpl.get(f"lss::k_peel<{rows}, {rs}, false>", bpl.get(f"lss::k_peel<{rows}, {rs}>"))
+        # the production instance: no phase clock, Kilosort rules
+        traffic = bpl.get(f"lss::k_peel<{rows}, {rs}, false, false>")
         traffic = round(traffic) if traffic else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
