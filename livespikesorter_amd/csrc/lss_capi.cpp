@@ -1914,6 +1914,7 @@ void group_capture(lss_group* g, GroupLane& L, const int32_t* ld) {
     const int n = g->n;
     L.wb = WhitenBatch{}; L.cb = PcconvBatch{}; L.pb = ProjectBatch{}; L.kb = PeelBatch{}; L.rb = ResetBatch{};
     L.wb.n = L.cb.n = L.pb.n = L.kb.n = L.rb.n = n;
+    L.kb.xcd = n > 1;                   // the peel's streams on their own XCDs
     PreBatch pre{};
     // lanes 1.. of a pipelined group work in buffers of their own
     const bool mine = g->pipelined && L.id > 0;

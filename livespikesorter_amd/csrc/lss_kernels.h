@@ -299,6 +299,10 @@ int peel_rows(int T, int CW);
 struct PeelBatch {
     int n;
     int g_start[MAX_JOBS + 1];     // first workgroup of each stream; g_start[n] = grid size
+    // nonzero: in a launch where every stream is active, XCD x runs the x-th eighth
+    // of the workgroups (streams, then column blocks), so a stream's neighbouring
+    // blocks share an L2; zero (or an idle stream): workgroup order
+    int xcd;
     PeelParams p[MAX_JOBS];
 };
 

@@ -1038,6 +1038,7 @@ __device__ __forceinline__ bool better(float v, int o, float bv, int bo) {
 // features of the final residual.
 // ---------------------------------------------------------------------------
 constexpr int PRE = 4;   // neighbour spikes fetched speculatively with the counts
+constexpr unsigned PEEL_XCDS = 8;   // XCDs of an MI355X (32 CUs and one L2 each)
 
 __device__ __forceinline__ uint64_t rt_now() { return __builtin_amdgcn_s_memrealtime(); }
 
@@ -1048,7 +1049,24 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     // Every product here is rounded before it is added or compared, as in the
     // FP32 reference; a fused multiply-add would move tie decisions by an ulp.
 #pragma clang fp contract(off)
-    const int sj = job_of(bt.g_start, bt.n, blockIdx.x);
+    // Workgroups are dispatched to the 8 XCDs round-robin (from some starting XCD),
+    // and each XCD places its share on its own 32 CUs.  With bt.xcd, and every stream
+    // of the grid active in this launch, XCD x runs the x-th eighth of the logical
+    // workgroups (streams, then column blocks), so neighbouring blocks poll each other
+    // and read the same spikes' ctc rows through one L2: with 8 streams, one stream
+    // per XCD.  With every stream active no other launch of the group has an active
+    // stream (a stream has one batch in flight), so no XCD holds more than
+    // ceil(G / 8) <= 32 waiting workgroups.  A launch with idle streams keeps the
+    // workgroup order, which spreads each stream over every XCD: launches on other
+    // lanes may run peels of their own beside it, and two streams placed on one XCD
+    // would not fit.  (A single stream's peel was slower on contiguous XCD shares:
+    // 264 against 245 us per 512-template, 3122-column batch.)
+    bool all_on = bt.xcd != 0;
+    for (int j = 0; all_on && j < bt.n; ++j) all_on = !bt.p[j].active || *bt.p[j].active;
+    const unsigned gq = (unsigned)bt.g_start[bt.n] / PEEL_XCDS, gr = (unsigned)bt.g_start[bt.n] % PEEL_XCDS;
+    const unsigned xcd = blockIdx.x % PEEL_XCDS;
+    const int gl = all_on ? (int)(xcd * gq + min(xcd, gr) + blockIdx.x / PEEL_XCDS) : (int)blockIdx.x;
+    const int sj = job_of(bt.g_start, bt.n, gl);
     const PeelParams& p = bt.p[sj];
     if (p.active && !*p.active) return;                         // stream idle: every workgroup leaves
     // LDS is kept under 64 KB so every table is reachable with ds_read's 16-bit
@@ -1097,7 +1115,7 @@ __global__ __launch_bounds__(PEEL_THREADS) void k_peel(PeelBatch bt) {
     static_assert(MAX_LIST <= PEEL_THREADS, "rank table reuses the reduction buffer");
     static_assert(2 * MAX_BAND <= PEEL_THREADS && MAX_BAND <= MAX_STAGE, "rescoring pairs fit the reused arrays");
 
-    const int g = blockIdx.x - bt.g_start[sj], tid = threadIdx.x;
+    const int g = gl - bt.g_start[sj], tid = threadIdx.x;
     // Workgroup g owns column block cb = g / S and template subset sb = g % S:
     // rows [sb * REG_NRG * R, (sb + 1) * REG_NRG * R) of the block's columns.
     // Subset 0 is the block's owner: it accepts spikes, publishes them and

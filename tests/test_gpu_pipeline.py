@@ -7,7 +7,8 @@ for, 8 streams of 1500-sample chunks (W = 1622), 12 ticks; the plain FP32
 group and the gated split-BF16 group.  Lane 1 then works in buffers of its own
 while lane 0 uses the handles', and every peel waits for the peel enqueued
 before it, so the device never holds two peel grids; the front stages of one
-tick run beside the other tick's peel.
+tick run beside the other tick's peel.  Also: two lanes of a plain group
+driven at once with disjoint halves of the streams active.
 """
 import threading
 from pathlib import Path
@@ -94,4 +95,46 @@ def test_pipelined_lanes_match_one_lane(streams, mode):
     for k in range(S):
         assert np.array_equal(again[k][0], ref[3][k][0])
     print(f"[pipeline {mode}] {n_ev} events identical over {TICKS} ticks x {S} streams")
+    assert n_ev > 1000
+
+
+def test_concurrent_lanes_disjoint_streams(streams):
+    """Two lanes of a plain (not pipelined) group driven at once from two threads,
+    each with half of the streams active and the others idle: the two launches'
+    peel grids run side by side on the device, and each stream's events and
+    features equal those of one lane taking every stream in turn (with every
+    stream active, the launch that places each stream's workgroups on XCDs of
+    its own)."""
+    d, bufs = streams
+    with StreamGroup([d] * S, batch=W, precision=PREC_FP32) as g:
+        print(f"[lanes] one lane, every stream active, {TICKS} ticks", flush=True)
+        ref = [g.process(_ptrs(bufs, j), [j * N] * S, device=True, lds=[C] * S) for j in range(TICKS)]
+        got = [[None] * S for _ in range(TICKS)]
+        errs = []
+
+        def run(lane):
+            mine = range(lane * S // 2, (lane + 1) * S // 2)
+            try:
+                for j in range(TICKS):
+                    p = [q if k in mine else None for k, q in enumerate(_ptrs(bufs, j))]
+                    out = g.process(p, [j * N] * S, device=True, lds=[C] * S, lane=lane)
+                    for k in mine:
+                        got[j][k] = out[k]
+            except Exception as e:              # reported by the main thread
+                errs.append(e)
+
+        print("[lanes] two lanes, two threads, disjoint halves of the streams", flush=True)
+        th = [threading.Thread(target=run, args=(lane,)) for lane in (0, 1)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert not errs, errs
+    n_ev = 0
+    for j in range(TICKS):
+        for k in range(S):
+            assert np.array_equal(got[j][k][0], ref[j][k][0]), (j, k)
+            assert np.array_equal(got[j][k][1], ref[j][k][1]), (j, k)
+            n_ev += len(ref[j][k][0])
+    print(f"[lanes] {n_ev} events identical over {TICKS} ticks x {S} streams")
     assert n_ev > 1000
